@@ -1,0 +1,1165 @@
+/*
+ * cylpic_oracle.c -- FP64 CPU restatement of cylindrical EPOCH's quasi-3D PIC
+ * timestep.  TEST INFRASTRUCTURE ONLY (see cylpic_oracle.h): the parity
+ * checker for the HIP path, never part of the product.
+ *
+ * Every routine follows the Fortran text of the cited reference lines in the
+ * same operation order (built with -ffp-contract=off), with one rank's
+ * semantics for the MPI exchanges: the r=0 axis and r_max boundary are local,
+ * x halos wrap onto this rank when x is periodic, r is never exchanged.
+ *
+ * Deviations from the reference, all at the last-ulp level:
+ *   - the complex reciprocal 1/exp(-i theta) (particles.F90:323) is done with
+ *     Smith's algorithm; the reference's compiler runtime may round it
+ *     differently;
+ *   - SQRT, ATAN2 come from this C library's libm.
+ * These are "parity unpinned" at ulp level; everything else is pinned by the
+ * tests under tests/ (closed forms, invariants, the worked deposition example
+ * of the reference's documentation).
+ */
+#include "cylpic_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define C_LIGHT 2.99792458e8                                  /* constants.F90:175 */
+#define EPSILON0 8.854187817620389850536563031710750e-12      /* constants.F90:179 */
+#define PI 3.141592653589793238462643383279503                /* constants.F90:172 */
+#define C_TINY 2.2250738585072014e-308                        /* TINY(1.0_num) */
+
+/* ------------------------------------------------------------------ */
+/* indexing                                                           */
+/* ------------------------------------------------------------------ */
+#define NX2(g) ((g)->nx + 2 * (g)->ng)
+#define NY2(g) ((g)->ny + 2 * (g)->ng)
+
+/* Fortran f(ix, iy, m) with lower bounds (1-ng, 1-ng, 0) */
+static inline size_t I3(const orc_grid *g, int ix, int iy, int m) {
+  return ((size_t)m * (size_t)NY2(g) + (size_t)(iy + g->ng - 1)) * (size_t)NX2(g) +
+         (size_t)(ix + g->ng - 1);
+}
+/* Fortran s(iy, m) with lower bounds (1-ng, 0) */
+static inline size_t I2(const orc_grid *g, int iy, int m) {
+  return (size_t)m * (size_t)NY2(g) + (size_t)(iy + g->ng - 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* complex arithmetic with Fortran's operand order                     */
+/* ------------------------------------------------------------------ */
+static inline ocplx cc(double re, double im) { ocplx z; z.re = re; z.im = im; return z; }
+static inline ocplx cadd(ocplx a, ocplx b) { return cc(a.re + b.re, a.im + b.im); }
+static inline ocplx csub(ocplx a, ocplx b) { return cc(a.re - b.re, a.im - b.im); }
+static inline ocplx cneg(ocplx a) { return cc(-a.re, -a.im); }
+static inline ocplx rmul(double s, ocplx a) { return cc(s * a.re, s * a.im); }
+static inline ocplx cmulr(ocplx a, double s) { return cc(a.re * s, a.im * s); }
+static inline ocplx cdivr(ocplx a, double s) { return cc(a.re / s, a.im / s); }
+static inline ocplx cmul(ocplx a, ocplx b) {
+  return cc(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+/* (0, s) * a : the product with a purely imaginary factor */
+static inline ocplx imul(double s, ocplx a) { return cc(-(s * a.im), s * a.re); }
+/* 1 / z by Smith's algorithm */
+static inline ocplx crecip(ocplx z) {
+  if (fabs(z.re) >= fabs(z.im)) {
+    double r = z.im / z.re, den = z.re + z.im * r;
+    return cc(1.0 / den, -r / den);
+  } else {
+    double r = z.re / z.im, den = z.re * r + z.im;
+    return cc(r / den, -1.0 / den);
+  }
+}
+static const ocplx CZERO = {0.0, 0.0};
+
+/* ------------------------------------------------------------------ */
+/* field solver: src/fields.f90                                       */
+/* ------------------------------------------------------------------ */
+
+/* update_e_field, fields.f90:53-182 (E advanced by dt/2) */
+void orc_update_e_field(const orc_grid *g, orc_fields *f) {
+  const int nx = g->nx, ny = g->ny, M = g->n_mode, ng = g->ng;
+  const double c2 = C_LIGHT * C_LIGHT, dt = g->dt, dx = g->dx, dy = g->dy;
+  ocplx *ex = f->ex, *er = f->er, *et = f->et;
+  const ocplx *bx = f->bx, *br = f->br, *bt = f->bt;
+  const ocplx *jx = f->jx, *jr = f->jr, *jt = f->jt;
+
+  /* interior, ir from 1 (r=0 axis is local) to ny, ix from 0 to nx  :62-113 */
+  for (int m = 0; m < M; ++m) {
+    for (int ir = 1; ir <= ny; ++ir) {
+      double r_d = fabs((double)(ir - 1) * dy + g->y_grid_min);
+      double r_p = r_d + 0.5 * dy;
+      double fac_x = c2 / r_p;
+      double a_x = (double)m * fac_x;       /* im_fac_x = (0, a_x) */
+      double a_r = ((double)m * c2) / r_d;  /* im_fac_r = (0, a_r) */
+      for (int ix = 0; ix <= nx; ++ix) {
+        size_t k = I3(g, ix, ir, m);
+        size_t kr1 = I3(g, ix, ir + 1, m);
+        size_t kx1 = I3(g, ix + 1, ir, m);
+        /* (11) exm */
+        ocplx t1 = rmul(fac_x * 0.5, cadd(bt[kr1], bt[k]));
+        ocplx t2 = imul(a_x, br[k]);
+        ocplx t3 = cdivr(rmul(c2, csub(bt[kr1], bt[k])), dy);
+        ocplx t4 = cdivr(jx[k], EPSILON0);
+        ocplx s = csub(cadd(cadd(t1, t2), t3), t4);
+        ex[k] = cadd(ex[k], cmulr(cmulr(s, 0.5), dt));
+        /* (9) erm */
+        t1 = cneg(imul(a_r, bx[k]));
+        t2 = cdivr(rmul(c2, csub(bt[kx1], bt[k])), dx);
+        t3 = cdivr(jr[k], EPSILON0);
+        s = csub(csub(t1, t2), t3);
+        er[k] = cadd(er[k], cmulr(cmulr(s, 0.5), dt));
+        /* (10) etm */
+        t1 = cdivr(rmul(c2, csub(br[kx1], br[k])), dx);
+        t2 = cdivr(rmul(c2, csub(bx[kr1], bx[k])), dy);
+        t3 = cdivr(jt[k], EPSILON0);
+        s = csub(csub(t1, t2), t3);
+        et[k] = cadd(et[k], cmulr(cmulr(s, 0.5), dt));
+      }
+    }
+  }
+
+  /* r_min rows and reflected ghosts, fields.f90:116-180 */
+  for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
+    /* m = 0 */
+    {
+      const int m = 0;
+      size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m);
+      ocplx s = csub(rmul((4.0 * c2) / dy, bt[k1]), cdivr(jx[k0], EPSILON0));
+      ex[k0] = cadd(ex[k0], cmulr(cmulr(s, 0.5), dt));
+      et[k0] = CZERO;
+      er[k0] = cneg(er[k1]);
+      for (int ir = 1 - ng; ir <= -1; ++ir) {
+        et[I3(g, ix, ir, m)] = cneg(et[I3(g, ix, -ir, m)]);
+        er[I3(g, ix, ir, m)] = cneg(er[I3(g, ix, -ir + 1, m)]);
+        ex[I3(g, ix, ir, m)] = ex[I3(g, ix, -ir, m)];
+      }
+    }
+    /* m = 1 */
+    if (M > 1) {
+      const int m = 1;
+      size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m), k2 = I3(g, ix, 2, m);
+      ex[k0] = CZERO;
+      er[k0] = csub(imul(2.0, et[k0]), er[k1]);
+      ocplx w = csub(rmul(9.0, er[k1]), er[k2]);
+      et[k0] = cc(0.125 * w.im, -(0.125 * w.re));
+      for (int ir = 1 - ng; ir <= -1; ++ir) {
+        et[I3(g, ix, ir, m)] = et[I3(g, ix, -ir, m)];
+        er[I3(g, ix, ir, m)] = er[I3(g, ix, -ir + 1, m)];
+        ex[I3(g, ix, ir, m)] = cneg(ex[I3(g, ix, -ir, m)]);
+      }
+    }
+    /* m > 1 */
+    double mode_sign = 1.0;
+    for (int m = 2; m < M; ++m) {
+      size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m), k2 = I3(g, ix, 2, m);
+      ex[k0] = CZERO;
+      et[k0] = CZERO;
+      er[k0] = cneg(er[k1]);
+      er[k1] = cdivr(er[k2], 9.0);
+      for (int ir = 1 - ng; ir <= -1; ++ir) {
+        et[I3(g, ix, ir, m)] = cneg(rmul(mode_sign, et[I3(g, ix, -ir, m)]));
+        er[I3(g, ix, ir, m)] = cneg(rmul(mode_sign, er[I3(g, ix, -ir + 1, m)]));
+        ex[I3(g, ix, ir, m)] = rmul(mode_sign, ex[I3(g, ix, -ir, m)]);
+      }
+      mode_sign = -mode_sign;
+    }
+  }
+}
+
+/* update_b_field, fields.f90:186-312 (B advanced by dt/2) */
+void orc_update_b_field(const orc_grid *g, orc_fields *f) {
+  const int nx = g->nx, ny = g->ny, M = g->n_mode, ng = g->ng;
+  const double dt = g->dt, dx = g->dx, dy = g->dy;
+  ocplx *bx = f->bx, *br = f->br, *bt = f->bt;
+  const ocplx *ex = f->ex, *er = f->er, *et = f->et;
+
+  /* interior, ir from 1 to ny-1 (r_max is local), ix from 0 to nx  :203-241 */
+  for (int m = 0; m < M; ++m) {
+    for (int ir = 1; ir <= ny - 1; ++ir) {
+      double r_d = fabs((double)(ir - 1) * dy + g->y_grid_min);
+      double r_p = r_d + 0.5 * dy;
+      double a_x = (double)m / r_d;  /* im_fac_x = (0, a_x) */
+      double a_r = (double)m / r_p;  /* im_fac_r = (0, a_r) */
+      for (int ix = 0; ix <= nx; ++ix) {
+        size_t k = I3(g, ix, ir, m);
+        size_t krm = I3(g, ix, ir - 1, m);
+        size_t kxm = I3(g, ix - 1, ir, m);
+        /* (8) bxm */
+        ocplx t1 = imul(a_x, er[k]);
+        ocplx t2 = cdivr(rmul(0.5, cadd(et[k], et[krm])), r_d);
+        ocplx t3 = cdivr(csub(et[k], et[krm]), dy);
+        ocplx s = cadd(cadd(t1, t2), t3);
+        bx[k] = csub(bx[k], cmulr(cmulr(s, 0.5), dt));
+        /* (6) brm */
+        t1 = imul(a_r, ex[k]);
+        t2 = cdivr(csub(et[k], et[kxm]), dx);
+        s = cadd(t1, t2);
+        br[k] = cadd(br[k], cmulr(cmulr(s, 0.5), dt));
+        /* (7) btm */
+        t1 = cneg(cdivr(csub(er[k], er[kxm]), dx));
+        t2 = cdivr(csub(ex[k], ex[krm]), dy);
+        s = cadd(t1, t2);
+        bt[k] = cadd(bt[k], cmulr(cmulr(s, 0.5), dt));
+      }
+    }
+  }
+
+  /* r_min rows and reflected ghosts, fields.f90:249-310 */
+  const double idy = 1.0 / dy;  /* imagi/dy = (0, 1/dy) */
+  for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
+    {
+      const int m = 0;
+      size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m);
+      br[k0] = CZERO;
+      bx[k0] = bx[k1];
+      bt[k0] = cneg(bt[k1]);
+      for (int ir = 1 - ng; ir <= -1; ++ir) {
+        bt[I3(g, ix, ir, m)] = cneg(bt[I3(g, ix, -ir + 1, m)]);
+        br[I3(g, ix, ir, m)] = cneg(br[I3(g, ix, -ir, m)]);
+        bx[I3(g, ix, ir, m)] = bx[I3(g, ix, -ir + 1, m)];
+      }
+    }
+    if (M > 1) {
+      const int m = 1;
+      size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m);
+      bx[k0] = cneg(bx[k1]);
+      if (ix >= 2 - ng) {  /* (22) */
+        ocplx t1 = imul(idy, ex[k0]);
+        ocplx t2 = cdivr(csub(et[k0], et[I3(g, ix - 1, 0, m)]), dx);
+        ocplx s = cadd(t1, t2);
+        br[k0] = cadd(br[k0], cmulr(cmulr(s, 0.5), dt));
+      }
+      bt[k0] = csub(imul(-2.0, br[k0]), bt[k1]);
+      for (int ir = 1 - ng; ir <= -1; ++ir) {
+        bt[I3(g, ix, ir, m)] = bt[I3(g, ix, -ir + 1, m)];
+        br[I3(g, ix, ir, m)] = br[I3(g, ix, -ir, m)];
+        bx[I3(g, ix, ir, m)] = cneg(bx[I3(g, ix, -ir + 1, m)]);
+      }
+    }
+    double mode_sign = 1.0;
+    for (int m = 2; m < M; ++m) {
+      size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m);
+      bx[k0] = cneg(bx[k1]);
+      br[k0] = CZERO;
+      bt[k0] = cneg(bt[k1]);
+      for (int ir = 1 - ng; ir <= -1; ++ir) {
+        bt[I3(g, ix, ir, m)] = cneg(rmul(mode_sign, bt[I3(g, ix, -ir + 1, m)]));
+        br[I3(g, ix, ir, m)] = cneg(rmul(mode_sign, br[I3(g, ix, -ir, m)]));
+        bx[I3(g, ix, ir, m)] = rmul(mode_sign, bx[I3(g, ix, -ir + 1, m)]);
+      }
+      mode_sign = -mode_sign;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* field boundaries: src/boundary.F90                                 */
+/* ------------------------------------------------------------------ */
+
+/* Stagger flags (stagger(c_dir_x/y, c_stagger_*m)), constants.F90:291-299
+ * with the table of setup.F90:127-136. */
+enum { S_EX, S_ER, S_ET, S_BX, S_BR, S_BT };
+static const int STAG_X[6] = {0, 1, 1, 1, 0, 0};
+static const int STAG_Y[6] = {1, 0, 1, 0, 1, 0};
+
+/* do_field_mpi_with_lengths_cmplx on one rank, boundary.F90:500-593:
+ * only the x exchange can be non-trivial (periodic onto itself).
+ * rows [row_lo, row_hi] are the rows that reach the caller (the r-shift
+ * copy of efield_bcs/bfield_bcs drops the top row, boundary.F90:1362-1371). */
+static void halo_x(const orc_grid *g, ocplx *F, int row_lo, int row_hi) {
+  const int nx = g->nx, ng = g->ng;
+  for (int m = 0; m < g->n_mode; ++m)
+    for (int j = row_lo; j <= row_hi; ++j) {
+      if (g->bc_field[ORC_XMAX] == ORC_BC_PERIODIC)
+        for (int i = 1; i <= ng; ++i) F[I3(g, nx + i, j, m)] = F[I3(g, i, j, m)];
+      if (g->bc_field[ORC_XMIN] == ORC_BC_PERIODIC)
+        for (int i = 1; i <= ng; ++i) F[I3(g, i - ng, j, m)] = F[I3(g, nx - ng + i, j, m)];
+    }
+}
+
+/* field_mode_clamp_zero (sign = -1) and field_mode_zero_gradient (sign = +1),
+ * boundary.F90:654-707 and :772-829; called only for x_min, x_max, y_max. */
+static void mode_bc(const orc_grid *g, ocplx *F, int stag, int bnd, int clamp) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng, M = g->n_mode;
+  if (g->bc_field[bnd] == ORC_BC_PERIODIC) return;
+#define SETV(a, b) F[a] = clamp ? cneg(F[b]) : F[b]
+  for (int m = 0; m < M; ++m) {
+    if (bnd == ORC_XMIN) {
+      for (int j = 1 - ng; j <= ny + ng; ++j) {
+        if (STAG_X[stag]) {
+          for (int i = 1; i <= ng - 1; ++i) SETV(I3(g, i - ng, j, m), I3(g, ng - i, j, m));
+          if (clamp) F[I3(g, 0, j, m)] = CZERO;
+        } else {
+          for (int i = 1; i <= ng; ++i) SETV(I3(g, i - ng, j, m), I3(g, ng + 1 - i, j, m));
+        }
+      }
+    } else if (bnd == ORC_XMAX) {
+      for (int j = 1 - ng; j <= ny + ng; ++j) {
+        if (STAG_X[stag]) {
+          if (clamp) F[I3(g, nx, j, m)] = CZERO;
+          for (int i = 1; i <= ng - 1; ++i) SETV(I3(g, nx + i, j, m), I3(g, nx - i, j, m));
+        } else {
+          for (int i = 1; i <= ng; ++i) SETV(I3(g, nx + i, j, m), I3(g, nx + 1 - i, j, m));
+        }
+      }
+    } else if (bnd == ORC_YMAX) {
+      if (STAG_Y[stag]) {
+        if (clamp)
+          for (int ix = 1 - ng; ix <= nx + ng; ++ix) F[I3(g, ix, ny, m)] = CZERO;
+        for (int i = 1; i <= ng - 1; ++i)
+          for (int ix = 1 - ng; ix <= nx + ng; ++ix) SETV(I3(g, ix, ny + i, m), I3(g, ix, ny - i, m));
+      } else {
+        for (int i = 1; i <= ng; ++i)
+          for (int ix = 1 - ng; ix <= nx + ng; ++ix) SETV(I3(g, ix, ny + i, m), I3(g, ix, ny + 1 - i, m));
+      }
+    }
+  }
+#undef SETV
+}
+
+/* the generic clamp / zero-gradient loop shared by efield_bcs and bfield_bcs */
+static void generic_edges(const orc_grid *g, ocplx *a, ocplx *b, ocplx *c,
+                          int sa, int sb, int sc) {
+  for (int i = ORC_XMIN; i <= ORC_YMAX; ++i) {
+    if (i == ORC_YMIN) continue;
+    int bc = g->bc_field[i];
+    if (bc == ORC_BC_CLAMP || bc == ORC_BC_SIMPLE_LASER || bc == ORC_BC_SIMPLE_OUTFLOW) {
+      mode_bc(g, a, sa, i, 1);
+      mode_bc(g, b, sb, i, 1);
+      mode_bc(g, c, sc, i, 1);
+    }
+    if (bc == ORC_BC_ZERO_GRADIENT) {
+      mode_bc(g, a, sa, i, 0);
+      mode_bc(g, b, sb, i, 0);
+      mode_bc(g, c, sc, i, 0);
+    }
+  }
+}
+
+/* efield_bcs, boundary.F90:1355-1413 */
+void orc_efield_bcs(const orc_grid *g, orc_fields *f) {
+  const int ny = g->ny, ng = g->ng;
+  halo_x(g, f->ex, 1 - ng, ny + ng - 1);
+  halo_x(g, f->er, 1 - ng, ny + ng);
+  halo_x(g, f->et, 1 - ng, ny + ng - 1);
+  for (int i = ORC_XMIN; i <= ORC_XMAX; ++i) {
+    if (g->bc_field[i] == ORC_BC_CONDUCT) {
+      mode_bc(g, f->ex, S_EX, i, 1);
+      mode_bc(g, f->er, S_ER, i, 0);
+      mode_bc(g, f->et, S_ET, i, 0);
+    }
+  }
+  if (g->bc_field[ORC_YMAX] == ORC_BC_CONDUCT) {
+    mode_bc(g, f->ex, S_EX, ORC_YMAX, 0);
+    mode_bc(g, f->er, S_ER, ORC_YMAX, 1);
+    mode_bc(g, f->et, S_ET, ORC_YMAX, 0);
+  }
+  generic_edges(g, f->ex, f->er, f->et, S_EX, S_ER, S_ET);
+}
+
+/* bfield_bcs, boundary.F90:1417-1476 */
+void orc_bfield_bcs(const orc_grid *g, orc_fields *f, int mpi_only) {
+  const int ny = g->ny, ng = g->ng;
+  halo_x(g, f->bx, 1 - ng, ny + ng);
+  halo_x(g, f->br, 1 - ng, ny + ng - 1);
+  halo_x(g, f->bt, 1 - ng, ny + ng);
+  if (mpi_only) return;
+  for (int i = ORC_XMIN; i <= ORC_XMAX; ++i) {
+    if (g->bc_field[i] == ORC_BC_CONDUCT) {
+      mode_bc(g, f->bx, S_BX, i, 0);
+      mode_bc(g, f->br, S_BR, i, 1);
+      mode_bc(g, f->bt, S_BT, i, 1);
+    }
+  }
+  if (g->bc_field[ORC_YMAX] == ORC_BC_CONDUCT) {
+    mode_bc(g, f->bx, S_BX, ORC_YMAX, 1);
+    mode_bc(g, f->br, S_BR, ORC_YMAX, 0);
+    mode_bc(g, f->bt, S_BT, ORC_YMAX, 1);
+  }
+  generic_edges(g, f->bx, f->br, f->bt, S_BX, S_BR, S_BT);
+}
+
+/* setup_field_boundaries, setup.F90:393-420 (no CPML: nx0 = 1, nx1 = nx) */
+void orc_setup_field_boundaries(const orc_grid *g, orc_fields *f) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng;
+  for (int m = 0; m < g->n_mode; ++m)
+    for (int j = 1 - ng; j <= ny + ng; ++j) {
+      size_t s = I2(g, j, m);
+      f->bnd_xmin[0][s] = rmul(0.5, cadd(f->ex[I3(g, 1, j, m)], f->ex[I3(g, 0, j, m)]));
+      f->bnd_xmin[1][s] = f->er[I3(g, 0, j, m)];
+      f->bnd_xmin[2][s] = f->et[I3(g, 0, j, m)];
+      f->bnd_xmax[0][s] = rmul(0.5, cadd(f->ex[I3(g, nx, j, m)], f->ex[I3(g, nx + 1, j, m)]));
+      f->bnd_xmax[1][s] = f->er[I3(g, nx, j, m)];
+      f->bnd_xmax[2][s] = f->et[I3(g, nx, j, m)];
+      f->bnd_xmin[3][s] = f->bx[I3(g, 0, j, m)];
+      f->bnd_xmin[4][s] = rmul(0.5, cadd(f->br[I3(g, 1, j, m)], f->br[I3(g, 0, j, m)]));
+      f->bnd_xmin[5][s] = rmul(0.5, cadd(f->bt[I3(g, 1, j, m)], f->bt[I3(g, 0, j, m)]));
+      f->bnd_xmax[3][s] = f->bx[I3(g, nx, j, m)];
+      f->bnd_xmax[4][s] = rmul(0.5, cadd(f->br[I3(g, nx, j, m)], f->br[I3(g, nx + 1, j, m)]));
+      f->bnd_xmax[5][s] = rmul(0.5, cadd(f->bt[I3(g, nx, j, m)], f->bt[I3(g, nx + 1, j, m)]));
+    }
+}
+
+/* The m=1 plane-wave sources of laser.f90:476-489: source_t = s1 + i s2,
+ * source_r = -i s1 + s2. */
+static inline ocplx src_t(const double *src, int ny, int ir) {
+  return cc(src[ir], src[ny + 1 + ir]);
+}
+static inline ocplx src_r(const double *src, int ny, int ir) {
+  return cc(src[ny + 1 + ir], -src[ir]);
+}
+
+/* outflow_bcs_x_min, laser.f90:411-520.  src = source1(0:ny), source2(0:ny)
+ * as evaluated by the host, or NULL (no laser attached: zero sources).
+ * The r_d_vals(0:ny) term of the Btm update pairs with bxm(1,1:ny): the
+ * arrays are non-conformable and the reference's element-wise evaluation
+ * pairs row ir with r_d_vals(ir-1); that pairing is kept. */
+void orc_outflow_bcs_x_min(const orc_grid *g, orc_fields *f, const double *src) {
+  const int ny = g->ny, M = g->n_mode;
+  const double c = C_LIGHT, dt = g->dt;
+  const double dtc2 = dt * (c * c), lx = dtc2 / g->dx, lr = dtc2 / g->dy;
+  const double sum = 1.0 / (lx + c), diff = lx - c, dt_eps = dt / EPSILON0;
+  for (int m = 0; m < M; ++m)
+    for (int ir = 0; ir <= ny; ++ir) f->bx[I3(g, 0, ir, m)] = f->bnd_xmin[3][I2(g, ir, m)];
+  for (int m = 0; m < M; ++m) {
+    for (int ir = 1; ir <= ny; ++ir) {
+      ocplx st = (m == 1 && src) ? src_t(src, ny, ir) : CZERO;
+      double r_d = fabs((double)(ir - 1 - 1) * g->dy + g->y_grid_min);
+      ocplx a = rmul(4.0, st);
+      ocplx b = rmul(2.0, cadd(f->bnd_xmin[1][I2(g, ir, m)], rmul(c, f->bnd_xmin[5][I2(g, ir, m)])));
+      ocplx d = rmul(2.0, f->er[I3(g, 1, ir, m)]);
+      ocplx e = cdivr(imul(((double)m * (c * c)) * dt, f->bx[I3(g, 1, ir, m)]), r_d);
+      ocplx h = rmul(dt_eps, f->jr[I3(g, 1, ir, m)]);
+      ocplx k = rmul(diff, f->bt[I3(g, 2, ir, m)]);
+      f->bt[I3(g, 1, ir, m)] = rmul(sum, cadd(cadd(cadd(csub(cadd(a, b), d), e), h), k));
+    }
+    const int ir_l = 1, ir_h = ny - 1;
+    for (int ir = ir_l; ir <= ir_h; ++ir) {
+      ocplx sr = (m == 1 && src) ? src_r(src, ny, ir) : CZERO;
+      ocplx a = rmul(-4.0, sr);
+      ocplx b = rmul(2.0, cadd(f->bnd_xmin[2][I2(g, ir, m)], rmul(c, f->bnd_xmin[4][I2(g, ir, m)])));
+      ocplx d = rmul(2.0, f->et[I3(g, 1, ir, m)]);
+      ocplx e = rmul(lr, csub(f->bx[I3(g, 1, ir + 1, m)], f->bx[I3(g, 1, ir, m)]));
+      ocplx h = rmul(dt_eps, f->jt[I3(g, 1, ir, m)]);
+      ocplx k = rmul(diff, f->br[I3(g, 2, ir, m)]);
+      f->br[I3(g, 1, ir, m)] = rmul(sum, cadd(csub(csub(cadd(csub(a, b), d), e), h), k));
+    }
+  }
+}
+
+/* outflow_bcs_x_max, laser.f90:524-633.  Both source_t(0:ny) and
+ * r_d_vals(0:ny) pair with rows 1:ny offset by one, as in the reference. */
+void orc_outflow_bcs_x_max(const orc_grid *g, orc_fields *f, const double *src) {
+  const int nx = g->nx, ny = g->ny, M = g->n_mode;
+  const double c = C_LIGHT, dt = g->dt;
+  const double dtc2 = dt * (c * c), lx = dtc2 / g->dx, lr = dtc2 / g->dy;
+  const double sum = 1.0 / (lx + c), diff = lx - c, dt_eps = dt / EPSILON0;
+  for (int m = 0; m < M; ++m)
+    for (int ir = 0; ir <= ny; ++ir) f->bx[I3(g, nx, ir, m)] = f->bnd_xmax[3][I2(g, ir, m)];
+  for (int m = 0; m < M; ++m) {
+    for (int ir = 1; ir <= ny; ++ir) {
+      ocplx st = (m == 1 && src) ? src_t(src, ny, ir - 1) : CZERO;
+      double r_d = fabs((double)(ir - 1 - 1) * g->dy + g->y_grid_min);
+      ocplx a = rmul(-4.0, st);
+      ocplx b = rmul(2.0, cadd(f->bnd_xmax[1][I2(g, ir, m)], rmul(c, f->bnd_xmax[5][I2(g, ir, m)])));
+      ocplx d = rmul(2.0, f->er[I3(g, nx - 1, ir, m)]);
+      ocplx e = cdivr(imul(((double)m * (c * c)) * dt, f->bx[I3(g, nx - 1, ir, m)]), r_d);
+      ocplx h = rmul(dt_eps, f->jr[I3(g, nx - 1, ir, m)]);
+      ocplx k = rmul(diff, f->bt[I3(g, nx - 1, ir, m)]);
+      f->bt[I3(g, nx, ir, m)] = rmul(sum, cadd(csub(csub(cadd(csub(a, b), d), e), h), k));
+    }
+    const int ir_l = 1, ir_h = ny - 1;
+    for (int ir = ir_l; ir <= ir_h; ++ir) {
+      ocplx sr = (m == 1 && src) ? src_r(src, ny, ir) : CZERO;
+      ocplx a = rmul(4.0, sr);
+      ocplx b = rmul(2.0, cadd(f->bnd_xmax[2][I2(g, ir, m)], rmul(c, f->bnd_xmax[4][I2(g, ir, m)])));
+      ocplx d = rmul(2.0, f->et[I3(g, nx - 1, ir, m)]);
+      ocplx e = rmul(lr, csub(f->bx[I3(g, nx - 1, ir + 1, m)], f->bx[I3(g, nx - 1, ir, m)]));
+      ocplx h = rmul(dt_eps, f->jt[I3(g, nx - 1, ir, m)]);
+      ocplx k = rmul(diff, f->br[I3(g, nx - 1, ir, m)]);
+      f->br[I3(g, nx, ir, m)] = rmul(sum, cadd(cadd(cadd(csub(cadd(a, b), d), e), h), k));
+    }
+  }
+}
+
+/* outflow_bcs_r_max, laser.f90:637-690 */
+void orc_outflow_bcs_r_max(const orc_grid *g, orc_fields *f) {
+  const int nx = g->nx, ny = g->ny, M = g->n_mode;
+  const double c = C_LIGHT, dt = g->dt;
+  const double dtc2 = dt * (c * c);
+  const double inv_r = 1.0 / (((double)ny - 1.5) * g->dy + g->y_grid_min);
+  const double dtc2_4r = 0.25 * dtc2 * inv_r;
+  const double icdt = ((0.5 * c) * dt) * inv_r;  /* icdt_2r = (0, icdt) */
+  const double lx = dtc2 / g->dx, ly = dtc2 / g->dy;
+  const double sum_x = 1.0 / (ly + c), sum_t = 1.0 / ((ly + c) + dtc2_4r);
+  const double dt_2eps = (0.5 * dt) / EPSILON0;
+  const int ix_l = 1, ix_h = nx - 1;
+  for (int m = 0; m < M; ++m) {
+    for (int ix = ix_l; ix <= ix_h; ++ix) {
+      ocplx a = cneg(cmulr(f->bx[I3(g, ix, ny - 1, m)], c - ly));
+      ocplx b = cmulr(f->bx_old[I3(g, ix, ny, m)], -c + ly);
+      ocplx d = cmulr(f->bx_old[I3(g, ix, ny - 1, m)], -c - ly);
+      ocplx e = rmul((c * dt) * inv_r, f->et[I3(g, ix, ny - 1, m)]);
+      ocplx h = rmul(0.5 * lx,
+                     csub(cadd(csub(f->br[I3(g, ix + 1, ny - 1, m)], f->br[I3(g, ix, ny - 1, m)]),
+                               f->br_old[I3(g, ix + 1, ny - 1, m)]),
+                          f->br_old[I3(g, ix, ny - 1, m)]));
+      ocplx k = imul(icdt * (double)m, cadd(f->er[I3(g, ix, ny, m)], f->er[I3(g, ix, ny - 1, m)]));
+      ocplx l = rmul(dt_2eps, cadd(f->jt[I3(g, ix, ny - 1, m)], f->jt_old[I3(g, ix, ny - 1, m)]));
+      f->bx[I3(g, ix, ny, m)] =
+          rmul(sum_x, csub(csub(cadd(csub(csub(csub(a, b), d), e), h), k), l));
+    }
+    for (int ix = 1; ix <= nx; ++ix) {
+      ocplx a = cneg(cmulr(f->bt[I3(g, ix, ny - 1, m)], (c - ly) + dtc2_4r));
+      ocplx b = cmulr(f->bt_old[I3(g, ix, ny, m)], (-c + ly) + dtc2_4r);
+      ocplx d = cmulr(f->bt_old[I3(g, ix, ny - 1, m)], (-c - ly) + dtc2_4r);
+      ocplx e = rmul((0.5 * lx) / c,
+                     csub(csub(cadd(f->er[I3(g, ix, ny, m)], f->er[I3(g, ix, ny - 1, m)]),
+                               f->er[I3(g, ix - 1, ny, m)]),
+                          f->er[I3(g, ix - 1, ny - 1, m)]));
+      ocplx k = imul((icdt * (double)m) * c,
+                     cadd(f->br[I3(g, ix, ny - 1, m)], f->br_old[I3(g, ix, ny - 1, m)]));
+      ocplx l = rmul(dt_2eps, cadd(f->jx[I3(g, ix, ny - 1, m)], f->jx_old[I3(g, ix, ny - 1, m)]));
+      f->bt[I3(g, ix, ny, m)] = rmul(sum_t, cadd(csub(csub(csub(csub(a, b), d), e), k), l));
+    }
+  }
+}
+
+/* bfield_final_bcs, boundary.F90:1505-1537 (laser phases are host-side) */
+void orc_bfield_final_bcs(const orc_grid *g, orc_fields *f,
+                          const double *src_xmin, const double *src_xmax) {
+  orc_bfield_bcs(g, f, 0);
+  int b = g->bc_field[ORC_XMIN];
+  if (b == ORC_BC_SIMPLE_LASER || b == ORC_BC_SIMPLE_OUTFLOW)
+    orc_outflow_bcs_x_min(g, f, b == ORC_BC_SIMPLE_LASER ? src_xmin : NULL);
+  b = g->bc_field[ORC_XMAX];
+  if (b == ORC_BC_SIMPLE_LASER || b == ORC_BC_SIMPLE_OUTFLOW)
+    orc_outflow_bcs_x_max(g, f, b == ORC_BC_SIMPLE_LASER ? src_xmax : NULL);
+  b = g->bc_field[ORC_YMAX];
+  if (b == ORC_BC_SIMPLE_OUTFLOW) {
+    orc_outflow_bcs_r_max(g, f);
+  } else if (b == ORC_BC_ZERO_B) {
+    for (int m = 0; m < g->n_mode; ++m)
+      for (int ix = 1 - g->ng; ix <= g->nx + g->ng; ++ix) {
+        size_t k = I3(g, ix, g->ny, m);
+        f->bx[k] = CZERO; f->br[k] = CZERO; f->bt[k] = CZERO;
+      }
+  }
+  orc_bfield_bcs(g, f, 1);
+}
+
+/* update_eb_fields_half, fields.f90:316-337 */
+void orc_update_eb_fields_half(const orc_grid *g, orc_fields *f) {
+  size_t n = (size_t)NX2(g) * NY2(g) * g->n_mode * sizeof(ocplx);
+  orc_update_e_field(g, f);
+  orc_efield_bcs(g, f);
+  memcpy(f->bx_old, f->bx, n);
+  memcpy(f->br_old, f->br, n);
+  memcpy(f->bt_old, f->bt, n);
+  orc_update_b_field(g, f);
+  orc_bfield_bcs(g, f, 1);
+}
+
+/* update_eb_fields_final, fields.f90:341-353 */
+void orc_update_eb_fields_final(const orc_grid *g, orc_fields *f,
+                                const double *src_xmin, const double *src_xmax) {
+  orc_update_b_field(g, f);
+  orc_bfield_final_bcs(g, f, src_xmin, src_xmax);
+  orc_update_e_field(g, f);
+  orc_efield_bcs(g, f);
+}
+
+/* ------------------------------------------------------------------ */
+/* currents: src/boundary.F90, src/housekeeping/current_smooth.F90     */
+/* ------------------------------------------------------------------ */
+
+/* particle_reflection_bcs_complex, boundary.F90:918-1015, one mode plane */
+static void reflect_current(const orc_grid *g, ocplx *J, int m, int flip_dir) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng;
+  if (g->bc_particle[ORC_XMIN] == ORC_BC_REFLECT) {
+    for (int j = 1 - ng; j <= ny + ng; ++j) {
+      if (flip_dir == 1) {
+        for (int i = 1; i <= ng - 1; ++i) {
+          J[I3(g, i, j, m)] = csub(J[I3(g, i, j, m)], J[I3(g, 1 - i, j, m)]);
+          J[I3(g, 1 - i, j, m)] = CZERO;
+        }
+      } else {
+        for (int i = 1; i <= ng - 1; ++i) {
+          J[I3(g, i, j, m)] = cadd(J[I3(g, i, j, m)], J[I3(g, -i, j, m)]);
+          J[I3(g, -i, j, m)] = CZERO;
+        }
+      }
+    }
+  }
+  if (g->bc_particle[ORC_XMAX] == ORC_BC_REFLECT) {
+    for (int j = 1 - ng; j <= ny + ng; ++j) {
+      if (flip_dir == 1) {
+        for (int i = 1; i <= ng; ++i) {
+          J[I3(g, nx + 1 - i, j, m)] = csub(J[I3(g, nx + 1 - i, j, m)], J[I3(g, nx + i, j, m)]);
+          J[I3(g, nx + i, j, m)] = CZERO;
+        }
+      } else {
+        for (int i = 1; i <= ng; ++i) {
+          J[I3(g, nx - i, j, m)] = cadd(J[I3(g, nx - i, j, m)], J[I3(g, nx + i, j, m)]);
+          J[I3(g, nx + i, j, m)] = CZERO;
+        }
+      }
+    }
+  }
+  if (g->bc_particle[ORC_YMAX] == ORC_BC_REFLECT) {
+    const double r_max = g->y_grid_min + ((double)ny - 0.5) * g->dy;
+    for (int i = 1; i <= ng; ++i) {
+      for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
+        if (flip_dir == 2) {
+          double num = r_max + ((double)i - 0.5) * g->dy;
+          double den = r_max - ((double)i - 0.5) * g->dy;
+          J[I3(g, ix, ny + 1 - i, m)] =
+              csub(J[I3(g, ix, ny + 1 - i, m)], cdivr(cmulr(J[I3(g, ix, ny + i, m)], num), den));
+        } else if (flip_dir == 1) {
+          double num = r_max + (double)i * g->dy;
+          double den = r_max - (double)i * g->dy;
+          J[I3(g, ix, ny - i, m)] =
+              cadd(J[I3(g, ix, ny - i, m)], cdivr(cmulr(J[I3(g, ix, ny + i, m)], num), den));
+        } else {
+          J[I3(g, ix, ny - i, m)] = cadd(J[I3(g, ix, ny - i, m)], J[I3(g, ix, ny + i, m)]);
+        }
+        J[I3(g, ix, ny + i, m)] = CZERO;
+      }
+    }
+  }
+}
+
+/* particle_periodic_bcs_complex on one rank, boundary.F90:1133-1245 */
+static void periodic_current(const orc_grid *g, ocplx *J, int m) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng;
+  if (!(g->bc_particle[ORC_XMIN] == ORC_BC_PERIODIC && g->bc_particle[ORC_XMAX] == ORC_BC_PERIODIC))
+    return;
+  for (int j = 1 - ng; j <= ny + ng; ++j)
+    for (int i = 1; i <= ng; ++i)
+      J[I3(g, i, j, m)] = cadd(J[I3(g, i, j, m)], J[I3(g, nx + i, j, m)]);
+  for (int j = 1 - ng; j <= ny + ng; ++j)
+    for (int i = 1; i <= ng; ++i)
+      J[I3(g, nx - ng + i, j, m)] = cadd(J[I3(g, nx - ng + i, j, m)], J[I3(g, i - ng, j, m)]);
+}
+
+/* current_bcs without a species, boundary.F90:1893-1905 -> :1330-1351 */
+void orc_current_bcs(const orc_grid *g, orc_fields *f) {
+  for (int m = 0; m < g->n_mode; ++m) {
+    reflect_current(g, f->jx, m, 1);
+    periodic_current(g, f->jx, m);
+    reflect_current(g, f->jr, m, 2);
+    periodic_current(g, f->jr, m);
+    reflect_current(g, f->jt, m, 3);
+    periodic_current(g, f->jt, m);
+  }
+}
+
+/* current_bcs_r_min_final, boundary.F90:1909-1959 */
+void orc_current_bcs_r_min_final(const orc_grid *g, orc_fields *f) {
+  const int nx = g->nx, ng = g->ng, M = g->n_mode;
+  ocplx *jx = f->jx, *jr = f->jr, *jt = f->jt;
+  double mode_sign = 1.0;
+  for (int m = 0; m < M; ++m) {
+    for (int j = 2; j <= ng; ++j)
+      for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
+        size_t a = I3(g, ix, 1 - j, m);
+        jx[I3(g, ix, j - 1, m)] = cadd(jx[I3(g, ix, j - 1, m)], rmul(mode_sign, jx[a]));
+        jr[I3(g, ix, j, m)] = csub(jr[I3(g, ix, j, m)], rmul(mode_sign, jr[a]));
+        jt[I3(g, ix, j - 1, m)] = csub(jt[I3(g, ix, j - 1, m)], rmul(mode_sign, jt[a]));
+        jx[a] = CZERO; jr[a] = CZERO; jt[a] = CZERO;
+      }
+    for (int ix = 1 - ng; ix <= nx + ng; ++ix)
+      jr[I3(g, ix, 1, m)] = csub(jr[I3(g, ix, 1, m)], rmul(mode_sign, jr[I3(g, ix, 0, m)]));
+    mode_sign = -mode_sign;
+  }
+  for (int m = 0; m < M; ++m)
+    for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
+      size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m), k2 = I3(g, ix, 2, m);
+      if (m > 0) jx[k0] = CZERO;
+      else jx[k0] = cdivr(csub(rmul(4.0, jx[k1]), jx[k2]), 3.0);
+      if (m == 1) {
+        ocplx w = csub(rmul(9.0, jr[k1]), jr[k2]);
+        jt[k0] = cc(w.im / 8.0, -(w.re / 8.0));
+        jr[k0] = csub(imul(2.0, jt[k0]), jr[k1]);
+      } else {
+        jt[k0] = CZERO;
+        jr[k0] = cneg(jr[k1]);
+      }
+    }
+}
+
+/* current_finish, current_smooth.F90:29-46 (no smoothing, no correction) */
+void orc_current_finish(const orc_grid *g, orc_fields *f) {
+  orc_current_bcs(g, f);
+  halo_x(g, f->jx, 1 - g->ng, g->ny + g->ng);
+  halo_x(g, f->jr, 1 - g->ng, g->ny + g->ng);
+  halo_x(g, f->jt, 1 - g->ng, g->ny + g->ng);
+}
+
+/* ------------------------------------------------------------------ */
+/* particle push and deposit: src/particles.F90                       */
+/* ------------------------------------------------------------------ */
+
+/* per-step geometry tables, particles.F90:190-217.  Arrays are offset so
+ * that element [iy + ng] is Fortran index iy (inv_area_xt from 0-jng). */
+void orc_geometry(const orc_grid *g, double *inv_area_rt, double *inv_area_xt,
+                  double *inv_volume, double *ratio_area_xt) {
+  const int ny = g->ny, jng = g->ng;
+  const double dx = g->dx, dy = g->dy;
+  double r_low = g->y_grid_min - (double)jng * dy;
+  inv_area_xt[0 - jng + jng] = 1.0 / (((2.0 * PI) * fabs(r_low)) * dx);
+  for (int iy = 1 - jng; iy <= ny + jng; ++iy) {
+    if (lround((2.0 * r_low) / dy) == -1) {
+      double h = 0.5 * dy;
+      inv_area_rt[iy + jng] = 1.0 / (PI * (h * h));
+    } else {
+      double rh = r_low + dy;
+      inv_area_rt[iy + jng] = 1.0 / (PI * fabs(rh * rh - r_low * r_low));
+    }
+    inv_area_xt[iy + jng] = 1.0 / (((2.0 * PI) * fabs(r_low + dy)) * dx);
+    r_low = r_low + dy;
+  }
+  for (int iy = 1 - jng; iy <= ny + jng; ++iy) {
+    inv_volume[iy + jng] = inv_area_rt[iy + jng] / dx;
+    ratio_area_xt[iy + jng] = inv_area_xt[iy + jng] / inv_area_xt[iy - 1 + jng];
+  }
+}
+
+/* triangle/gx.inc (centred weights, sum 2) */
+static inline void weights3(double frac, double *w /* [-1..1] */) {
+  double cf2 = frac * frac;
+  w[-1] = (0.25 + cf2) + frac;
+  w[0] = 1.5 - 2.0 * cf2;
+  w[1] = (0.25 + cf2) - frac;
+}
+
+/* The DO im / DO iy / DO ix nest of particles.F90:586-665. */
+void orc_deposit_from_weights(const orc_grid *g, orc_fields *f,
+                              const double *gx_, const double *hx_,
+                              const double *gy_, const double *hy_,
+                              int cell_x2, int cell_y2, int dcellx, int dcelly,
+                              double fcx, double fcy, double fcz,
+                              ocplx exp_itheta_05, ocplx exp_idtheta,
+                              double dtheta,
+                              const double *inv_area_rt,
+                              const double *inv_area_xt,
+                              const double *inv_volume,
+                              const double *ratio_area_xt) {
+  const double *gx = gx_ + 2, *hx = hx_ + 2, *gy = gy_ + 2, *hy = hy_ + 2;
+  const int jng = g->ng;
+  const double third = 1.0 / 3.0, sixth = 0.5 * third;
+  const int xmin = -1 + (dcellx - 1) / 2, xmax = 1 + (dcellx + 1) / 2;
+  const int ymin = -1 + (dcelly - 1) / 2, ymax = 1 + (dcelly + 1) / 2;
+  ocplx exp_imtheta0 = cc(1.0, 0.0), exp_imdtheta = cc(1.0, 0.0);
+  ocplx m_fac_1 = CZERO, m_fac_2 = CZERO, m_fac_3 = CZERO, m_fac_4 = CZERO;
+  double inv_mdth = 0.0, inv_m2dth2 = 0.0;
+
+  for (int im = 0; im < g->n_mode; ++im) {
+    double mdth = (double)im * dtheta;
+    double m2dth2 = mdth * mdth;
+    int small_angle_change = fabs(mdth) < 1.0e-4;
+    if (!small_angle_change && im > 0) {
+      inv_mdth = 1.0 / mdth;
+      inv_m2dth2 = inv_mdth * inv_mdth;
+    }
+    if (im == 0) {
+      exp_imtheta0 = cc(1.0, 0.0);
+      exp_imdtheta = cc(1.0, 0.0);
+    } else {
+      exp_imtheta0 = cmul(exp_imtheta0, exp_itheta_05);
+      exp_imdtheta = cmul(exp_imdtheta, exp_idtheta);
+      if (small_angle_change) {
+        m_fac_1 = rmul(2.0, exp_imtheta0);
+        m_fac_2 = cmul(m_fac_1, cc(1.0 - sixth * m2dth2, 0.5 * mdth));
+        m_fac_3 = cmul(m_fac_1, cc(0.5 - 0.125 * m2dth2, third * mdth));
+        m_fac_4 = cmul(m_fac_1, cc(third - 0.1 * m2dth2, 0.25 * mdth));
+      } else {
+        m_fac_1 = rmul(2.0 * inv_mdth, exp_imtheta0);
+        m_fac_2 = cmul(m_fac_1, cc(exp_imdtheta.im, -(exp_imdtheta.re - 1.0)));
+        ocplx t = cmul(exp_imdtheta, cc(1.0, -mdth));
+        m_fac_3 = cmul(m_fac_1, rmul(inv_mdth, cc(t.re - 1.0, t.im)));
+        ocplx u = cmul(exp_imdtheta, cc(-m2dth2 + 2.0, -(2.0 * mdth)));
+        u.re = u.re - 2.0;
+        m_fac_4 = cmul(m_fac_1, imul(inv_m2dth2, u));
+      }
+    }
+
+    ocplx jyh_[5] = {CZERO, CZERO, CZERO, CZERO, CZERO};
+    ocplx *jyh = jyh_ + 2;
+    for (int iy = ymin; iy <= ymax; ++iy) {
+      int cy = cell_y2 + iy;
+      ocplx w_rt, ym_fac_1;
+      if (im == 0) {
+        w_rt = cc(gy[iy] + 0.5 * hy[iy], 0.0);
+        ym_fac_1 = cc(0.5 * gy[iy] + third * hy[iy], 0.0);
+      } else {
+        w_rt = cadd(cmulr(m_fac_2, gy[iy]), cmulr(m_fac_3, hy[iy]));
+        ym_fac_1 = cadd(cmulr(m_fac_3, gy[iy]), cmulr(m_fac_4, hy[iy]));
+      }
+      double fjx = fcx * inv_area_rt[cy + jng];
+      double fjy = (fcy * hy[iy]) * inv_area_xt[cy + jng];
+      double fjz = fcz * inv_volume[cy + jng];
+      ocplx jxh = CZERO;
+      for (int ix = xmin; ix <= xmax; ++ix) {
+        int cx = cell_x2 + ix;
+        ocplx w_xt;
+        if (im == 0) w_xt = cc(gx[ix] + 0.5 * hx[ix], 0.0);
+        else w_xt = cadd(cmulr(m_fac_2, gx[ix]), cmulr(m_fac_3, hx[ix]));
+        ocplx w_xr = cadd(rmul(gx[ix], w_rt), rmul(hx[ix], ym_fac_1));
+        jxh = csub(jxh, rmul(fjx * hx[ix], w_rt));
+        jyh[ix] = csub(cmulr(jyh[ix], ratio_area_xt[cy + jng]), rmul(fjy, w_xt));
+        ocplx jzh = rmul(fjz, w_xr);
+        size_t kx = I3(g, cx + 1, cy, im), kr = I3(g, cx, cy + 1, im), kt = I3(g, cx, cy, im);
+        f->jx[kx] = cadd(f->jx[kx], jxh);
+        f->jr[kr] = cadd(f->jr[kr], jyh[ix]);
+        f->jt[kt] = cadd(f->jt[kt], jzh);
+      }
+    }
+  }
+}
+
+/* Gather of one field component: sum over a 3x3 patch of mode m,
+ * triangle/e_part.inc and b_part.inc. */
+static inline ocplx patch(const orc_grid *g, const ocplx *F, int cx, int cy, int m,
+                          const double *wx, const double *wy) {
+  ocplx acc = CZERO;
+  for (int jy = -1; jy <= 1; ++jy) {
+    ocplx row = cadd(cadd(rmul(wx[-1], F[I3(g, cx - 1, cy + jy, m)]),
+                          rmul(wx[0], F[I3(g, cx, cy + jy, m)])),
+                     rmul(wx[1], F[I3(g, cx + 1, cy + jy, m)]));
+    acc = (jy == -1) ? rmul(wy[jy], row) : cadd(acc, rmul(wy[jy], row));
+  }
+  return acc;
+}
+static inline double re_mul(ocplx a, ocplx b) { return a.re * b.re - a.im * b.im; }
+
+/* push_particles, particles.F90:159-731 (probes, tracers, photons,
+ * HC_PUSH, delta-f and work-done options are not part of the default build
+ * and are not restated). */
+void orc_push_particles(const orc_grid *g, orc_fields *f, orc_species *sp, int n_species) {
+  const int ny = g->ny, M = g->n_mode, jng = g->ng;
+  size_t nbytes = (size_t)NX2(g) * NY2(g) * M * sizeof(ocplx);
+  memcpy(f->jx_old, f->jx, nbytes);
+  memcpy(f->jr_old, f->jr, nbytes);
+  memcpy(f->jt_old, f->jt, nbytes);
+  memset(f->jx, 0, nbytes);
+  memset(f->jr, 0, nbytes);
+  memset(f->jt, 0, nbytes);
+
+  const double c = C_LIGHT, dt = g->dt;
+  const double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idt = 1.0 / dt;
+  const double dto2 = dt / 2.0, dtco2 = c * dto2;
+  const double fac = 0.25;                 /* (0.5)**c_ndims */
+  const double dtfac = (0.5 * dt) * fac;
+
+  int ntab = ny + 2 * jng + 1;
+  double *inv_area_rt = calloc(ntab, sizeof(double));
+  double *inv_area_xt = calloc(ntab, sizeof(double));
+  double *inv_volume = calloc(ntab, sizeof(double));
+  double *ratio_area_xt = calloc(ntab, sizeof(double));
+  orc_geometry(g, inv_area_rt, inv_area_xt, inv_volume, ratio_area_xt);
+
+  for (int is = 0; is < n_species; ++is) {
+    orc_species *s = &sp[is];
+    if (s->immobile) continue;
+    const double part_q = s->q, part_mc = c * s->m, ipart_mc = 1.0 / part_mc;
+    const double cmratio = (part_q * dtfac) * ipart_mc, ccmratio = c * cmratio;
+
+    for (int64_t ip = 0; ip < s->n; ++ip) {
+      double part_weight = s->w[ip];
+      double part_x = s->x[ip], part_y = s->y[ip], part_z = s->z[ip];
+      double part_ux = s->px[ip] * ipart_mc;
+      double part_uy = s->py[ip] * ipart_mc;
+      double part_uz = s->pz[ip] * ipart_mc;
+
+      double gamma_rel = sqrt(((part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz) + 1.0);
+      double root = dtco2 / gamma_rel;
+      part_x = part_x + part_ux * root;
+      part_y = part_y + part_uy * root;
+      part_z = part_z + part_uz * root;
+
+      double part_x_local = part_x - g->x_grid_min;
+      double part_r = sqrt(part_y * part_y + part_z * part_z);
+      double part_r_local = part_r - g->y_grid_min;
+      ocplx exp_min_itheta = cc(part_y / part_r, (-part_z) / part_r);
+      double theta_05 = atan2(part_z, part_y);
+      ocplx exp_min_itheta_05 = exp_min_itheta;
+      ocplx exp_itheta_05 = crecip(exp_min_itheta_05);
+
+      double cell_x_r = part_x_local * idx;
+      double cell_y_r = part_r_local * idy;
+      int cell_x1 = (int)floor(cell_x_r + 0.5);
+      double cell_frac_x = (double)cell_x1 - cell_x_r;
+      cell_x1 += 1;
+      int cell_y1 = (int)floor(cell_y_r + 0.5);
+      double cell_frac_y = (double)cell_y1 - cell_y_r;
+      cell_y1 += 1;
+      double gx_[5] = {0}, gy_[5] = {0}, hx_[5] = {0}, hy_[5] = {0};
+      double *gx = gx_ + 2, *gy = gy_ + 2, *hx = hx_ + 2, *hy = hy_ + 2;
+      weights3(cell_frac_x, gx);
+      weights3(cell_frac_y, gy);
+
+      int cell_x2 = (int)floor(cell_x_r);
+      cell_frac_x = ((double)cell_x2 - cell_x_r) + 0.5;
+      cell_x2 += 1;
+      int cell_y2 = (int)floor(cell_y_r);
+      cell_frac_y = ((double)cell_y2 - cell_y_r) + 0.5;
+      cell_y2 += 1;
+      weights3(cell_frac_x, hx);
+      weights3(cell_frac_y, hy);
+
+      /* e_part.inc / b_part.inc */
+      ocplx e_m = cc(1.0, 0.0);
+      double ex_part = 0.0, er_part = 0.0, et_part = 0.0;
+      for (int im = 0; im < M; ++im) {
+        ex_part = ex_part + re_mul(e_m, patch(g, f->ex, cell_x1, cell_y2, im, gx, hy));
+        er_part = er_part + re_mul(e_m, patch(g, f->er, cell_x2, cell_y1, im, hx, gy));
+        et_part = et_part + re_mul(e_m, patch(g, f->et, cell_x2, cell_y2, im, hx, hy));
+        e_m = cmul(e_m, exp_min_itheta);
+      }
+      double ey_part = er_part * exp_min_itheta.re + et_part * exp_min_itheta.im;
+      double ez_part = -(er_part * exp_min_itheta.im) + et_part * exp_min_itheta.re;
+      e_m = cc(1.0, 0.0);
+      double bx_part = 0.0, br_part = 0.0, bt_part = 0.0;
+      for (int im = 0; im < M; ++im) {
+        bx_part = bx_part + re_mul(e_m, patch(g, f->bx, cell_x2, cell_y1, im, hx, gy));
+        br_part = br_part + re_mul(e_m, patch(g, f->br, cell_x1, cell_y2, im, gx, hy));
+        bt_part = bt_part + re_mul(e_m, patch(g, f->bt, cell_x1, cell_y1, im, gx, gy));
+        e_m = cmul(e_m, exp_min_itheta);
+      }
+      double by_part = br_part * exp_min_itheta.re + bt_part * exp_min_itheta.im;
+      double bz_part = -(br_part * exp_min_itheta.im) + bt_part * exp_min_itheta.re;
+
+      /* Boris, particles.F90:404-451 */
+      double uxm = part_ux + cmratio * ex_part;
+      double uym = part_uy + cmratio * ey_part;
+      double uzm = part_uz + cmratio * ez_part;
+      gamma_rel = sqrt(((uxm * uxm + uym * uym) + uzm * uzm) + 1.0);
+      root = ccmratio / gamma_rel;
+      double taux = bx_part * root, tauy = by_part * root, tauz = bz_part * root;
+      double taux2 = taux * taux, tauy2 = tauy * tauy, tauz2 = tauz * tauz;
+      double tau = 1.0 / (((1.0 + taux2) + tauy2) + tauz2);
+      double uxp = ((((1.0 + taux2) - tauy2) - tauz2) * uxm +
+                    2.0 * ((taux * tauy + tauz) * uym + (taux * tauz - tauy) * uzm)) * tau;
+      double uyp = ((((1.0 - taux2) + tauy2) - tauz2) * uym +
+                    2.0 * ((tauy * tauz + taux) * uzm + (tauy * taux - tauz) * uxm)) * tau;
+      double uzp = ((((1.0 - taux2) - tauy2) + tauz2) * uzm +
+                    2.0 * ((tauz * taux + tauy) * uxm + (tauz * tauy - taux) * uym)) * tau;
+      part_ux = uxp + cmratio * ex_part;
+      part_uy = uyp + cmratio * ey_part;
+      part_uz = uzp + cmratio * ez_part;
+
+      double part_u2 = (part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz;
+      gamma_rel = sqrt(part_u2 + 1.0);
+      double igamma = 1.0 / gamma_rel;
+      root = dtco2 * igamma;
+      double delta_x = part_ux * root, delta_y = part_uy * root, delta_z = part_uz * root;
+      part_x = part_x + delta_x;
+      part_y = part_y + delta_y;
+      part_z = part_z + delta_z;
+      s->x[ip] = part_x; s->y[ip] = part_y; s->z[ip] = part_z;
+      s->px[ip] = part_mc * part_ux; s->py[ip] = part_mc * part_uy; s->pz[ip] = part_mc * part_uz;
+
+      double part_vy = (part_uy * c) * igamma;
+      double part_vz = (part_uz * c) * igamma;
+      part_r = sqrt(part_y * part_y + part_z * part_z);
+      ocplx exp_itheta_10 = cc(part_y / part_r, part_z / part_r);
+      double part_vt = -(part_vy * exp_itheta_10.im) + part_vz * exp_itheta_10.re;
+
+      /* current at t+1.5dt, particles.F90:504-584 */
+      part_x_local = (part_x + delta_x) - g->x_grid_min;
+      part_y = part_y + delta_y;
+      part_z = part_z + delta_z;
+      part_r = sqrt(part_y * part_y + part_z * part_z);
+      part_r_local = part_r - g->y_grid_min;
+      ocplx exp_itheta_15 = cc(part_y / part_r, part_z / part_r);
+      double theta_15 = atan2(part_z, part_y);
+      ocplx exp_idtheta = cmul(exp_itheta_15, exp_min_itheta_05);
+      double dtheta = theta_15 - theta_05;
+
+      for (int k = -2; k <= 2; ++k) { gx[k] = hx[k]; gy[k] = hy[k]; }
+      cell_x_r = part_x_local * idx;
+      cell_y_r = part_r_local * idy;
+      int cell_x3 = (int)floor(cell_x_r);
+      cell_frac_x = ((double)cell_x3 - cell_x_r) + 0.5;
+      cell_x3 += 1;
+      int cell_y3 = (int)floor(cell_y_r);
+      cell_frac_y = ((double)cell_y3 - cell_y_r) + 0.5;
+      cell_y3 += 1;
+      for (int k = -2; k <= 2; ++k) { hx[k] = 0.0; hy[k] = 0.0; }
+      int dcellx = cell_x3 - cell_x2, dcelly = cell_y3 - cell_y2;
+      weights3(cell_frac_x, hx + dcellx);
+      weights3(cell_frac_y, hy + dcelly);
+      for (int k = -2; k <= 2; ++k) { hx[k] = hx[k] - gx[k]; hy[k] = hy[k] - gy[k]; }
+
+      double q_weight_fac = (part_q * part_weight) * fac;
+      double fcx = q_weight_fac * idt, fcy = q_weight_fac * idt, fcz = q_weight_fac * part_vt;
+      orc_deposit_from_weights(g, f, gx_, hx_, gy_, hy_, cell_x2, cell_y2, dcellx, dcelly,
+                               fcx, fcy, fcz, exp_itheta_05, exp_idtheta, dtheta,
+                               inv_area_rt, inv_area_xt, inv_volume, ratio_area_xt);
+    }
+    /* current_bcs(species=ispecies): a no-op unless particle bcs are mixed */
+  }
+  free(inv_area_rt); free(inv_area_xt); free(inv_volume); free(ratio_area_xt);
+
+  orc_current_bcs_r_min_final(g, f);
+  orc_particle_bcs(g, sp, n_species);
+}
+
+/* particle_bcs on one rank, boundary.F90:1541-1889 (periodic, reflect,
+ * open; thermal boundaries are not part of this path).  Particles that wrap
+ * through a periodic boundary are re-appended at the end of their list,
+ * first those that left through x_min then through x_max, which is the
+ * order the reference's self-exchange produces. */
+void orc_particle_bcs(const orc_grid *g, orc_species *sp, int n_species) {
+  const double dx = g->dx, dy = g->dy;
+  const int png = 3;
+  const double bshift_x = dx * (double)((1 + png + 0) / 2);
+  const double x_min_outer = g->x_min - bshift_x, x_max_outer = g->x_max + bshift_x;
+  const double x_shift = g->x_max - g->x_min;
+  const double bshift_y = dy * (double)((1 + png + 0) / 2);
+  const double y_max_outer = g->y_max + bshift_y;
+
+  for (int is = 0; is < n_species; ++is) {
+    orc_species *s = &sp[is];
+    int64_t n = s->n;
+    int8_t *state = malloc((size_t)(n > 0 ? n : 1));  /* 0 keep, 1 remove, 2 left -x, 3 left +x */
+    for (int64_t ip = 0; ip < n; ++ip) {
+      int xbd = 0, out_of_bounds = 0, bc = 0;
+      double part_pos = s->x[ip];
+      if (part_pos < g->x_min) {
+        xbd = -1;
+        xbd = 0;
+        bc = g->bc_particle[ORC_XMIN];
+        if (bc == ORC_BC_REFLECT) {
+          s->x[ip] = 2.0 * g->x_min - part_pos;
+          s->px[ip] = -s->px[ip];
+        } else if (bc == ORC_BC_PERIODIC) {
+          xbd = -1;
+          s->x[ip] = part_pos - (-1.0) * x_shift;
+        }
+        if (part_pos < x_min_outer && bc != ORC_BC_PERIODIC) out_of_bounds = 1;
+      }
+      if (part_pos >= g->x_max) {
+        xbd = 1;
+        xbd = 0;
+        bc = g->bc_particle[ORC_XMAX];
+        if (bc == ORC_BC_REFLECT) {
+          s->x[ip] = 2.0 * g->x_max - part_pos;
+          s->px[ip] = -s->px[ip];
+        } else if (bc == ORC_BC_PERIODIC) {
+          xbd = 1;
+          s->x[ip] = part_pos - 1.0 * x_shift;
+        }
+        if (part_pos >= x_max_outer && bc != ORC_BC_PERIODIC) out_of_bounds = 1;
+      }
+      part_pos = sqrt(s->y[ip] * s->y[ip] + s->z[ip] * s->z[ip]);
+      if (part_pos >= g->y_max) {
+        bc = g->bc_particle[ORC_YMAX];
+        if (bc == ORC_BC_REFLECT) {
+          double radial_reduction = 2.0 * g->y_max / part_pos - 1.0;
+          s->y[ip] = s->y[ip] * radial_reduction;
+          s->z[ip] = s->z[ip] * radial_reduction;
+          double inv_final_r = 1.0 / sqrt(s->y[ip] * s->y[ip] + s->z[ip] * s->z[ip]);
+          double cos_theta = s->y[ip] * inv_final_r;
+          double sin_theta = s->z[ip] * inv_final_r;
+          double part_pr = s->py[ip] * cos_theta + s->pz[ip] * sin_theta;
+          double part_pt = -(s->py[ip] * sin_theta) + s->pz[ip] * cos_theta;
+          s->py[ip] = -(part_pr * cos_theta) - part_pt * sin_theta;
+          s->pz[ip] = -(part_pr * sin_theta) + part_pt * cos_theta;
+        }
+        if (part_pos >= y_max_outer && bc != ORC_BC_PERIODIC) out_of_bounds = 1;
+      }
+      state[ip] = out_of_bounds ? 1 : (xbd < 0 ? 2 : (xbd > 0 ? 3 : 0));
+    }
+    /* compact, then append the two self-exchanged groups */
+    double *arrs[7] = {s->x, s->y, s->z, s->px, s->py, s->pz, s->w};
+    int64_t *tmpi = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t cnt = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+      int want = pass == 0 ? 0 : (pass == 1 ? 2 : 3);
+      for (int64_t ip = 0; ip < n; ++ip)
+        if (state[ip] == want) tmpi[cnt++] = ip;
+    }
+    double *tmp = malloc(sizeof(double) * (size_t)(cnt > 0 ? cnt : 1));
+    for (int a = 0; a < 7; ++a) {
+      for (int64_t k = 0; k < cnt; ++k) tmp[k] = arrs[a][tmpi[k]];
+      memcpy(arrs[a], tmp, sizeof(double) * (size_t)cnt);
+    }
+    if (s->id) {
+      int64_t *tid = malloc(sizeof(int64_t) * (size_t)(cnt > 0 ? cnt : 1));
+      for (int64_t k = 0; k < cnt; ++k) tid[k] = s->id[tmpi[k]];
+      memcpy(s->id, tid, sizeof(int64_t) * (size_t)cnt);
+      free(tid);
+    }
+    s->n = cnt;
+    free(tmp); free(tmpi); free(state);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* cell key (split_particle.F90:61-62) and stable counting sort         */
+/* ------------------------------------------------------------------ */
+void orc_cell_keys(const orc_grid *g, int64_t n, const double *x, const double *y,
+                   const double *z, int tile_x, int tile_r, int32_t *keys) {
+  const int nx = g->nx, ny = g->ny;
+  const int ntx = (nx + tile_x - 1) / tile_x;
+  for (int64_t i = 0; i < n; ++i) {
+    double r = sqrt(y[i] * y[i] + z[i] * z[i]);
+    int cx = (int)floor((x[i] - g->x_grid_min) / g->dx + 1.5);
+    int cy = (int)floor((r - g->y_grid_min) / g->dy + 1.5);
+    if (cx < 1 || cx > nx || cy < 1 || cy > ny) { keys[i] = -1; continue; }
+    int lx = cx - 1, ly = cy - 1;
+    int tx = lx / tile_x, ty = ly / tile_r;
+    keys[i] = (int32_t)((((int64_t)ty * ntx + tx) * tile_r + (ly - ty * tile_r)) * tile_x +
+                        (lx - tx * tile_x));
+  }
+}
+
+void orc_stable_sort_keys(int64_t n, const int32_t *keys, int32_t nkeys, int64_t *perm) {
+  int64_t *start = calloc((size_t)nkeys + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < n; ++i) start[keys[i] + 1]++;
+  for (int32_t k = 0; k < nkeys; ++k) start[k + 1] += start[k];
+  for (int64_t i = 0; i < n; ++i) perm[start[keys[i]]++] = i;
+  free(start);
+}
+
+/* ------------------------------------------------------------------ */
+/* set_dt, setup.F90:629-673                                          */
+/* ------------------------------------------------------------------ */
+double orc_set_dt(double dx, double dy, double dt_multiplier,
+                  double dt_plasma_frequency, double dt_laser) {
+  double dt = ((0.9 * (dx < dy ? dx : dy)) / sqrt(2.0)) / C_LIGHT;
+  if (dt_plasma_frequency > C_TINY && dt_plasma_frequency < dt) dt = dt_plasma_frequency;
+  if (dt_laser > C_TINY && dt_laser < dt) dt = dt_laser;
+  return dt_multiplier * dt;
+}
+
+/* ------------------------------------------------------------------ */
+/* mode energies: W_m = kappa_m * sum (eps0/2 |E_m|^2 + |B_m|^2/(2 mu0))  */
+/* * r dr dx over interior cells, each component at its own staggered   */
+/* radius; kappa_0 = 2 pi, kappa_m>0 = pi (the theta integral of         */
+/* Re[F_m exp(-i m theta)]^2).                                           */
+/* ------------------------------------------------------------------ */
+void orc_mode_energy(const orc_grid *g, const orc_fields *f, double *w_e, double *w_b) {
+  const double mu0 = 4.0e-7 * PI;
+  for (int m = 0; m < g->n_mode; ++m) {
+    double kappa = (m == 0) ? 2.0 * PI : PI;
+    double se = 0.0, sb = 0.0;
+    for (int ir = 1; ir <= g->ny; ++ir) {
+      double r_c = ((double)ir - 0.5) * g->dy;   /* er, bx, bt */
+      double r_e = (double)ir * g->dy;           /* ex, et, br */
+      for (int ix = 1; ix <= g->nx; ++ix) {
+        size_t k = I3(g, ix, ir, m);
+        ocplx a = f->ex[k], b = f->er[k], c = f->et[k];
+        se += (a.re * a.re + a.im * a.im) * r_e + (b.re * b.re + b.im * b.im) * r_c +
+              (c.re * c.re + c.im * c.im) * r_e;
+        a = f->bx[k]; b = f->br[k]; c = f->bt[k];
+        sb += (a.re * a.re + a.im * a.im) * r_c + (b.re * b.re + b.im * b.im) * r_e +
+              (c.re * c.re + c.im * c.im) * r_c;
+      }
+    }
+    w_e[m] = kappa * 0.5 * EPSILON0 * se * g->dx * g->dy;
+    w_b[m] = kappa * 0.5 / mu0 * sb * g->dx * g->dy;
+  }
+}

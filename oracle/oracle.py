@@ -1,0 +1,270 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/cylpic_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg as the parity checker.  The product library
+(cylindrical_epoch_amd) never imports this module.
+
+Arrays follow the reference's Fortran layout COMPLEX(8) f(1-ng:nx+ng,
+1-ng:ny+ng, 0:n_mode-1) (src/shared_data.F90:474-478), held here as numpy
+complex128 arrays of shape (n_mode, ny+2ng, nx+2ng): element [m, j+ng-1,
+i+ng-1] is Fortran f(i, j, m).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libcylpic_oracle.so")
+_lib = None
+
+BC = dict(periodic=1, other=2, simple_laser=3, simple_outflow=4, open=5,
+          zero_gradient=7, clamp=8, reflect=9, conduct=10, thermal=11,
+          zero_b=16)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("n_mode", C.c_int), ("ng", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double),
+                ("x_min", C.c_double), ("x_max", C.c_double),
+                ("y_min", C.c_double), ("y_max", C.c_double),
+                ("x_grid_min", C.c_double), ("y_grid_min", C.c_double),
+                ("bc_field", C.c_int * 4), ("bc_particle", C.c_int * 4),
+                ("time", C.c_double)]
+
+
+_P = C.c_void_p
+
+
+class OCplx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class Fields(C.Structure):
+    _fields_ = [(n, _P) for n in ("ex", "er", "et", "bx", "br", "bt", "jx", "jr", "jt",
+                                   "bx_old", "br_old", "bt_old", "jx_old", "jr_old", "jt_old")] + [
+        ("bnd_xmin", _P * 6), ("bnd_xmax", _P * 6)]
+
+
+class Species(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(n, _P) for n in ("x", "y", "z", "px", "py", "pz", "w", "id")] + [
+        ("q", C.c_double), ("m", C.c_double), ("immobile", C.c_int)]
+
+
+def _declare(L):
+    gp, fp, sp = C.POINTER(Grid), C.POINTER(Fields), C.POINTER(Species)
+    dp = C.c_void_p
+    sig = {
+        "orc_update_e_field": [gp, fp], "orc_update_b_field": [gp, fp],
+        "orc_update_eb_fields_half": [gp, fp],
+        "orc_update_eb_fields_final": [gp, fp, dp, dp],
+        "orc_efield_bcs": [gp, fp], "orc_bfield_bcs": [gp, fp, C.c_int],
+        "orc_bfield_final_bcs": [gp, fp, dp, dp],
+        "orc_outflow_bcs_x_min": [gp, fp, dp], "orc_outflow_bcs_x_max": [gp, fp, dp],
+        "orc_outflow_bcs_r_max": [gp, fp], "orc_setup_field_boundaries": [gp, fp],
+        "orc_push_particles": [gp, fp, sp, C.c_int],
+        "orc_current_bcs": [gp, fp], "orc_current_bcs_r_min_final": [gp, fp],
+        "orc_current_finish": [gp, fp], "orc_particle_bcs": [gp, sp, C.c_int],
+        "orc_geometry": [gp, dp, dp, dp, dp],
+        "orc_deposit_from_weights": [gp, fp, dp, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_double, C.c_double, C.c_double,
+                                     OCplx, OCplx, C.c_double,
+                                     dp, dp, dp, dp],
+        "orc_cell_keys": [gp, C.c_int64, dp, dp, dp, C.c_int, C.c_int, dp],
+        "orc_stable_sort_keys": [C.c_int64, dp, C.c_int32, dp],
+        "orc_mode_energy": [gp, fp, dp, dp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = None
+    L.orc_set_dt.argtypes = [C.c_double] * 5
+    L.orc_set_dt.restype = C.c_double
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+FIELD_NAMES = ("ex", "er", "et", "bx", "br", "bt", "jx", "jr", "jt",
+               "bx_old", "br_old", "bt_old", "jx_old", "jr_old", "jt_old")
+
+
+class State:
+    """Grid + fields + species for the oracle, numpy-owned."""
+
+    def __init__(self, nx, ny, n_mode, dx, dy, dt, x_min=0.0, y_max=None,
+                 bc_field=("periodic", "periodic", "other", "conduct"),
+                 bc_particle=("periodic", "periodic", "other", "reflect"), ng=5):
+        self.g = Grid()
+        g = self.g
+        g.nx, g.ny, g.n_mode, g.ng = nx, ny, n_mode, ng
+        g.dx, g.dy, g.dt = dx, dy, dt
+        g.x_min = x_min
+        g.x_max = x_min + nx * dx
+        g.y_min = 0.0
+        g.y_max = ny * dy if y_max is None else y_max
+        g.x_grid_min = g.x_min + dx / 2.0
+        g.y_grid_min = g.y_min + dy / 2.0
+        g.bc_field[:] = [map_field_bc(b) for b in bc_field]
+        g.bc_particle[:] = [map_particle_bc(b) for b in bc_particle]
+        g.time = 0.0
+        shape = (n_mode, ny + 2 * ng, nx + 2 * ng)
+        self.shape = shape
+        self.arr = {n: np.zeros(shape, np.complex128) for n in FIELD_NAMES}
+        self.bnd = {s: [np.zeros((n_mode, ny + 2 * ng), np.complex128) for _ in range(6)]
+                    for s in ("xmin", "xmax")}
+        self.f = Fields()
+        for n in FIELD_NAMES:
+            setattr(self.f, n, self.arr[n].ctypes.data)
+        for k in range(6):
+            self.f.bnd_xmin[k] = self.bnd["xmin"][k].ctypes.data
+            self.f.bnd_xmax[k] = self.bnd["xmax"][k].ctypes.data
+        self.species = []
+
+    # Fortran-index view helpers
+    def at(self, name, i, j, m):
+        ng = self.g.ng
+        return self.arr[name][m, j + ng - 1, i + ng - 1]
+
+    def add_species(self, q, m, x, y, z, px, py, pz, w, immobile=False, ids=None):
+        n = len(x)
+        d = dict(x=x, y=y, z=z, px=px, py=py, pz=pz, w=w)
+        d = {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in d.items()}
+        d["id"] = (np.arange(n, dtype=np.int64) if ids is None
+                   else np.ascontiguousarray(ids, dtype=np.int64).copy())
+        sp = dict(q=q, m=m, immobile=bool(immobile), n=n, **d)
+        self.species.append(sp)
+        return sp
+
+    def _species_array(self):
+        arr = (Species * max(1, len(self.species)))()
+        for k, s in enumerate(self.species):
+            a = arr[k]
+            a.n = s["n"]
+            for nm in ("x", "y", "z", "px", "py", "pz", "w", "id"):
+                setattr(a, nm, s[nm].ctypes.data)
+            a.q, a.m, a.immobile = s["q"], s["m"], int(s["immobile"])
+        return arr
+
+    def _species_back(self, arr):
+        for k, s in enumerate(self.species):
+            n = arr[k].n
+            s["n"] = n
+            for nm in ("x", "y", "z", "px", "py", "pz", "w", "id"):
+                s[nm] = s[nm][:n].copy()
+
+    # reference routines
+    def update_eb_fields_half(self):
+        lib().orc_update_eb_fields_half(C.byref(self.g), C.byref(self.f))
+
+    def update_eb_fields_final(self, src_xmin=None, src_xmax=None):
+        a = None if src_xmin is None else np.ascontiguousarray(src_xmin, np.float64)
+        b = None if src_xmax is None else np.ascontiguousarray(src_xmax, np.float64)
+        lib().orc_update_eb_fields_final(C.byref(self.g), C.byref(self.f),
+                                         None if a is None else _ptr(a),
+                                         None if b is None else _ptr(b))
+
+    def push_particles(self):
+        arr = self._species_array()
+        lib().orc_push_particles(C.byref(self.g), C.byref(self.f), arr, len(self.species))
+        self._species_back(arr)
+
+    def current_finish(self):
+        lib().orc_current_finish(C.byref(self.g), C.byref(self.f))
+
+    def setup_field_boundaries(self):
+        lib().orc_setup_field_boundaries(C.byref(self.g), C.byref(self.f))
+
+    def step(self, src_xmin=None, src_xmax=None):
+        """One timestep of the reference main loop, epoch2d.F90:213-265
+        (no moving window, no optional physics)."""
+        self.update_eb_fields_half()
+        self.push_particles()
+        self.current_finish()
+        self.g.time += self.g.dt
+        self.update_eb_fields_final(src_xmin, src_xmax)
+
+    def call(self, name, *args):
+        fn = getattr(lib(), name)
+        fn(C.byref(self.g), C.byref(self.f), *args)
+
+    def mode_energy(self):
+        M = self.g.n_mode
+        we, wb = np.zeros(M), np.zeros(M)
+        lib().orc_mode_energy(C.byref(self.g), C.byref(self.f), _ptr(we), _ptr(wb))
+        return we, wb
+
+    def geometry(self):
+        n = self.g.ny + 2 * self.g.ng + 1
+        a = [np.zeros(n) for _ in range(4)]
+        lib().orc_geometry(C.byref(self.g), *[_ptr(x) for x in a])
+        return a
+
+
+def map_field_bc(name):
+    """setup_boundaries' field mapping, boundary.F90:30-68."""
+    b = BC[name] if isinstance(name, str) else int(name)
+    if b == BC["other"]:
+        b = BC["clamp"]
+    if b == BC["reflect"]:
+        b = BC["clamp"]
+    if b == BC["open"]:
+        b = BC["simple_outflow"]
+    return b
+
+
+def map_particle_bc(name):
+    """setup_particle_boundary, boundary.F90:98-131."""
+    b = BC[name] if isinstance(name, str) else int(name)
+    if b in (BC["other"], BC["conduct"]):
+        b = BC["reflect"]
+    if b in (BC["simple_laser"], BC["simple_outflow"]):
+        b = BC["open"]
+    return b
+
+
+def set_dt(dx, dy, dt_multiplier=1.0, dt_plasma=0.0, dt_laser=0.0):
+    return lib().orc_set_dt(dx, dy, dt_multiplier, dt_plasma, dt_laser)
+
+
+def cell_keys(state, x, y, z, tile_x, tile_r):
+    n = len(x)
+    keys = np.zeros(n, np.int32)
+    x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+    lib().orc_cell_keys(C.byref(state.g), n, _ptr(x), _ptr(y), _ptr(z), tile_x, tile_r, _ptr(keys))
+    return keys
+
+
+def stable_sort(keys, nkeys):
+    keys = np.ascontiguousarray(keys, np.int32)
+    perm = np.zeros(len(keys), np.int64)
+    lib().orc_stable_sort_keys(len(keys), _ptr(keys), nkeys, _ptr(perm))
+    return perm
+
+
+def deposit_from_weights(state, gx, hx, gy, hy, cell_x2, cell_y2, dcellx, dcelly,
+                         fcx, fcy, fcz, exp_itheta_05=(1.0, 0.0), exp_idtheta=(1.0, 0.0),
+                         dtheta=0.0):
+    geo = state.geometry()
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (gx, hx, gy, hy)]
+    lib().orc_deposit_from_weights(C.byref(state.g), C.byref(state.f),
+                                   *[_ptr(a) for a in arrs], cell_x2, cell_y2, dcellx, dcelly,
+                                   fcx, fcy, fcz, OCplx(*exp_itheta_05),
+                                   OCplx(*exp_idtheta), dtheta,
+                                   *[_ptr(a) for a in geo])
