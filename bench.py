@@ -1,0 +1,214 @@
+"""Throughput of the quasi-3D PIC timestep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed|both]
+
+A step is one full timestep of the reference's main loop (update_eb_fields_half,
+push_particles, current_finish, update_eb_fields_final; src/epoch2d.F90:213-265)
+over the whole particle load, with every input resident in HBM.  Workload: the
+8192 x 512 cell, 3-mode, 32 particles-per-cell single run (134,217,728
+electrons, n_e = 5e23 m^-3 plateau, dx = 0.02 um, dy = 0.1 um) with a Gaussian
+laser (lambda 0.8 um, a0 = 3, waist 15 um, 25 fs) entering through x_min, open
+particle boundaries, no moving window.  Synthetic data: particles from a seeded
+generator with a 100 eV thermal spread.
+
+Prints one JSON line.  The CPU baseline is the FP64 oracle (one host core) timed
+on a sample of the same workload and extrapolated per particle.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from cylindrical_epoch_amd import deck  # noqa: E402
+from cylindrical_epoch_amd.host import Simulation  # noqa: E402
+
+METRIC = "particle-steps/s (push+deposit+FDTD), mixed vs FP64, 1 MI355X; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "wakefield_8192x512_m3_ppc32"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Laser:
+    """Host-side laser source of one boundary (laser.f90:444-461 with a
+    Gaussian transverse profile and Gaussian temporal envelope)."""
+
+    def __init__(self, ny, dy, a0=3.0, lam=0.8e-6, waist=15e-6, fwhm=25e-15, pol=0.0):
+        self.omega = 2 * math.pi * deck.C_LIGHT / lam
+        self.amp = a0 * deck.M0 * deck.C_LIGHT * self.omega / deck.Q0
+        r = np.abs((np.arange(ny + 1) - 1) * dy + dy / 2)
+        self.profile = np.exp(-(r / waist) ** 2)
+        self.tw = fwhm / (2 * math.sqrt(math.log(2)))
+        self.t0 = 2 * fwhm
+        self.pol = pol
+        self.ny = ny
+
+    def sources(self, t):
+        env = math.exp(-((t - self.t0) / self.tw) ** 2) * self.amp
+        base = env * self.profile * math.sin(self.omega * t)
+        return np.concatenate([base * math.cos(self.pol), base * math.sin(self.pol)])
+
+
+def make_sim(w, precision, particles):
+    dx, dy = w["dx"], w["dy"]
+    laser = Laser(w["ny"], dy)
+    dt = deck.set_dt(dx, dy, 0.95, plasma=[(w["density"], 100.0 * deck.EV / deck.KB, deck.M0)],
+                     laser_omegas=[laser.omega])
+    sim = Simulation(w["nx"], w["ny"], w["n_mode"], dx, dy, dt,
+                     bc_field=("simple_laser", "simple_outflow", "other", "open"),
+                     bc_particle=("open", "open", "other", "open"), precision=precision)
+    n = len(particles["x"])
+    sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=n)
+    sim.upload_particles(sid, particles["x"], particles["y"], particles["z"], particles["px"],
+                         particles["py"], particles["pz"], particles["w"])
+    sim.setup_field_boundaries()
+    return sim, laser, dt
+
+
+def run(sim, laser, dt, steps, warmup):
+    t = 0.0
+    for _ in range(warmup):
+        t += dt
+        sim.step(laser.sources(t))
+    sim.synchronize()
+    sim.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t += dt
+        sim.step(laser.sources(t))
+    sim.synchronize()
+    t1 = time.perf_counter()
+    prof = {k: sim.kernel_time(k) for k in ("push_deposit", "update_e", "update_b", "field_bcs",
+                                            "copy_old", "zero_j", "current_axis", "particle_bcs",
+                                            "current_finish", "sort")}
+    sim.profile(False)
+    return (t1 - t0) / steps, prof
+
+
+def roofline(w, n, prof_push, precision):
+    """Algorithmic bytes of one push_deposit launch / its average duration."""
+    ms, cnt = prof_push
+    if cnt == 0:
+        return None
+    cells = w["nx"] * w["ny"]
+    if precision == "fp64":
+        per_particle = 104.0                       # read 7 doubles, write 6
+        per_cell_mode = 16.0 * (6 + 6)             # gather 6 comps, J 3 comps read+write
+    else:
+        per_particle = 60.0
+        per_cell_mode = 8.0 * (6 + 6)
+    bytes_ = per_particle * n + per_cell_mode * w["n_mode"] * cells
+    avg_s = ms / cnt / 1e3
+    achieved = bytes_ / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "push_deposit", "avg_launch_ms": round(ms / cnt, 4),
+            "algorithmic_bytes_per_launch": bytes_}
+
+
+def cpu_baseline(w, sample_cols):
+    """FP64 oracle on one core: push+deposit of the particles in the first
+    `sample_cols` x-columns, plus the full-grid field updates, extrapolated
+    to the whole load."""
+    try:
+        from oracle import oracle as O
+    except Exception as e:  # pragma: no cover
+        return {"error": f"oracle unavailable: {e}"}
+    dx, dy = w["dx"], w["dy"]
+    dt = deck.set_dt(dx, dy, 0.95)
+    st = O.State(w["nx"], w["ny"], w["n_mode"], dx, dy, dt,
+                 bc_field=("simple_laser", "simple_outflow", "other", "open"),
+                 bc_particle=("open", "open", "other", "open"))
+    p = deck.uniform_plasma(w["nx"], w["ny"], dx, dy, w["ppc"], w["density"], 100.0,
+                            seed=1, cells_x=np.arange(sample_cols))
+    ns = len(p["x"])
+    st.add_species(-deck.Q0, deck.M0, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+    st.setup_field_boundaries()
+    t0 = time.perf_counter()
+    st.update_eb_fields_half()
+    t1 = time.perf_counter()
+    st.push_particles()
+    t2 = time.perf_counter()
+    st.current_finish()
+    st.update_eb_fields_final()
+    t3 = time.perf_counter()
+    n_full = w["nx"] * w["ny"] * w["ppc"]
+    t_fields = (t1 - t0) + (t3 - t2)
+    t_push = (t2 - t1) * n_full / ns
+    value = n_full / (t_fields + t_push)
+    return {"value": round(value, 1), "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"one step of the FP64 oracle: field updates on the full {w['nx']}x{w['ny']}x"
+                       f"{w['n_mode']} grid ({t_fields:.2f} s) + push/deposit of the {ns} particles in "
+                       f"x-columns 1..{sample_cols} ({t2 - t1:.2f} s), push time scaled to {n_full} particles"),
+            "push_particle_steps_per_s": round(ns / (t2 - t1), 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--workload", default=WORKLOAD, choices=sorted(deck.WORKLOADS))
+    ap.add_argument("--cpu-sample-cols", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    w = deck.WORKLOADS[a.workload]
+    log("generating particles", w)
+    t0 = time.perf_counter()
+    parts = deck.uniform_plasma(w["nx"], w["ny"], w["dx"], w["dy"], w["ppc"], w["density"], 100.0, seed=7)
+    n = len(parts["x"])
+    log(f"{n} particles in {time.perf_counter() - t0:.1f} s")
+    precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    results = {}
+    for prec in precs:
+        sim, laser, dt = make_sim(w, prec, parts)
+        log(f"{prec}: uploaded, dt = {dt:.4e}")
+        ms, prof = run(sim, laser, dt, a.steps, a.warmup)
+        sim.close()
+        results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof)
+        log(f"{prec}: {ms * 1e3:.3f} ms/step, {n / ms:.4e} particle-steps/s")
+        for k, (kms, cnt) in prof.items():
+            if cnt:
+                log(f"   {k:15s} {kms / a.steps:8.3f} ms/step ({cnt} launches)")
+    del parts
+    main_prec = "mixed" if "mixed" in results else "fp64"
+    r = results[main_prec]
+    cells = w["nx"] * w["ny"] * w["n_mode"]
+    out = {
+        "metric": METRIC, "value": round(r["value"], 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(r["ms"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if main_prec == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "nx": w["nx"], "ny": w["ny"], "n_mode": w["n_mode"],
+                   "ppc": w["ppc"], "particles": n, "precision": main_prec, "parallelism": "single-gpu",
+                   "laser": "x_min, a0=3, lambda 0.8 um, waist 15 um, 25 fs", "moving_window": False},
+        "field_cell_updates_per_s": round(cells / (r["ms"] / 1e3), 1),
+        "per_precision": {k: {"ms_per_step": round(v["ms"], 4), "particle_steps_per_s": round(v["value"], 1)}
+                          for k, v in results.items()},
+        "roofline": roofline(w, n, r["prof"]["push_deposit"], main_prec),
+    }
+    if "fp64" in results and "mixed" in results:
+        out["mixed_over_fp64"] = round(results["mixed"]["value"] / results["fp64"]["value"], 3)
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_sample_cols)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,827 @@
+// C ABI of the gfx950 quasi-3D PIC library (declared in include/cylpic.h).
+// Host-side state management: device allocations, the call sequence of the
+// reference's main loop (src/epoch2d.F90:213-265) and the sort schedule.
+#include "../../include/cylpic.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "launch.h"
+
+using namespace cylpic;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(CYLPIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+struct Species {
+  double q = 0, m = 0;
+  int immobile = 0;
+  long cap = 0;
+  long n = 0;             // particles held (upper bound until the next sort)
+  bool ids = false;
+  // FP64: x, y, z, px, py, pz, w (cur / alternate)
+  double* a[7] = {};
+  double* b[7] = {};
+  // mixed: cell (int32 stored as uint32) + ox, y, z, ux, uy, uz, w (float)
+  uint32_t* ma[8] = {};
+  uint32_t* mb[8] = {};
+  int64_t* id[2] = {};
+  int* key = nullptr;
+  int* tile_start = nullptr;  // ntiles + 1
+  int* d_count = nullptr;     // == tile_start + ntiles
+};
+
+struct ProfEntry {
+  double ms = 0;
+  int64_t count = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+}  // namespace
+
+struct cylpic_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Grid g{};
+  Tiles tl{};
+  int prec = CYLPIC_FP64;
+  FieldSet<double> f64;
+  FieldSet<float> f32;
+  double* geo = nullptr;
+  int ntab = 0;
+  double* src_dev[2] = {};
+  double* src_host[2] = {};
+  int* d_err = nullptr;
+  double* d_energy = nullptr;
+  std::vector<Species> sp;
+  SortScratch sc{};
+  uint32_t* sc_mem = nullptr;
+  long sc_cap = -1;
+  int sort_interval = 1;
+  long push_count = 0;
+  bool sorted_once = false;
+  bool profiling = false;
+  std::map<std::string, ProfEntry> prof;
+  std::vector<hipEvent_t> event_pool;
+  double time = 0;
+};
+
+namespace {
+
+// -------------------------------------------------------------- profiling
+hipEvent_t get_event(cylpic_ctx* h) {
+  if (!h->event_pool.empty()) {
+    hipEvent_t e = h->event_pool.back();
+    h->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct Scope {
+  cylpic_ctx* h;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  Scope(cylpic_ctx* h_, const char* n) : h(h_), name(n) {
+    if (h->profiling) {
+      a = get_event(h);
+      b = get_event(h);
+      if (a) (void)hipEventRecord(a, h->stream);
+    }
+  }
+  ~Scope() {
+    if (h->profiling && a && b) {
+      (void)hipEventRecord(b, h->stream);
+      h->prof[name].pending.emplace_back(a, b);
+    }
+  }
+};
+
+void resolve_profile(cylpic_ctx* h) {
+  for (auto& kv : h->prof) {
+    for (auto& ev : kv.second.pending) {
+      float ms = 0;
+      (void)hipEventSynchronize(ev.second);
+      if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+        kv.second.ms += ms;
+        kv.second.count += 1;
+      }
+      h->event_pool.push_back(ev.first);
+      h->event_pool.push_back(ev.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+// -------------------------------------------------------------- helpers
+int map_field_bc(int b) {  // boundary.F90:30-68
+  if (b == BC_OTHER) b = BC_CLAMP;
+  if (b == BC_REFLECT) b = BC_CLAMP;
+  if (b == BC_OPEN) b = BC_SIMPLE_OUTFLOW;
+  return b;
+}
+int map_particle_bc(int b) {  // boundary.F90:98-131
+  if (b == BC_OTHER || b == BC_CONDUCT) b = BC_REFLECT;
+  if (b == BC_SIMPLE_LASER || b == BC_SIMPLE_OUTFLOW || b == BC_CPML_LASER || b == BC_CPML_OUTFLOW) b = BC_OPEN;
+  return b;
+}
+
+// Geometry tables of push_particles, particles.F90:190-217
+void geometry(const Grid& g, std::vector<double>& t) {
+  const int ny = g.ny, jng = g.ng, n = ny + 2 * jng + 1;
+  t.assign(4 * (size_t)n, 0.0);
+  double* rt = t.data();
+  double* xt = rt + n;
+  double* vol = xt + n;
+  double* rat = vol + n;
+  const double dx = g.dx, dy = g.dy;
+  double r_low = g.y_grid_min - (double)jng * dy;
+  xt[0] = 1.0 / (((2.0 * PI) * std::fabs(r_low)) * dx);
+  for (int iy = 1 - jng; iy <= ny + jng; ++iy) {
+    if (std::lround((2.0 * r_low) / dy) == -1) {
+      const double hh = 0.5 * dy;
+      rt[iy + jng] = 1.0 / (PI * (hh * hh));
+    } else {
+      const double rh = r_low + dy;
+      rt[iy + jng] = 1.0 / (PI * std::fabs(rh * rh - r_low * r_low));
+    }
+    xt[iy + jng] = 1.0 / (((2.0 * PI) * std::fabs(r_low + dy)) * dx);
+    r_low = r_low + dy;
+  }
+  for (int iy = 1 - jng; iy <= ny + jng; ++iy) {
+    vol[iy + jng] = rt[iy + jng] / dx;
+    rat[iy + jng] = xt[iy + jng] / xt[iy - 1 + jng];
+  }
+}
+
+Geo geo_of(cylpic_ctx* h) {
+  Geo g;
+  g.inv_area_rt = h->geo;
+  g.inv_area_xt = h->geo + h->ntab;
+  g.inv_volume = h->geo + 2 * h->ntab;
+  g.ratio_area_xt = h->geo + 3 * h->ntab;
+  return g;
+}
+
+long nfield(const cylpic_ctx* h) { return h->g.plane * h->g.M; }
+
+int alloc_fields(cylpic_ctx* h) {
+  const long n = nfield(h);
+  const long nb = (long)h->g.ny2 * h->g.M;
+  if (h->prec == CYLPIC_FP64) {
+    for (int k = 0; k < 15; ++k) {
+      HIPCHK(hipMalloc((void**)&h->f64.f[k], n * sizeof(cplx<double>)));
+      launch_fill_zero(h->f64.f[k], n, h->stream);
+    }
+    for (int k = 0; k < 6; ++k) {
+      HIPCHK(hipMalloc((void**)&h->f64.bnd_xmin[k], nb * sizeof(cplx<double>)));
+      HIPCHK(hipMalloc((void**)&h->f64.bnd_xmax[k], nb * sizeof(cplx<double>)));
+      launch_fill_zero(h->f64.bnd_xmin[k], nb, h->stream);
+      launch_fill_zero(h->f64.bnd_xmax[k], nb, h->stream);
+    }
+  } else {
+    for (int k = 0; k < 15; ++k) {
+      HIPCHK(hipMalloc((void**)&h->f32.f[k], n * sizeof(cplx<float>)));
+      launch_fill_zero(h->f32.f[k], n, h->stream);
+    }
+    for (int k = 0; k < 6; ++k) {
+      HIPCHK(hipMalloc((void**)&h->f32.bnd_xmin[k], nb * sizeof(cplx<float>)));
+      HIPCHK(hipMalloc((void**)&h->f32.bnd_xmax[k], nb * sizeof(cplx<float>)));
+      launch_fill_zero(h->f32.bnd_xmin[k], nb, h->stream);
+      launch_fill_zero(h->f32.bnd_xmax[k], nb, h->stream);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+int set_tiles(cylpic_ctx* h, int tx, int tr) {
+  if (tx < 1 || tr < 1) return fail(CYLPIC_EINVAL, "tile sizes must be >= 1");
+  Tiles& t = h->tl;
+  t.tx = tx;
+  t.tr = tr;
+  t.ntx = (h->g.nx + tx - 1) / tx;
+  t.ntr = (h->g.ny + tr - 1) / tr;
+  t.ntiles = t.ntx * t.ntr;
+  size_t lds = push_lds_bytes(h->g, t, h->prec);
+  if (lds > 64 * 1024)
+    return fail(CYLPIC_EINVAL, "tile too large: the LDS current box needs " + std::to_string(lds) +
+                                   " bytes (limit 65536)");
+  if (h->prec == CYLPIC_MIXED && lds > 0) {
+    // nothing extra: dynamic LDS below 64 KB needs no attribute
+  }
+  return CYLPIC_OK;
+}
+
+int ensure_sort_scratch(cylpic_ctx* h, long n) {
+  if (n <= h->sc_cap) return CYLPIC_OK;
+  if (h->sc_mem) (void)hipFree(h->sc_mem);
+  h->sc_mem = nullptr;
+  const size_t words = sort_scratch_words(n);
+  HIPCHK(hipMalloc((void**)&h->sc_mem, words * sizeof(uint32_t)));
+  uint32_t* p = h->sc_mem;
+  h->sc.k[0] = p;
+  p += n;
+  h->sc.k[1] = p;
+  p += n;
+  h->sc.v[0] = p;
+  p += n;
+  h->sc.v[1] = p;
+  p += n;
+  h->sc.counts = p;
+  const long nb = (n + 4095) / 4096;
+  p += nb * 256;
+  h->sc.scan_tmp = p;
+  h->sc.cap = n;
+  h->sc_cap = n;
+  return CYLPIC_OK;
+}
+
+// FP64 / mixed particle views
+PartF64 view64(Species& s) {
+  PartF64 p;
+  p.x = s.a[0];
+  p.y = s.a[1];
+  p.z = s.a[2];
+  p.px = s.a[3];
+  p.py = s.a[4];
+  p.pz = s.a[5];
+  p.w = s.a[6];
+  p.key = s.key;
+  return p;
+}
+
+int sort_species(cylpic_ctx* h, Species& s) {
+  Scope sc(h, "sort");
+  if (s.n <= 0) {
+    launch_tile_bounds(nullptr, 0, h->tl.ntiles, s.tile_start, h->stream);
+    return CYLPIC_OK;
+  }
+  int rc = ensure_sort_scratch(h, s.n);
+  if (rc) return rc;
+  uint32_t *perm = nullptr, *skeys = nullptr;
+  radix_sort(s.key, s.n, h->tl.ntiles, h->sc, &perm, &skeys, h->stream);
+  if (h->prec == CYLPIC_FP64) {
+    launch_gather_f64(perm, s.n, s.a, s.b, 7, h->stream);
+    for (int k = 0; k < 7; ++k) std::swap(s.a[k], s.b[k]);
+  } else {
+    launch_gather_32(perm, s.n, s.ma, s.mb, 8, h->stream);
+    for (int k = 0; k < 8; ++k) std::swap(s.ma[k], s.mb[k]);
+  }
+  if (s.ids) {
+    launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], h->stream);
+    std::swap(s.id[0], s.id[1]);
+  }
+  launch_copy_keys(skeys, s.key, s.n, h->stream);
+  launch_tile_bounds(skeys, s.n, h->tl.ntiles, s.tile_start, h->stream);
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+// read back the alive count after a sort (synchronising)
+int refresh_count(cylpic_ctx* h, Species& s) {
+  if (!s.tile_start) return CYLPIC_OK;
+  int c = 0;
+  HIPCHK(hipMemcpyAsync(&c, s.tile_start + h->tl.ntiles, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  s.n = c;
+  return CYLPIC_OK;
+}
+
+int check_device_errors(cylpic_ctx* h) {
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, h->d_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (e) {
+    HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(int), h->stream));
+    std::string m = "device reported:";
+    if (e & ERR_CELL_RANGE) m += " a particle outside the grid's guard cells (skipped);";
+    if (e & ERR_DCELL) m += " a particle crossing more than one cell in a step (CFL violated, deposit skipped);";
+    return fail(CYLPIC_EPHYS, m);
+  }
+  return CYLPIC_OK;
+}
+
+void free_all(cylpic_ctx* h) {
+  for (int k = 0; k < 15; ++k) {
+    if (h->f64.f[k]) (void)hipFree(h->f64.f[k]);
+    if (h->f32.f[k]) (void)hipFree(h->f32.f[k]);
+  }
+  for (int k = 0; k < 6; ++k) {
+    if (h->f64.bnd_xmin[k]) (void)hipFree(h->f64.bnd_xmin[k]);
+    if (h->f64.bnd_xmax[k]) (void)hipFree(h->f64.bnd_xmax[k]);
+    if (h->f32.bnd_xmin[k]) (void)hipFree(h->f32.bnd_xmin[k]);
+    if (h->f32.bnd_xmax[k]) (void)hipFree(h->f32.bnd_xmax[k]);
+  }
+  for (auto& s : h->sp) {
+    for (int k = 0; k < 7; ++k) {
+      if (s.a[k]) (void)hipFree(s.a[k]);
+      if (s.b[k]) (void)hipFree(s.b[k]);
+    }
+    for (int k = 0; k < 8; ++k) {
+      if (s.ma[k]) (void)hipFree(s.ma[k]);
+      if (s.mb[k]) (void)hipFree(s.mb[k]);
+    }
+    if (s.id[0]) (void)hipFree(s.id[0]);
+    if (s.id[1]) (void)hipFree(s.id[1]);
+    if (s.key) (void)hipFree(s.key);
+    if (s.tile_start) (void)hipFree(s.tile_start);
+  }
+  if (h->geo) (void)hipFree(h->geo);
+  for (int k = 0; k < 2; ++k) {
+    if (h->src_dev[k]) (void)hipFree(h->src_dev[k]);
+    if (h->src_host[k]) (void)hipHostFree(h->src_host[k]);
+  }
+  if (h->d_err) (void)hipFree(h->d_err);
+  if (h->d_energy) (void)hipFree(h->d_energy);
+  if (h->sc_mem) (void)hipFree(h->sc_mem);
+  for (auto& kv : h->prof)
+    for (auto& ev : kv.second.pending) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+  for (auto e : h->event_pool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+#define CHECK_H(h) \
+  if (!(h)) return fail(CYLPIC_EINVAL, "null handle")
+
+int alloc_species_arrays(cylpic_ctx* h, Species& s) {
+  const long cap = s.cap > 0 ? s.cap : 1;
+  if (h->prec == CYLPIC_FP64) {
+    for (int k = 0; k < 7; ++k) {
+      HIPCHK(hipMalloc((void**)&s.a[k], cap * sizeof(double)));
+      HIPCHK(hipMalloc((void**)&s.b[k], cap * sizeof(double)));
+    }
+  } else {
+    for (int k = 0; k < 8; ++k) {
+      HIPCHK(hipMalloc((void**)&s.ma[k], cap * sizeof(uint32_t)));
+      HIPCHK(hipMalloc((void**)&s.mb[k], cap * sizeof(uint32_t)));
+    }
+  }
+  HIPCHK(hipMalloc((void**)&s.key, cap * sizeof(int)));
+  HIPCHK(hipMalloc((void**)&s.tile_start, (h->tl.ntiles + 1) * sizeof(int)));
+  HIPCHK(hipMemsetAsync(s.tile_start, 0, (h->tl.ntiles + 1) * sizeof(int), h->stream));
+  s.d_count = s.tile_start + h->tl.ntiles;
+  return CYLPIC_OK;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+const char* cylpic_last_error(void) { return g_last_error.c_str(); }
+
+int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx, double dy, double dt,
+                double x_min, double x_max, double y_min, double y_max, const int bc_field[4],
+                const int bc_particle[4], int precision, int device) {
+  if (!out) return fail(CYLPIC_EINVAL, "null output handle");
+  *out = nullptr;
+  if (ng != 5) return fail(CYLPIC_EINVAL, "ng must be 5 (triangle shape: png = 3, ng = png + 2)");
+  if (nx < 2 * ng) return fail(CYLPIC_EINVAL, "nx must be >= 2*ng");
+  if (ny < 3) return fail(CYLPIC_EINVAL, "ny must be >= 3");
+  if (n_mode < 1 || n_mode > 8) return fail(CYLPIC_EINVAL, "n_mode must be in 1..8");
+  if (!(dx > 0) || !(dy > 0) || !(dt > 0)) return fail(CYLPIC_EINVAL, "dx, dy, dt must be positive");
+  if (precision != CYLPIC_FP64 && precision != CYLPIC_MIXED)
+    return fail(CYLPIC_EINVAL, "precision must be CYLPIC_FP64 or CYLPIC_MIXED");
+  if (precision == CYLPIC_MIXED) return fail(CYLPIC_EINVAL, "CYLPIC_MIXED is not available in this build");
+  if (!bc_field || !bc_particle) return fail(CYLPIC_EINVAL, "null boundary arrays");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CYLPIC_EHIP, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(CYLPIC_EINVAL, "device index out of range");
+
+  cylpic_ctx* h = new (std::nothrow) cylpic_ctx();
+  if (!h) return fail(CYLPIC_ENOMEM, "host allocation failed");
+  h->device = device;
+  h->prec = precision;
+  Grid& g = h->g;
+  g.nx = nx;
+  g.ny = ny;
+  g.M = n_mode;
+  g.ng = ng;
+  g.nx2 = nx + 2 * ng;
+  g.ny2 = ny + 2 * ng;
+  g.plane = (long)g.nx2 * g.ny2;
+  g.dx = dx;
+  g.dy = dy;
+  g.dt = dt;
+  g.x_min = x_min;
+  g.x_max = x_max;
+  g.y_min = y_min;
+  g.y_max = y_max;
+  g.x_grid_min = x_min + dx / 2.0;  // setup.F90:177-180
+  g.y_grid_min = y_min + dy / 2.0;
+  for (int i = 0; i < 4; ++i) {
+    g.bc_field[i] = map_field_bc(bc_field[i]);
+    g.bc_particle[i] = map_particle_bc(bc_particle[i]);
+  }
+  for (int i = 0; i < 4; ++i) {
+    if (i == YMIN) continue;
+    int b = g.bc_particle[i];
+    if (b == BC_THERMAL || b == BC_HEAT_BATH || b == BC_MIXED) {
+      delete h;
+      return fail(CYLPIC_EINVAL, "thermal / heat-bath / mixed particle boundaries are not supported");
+    }
+  }
+  if (g.bc_field[YMAX] == BC_PERIODIC || g.bc_particle[YMAX] == BC_PERIODIC) {
+    delete h;
+    return fail(CYLPIC_EINVAL, "r_max cannot be periodic");
+  }
+  if ((g.bc_particle[XMIN] == BC_PERIODIC) != (g.bc_particle[XMAX] == BC_PERIODIC)) {
+    delete h;
+    return fail(CYLPIC_EINVAL, "periodic particle boundaries must be set on both x ends");
+  }
+
+  auto bail = [&](int rc) {
+    free_all(h);
+    delete h;
+    return rc;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(fail(CYLPIC_EHIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(CYLPIC_EHIP, "stream creation failed"));
+  int rc = alloc_fields(h);
+  if (rc) return bail(rc);
+  rc = set_tiles(h, std::min(16, nx), std::min(8, ny));
+  if (rc) return bail(rc);
+  std::vector<double> t;
+  geometry(g, t);
+  h->ntab = ny + 2 * ng + 1;
+  if (hipMalloc((void**)&h->geo, t.size() * sizeof(double)) != hipSuccess)
+    return bail(fail(CYLPIC_ENOMEM, "geometry allocation failed"));
+  if (hipMemcpy(h->geo, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(CYLPIC_EHIP, "geometry upload failed"));
+  for (int k = 0; k < 2; ++k) {
+    if (hipMalloc((void**)&h->src_dev[k], 2 * (ny + 1) * sizeof(double)) != hipSuccess)
+      return bail(fail(CYLPIC_ENOMEM, "source allocation failed"));
+    if (hipHostMalloc((void**)&h->src_host[k], 2 * (ny + 1) * sizeof(double), 0) != hipSuccess)
+      return bail(fail(CYLPIC_ENOMEM, "pinned source allocation failed"));
+  }
+  if (hipMalloc((void**)&h->d_err, sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&h->d_energy, 2 * n_mode * sizeof(double)) != hipSuccess)
+    return bail(fail(CYLPIC_ENOMEM, "allocation failed"));
+  if (hipMemset(h->d_err, 0, sizeof(int)) != hipSuccess) return bail(fail(CYLPIC_EHIP, "memset failed"));
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(CYLPIC_EHIP, "init sync failed"));
+  *out = h;
+  return CYLPIC_OK;
+}
+
+int cylpic_finalize(cylpic_ctx* h) {
+  CHECK_H(h);
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+  return CYLPIC_OK;
+}
+
+int cylpic_set_dt(cylpic_ctx* h, double dt) {
+  CHECK_H(h);
+  if (!(dt > 0)) return fail(CYLPIC_EINVAL, "dt must be positive");
+  h->g.dt = dt;
+  return CYLPIC_OK;
+}
+
+int cylpic_set_time(cylpic_ctx* h, double time) {
+  CHECK_H(h);
+  h->time = time;
+  return CYLPIC_OK;
+}
+
+int cylpic_set_tile(cylpic_ctx* h, int tile_x, int tile_r) {
+  CHECK_H(h);
+  for (auto& s : h->sp)
+    if (s.cap > 0) return fail(CYLPIC_ESTATE, "set the tile shape before registering species");
+  return set_tiles(h, tile_x, tile_r);
+}
+
+int cylpic_set_sort_interval(cylpic_ctx* h, int interval) {
+  CHECK_H(h);
+  if (interval < 1) return fail(CYLPIC_EINVAL, "sort interval must be >= 1");
+  h->sort_interval = interval;
+  return CYLPIC_OK;
+}
+
+int cylpic_set_species(cylpic_ctx* h, int id, double charge, double mass, int immobile, int64_t capacity) {
+  CHECK_H(h);
+  if (id < 0 || id > (int)h->sp.size()) return fail(CYLPIC_EINVAL, "species ids must be dense from 0");
+  if (!(mass > 0)) return fail(CYLPIC_EINVAL, "mass must be positive");
+  if (capacity < 0 || capacity > 0x7fffffffL) return fail(CYLPIC_EINVAL, "capacity out of range");
+  HIPCHK(hipSetDevice(h->device));
+  if (id == (int)h->sp.size()) h->sp.emplace_back();
+  Species& s = h->sp[id];
+  if (s.cap > 0) return fail(CYLPIC_ESTATE, "species already registered");
+  s.q = charge;
+  s.m = mass;
+  s.immobile = immobile;
+  s.cap = capacity;
+  return alloc_species_arrays(h, s);
+}
+
+int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double* x, const double* y,
+                            const double* z, const double* px, const double* py, const double* pz,
+                            const double* w, const int64_t* ids) {
+  CHECK_H(h);
+  if (species < 0 || species >= (int)h->sp.size()) return fail(CYLPIC_EINVAL, "unknown species");
+  Species& s = h->sp[species];
+  if (n < 0 || n > s.cap) return fail(CYLPIC_EINVAL, "particle count exceeds the species capacity");
+  if (n > 0 && (!x || !y || !z || !px || !py || !pz || !w)) return fail(CYLPIC_EINVAL, "null particle array");
+  HIPCHK(hipSetDevice(h->device));
+  const double* src[7] = {x, y, z, px, py, pz, w};
+  for (int k = 0; k < 7; ++k)
+    if (n > 0) HIPCHK(hipMemcpyAsync(s.a[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (ids) {
+    if (!s.id[0]) {
+      HIPCHK(hipMalloc((void**)&s.id[0], (s.cap > 0 ? s.cap : 1) * sizeof(int64_t)));
+      HIPCHK(hipMalloc((void**)&s.id[1], (s.cap > 0 ? s.cap : 1) * sizeof(int64_t)));
+    }
+    if (n > 0) HIPCHK(hipMemcpyAsync(s.id[0], ids, n * sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+    s.ids = true;
+  } else {
+    s.ids = false;
+  }
+  s.n = n;
+  launch_keys_f64(h->g, h->tl, view64(s), n, h->stream);
+  int rc = sort_species(h, s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return refresh_count(h, s);
+}
+
+int cylpic_particle_count(cylpic_ctx* h, int species, int64_t* n_out) {
+  CHECK_H(h);
+  if (species < 0 || species >= (int)h->sp.size() || !n_out) return fail(CYLPIC_EINVAL, "bad argument");
+  HIPCHK(hipSetDevice(h->device));
+  Species& s = h->sp[species];
+  int rc = sort_species(h, s);  // drop removed particles first
+  if (rc) return rc;
+  rc = refresh_count(h, s);
+  if (rc) return rc;
+  *n_out = s.n;
+  return CYLPIC_OK;
+}
+
+int cylpic_download_particles(cylpic_ctx* h, int species, int64_t capacity, int64_t* n_out, double* x,
+                              double* y, double* z, double* px, double* py, double* pz, double* w,
+                              int64_t* ids) {
+  CHECK_H(h);
+  if (species < 0 || species >= (int)h->sp.size()) return fail(CYLPIC_EINVAL, "unknown species");
+  HIPCHK(hipSetDevice(h->device));
+  Species& s = h->sp[species];
+  int rc = sort_species(h, s);
+  if (rc) return rc;
+  rc = refresh_count(h, s);
+  if (rc) return rc;
+  if (n_out) *n_out = s.n;
+  if (s.n > capacity) return fail(CYLPIC_EINVAL, "output capacity smaller than the particle count");
+  double* dst[7] = {x, y, z, px, py, pz, w};
+  for (int k = 0; k < 7; ++k)
+    if (dst[k] && s.n > 0)
+      HIPCHK(hipMemcpyAsync(dst[k], s.a[k], s.n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (ids && s.ids && s.n > 0)
+    HIPCHK(hipMemcpyAsync(ids, s.id[0], s.n * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return check_device_errors(h);
+}
+
+int cylpic_upload_field(cylpic_ctx* h, int which, const double* f) {
+  CHECK_H(h);
+  if (which < 0 || which >= CYLPIC_NFIELDS || !f) return fail(CYLPIC_EINVAL, "bad field id or pointer");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(h->f64.f[which], f, nfield(h) * sizeof(cplx<double>), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CYLPIC_OK;
+}
+
+int cylpic_download_field(cylpic_ctx* h, int which, double* f) {
+  CHECK_H(h);
+  if (which < 0 || which >= CYLPIC_NFIELDS || !f) return fail(CYLPIC_EINVAL, "bad field id or pointer");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(f, h->f64.f[which], nfield(h) * sizeof(cplx<double>), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return check_device_errors(h);
+}
+
+int cylpic_setup_field_boundaries(cylpic_ctx* h) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  launch_setup_bnd(h->g, h->f64, h->stream);
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+int cylpic_update_eb_fields_half(cylpic_ctx* h) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  const size_t bytes = nfield(h) * sizeof(cplx<double>);
+  {
+    Scope s(h, "update_e");
+    launch_update_e(h->g, h->f64, h->stream);
+  }
+  {
+    Scope s(h, "field_bcs");
+    launch_efield_bcs(h->g, h->f64, h->stream);
+  }
+  {
+    Scope s(h, "copy_old");
+    for (int k = 0; k < 3; ++k)
+      HIPCHK(hipMemcpyAsync(h->f64.f[9 + k], h->f64.f[3 + k], bytes, hipMemcpyDeviceToDevice, h->stream));
+  }
+  {
+    Scope s(h, "update_b");
+    launch_update_b(h->g, h->f64, h->stream);
+  }
+  {
+    Scope s(h, "field_bcs");
+    launch_bfield_bcs(h->g, h->f64, 1, h->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+int cylpic_push_particles(cylpic_ctx* h) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  // jxm_old = jxm; jxm = 0 (particles.F90:163-169): swap roles, clear the new J
+  for (int k = 0; k < 3; ++k) std::swap(h->f64.f[6 + k], h->f64.f[12 + k]);
+  {
+    Scope s(h, "zero_j");
+    for (int k = 0; k < 3; ++k) launch_fill_zero(h->f64.f[6 + k], nfield(h), h->stream);
+  }
+  const double dt = h->g.dt;
+  for (auto& s : h->sp) {
+    if (s.immobile || s.n <= 0) continue;
+    PushConst pc;
+    pc.part_q = s.q;
+    pc.part_mc = C_LIGHT * s.m;
+    pc.ipart_mc = 1.0 / pc.part_mc;
+    const double dtfac = (0.5 * dt) * 0.25;
+    pc.cmratio = (pc.part_q * dtfac) * pc.ipart_mc;
+    pc.ccmratio = C_LIGHT * pc.cmratio;
+    pc.idx = 1.0 / h->g.dx;
+    pc.idy = 1.0 / h->g.dy;
+    pc.idt = 1.0 / dt;
+    pc.dtco2 = C_LIGHT * (dt / 2.0);
+    Scope sc(h, "push_deposit");
+    launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->d_err, h->stream);
+  }
+  {
+    Scope s(h, "current_axis");
+    launch_current_axis(h->g, h->f64, h->stream);
+  }
+  {
+    Scope sc(h, "particle_bcs");
+    for (auto& s : h->sp)
+      if (!s.immobile && s.n > 0) launch_particle_bcs_f64(h->g, h->tl, view64(s), s.n, s.d_count, h->stream);
+  }
+  HIPCHK(hipGetLastError());
+  h->push_count += 1;
+  if (h->push_count % h->sort_interval == 0) {
+    for (auto& s : h->sp) {
+      if (s.immobile || s.n <= 0) continue;
+      int rc = sort_species(h, s);
+      if (rc) return rc;
+    }
+  }
+  return CYLPIC_OK;
+}
+
+int cylpic_sort_particles(cylpic_ctx* h) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  for (auto& s : h->sp) {
+    int rc = sort_species(h, s);
+    if (rc) return rc;
+  }
+  return CYLPIC_OK;
+}
+
+int cylpic_current_finish(cylpic_ctx* h) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  Scope s(h, "current_finish");
+  launch_current_finish(h->g, h->f64, h->stream);
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+int cylpic_update_eb_fields_final(cylpic_ctx* h, const double* src_xmin, const double* src_xmax) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  const size_t sb = 2 * (size_t)(h->g.ny + 1) * sizeof(double);
+  const double* dsrc[2] = {nullptr, nullptr};
+  const double* hsrc[2] = {src_xmin, src_xmax};
+  for (int k = 0; k < 2; ++k) {
+    if (!hsrc[k]) continue;
+    // stage through pinned memory; the previous step's copy has completed
+    // once the stream reaches this point
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::memcpy(h->src_host[k], hsrc[k], sb);
+    HIPCHK(hipMemcpyAsync(h->src_dev[k], h->src_host[k], sb, hipMemcpyHostToDevice, h->stream));
+    dsrc[k] = h->src_dev[k];
+  }
+  {
+    Scope s(h, "update_b");
+    launch_update_b(h->g, h->f64, h->stream);
+  }
+  {
+    Scope s(h, "field_bcs");
+    launch_bfield_final_bcs(h->g, h->f64, dsrc[0], dsrc[1], h->stream);
+  }
+  {
+    Scope s(h, "update_e");
+    launch_update_e(h->g, h->f64, h->stream);
+  }
+  {
+    Scope s(h, "field_bcs");
+    launch_efield_bcs(h->g, h->f64, h->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+int cylpic_step(cylpic_ctx* h, const double* src_xmin, const double* src_xmax) {
+  int rc = cylpic_update_eb_fields_half(h);
+  if (rc) return rc;
+  rc = cylpic_push_particles(h);
+  if (rc) return rc;
+  rc = cylpic_current_finish(h);
+  if (rc) return rc;
+  h->time += h->g.dt;
+  return cylpic_update_eb_fields_final(h, src_xmin, src_xmax);
+}
+
+int cylpic_mode_energy(cylpic_ctx* h, double* w_e, double* w_b) {
+  CHECK_H(h);
+  if (!w_e || !w_b) return fail(CYLPIC_EINVAL, "null output");
+  HIPCHK(hipSetDevice(h->device));
+  const int M = h->g.M;
+  HIPCHK(hipMemsetAsync(h->d_energy, 0, 2 * M * sizeof(double), h->stream));
+  launch_energy(h->g, h->f64, h->d_energy, h->stream);
+  std::vector<double> out(2 * M);
+  HIPCHK(hipMemcpyAsync(out.data(), h->d_energy, 2 * M * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const double mu0 = 4.0e-7 * PI;
+  for (int m = 0; m < M; ++m) {
+    const double kappa = m == 0 ? 2.0 * PI : PI;
+    w_e[m] = kappa * 0.5 * EPSILON0 * out[2 * m] * h->g.dx * h->g.dy;
+    w_b[m] = kappa * 0.5 / mu0 * out[2 * m + 1] * h->g.dx * h->g.dy;
+  }
+  return CYLPIC_OK;
+}
+
+int cylpic_synchronize(cylpic_ctx* h) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return check_device_errors(h);
+}
+
+int cylpic_profile(cylpic_ctx* h, int enable) {
+  CHECK_H(h);
+  if (!enable) resolve_profile(h);
+  h->profiling = enable != 0;
+  if (enable) h->prof.clear();
+  return CYLPIC_OK;
+}
+
+int cylpic_kernel_time(cylpic_ctx* h, const char* name, double* ms, int64_t* count) {
+  CHECK_H(h);
+  if (!name || !ms || !count) return fail(CYLPIC_EINVAL, "null argument");
+  resolve_profile(h);
+  auto it = h->prof.find(name);
+  if (it == h->prof.end()) {
+    *ms = 0;
+    *count = 0;
+    return CYLPIC_OK;
+  }
+  *ms = it->second.ms;
+  *count = it->second.count;
+  return CYLPIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Shared definitions of the gfx950 quasi-3D PIC library (device + host).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace cylpic {
+
+constexpr double C_LIGHT = 2.99792458e8;                              // constants.F90:175
+constexpr double EPSILON0 = 8.854187817620389850536563031710750e-12;  // constants.F90:179
+constexpr double PI = 3.141592653589793238462643383279503;            // constants.F90:172
+
+// c_bc_* codes, constants.F90:56-72
+enum BcCode {
+  BC_PERIODIC = 1, BC_OTHER = 2, BC_SIMPLE_LASER = 3, BC_SIMPLE_OUTFLOW = 4,
+  BC_OPEN = 5, BC_ZERO_GRADIENT = 7, BC_CLAMP = 8, BC_REFLECT = 9,
+  BC_CONDUCT = 10, BC_THERMAL = 11, BC_CPML_LASER = 12, BC_CPML_OUTFLOW = 13,
+  BC_MIXED = 14, BC_HEAT_BATH = 15, BC_ZERO_B = 16
+};
+enum { XMIN = 0, XMAX = 1, YMIN = 2, YMAX = 3 };
+
+// device error flags (bitmask, read back by the host after a step)
+enum { ERR_CELL_RANGE = 1, ERR_DCELL = 2, ERR_SORT_OVERFLOW = 4 };
+
+template <class T>
+struct alignas(2 * sizeof(T)) cplx {
+  T re, im;
+};
+
+// Grid scalars, passed by value to every kernel.
+struct Grid {
+  int nx, ny, M, ng;
+  int nx2, ny2;          // nx + 2 ng, ny + 2 ng
+  long plane;            // nx2 * ny2
+  double dx, dy, dt;
+  double x_min, x_max, y_min, y_max;
+  double x_grid_min, y_grid_min;
+  int bc_field[4];
+  int bc_particle[4];
+  // Fortran f(ix, iy, m), lower bounds (1-ng, 1-ng, 0)
+  __host__ __device__ __forceinline__ long idx(int ix, int iy, int m) const {
+    return (long)m * plane + (long)(iy + ng - 1) * nx2 + (ix + ng - 1);
+  }
+  // boundary column s(iy, m), lower bounds (1-ng, 0)
+  __host__ __device__ __forceinline__ long idx2(int iy, int m) const {
+    return (long)m * ny2 + (iy + ng - 1);
+  }
+};
+
+// Tile decomposition of the push/deposit: tiles of tile_x x tile_r cells,
+// numbered row-major in x; H halo cells around each tile in the LDS current box.
+struct Tiles {
+  int tx, tr, ntx, ntr, ntiles;
+};
+constexpr int DEPOSIT_HALO = 3;
+
+}  // namespace cylpic

@@ -1,0 +1,114 @@
+// Device-state containers and kernel launcher declarations.
+#pragma once
+
+#include "common.h"
+
+namespace cylpic {
+
+// The fifteen mode arrays of the reference (shared_data.F90:474-478) in
+// the device precision T, plus the outflow-boundary snapshots.
+template <class T>
+struct FieldSet {
+  cplx<T>* f[15] = {};
+  cplx<T>* bnd_xmin[6] = {};
+  cplx<T>* bnd_xmax[6] = {};
+  cplx<T>* ex() { return f[0]; }
+  cplx<T>* er() { return f[1]; }
+  cplx<T>* et() { return f[2]; }
+  cplx<T>* bx() { return f[3]; }
+  cplx<T>* br() { return f[4]; }
+  cplx<T>* bt() { return f[5]; }
+  cplx<T>* jx() { return f[6]; }
+  cplx<T>* jr() { return f[7]; }
+  cplx<T>* jt() { return f[8]; }
+  cplx<T>* bx_old() { return f[9]; }
+  cplx<T>* br_old() { return f[10]; }
+  cplx<T>* bt_old() { return f[11]; }
+  cplx<T>* jx_old() { return f[12]; }
+  cplx<T>* jr_old() { return f[13]; }
+  cplx<T>* jt_old() { return f[14]; }
+};
+
+// Geometry tables of push_particles (particles.F90:190-217), FP64,
+// element [iy + jng] holds Fortran index iy.
+struct Geo {
+  const double* inv_area_rt;
+  const double* inv_area_xt;
+  const double* inv_volume;
+  const double* ratio_area_xt;
+};
+
+// Per-species constants of push_particles (particles.F90:159-261)
+struct PushConst {
+  double part_q, part_mc, ipart_mc, cmratio, ccmratio;
+  double idx, idy, idt, dtco2;
+};
+
+// FP64 particle arrays (SoA).  key = tile of the particle's cell, or
+// ntiles for a removed particle.
+struct PartF64 {
+  double *x, *y, *z, *px, *py, *pz, *w;
+  int* key;
+};
+
+// Mixed-precision particle arrays: x as integer cell (1..nx) plus an FP32
+// offset from the cell centre in units of dx; y, z in units of dy; momenta
+// as u = p / (m c); FP32 weight.
+struct PartMixed {
+  int* cell;
+  float *ox, *y, *z, *ux, *uy, *uz, *w;
+  int* key;
+};
+
+// fields.hip
+template <class T> void launch_update_e(const Grid&, FieldSet<T>&, hipStream_t);
+template <class T> void launch_update_b(const Grid&, FieldSet<T>&, hipStream_t);
+template <class T> void launch_efield_bcs(const Grid&, FieldSet<T>&, hipStream_t);
+template <class T> void launch_bfield_bcs(const Grid&, FieldSet<T>&, int mpi_only, hipStream_t);
+template <class T> void launch_setup_bnd(const Grid&, FieldSet<T>&, hipStream_t);
+template <class T>
+void launch_bfield_final_bcs(const Grid&, FieldSet<T>&, const double* src_xmin, const double* src_xmax,
+                             hipStream_t);
+template <class T> void launch_current_axis(const Grid&, FieldSet<T>&, hipStream_t);
+template <class T> void launch_current_finish(const Grid&, FieldSet<T>&, hipStream_t);
+template <class T> void launch_fill_zero(cplx<T>*, long n, hipStream_t);
+template <class T> void launch_energy(const Grid&, FieldSet<T>&, double* out, hipStream_t);
+void launch_d2f(const cplx<double>* src, cplx<float>* dst, long n, hipStream_t);
+void launch_f2d(const cplx<float>* src, cplx<double>* dst, long n, hipStream_t);
+
+// push.hip
+void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const PartF64&,
+                             const int* tile_start, FieldSet<double>&, const Geo&, int* err,
+                             hipStream_t);
+void launch_particle_bcs_f64(const Grid&, const Tiles&, const PartF64&, long n, const int* count,
+                             hipStream_t);
+void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, hipStream_t);
+void launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, const PartMixed&,
+                               const int* tile_start, FieldSet<float>&, const Geo&, int* err,
+                               hipStream_t);
+void launch_particle_bcs_mixed(const Grid&, const Tiles&, const PartMixed&, long n, const int* count,
+                               hipStream_t);
+void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, hipStream_t);
+size_t push_lds_bytes(const Grid&, const Tiles&, int precision);
+
+// sort.hip
+struct SortScratch {
+  uint32_t *k[2], *v[2];
+  uint32_t* counts;
+  uint32_t* scan_tmp;
+  long cap;
+};
+size_t sort_scratch_words(long n);
+// Stable sort of int keys in [0, nkeys] (n elements); on return perm[i] is
+// the source index of sorted position i and keys_out the sorted keys.
+void radix_sort(const int* keys, long n, int nkeys, SortScratch& s, uint32_t** perm_out,
+                uint32_t** keys_out, hipStream_t);
+void launch_tile_bounds(const uint32_t* sorted_keys, long n, int ntiles, int* tile_start, hipStream_t);
+void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst,
+                       int narr, hipStream_t);
+void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst,
+                      int narr, hipStream_t);
+void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, hipStream_t);
+void launch_copy_keys(const uint32_t* src, int* dst, long n, hipStream_t);
+
+}  // namespace cylpic

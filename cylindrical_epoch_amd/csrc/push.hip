@@ -1,0 +1,451 @@
+// Particle push + charge-conserving mode deposit + particle boundaries on
+// gfx950 (push_particles, src/particles.F90:28-734; particle_bcs,
+// src/boundary.F90:1541-1889).
+//
+// Work decomposition: one workgroup per tile of tile_x x tile_r cells.
+// Particles are kept sorted by tile (stable), so a workgroup streams one
+// contiguous slice of the SoA arrays (coalesced).  The current of the tile
+// plus a DEPOSIT_HALO-cell border (the reach of a triangle-shape particle
+// that started in the tile: 2 cells of shape + 1 cell of motion) is
+// accumulated in LDS with LDS atomics and added to global J once per tile.
+// A particle that drifted further (sorting less often than every step)
+// deposits straight into global J with global atomics.  Fields are gathered
+// from global memory: a tile's 3x3-stencil footprint is a few KB per mode
+// and stays in L1/L2 across the tile's particles.
+#include "cplx_ops.h"
+#include "launch.h"
+
+namespace cylpic {
+
+// triangle/gx.inc: centred quadratic-spline weights (sum 2)
+template <class T>
+__device__ __forceinline__ void weights3(T frac, T& wm, T& w0, T& wp) {
+  T cf2 = frac * frac;
+  wm = ((T)0.25 + cf2) + frac;
+  w0 = (T)1.5 - (T)2 * cf2;
+  wp = ((T)0.25 + cf2) - frac;
+}
+
+// One 3x3 patch of mode m weighted by wx (x) and wy (r), e_part.inc order.
+template <class T, class S>
+__device__ __forceinline__ cplx<T> patch(const Grid& g, const cplx<S>* __restrict__ F, int cx, int cy, int m,
+                                         const T* wx, const T* wy) {
+  cplx<T> acc;
+#pragma unroll
+  for (int jy = -1; jy <= 1; ++jy) {
+    const long k = g.idx(cx, cy + jy, m);
+    cplx<T> a = conv<T>(F[k - 1]), b = conv<T>(F[k]), c = conv<T>(F[k + 1]);
+    cplx<T> row = cadd(cadd(rmul(wx[0], a), rmul(wx[1], b)), rmul(wx[2], c));
+    acc = (jy == -1) ? rmul(wy[0], row) : cadd(acc, rmul(wy[jy + 1], row));
+  }
+  return acc;
+}
+
+struct LdsBox {
+  int x0, r0, LX, LR, M;
+};
+
+__device__ __forceinline__ void dep_add_f64(double* lds, const LdsBox& b, int comp, int m, int cx, int cy,
+                                            cplx<double> v, cplx<double>* __restrict__ gJ, const Grid& g) {
+  const int lx = cx - b.x0, lr = cy - b.r0;
+  if ((unsigned)lx < (unsigned)b.LX && (unsigned)lr < (unsigned)b.LR) {
+    double* p = lds + 2 * (((comp * b.M + m) * b.LR + lr) * b.LX + lx);
+    atomicAdd(p, v.re);
+    if (m) atomicAdd(p + 1, v.im);
+  } else {
+    double* p = (double*)(gJ + g.idx(cx, cy, m));
+    unsafeAtomicAdd(p, v.re);
+    if (m) unsafeAtomicAdd(p + 1, v.im);
+  }
+}
+
+// ------------------------------------------------------------------------
+// FP64 push + deposit: the arithmetic of particles.F90:268-665 in the same
+// operation order (this file is compiled without FP contraction).
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_push_deposit_f64(
+    Grid g, Tiles tl, PushConst pc, PartF64 sp, const int* __restrict__ tile_start,
+    const cplx<double>* __restrict__ ex, const cplx<double>* __restrict__ er,
+    const cplx<double>* __restrict__ et, const cplx<double>* __restrict__ bx,
+    const cplx<double>* __restrict__ br, const cplx<double>* __restrict__ bt,
+    cplx<double>* __restrict__ jx, cplx<double>* __restrict__ jr, cplx<double>* __restrict__ jt, Geo geo,
+    int* __restrict__ err) {
+  extern __shared__ double lds[];
+  const int tile = blockIdx.x;
+  const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
+  LdsBox box;
+  box.x0 = ttx * tl.tx + 1 - DEPOSIT_HALO;
+  box.r0 = tty * tl.tr + 1 - DEPOSIT_HALO;
+  box.LX = tl.tx + 2 * DEPOSIT_HALO;
+  box.LR = tl.tr + 2 * DEPOSIT_HALO;
+  box.M = g.M;
+  const int nl = 3 * g.M * box.LR * box.LX;
+  for (int e = threadIdx.x; e < 2 * nl; e += blockDim.x) lds[e] = 0.0;
+  __syncthreads();
+
+  const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
+  const int jng = g.ng;
+  const double fac = 0.25;
+  const double third = 1.0 / 3.0, sixth = 0.5 * third;
+  const int M = g.M;
+
+  for (int ip = p0 + (int)threadIdx.x; ip < p1; ip += blockDim.x) {
+    if (sp.key[ip] >= tl.ntiles) continue;  // removed since the last sort
+    const double part_weight = sp.w[ip];
+    double part_x = sp.x[ip], part_y = sp.y[ip], part_z = sp.z[ip];
+    double part_ux = sp.px[ip] * pc.ipart_mc;
+    double part_uy = sp.py[ip] * pc.ipart_mc;
+    double part_uz = sp.pz[ip] * pc.ipart_mc;
+
+    double gamma_rel = sqrt(((part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz) + 1.0);
+    double root = pc.dtco2 / gamma_rel;
+    part_x = part_x + part_ux * root;
+    part_y = part_y + part_uy * root;
+    part_z = part_z + part_uz * root;
+
+    const double part_x_local = part_x - g.x_grid_min;
+    double part_r = sqrt(part_y * part_y + part_z * part_z);
+    const double part_r_local = part_r - g.y_grid_min;
+    const cplx<double> emi = mk(part_y / part_r, (-part_z) / part_r);  // exp(-i theta) at t+dt/2
+    const double theta_05 = atan2(part_z, part_y);
+    // 1 / exp(-i theta) by Smith's algorithm
+    cplx<double> ei05;
+    if (fabs(emi.re) >= fabs(emi.im)) {
+      double r = emi.im / emi.re, den = emi.re + emi.im * r;
+      ei05 = mk(1.0 / den, -r / den);
+    } else {
+      double r = emi.re / emi.im, den = emi.re * r + emi.im;
+      ei05 = mk(r / den, -1.0 / den);
+    }
+
+    double cell_x_r = part_x_local * pc.idx;
+    double cell_y_r = part_r_local * pc.idy;
+    int cell_x1 = (int)floor(cell_x_r + 0.5);
+    double frac = (double)cell_x1 - cell_x_r;
+    cell_x1 += 1;
+    double gx[5], gy[5], hx[5], hy[5];
+    gx[0] = gx[4] = gy[0] = gy[4] = 0.0;
+    weights3(frac, gx[1], gx[2], gx[3]);
+    int cell_y1 = (int)floor(cell_y_r + 0.5);
+    frac = (double)cell_y1 - cell_y_r;
+    cell_y1 += 1;
+    weights3(frac, gy[1], gy[2], gy[3]);
+    int cell_x2 = (int)floor(cell_x_r);
+    double frac_x2 = ((double)cell_x2 - cell_x_r) + 0.5;
+    cell_x2 += 1;
+    int cell_y2 = (int)floor(cell_y_r);
+    double frac_y2 = ((double)cell_y2 - cell_y_r) + 0.5;
+    cell_y2 += 1;
+    hx[0] = hx[4] = hy[0] = hy[4] = 0.0;
+    weights3(frac_x2, hx[1], hx[2], hx[3]);
+    weights3(frac_y2, hy[1], hy[2], hy[3]);
+
+    // every index the gather and deposit touch must stay inside the arrays
+    if (cell_x1 < 2 - jng || cell_x1 > g.nx + jng - 1 || cell_y1 < 2 - jng || cell_y1 > g.ny + jng - 1 ||
+        cell_x2 < -2 || cell_x2 > g.nx + 2 || cell_y2 < -2 || cell_y2 > g.ny + 2) {
+      atomicOr(err, ERR_CELL_RANGE);
+      continue;
+    }
+
+    // e_part.inc / b_part.inc
+    double ex_part = 0.0, er_part = 0.0, et_part = 0.0;
+    double bx_part = 0.0, br_part = 0.0, bt_part = 0.0;
+    {
+      cplx<double> e_m = mk(1.0, 0.0);
+      for (int im = 0; im < M; ++im) {
+        ex_part = ex_part + re_mul(e_m, patch<double>(g, ex, cell_x1, cell_y2, im, gx + 1, hy + 1));
+        er_part = er_part + re_mul(e_m, patch<double>(g, er, cell_x2, cell_y1, im, hx + 1, gy + 1));
+        et_part = et_part + re_mul(e_m, patch<double>(g, et, cell_x2, cell_y2, im, hx + 1, hy + 1));
+        e_m = cmul(e_m, emi);
+      }
+      e_m = mk(1.0, 0.0);
+      for (int im = 0; im < M; ++im) {
+        bx_part = bx_part + re_mul(e_m, patch<double>(g, bx, cell_x2, cell_y1, im, hx + 1, gy + 1));
+        br_part = br_part + re_mul(e_m, patch<double>(g, br, cell_x1, cell_y2, im, gx + 1, hy + 1));
+        bt_part = bt_part + re_mul(e_m, patch<double>(g, bt, cell_x1, cell_y1, im, gx + 1, gy + 1));
+        e_m = cmul(e_m, emi);
+      }
+    }
+    const double ey_part = er_part * emi.re + et_part * emi.im;
+    const double ez_part = -(er_part * emi.im) + et_part * emi.re;
+    const double by_part = br_part * emi.re + bt_part * emi.im;
+    const double bz_part = -(br_part * emi.im) + bt_part * emi.re;
+
+    // Boris rotation, particles.F90:404-451
+    const double cmratio = pc.cmratio;
+    const double uxm = part_ux + cmratio * ex_part;
+    const double uym = part_uy + cmratio * ey_part;
+    const double uzm = part_uz + cmratio * ez_part;
+    gamma_rel = sqrt(((uxm * uxm + uym * uym) + uzm * uzm) + 1.0);
+    root = pc.ccmratio / gamma_rel;
+    const double taux = bx_part * root, tauy = by_part * root, tauz = bz_part * root;
+    const double taux2 = taux * taux, tauy2 = tauy * tauy, tauz2 = tauz * tauz;
+    const double tau = 1.0 / (((1.0 + taux2) + tauy2) + tauz2);
+    const double uxp = ((((1.0 + taux2) - tauy2) - tauz2) * uxm +
+                        2.0 * ((taux * tauy + tauz) * uym + (taux * tauz - tauy) * uzm)) * tau;
+    const double uyp = ((((1.0 - taux2) + tauy2) - tauz2) * uym +
+                        2.0 * ((tauy * tauz + taux) * uzm + (tauy * taux - tauz) * uxm)) * tau;
+    const double uzp = ((((1.0 - taux2) - tauy2) + tauz2) * uzm +
+                        2.0 * ((tauz * taux + tauy) * uxm + (tauz * tauy - taux) * uym)) * tau;
+    part_ux = uxp + cmratio * ex_part;
+    part_uy = uyp + cmratio * ey_part;
+    part_uz = uzp + cmratio * ez_part;
+
+    const double part_u2 = (part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz;
+    gamma_rel = sqrt(part_u2 + 1.0);
+    const double igamma = 1.0 / gamma_rel;
+    root = pc.dtco2 * igamma;
+    const double delta_x = part_ux * root, delta_y = part_uy * root, delta_z = part_uz * root;
+    part_x = part_x + delta_x;
+    part_y = part_y + delta_y;
+    part_z = part_z + delta_z;
+    sp.x[ip] = part_x;
+    sp.y[ip] = part_y;
+    sp.z[ip] = part_z;
+    sp.px[ip] = pc.part_mc * part_ux;
+    sp.py[ip] = pc.part_mc * part_uy;
+    sp.pz[ip] = pc.part_mc * part_uz;
+
+    const double part_vy = (part_uy * C_LIGHT) * igamma;
+    const double part_vz = (part_uz * C_LIGHT) * igamma;
+    part_r = sqrt(part_y * part_y + part_z * part_z);
+    const double part_vt = -(part_vy * (part_z / part_r)) + part_vz * (part_y / part_r);
+
+    // current from the motion t+dt/2 -> t+3dt/2, particles.F90:508-584
+    const double px15 = (part_x + delta_x) - g.x_grid_min;
+    const double py15 = part_y + delta_y, pz15 = part_z + delta_z;
+    const double r15 = sqrt(py15 * py15 + pz15 * pz15);
+    const cplx<double> ei15 = mk(py15 / r15, pz15 / r15);
+    const double theta_15 = atan2(pz15, py15);
+    const cplx<double> eidth = cmul(ei15, emi);
+    const double dtheta = theta_15 - theta_05;
+
+    for (int k = 0; k < 5; ++k) {
+      gx[k] = hx[k];
+      gy[k] = hy[k];
+    }
+    cell_x_r = px15 * pc.idx;
+    cell_y_r = (r15 - g.y_grid_min) * pc.idy;
+    int cell_x3 = (int)floor(cell_x_r);
+    const double frac_x3 = ((double)cell_x3 - cell_x_r) + 0.5;
+    cell_x3 += 1;
+    int cell_y3 = (int)floor(cell_y_r);
+    const double frac_y3 = ((double)cell_y3 - cell_y_r) + 0.5;
+    cell_y3 += 1;
+    const int dcellx = cell_x3 - cell_x2, dcelly = cell_y3 - cell_y2;
+    if (dcellx < -1 || dcellx > 1 || dcelly < -1 || dcelly > 1) {
+      atomicOr(err, ERR_DCELL);
+      continue;
+    }
+    for (int k = 0; k < 5; ++k) {
+      hx[k] = 0.0;
+      hy[k] = 0.0;
+    }
+    weights3(frac_x3, hx[1 + dcellx], hx[2 + dcellx], hx[3 + dcellx]);
+    weights3(frac_y3, hy[1 + dcelly], hy[2 + dcelly], hy[3 + dcelly]);
+    for (int k = 0; k < 5; ++k) {
+      hx[k] = hx[k] - gx[k];
+      hy[k] = hy[k] - gy[k];
+    }
+
+    const double q_weight_fac = (pc.part_q * part_weight) * fac;
+    const double fcx = q_weight_fac * pc.idt, fcy = q_weight_fac * pc.idt;
+    const double fcz = q_weight_fac * part_vt;
+    const int xmin = -1 + (dcellx - 1) / 2, xmax = 1 + (dcellx + 1) / 2;
+    const int ymin = -1 + (dcelly - 1) / 2, ymax = 1 + (dcelly + 1) / 2;
+
+    cplx<double> exp_imtheta0 = mk(1.0, 0.0), exp_imdtheta = mk(1.0, 0.0);
+    cplx<double> m_fac_2 = mk(0.0, 0.0), m_fac_3 = mk(0.0, 0.0), m_fac_4 = mk(0.0, 0.0);
+    double inv_mdth = 0.0, inv_m2dth2 = 0.0;
+    for (int im = 0; im < M; ++im) {
+      const double mdth = (double)im * dtheta;
+      const double m2dth2 = mdth * mdth;
+      const bool small_angle_change = fabs(mdth) < 1.0e-4;
+      if (!small_angle_change && im > 0) {
+        inv_mdth = 1.0 / mdth;
+        inv_m2dth2 = inv_mdth * inv_mdth;
+      }
+      if (im > 0) {
+        exp_imtheta0 = cmul(exp_imtheta0, ei05);
+        exp_imdtheta = cmul(exp_imdtheta, eidth);
+        cplx<double> m_fac_1;
+        if (small_angle_change) {
+          m_fac_1 = rmul(2.0, exp_imtheta0);
+          m_fac_2 = cmul(m_fac_1, mk(1.0 - sixth * m2dth2, 0.5 * mdth));
+          m_fac_3 = cmul(m_fac_1, mk(0.5 - 0.125 * m2dth2, third * mdth));
+          m_fac_4 = cmul(m_fac_1, mk(third - 0.1 * m2dth2, 0.25 * mdth));
+        } else {
+          m_fac_1 = rmul(2.0 * inv_mdth, exp_imtheta0);
+          m_fac_2 = cmul(m_fac_1, mk(exp_imdtheta.im, -(exp_imdtheta.re - 1.0)));
+          cplx<double> t = cmul(exp_imdtheta, mk(1.0, -mdth));
+          m_fac_3 = cmul(m_fac_1, rmul(inv_mdth, mk(t.re - 1.0, t.im)));
+          cplx<double> u = cmul(exp_imdtheta, mk(-m2dth2 + 2.0, -(2.0 * mdth)));
+          u.re = u.re - 2.0;
+          m_fac_4 = cmul(m_fac_1, imul(inv_m2dth2, u));
+        }
+      }
+      cplx<double> jyh[5];
+      for (int k = 0; k < 5; ++k) jyh[k] = mk(0.0, 0.0);
+      for (int iy = ymin; iy <= ymax; ++iy) {
+        const int cy = cell_y2 + iy;
+        const double gyv = gy[iy + 2], hyv = hy[iy + 2];
+        cplx<double> w_rt, ym_fac_1;
+        if (im == 0) {
+          w_rt = mk(gyv + 0.5 * hyv, 0.0);
+          ym_fac_1 = mk(0.5 * gyv + third * hyv, 0.0);
+        } else {
+          w_rt = cadd(cmulr(m_fac_2, gyv), cmulr(m_fac_3, hyv));
+          ym_fac_1 = cadd(cmulr(m_fac_3, gyv), cmulr(m_fac_4, hyv));
+        }
+        const double fjx = fcx * geo.inv_area_rt[cy + jng];
+        const double fjy = (fcy * hyv) * geo.inv_area_xt[cy + jng];
+        const double fjz = fcz * geo.inv_volume[cy + jng];
+        const double rat = geo.ratio_area_xt[cy + jng];
+        cplx<double> jxh = mk(0.0, 0.0);
+        for (int ix = xmin; ix <= xmax; ++ix) {
+          const int cx = cell_x2 + ix;
+          const double gxv = gx[ix + 2], hxv = hx[ix + 2];
+          cplx<double> w_xt;
+          if (im == 0) w_xt = mk(gxv + 0.5 * hxv, 0.0);
+          else w_xt = cadd(cmulr(m_fac_2, gxv), cmulr(m_fac_3, hxv));
+          const cplx<double> w_xr = cadd(rmul(gxv, w_rt), rmul(hxv, ym_fac_1));
+          jxh = csub(jxh, rmul(fjx * hxv, w_rt));
+          jyh[ix + 2] = csub(cmulr(jyh[ix + 2], rat), rmul(fjy, w_xt));
+          const cplx<double> jzh = rmul(fjz, w_xr);
+          dep_add_f64(lds, box, 0, im, cx + 1, cy, jxh, jx, g);
+          dep_add_f64(lds, box, 1, im, cx, cy + 1, jyh[ix + 2], jr, g);
+          dep_add_f64(lds, box, 2, im, cx, cy, jzh, jt, g);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // one pass of global atomics per tile
+  for (int e = threadIdx.x; e < nl; e += blockDim.x) {
+    const double re = lds[2 * e], imv = lds[2 * e + 1];
+    if (re == 0.0 && imv == 0.0) continue;
+    int rem = e;
+    const int lx = rem % box.LX;
+    rem /= box.LX;
+    const int lr = rem % box.LR;
+    rem /= box.LR;
+    const int m = rem % M;
+    const int comp = rem / M;
+    cplx<double>* J = comp == 0 ? jx : (comp == 1 ? jr : jt);
+    double* p = (double*)(J + g.idx(box.x0 + lx, box.r0 + lr, m));
+    if (re != 0.0) unsafeAtomicAdd(p, re);
+    if (imv != 0.0) unsafeAtomicAdd(p + 1, imv);
+  }
+}
+
+// ------------------------------------------------------------------------
+// particle_bcs (one rank) and the tile key of the new position.
+// key = tile of the cell (src/housekeeping/split_particle.F90:61-62 with the
+// cell clamped into 1..nx, 1..ny), or ntiles once removed.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int tile_of_cell(const Grid& g, const Tiles& tl, int cx, int cy) {
+  cx = cx < 1 ? 1 : (cx > g.nx ? g.nx : cx);
+  cy = cy < 1 ? 1 : (cy > g.ny ? g.ny : cy);
+  return ((cy - 1) / tl.tr) * tl.ntx + (cx - 1) / tl.tx;
+}
+
+__device__ __forceinline__ int key_f64(const Grid& g, const Tiles& tl, double x, double y, double z) {
+  const double r = sqrt(y * y + z * z);
+  const double fx = floor((x - g.x_grid_min) / g.dx + 1.5);
+  const double fy = floor((r - g.y_grid_min) / g.dy + 1.5);
+  const int cx = fx < -1.0e9 ? -1000000000 : (fx > 1.0e9 ? 1000000000 : (int)fx);
+  const int cy = fy < -1.0e9 ? -1000000000 : (fy > 1.0e9 ? 1000000000 : (int)fy);
+  return tile_of_cell(g, tl, cx, cy);
+}
+
+__global__ void k_particle_bcs_f64(Grid g, Tiles tl, PartF64 sp, long n, const int* __restrict__ count) {
+  const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n || ip >= (long)*count) return;
+  if (sp.key[ip] >= tl.ntiles) return;
+  const int png = 3;
+  const double bshift_x = g.dx * (double)((1 + png + 0) / 2);
+  const double x_min_outer = g.x_min - bshift_x, x_max_outer = g.x_max + bshift_x;
+  const double x_shift = g.x_max - g.x_min;
+  const double y_max_outer = g.y_max + g.dy * (double)((1 + png + 0) / 2);
+  bool out_of_bounds = false;
+  int bc = 0;
+  double part_pos = sp.x[ip];
+  if (part_pos < g.x_min) {
+    bc = g.bc_particle[XMIN];
+    if (bc == BC_REFLECT) {
+      sp.x[ip] = 2.0 * g.x_min - part_pos;
+      sp.px[ip] = -sp.px[ip];
+    } else if (bc == BC_PERIODIC) {
+      sp.x[ip] = part_pos - (-1.0) * x_shift;
+    }
+    if (part_pos < x_min_outer && bc != BC_PERIODIC) out_of_bounds = true;
+  }
+  if (part_pos >= g.x_max) {
+    bc = g.bc_particle[XMAX];
+    if (bc == BC_REFLECT) {
+      sp.x[ip] = 2.0 * g.x_max - part_pos;
+      sp.px[ip] = -sp.px[ip];
+    } else if (bc == BC_PERIODIC) {
+      sp.x[ip] = part_pos - 1.0 * x_shift;
+    }
+    if (part_pos >= x_max_outer && bc != BC_PERIODIC) out_of_bounds = true;
+  }
+  double y = sp.y[ip], z = sp.z[ip];
+  part_pos = sqrt(y * y + z * z);
+  if (part_pos >= g.y_max) {
+    bc = g.bc_particle[YMAX];
+    if (bc == BC_REFLECT) {
+      const double radial_reduction = 2.0 * g.y_max / part_pos - 1.0;
+      y = y * radial_reduction;
+      z = z * radial_reduction;
+      sp.y[ip] = y;
+      sp.z[ip] = z;
+      const double inv_final_r = 1.0 / sqrt(y * y + z * z);
+      const double cos_theta = y * inv_final_r, sin_theta = z * inv_final_r;
+      const double py = sp.py[ip], pz = sp.pz[ip];
+      const double part_pr = py * cos_theta + pz * sin_theta;
+      const double part_pt = -(py * sin_theta) + pz * cos_theta;
+      sp.py[ip] = -(part_pr * cos_theta) - part_pt * sin_theta;
+      sp.pz[ip] = -(part_pr * sin_theta) + part_pt * cos_theta;
+    }
+    if (part_pos >= y_max_outer && bc != BC_PERIODIC) out_of_bounds = true;
+  }
+  sp.key[ip] = out_of_bounds ? tl.ntiles : key_f64(g, tl, sp.x[ip], y, z);
+}
+
+__global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n) {
+  const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n) return;
+  sp.key[ip] = key_f64(g, tl, sp.x[ip], sp.y[ip], sp.z[ip]);
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+size_t push_lds_bytes(const Grid& g, const Tiles& tl, int precision) {
+  size_t cells = (size_t)(tl.tx + 2 * DEPOSIT_HALO) * (size_t)(tl.tr + 2 * DEPOSIT_HALO);
+  size_t per = precision == 0 ? 2 * sizeof(double) : 4 * sizeof(float);  // mixed: hi + lo pairs
+  return 3 * (size_t)g.M * cells * per;
+}
+
+void launch_push_deposit_f64(const Grid& g, const Tiles& tl, const PushConst& pc, const PartF64& sp,
+                             const int* tile_start, FieldSet<double>& F, const Geo& geo, int* err,
+                             hipStream_t s) {
+  size_t lds = push_lds_bytes(g, tl, 0);
+  hipLaunchKernelGGL(k_push_deposit_f64, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
+                     F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err);
+}
+
+void launch_particle_bcs_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, const int* count,
+                             hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_particle_bcs_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, sp, n,
+                     count);
+}
+
+void launch_keys_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_keys_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, sp, n);
+}
+
+}  // namespace cylpic

@@ -1,0 +1,174 @@
+"""Host-side mirror of cylindrical EPOCH's hot-path interface.
+
+The reference's main loop (src/epoch2d.F90:213-265) calls argument-less
+subroutines on module globals: update_eb_fields_half, push_particles,
+current_finish, update_eb_fields_final.  `Simulation` exposes the same
+routines, by the same names, over a handle of the HIP library
+(include/cylpic.h); the state (mode fields, particles) lives on the GPU.
+
+Field arrays cross the boundary in the reference's Fortran layout
+COMPLEX(8) f(1-ng:nx+ng, 1-ng:ny+ng, 0:n_mode-1) (src/shared_data.F90:
+474-478), here numpy complex128 arrays of shape (n_mode, ny+2ng, nx+2ng).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CylpicError, check, lib
+
+# c_bc_* codes, src/constants.F90:56-72
+BC = dict(periodic=1, other=2, simple_laser=3, simple_outflow=4, open=5,
+          zero_gradient=7, clamp=8, reflect=9, conduct=10, thermal=11,
+          cpml_laser=12, cpml_outflow=13, zero_b=16)
+
+FIELDS = ("ex", "er", "et", "bx", "br", "bt", "jx", "jr", "jt",
+          "bx_old", "br_old", "bt_old", "jx_old", "jr_old", "jt_old")
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+def _code(b):
+    return BC[b] if isinstance(b, str) else int(b)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Simulation:
+    """One rank's share of the domain (the r=0 axis and r_max are local)."""
+
+    def __init__(self, nx, ny, n_mode, dx, dy, dt, x_min=0.0, y_max=None,
+                 bc_field=("periodic", "periodic", "other", "conduct"),
+                 bc_particle=("periodic", "periodic", "other", "reflect"),
+                 precision="fp64", device=0, ng=5, tile=None, sort_interval=1):
+        self.nx, self.ny, self.n_mode, self.ng = nx, ny, n_mode, ng
+        self.dx, self.dy, self.dt = dx, dy, dt
+        self.x_min, self.x_max = x_min, x_min + nx * dx
+        self.y_min = 0.0
+        self.y_max = ny * dy if y_max is None else y_max
+        self.shape = (n_mode, ny + 2 * ng, nx + 2 * ng)
+        self.precision = precision
+        self.time = 0.0
+        h = C.c_void_p()
+        bf = (C.c_int * 4)(*[_code(b) for b in bc_field])
+        bp = (C.c_int * 4)(*[_code(b) for b in bc_particle])
+        check(lib().cylpic_init(C.byref(h), nx, ny, n_mode, ng, dx, dy, dt, x_min, self.x_max,
+                                self.y_min, self.y_max, C.cast(bf, C.c_void_p), C.cast(bp, C.c_void_p),
+                                PRECISION[precision], device))
+        self.h = h
+        if tile is not None:
+            check(lib().cylpic_set_tile(self.h, int(tile[0]), int(tile[1])))
+        if sort_interval != 1:
+            check(lib().cylpic_set_sort_interval(self.h, int(sort_interval)))
+        self.species = []
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().cylpic_finalize(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ species
+    def set_species(self, charge, mass, immobile=False, capacity=0):
+        sid = len(self.species)
+        check(lib().cylpic_set_species(self.h, sid, charge, mass, int(bool(immobile)), int(capacity)))
+        self.species.append(dict(charge=charge, mass=mass, immobile=immobile, capacity=capacity))
+        return sid
+
+    def upload_particles(self, sid, x, y, z, px, py, pz, w, ids=None):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, px, py, pz, w)]
+        n = len(arrs[0])
+        idarr = None if ids is None else np.ascontiguousarray(ids, np.int64)
+        check(lib().cylpic_upload_particles(self.h, sid, n, *[_ptr(a) for a in arrs], _ptr(idarr)))
+
+    def particle_count(self, sid):
+        n = C.c_int64()
+        check(lib().cylpic_particle_count(self.h, sid, C.byref(n)))
+        return n.value
+
+    def download_particles(self, sid, with_ids=True):
+        cap = max(1, self.species[sid]["capacity"])
+        out = {k: np.empty(cap, np.float64) for k in ("x", "y", "z", "px", "py", "pz", "w")}
+        ids = np.empty(cap, np.int64) if with_ids else None
+        n = C.c_int64()
+        check(lib().cylpic_download_particles(self.h, sid, cap, C.byref(n),
+                                              *[_ptr(out[k]) for k in ("x", "y", "z", "px", "py", "pz", "w")],
+                                              _ptr(ids)))
+        res = {k: v[:n.value].copy() for k, v in out.items()}
+        if with_ids:
+            res["id"] = ids[:n.value].copy()
+        return res
+
+    # ------------------------------------------------------------ fields
+    def upload_field(self, name, arr):
+        a = np.ascontiguousarray(arr, np.complex128)
+        if a.shape != self.shape:
+            raise CylpicError(f"field {name}: shape {a.shape} != {self.shape}")
+        check(lib().cylpic_upload_field(self.h, FIELDS.index(name), _ptr(a)))
+
+    def download_field(self, name):
+        a = np.empty(self.shape, np.complex128)
+        check(lib().cylpic_download_field(self.h, FIELDS.index(name), _ptr(a)))
+        return a
+
+    def setup_field_boundaries(self):
+        check(lib().cylpic_setup_field_boundaries(self.h))
+
+    # ------------------------------------------------------------ hot path
+    def update_eb_fields_half(self):
+        check(lib().cylpic_update_eb_fields_half(self.h))
+
+    def push_particles(self):
+        check(lib().cylpic_push_particles(self.h))
+
+    def current_finish(self):
+        check(lib().cylpic_current_finish(self.h))
+
+    def update_eb_fields_final(self, src_xmin=None, src_xmax=None):
+        a = None if src_xmin is None else np.ascontiguousarray(src_xmin, np.float64)
+        b = None if src_xmax is None else np.ascontiguousarray(src_xmax, np.float64)
+        for s in (a, b):
+            if s is not None and s.size != 2 * (self.ny + 1):
+                raise CylpicError("laser sources hold source1(0:ny) then source2(0:ny)")
+        check(lib().cylpic_update_eb_fields_final(self.h, _ptr(a), _ptr(b)))
+
+    def step(self, src_xmin=None, src_xmax=None):
+        """One timestep in the reference's main-loop order."""
+        a = None if src_xmin is None else np.ascontiguousarray(src_xmin, np.float64)
+        b = None if src_xmax is None else np.ascontiguousarray(src_xmax, np.float64)
+        check(lib().cylpic_step(self.h, _ptr(a), _ptr(b)))
+        self.time += self.dt
+
+    def sort_particles(self):
+        check(lib().cylpic_sort_particles(self.h))
+
+    def synchronize(self):
+        check(lib().cylpic_synchronize(self.h))
+
+    # ------------------------------------------------------------ diagnostics
+    def mode_energy(self):
+        we = np.zeros(self.n_mode)
+        wb = np.zeros(self.n_mode)
+        check(lib().cylpic_mode_energy(self.h, _ptr(we), _ptr(wb)))
+        return we, wb
+
+    def profile(self, enable=True):
+        check(lib().cylpic_profile(self.h, int(bool(enable))))
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        check(lib().cylpic_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,154 @@
+/*
+ * cylpic.h -- C ABI of the MI355X (gfx950) quasi-3D PIC timestep library.
+ *
+ * The library replaces the argument-less Fortran hot-path subroutines of
+ * cylindrical EPOCH (src/epoch2d.F90:213-265 main loop) with calls on an
+ * opaque handle that owns GPU-resident fields and particles.  Plain C types
+ * only: doubles, ints, pointers and sizes.  Every call returns 0 on success
+ * or a nonzero CYLPIC_E* code; cylpic_last_error() gives the message.  No C++
+ * exception crosses this boundary.  The reference aborts through
+ * abort_code(errcode) (src/housekeeping/utilities.f90:261-281); a Fortran
+ * caller maps a nonzero return to abort_code(c_err_other).
+ *
+ * Conventions (src/shared_data.F90:474-478, src/constants.F90:524-545):
+ *   x is the cylinder axis, y the radius; particles carry Cartesian
+ *   (x, y, z) and momenta (px, py, pz) in SI units, weight in real
+ *   particles per macro-particle.  Mode fields are COMPLEX(8)
+ *   f(1-ng:nx+ng, 1-ng:ny+ng, 0:n_mode-1), column-major, interleaved
+ *   (re, im) doubles: exactly what c_loc(exm) hands over from Fortran.
+ *   ng = jng = 5 (triangle shape).
+ */
+#ifndef CYLPIC_H
+#define CYLPIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cylpic_ctx cylpic_ctx;
+
+/* return codes */
+#define CYLPIC_OK 0
+#define CYLPIC_EINVAL 1     /* bad argument                         */
+#define CYLPIC_EHIP 2       /* a HIP runtime call failed            */
+#define CYLPIC_ESTATE 3     /* call out of order / handle not ready */
+#define CYLPIC_ENOMEM 4     /* device allocation failed             */
+#define CYLPIC_EPHYS 5      /* a particle moved more than the CFL allows */
+
+/* precision of the device path */
+#define CYLPIC_FP64 0       /* FP64 everywhere, the reference's method          */
+#define CYLPIC_MIXED 1      /* int32 cell + FP32 offsets, FP32 fields/momenta,
+                               compensated FP32 current tiles, FP64 axis rows   */
+
+/* boundary codes = the reference's c_bc_* (src/constants.F90:56-72), given
+ * as the deck sets them; the library applies setup_boundaries' mapping
+ * (src/boundary.F90:30-131) itself. */
+
+/* field identifiers for upload/download (src/shared_data.F90:474-478) */
+enum {
+  CYLPIC_EXM = 0, CYLPIC_ERM, CYLPIC_ETM, CYLPIC_BXM, CYLPIC_BRM, CYLPIC_BTM,
+  CYLPIC_JXM, CYLPIC_JRM, CYLPIC_JTM,
+  CYLPIC_BXM_OLD, CYLPIC_BRM_OLD, CYLPIC_BTM_OLD,
+  CYLPIC_JXM_OLD, CYLPIC_JRM_OLD, CYLPIC_JTM_OLD,
+  CYLPIC_NFIELDS
+};
+
+/* Create a handle on HIP device `device`.  Replaces the field allocation of
+ * mpi_initialise (src/housekeeping/mpi_routines.F90:373-403) and the grid
+ * scalars of setup_grid (src/housekeeping/setup.F90:164-208) for one rank
+ * that owns the whole domain (the r=0 axis and r_max are local).
+ * bc_field / bc_particle: x_min, x_max, y_min, y_max codes as in the deck.
+ * All fields start at zero. */
+int cylpic_init(cylpic_ctx **out, int nx, int ny, int n_mode, int ng,
+                double dx, double dy, double dt,
+                double x_min, double x_max, double y_min, double y_max,
+                const int bc_field[4], const int bc_particle[4],
+                int precision, int device);
+int cylpic_finalize(cylpic_ctx *h);
+const char *cylpic_last_error(void);
+
+/* dt after set_dt (src/housekeeping/setup.F90:629-673) on the host */
+int cylpic_set_dt(cylpic_ctx *h, double dt);
+int cylpic_set_time(cylpic_ctx *h, double time);
+
+/* Register species `id` (0-based, ids dense): charge, mass, immobile flag
+ * (src/shared_data.F90:190-280); capacity = maximum particle count. */
+int cylpic_set_species(cylpic_ctx *h, int id, double charge, double mass,
+                       int immobile, int64_t capacity);
+
+/* Copy a species' particles to / from the device, SoA staging arrays filled
+ * from the linked list (src/housekeeping/partlist.F90:414-488 wire order:
+ * pos3, p3, weight).  ids may be NULL (no identity tracking). */
+int cylpic_upload_particles(cylpic_ctx *h, int species, int64_t n,
+                            const double *x, const double *y, const double *z,
+                            const double *px, const double *py, const double *pz,
+                            const double *w, const int64_t *ids);
+int cylpic_download_particles(cylpic_ctx *h, int species, int64_t capacity,
+                              int64_t *n_out, double *x, double *y, double *z,
+                              double *px, double *py, double *pz, double *w,
+                              int64_t *ids);
+int cylpic_particle_count(cylpic_ctx *h, int species, int64_t *n_out);
+
+/* One mode array in the Fortran layout above (c_loc(exm) etc.). */
+int cylpic_upload_field(cylpic_ctx *h, int which, const double *f);
+int cylpic_download_field(cylpic_ctx *h, int which, double *f);
+
+/* setup_field_boundaries (src/housekeeping/setup.F90:393-420): snapshot the
+ * x_min/x_max columns the outflow boundaries use.  Call after the initial
+ * fields are uploaded. */
+int cylpic_setup_field_boundaries(cylpic_ctx *h);
+
+/* update_eb_fields_half (src/fields.f90:316-337): E to t+dt/2, efield_bcs,
+ * B_old = B, B to t+dt/2, B halos. */
+int cylpic_update_eb_fields_half(cylpic_ctx *h);
+
+/* push_particles (src/particles.F90:28-734) for all mobile species:
+ * J_old = J, J = 0, Boris push, charge-conserving mode deposit,
+ * current_bcs_r_min_final (src/boundary.F90:1909-1959) and particle_bcs
+ * (src/boundary.F90:1541-1889: periodic, reflect, open). */
+int cylpic_push_particles(cylpic_ctx *h);
+
+/* current_finish (src/housekeeping/current_smooth.F90:29-46): current_bcs
+ * (reflection + periodic ghost sums) and J halos; no smoothing. */
+int cylpic_current_finish(cylpic_ctx *h);
+
+/* update_eb_fields_final (src/fields.f90:341-353) with bfield_final_bcs
+ * (src/boundary.F90:1505-1537).  src_xmin / src_xmax: the laser sources the
+ * host evaluated for this step through the deck parser, source1(0:ny) then
+ * source2(0:ny) (src/laser.f90:444-461), 2*(ny+1) doubles; NULL = no laser. */
+int cylpic_update_eb_fields_final(cylpic_ctx *h, const double *src_xmin,
+                                  const double *src_xmax);
+
+/* The four calls above in main-loop order plus time += dt
+ * (src/epoch2d.F90:213-265 without optional physics, output or window). */
+int cylpic_step(cylpic_ctx *h, const double *src_xmin, const double *src_xmax);
+
+/* Re-sort every species' particles by tile key (stable), dropping removed
+ * particles.  The key of a particle is its cell (src/housekeeping/
+ * split_particle.F90:61-62) mapped to its tile.  Sorting happens on its own
+ * every `interval` steps (default 1); this forces one now. */
+int cylpic_sort_particles(cylpic_ctx *h);
+int cylpic_set_sort_interval(cylpic_ctx *h, int interval);
+/* Tile shape (cells) of the push/deposit work decomposition. */
+int cylpic_set_tile(cylpic_ctx *h, int tile_x, int tile_r);
+
+/* Per-mode field energies (FP64 reduction; the reference's Cartesian
+ * energy sums are never updated in cylindrical mode, src/housekeeping/
+ * setup.F90:526-571), w_e[n_mode], w_b[n_mode] in joules. */
+int cylpic_mode_energy(cylpic_ctx *h, double *w_e, double *w_b);
+
+/* Wait for all queued device work of this handle. */
+int cylpic_synchronize(cylpic_ctx *h);
+
+/* Kernel timing (HIP events on the library's stream).  After enabling,
+ * cylpic_kernel_time(h, name, &ms, &count) returns the accumulated time and
+ * launch count of a kernel family ("push_deposit", "update_e", ...). */
+int cylpic_profile(cylpic_ctx *h, int enable);
+int cylpic_kernel_time(cylpic_ctx *h, const char *name, double *ms, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

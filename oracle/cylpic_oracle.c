@@ -1108,7 +1108,9 @@ void orc_cell_keys(const orc_grid *g, int64_t n, const double *x, const double *
     double r = sqrt(y[i] * y[i] + z[i] * z[i]);
     int cx = (int)floor((x[i] - g->x_grid_min) / g->dx + 1.5);
     int cy = (int)floor((r - g->y_grid_min) / g->dy + 1.5);
-    if (cx < 1 || cx > nx || cy < 1 || cy > ny) { keys[i] = -1; continue; }
+    /* a particle on the domain edge within rounding maps to the edge cell */
+    cx = cx < 1 ? 1 : (cx > nx ? nx : cx);
+    cy = cy < 1 ? 1 : (cy > ny ? ny : cy);
     int lx = cx - 1, ly = cy - 1;
     int tx = lx / tile_x, ty = ly / tile_r;
     keys[i] = (int32_t)((((int64_t)ty * ntx + tx) * tile_r + (ly - ty * tile_r)) * tile_x +

@@ -114,8 +114,8 @@ void orc_deposit_from_weights(const orc_grid *g, orc_fields *f,
 
 /* ---- cell key sort (key: src/housekeeping/split_particle.F90:61-62) --- */
 /* Tile-major cell key: tiles of tile_x * tile_r cells, tiles row-major in x,
- * cells inside a tile row-major in x.  Returns -1 for a particle outside
- * 1..nx x 1..ny. */
+ * cells inside a tile row-major in x.  Cells are clamped into 1..nx, 1..ny
+ * (a particle on the outer edge within rounding). */
 void orc_cell_keys(const orc_grid *g, int64_t n, const double *x,
                    const double *y, const double *z, int tile_x, int tile_r,
                    int32_t *keys);
