@@ -5,8 +5,13 @@ oracle.
 Particle positions and momenta: bit-identical (same operation order, no
 fused multiply-add, correctly rounded sqrt and division on both sides).
 Currents: the GPU adds each particle's contributions with atomics in a
-different order, and atan2 comes from a different math library, so the
-tolerance is |J_gpu - J_cpu| <= 1e-12 * max|J|."""
+different order and atan2 comes from a different math library.  For m = 0
+the tolerance is |J_gpu - J_cpu| <= 1e-12 * max|J_0|.  For m > 0 the
+reference's exact-branch factors (particles.F90:619-624) lose ~12 eps /
+(m dtheta)^3 of relative accuracy near the |m dtheta| = 1e-4 switch, so an
+ulp of difference in atan2 moves those currents by up to ~1e-9 of max|J_m|
+(tests/test_oracle_invariants.py::test_mode_factor_sensitivity measures
+this on the CPU); the tolerance there is 1e-7 * max|J_m|."""
 import numpy as np
 import pytest
 
@@ -14,7 +19,8 @@ from gpu_helpers import (add_plasma, by_id, load_fields, log, make_pair, max_rel
                          random_fields)
 
 pytestmark = pytest.mark.gpu
-J_TOL = 1e-12
+J_TOL_M0 = 1e-12
+J_TOL_M = 1e-7
 
 
 def _compare_particles(st, sim, sid=0):
@@ -25,11 +31,12 @@ def _compare_particles(st, sim, sid=0):
         assert np.array_equal(got[k], want[k]), (k, max_rel(got[k], want[k]))
 
 
-def _compare_currents(st, sim, tol=J_TOL):
+def _compare_currents(st, sim):
     for n in ("jx", "jr", "jt", "jx_old", "jr_old", "jt_old"):
         got = sim.download_field(n)
-        err = max_rel(got, st.arr[n])
-        assert err <= tol, (n, err)
+        for m in range(got.shape[0]):
+            err = max_rel(got[m], st.arr[n][m])
+            assert err <= (J_TOL_M0 if m == 0 else J_TOL_M), (n, m, err)
 
 
 @pytest.mark.parametrize("M", [1, 2, 3])
@@ -96,8 +103,10 @@ def test_multi_step_thermal(oracle):
         st.step()
         sim.step()
     for n in ("ex", "er", "et", "bx", "br", "bt", "jx", "jr", "jt"):
-        err = max_rel(sim.download_field(n), st.arr[n])
-        assert err < 1e-9, (n, err)
+        got = sim.download_field(n)
+        for m in range(got.shape[0]):
+            err = max_rel(got[m], st.arr[n][m])
+            assert err < (1e-10 if m == 0 else 1e-6), (n, m, err)
     got = by_id(sim.download_particles(0))
     want = by_id({k: st.species[0][k] for k in ("x", "y", "z", "px", "py", "pz", "w", "id")})
     for k in ("x", "y", "z", "px", "py", "pz"):

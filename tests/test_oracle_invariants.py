@@ -347,3 +347,27 @@ def test_particle_bcs_periodic_and_reflect(oracle):
     assert sp["y"][0] == pytest.approx(g.y_max - 0.25 * g.dy, rel=1e-14)
     assert sp["py"][0] == pytest.approx(-p[1], rel=1e-14)
     assert sp["pz"][0] == pytest.approx(p[2], rel=1e-14)
+
+
+def test_mode_factor_sensitivity(oracle):
+    """The m>0 deposit's exact-branch factors are ill-conditioned near the
+    small-angle switch: one ulp of dtheta (the difference two atan2
+    implementations may show) moves the m=1 current by far more than one ulp
+    of J.  This bounds what a GPU/CPU comparison of m>0 currents can expect."""
+    rng = np.random.default_rng(17)
+    worst = 0.0
+    for _ in range(200):
+        dth = 10 ** rng.uniform(-4, -2) * rng.choice([-1, 1])
+        out = []
+        for d in (dth, np.nextafter(dth, 1.0)):
+            st = oracle.State(16, 16, 2, 1e-7, 1e-7, 1e-16)
+            gx = np.array([0, 0.04, 1.28, 0.68, 0])
+            hx = np.array([0, 0.5, 0.12, -0.62, 0])
+            gy = np.array([0, 0.6, 1.34, 0.06, 0])
+            hy = np.array([0, -0.4, 0.14, 0.26, 0])
+            e05 = (math.cos(0.4), math.sin(0.4))
+            ed = (math.cos(dth), math.sin(dth))     # phasor from positions: unchanged
+            oracle.deposit_from_weights(st, gx, hx, gy, hy, 8, 8, 0, 0, 1.0, 1.0, 0.5, e05, ed, d)
+            out.append(np.concatenate([st.arr[c][1].ravel() for c in ("jx", "jr", "jt")]))
+        worst = max(worst, np.max(np.abs(out[0] - out[1])) / np.max(np.abs(out[0])))
+    assert 1e-13 < worst < 1e-8
