@@ -157,7 +157,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--workload", default=WORKLOAD, choices=sorted(deck.WORKLOADS))
     ap.add_argument("--cpu-sample-cols", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")

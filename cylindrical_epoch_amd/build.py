@@ -15,7 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcylpic.so")
-SOURCES = ["api.cpp", "fields.hip", "push.hip", "sort.hip"]
+SOURCES = ["api.cpp", "fields.hip", "push.hip", "push_mixed.hip", "sort.hip"]
+# the mixed-precision push contracts multiply-adds; everything else keeps the
+# reference's operation order
+CONTRACT = {"push_mixed.hip": "-ffp-contract=fast"}
 ARCH = "gfx950"
 
 
@@ -41,7 +44,10 @@ def flags():
 def compile_one(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     lang = ["-x", "hip"]
-    cmd = [hipcc()] + flags() + lang + ["-c", os.path.join(CSRC, src), "-o", out]
+    fl = flags()
+    if src in CONTRACT:
+        fl = [CONTRACT[src] if f == "-ffp-contract=off" else f for f in fl]
+    cmd = [hipcc()] + fl + lang + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -44,9 +44,10 @@ struct Species {
   uint32_t* ma[8] = {};
   uint32_t* mb[8] = {};
   int64_t* id[2] = {};
-  int* key = nullptr;
+  uint32_t* key[2] = {};      // tile key per particle (ntiles = removed), sort ping-pong
   int* tile_start = nullptr;  // ntiles + 1
   int* d_count = nullptr;     // == tile_start + ntiles
+  bool dirty = false;         // pushed since the last sort (removed particles may remain)
 };
 
 struct ProfEntry {
@@ -66,6 +67,7 @@ struct cylpic_ctx {
   FieldSet<double> f64;
   FieldSet<float> f32;
   double* geo = nullptr;
+  float* geof = nullptr;  // FP32 copy for the mixed path
   int ntab = 0;
   double* src_dev[2] = {};
   double* src_host[2] = {};
@@ -76,6 +78,7 @@ struct cylpic_ctx {
   uint32_t* sc_mem = nullptr;
   long sc_cap = -1;
   int sort_interval = 1;
+  int slot_bits = 6;
   long push_count = 0;
   bool sorted_once = false;
   bool profiling = false;
@@ -185,6 +188,19 @@ Geo geo_of(cylpic_ctx* h) {
 
 long nfield(const cylpic_ctx* h) { return h->g.plane * h->g.M; }
 
+// run fn on the field set of the handle's precision
+template <class Fn>
+void with_fields(cylpic_ctx* h, Fn fn) {
+  if (h->prec == CYLPIC_FP64) fn(h->f64);
+  else fn(h->f32);
+}
+size_t cplx_bytes(const cylpic_ctx* h) { return h->prec == CYLPIC_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>); }
+void* field_ptr(cylpic_ctx* h, int k) { return h->prec == CYLPIC_FP64 ? (void*)h->f64.f[k] : (void*)h->f32.f[k]; }
+void swap_fields(cylpic_ctx* h, int a, int b) {
+  std::swap(h->f64.f[a], h->f64.f[b]);
+  std::swap(h->f32.f[a], h->f32.f[b]);
+}
+
 int alloc_fields(cylpic_ctx* h) {
   const long n = nfield(h);
   const long nb = (long)h->g.ny2 * h->g.M;
@@ -223,13 +239,10 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   t.ntx = (h->g.nx + tx - 1) / tx;
   t.ntr = (h->g.ny + tr - 1) / tr;
   t.ntiles = t.ntx * t.ntr;
-  size_t lds = push_lds_bytes(h->g, t, h->prec);
+  size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t) : push_lds_bytes(h->g, t, 0);
   if (lds > 64 * 1024)
-    return fail(CYLPIC_EINVAL, "tile too large: the LDS current box needs " + std::to_string(lds) +
-                                   " bytes (limit 65536)");
-  if (h->prec == CYLPIC_MIXED && lds > 0) {
-    // nothing extra: dynamic LDS below 64 KB needs no attribute
-  }
+    return fail(CYLPIC_EINVAL, "tile too large: the push needs " + std::to_string(lds) +
+                                   " bytes of LDS (limit 65536)");
   return CYLPIC_OK;
 }
 
@@ -258,6 +271,20 @@ int ensure_sort_scratch(cylpic_ctx* h, long n) {
 }
 
 // FP64 / mixed particle views
+PartMixed viewm(Species& s) {
+  PartMixed p;
+  p.cell = (int*)s.ma[0];
+  p.ox = (float*)s.ma[1];
+  p.y = (float*)s.ma[2];
+  p.z = (float*)s.ma[3];
+  p.ux = (float*)s.ma[4];
+  p.uy = (float*)s.ma[5];
+  p.uz = (float*)s.ma[6];
+  p.w = (float*)s.ma[7];
+  p.key = (int*)s.key[0];
+  return p;
+}
+
 PartF64 view64(Species& s) {
   PartF64 p;
   p.x = s.a[0];
@@ -267,20 +294,22 @@ PartF64 view64(Species& s) {
   p.py = s.a[4];
   p.pz = s.a[5];
   p.w = s.a[6];
-  p.key = s.key;
+  p.key = (int*)s.key[0];
   return p;
 }
 
 int sort_species(cylpic_ctx* h, Species& s) {
   Scope sc(h, "sort");
+  s.dirty = false;
   if (s.n <= 0) {
     launch_tile_bounds(nullptr, 0, h->tl.ntiles, s.tile_start, h->stream);
     return CYLPIC_OK;
   }
   int rc = ensure_sort_scratch(h, s.n);
   if (rc) return rc;
-  uint32_t *perm = nullptr, *skeys = nullptr;
-  radix_sort(s.key, s.n, h->tl.ntiles, h->sc, &perm, &skeys, h->stream);
+  uint32_t* perm = nullptr;
+  radix_sort(s.key[0], s.n, h->tl.ntiles, h->slot_bits, h->sc, s.key[1], &perm, h->stream);
+  std::swap(s.key[0], s.key[1]);
   if (h->prec == CYLPIC_FP64) {
     launch_gather_f64(perm, s.n, s.a, s.b, 7, h->stream);
     for (int k = 0; k < 7; ++k) std::swap(s.a[k], s.b[k]);
@@ -292,8 +321,7 @@ int sort_species(cylpic_ctx* h, Species& s) {
     launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], h->stream);
     std::swap(s.id[0], s.id[1]);
   }
-  launch_copy_keys(skeys, s.key, s.n, h->stream);
-  launch_tile_bounds(skeys, s.n, h->tl.ntiles, s.tile_start, h->stream);
+  launch_tile_bounds(s.key[0], s.n, h->tl.ntiles, s.tile_start, h->stream);
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;
 }
@@ -344,10 +372,12 @@ void free_all(cylpic_ctx* h) {
     }
     if (s.id[0]) (void)hipFree(s.id[0]);
     if (s.id[1]) (void)hipFree(s.id[1]);
-    if (s.key) (void)hipFree(s.key);
+    if (s.key[0]) (void)hipFree(s.key[0]);
+    if (s.key[1]) (void)hipFree(s.key[1]);
     if (s.tile_start) (void)hipFree(s.tile_start);
   }
   if (h->geo) (void)hipFree(h->geo);
+  if (h->geof) (void)hipFree(h->geof);
   for (int k = 0; k < 2; ++k) {
     if (h->src_dev[k]) (void)hipFree(h->src_dev[k]);
     if (h->src_host[k]) (void)hipHostFree(h->src_host[k]);
@@ -380,7 +410,8 @@ int alloc_species_arrays(cylpic_ctx* h, Species& s) {
       HIPCHK(hipMalloc((void**)&s.mb[k], cap * sizeof(uint32_t)));
     }
   }
-  HIPCHK(hipMalloc((void**)&s.key, cap * sizeof(int)));
+  HIPCHK(hipMalloc((void**)&s.key[0], cap * sizeof(uint32_t)));
+  HIPCHK(hipMalloc((void**)&s.key[1], cap * sizeof(uint32_t)));
   HIPCHK(hipMalloc((void**)&s.tile_start, (h->tl.ntiles + 1) * sizeof(int)));
   HIPCHK(hipMemsetAsync(s.tile_start, 0, (h->tl.ntiles + 1) * sizeof(int), h->stream));
   s.d_count = s.tile_start + h->tl.ntiles;
@@ -408,7 +439,7 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
   if (!(dx > 0) || !(dy > 0) || !(dt > 0)) return fail(CYLPIC_EINVAL, "dx, dy, dt must be positive");
   if (precision != CYLPIC_FP64 && precision != CYLPIC_MIXED)
     return fail(CYLPIC_EINVAL, "precision must be CYLPIC_FP64 or CYLPIC_MIXED");
-  if (precision == CYLPIC_MIXED) return fail(CYLPIC_EINVAL, "CYLPIC_MIXED is not available in this build");
+  if (precision == CYLPIC_MIXED && n_mode > 4) return fail(CYLPIC_EINVAL, "CYLPIC_MIXED supports n_mode <= 4");
   if (!bc_field || !bc_particle) return fail(CYLPIC_EINVAL, "null boundary arrays");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CYLPIC_EHIP, "no HIP device");
@@ -466,8 +497,16 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
     return bail(fail(CYLPIC_EHIP, "stream creation failed"));
   int rc = alloc_fields(h);
   if (rc) return bail(rc);
-  rc = set_tiles(h, std::min(16, nx), std::min(8, ny));
-  if (rc) return bail(rc);
+  {
+    // default tile: the largest of these whose LDS needs fit in 64 KB
+    const int cand[][2] = {{16, 8}, {16, 4}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}};
+    rc = CYLPIC_EINVAL;
+    for (auto& c : cand) {
+      rc = set_tiles(h, std::min(c[0], nx), std::min(c[1], ny));
+      if (rc == CYLPIC_OK) break;
+    }
+    if (rc) return bail(rc);
+  }
   std::vector<double> t;
   geometry(g, t);
   h->ntab = ny + 2 * ng + 1;
@@ -475,6 +514,12 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
     return bail(fail(CYLPIC_ENOMEM, "geometry allocation failed"));
   if (hipMemcpy(h->geo, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(CYLPIC_EHIP, "geometry upload failed"));
+  {
+    std::vector<float> tf(t.begin(), t.end());
+    if (hipMalloc((void**)&h->geof, tf.size() * sizeof(float)) != hipSuccess ||
+        hipMemcpy(h->geof, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(CYLPIC_ENOMEM, "geometry upload failed"));
+  }
   for (int k = 0; k < 2; ++k) {
     if (hipMalloc((void**)&h->src_dev[k], 2 * (ny + 1) * sizeof(double)) != hipSuccess)
       return bail(fail(CYLPIC_ENOMEM, "source allocation failed"));
@@ -552,8 +597,21 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
   if (n > 0 && (!x || !y || !z || !px || !py || !pz || !w)) return fail(CYLPIC_EINVAL, "null particle array");
   HIPCHK(hipSetDevice(h->device));
   const double* src[7] = {x, y, z, px, py, pz, w};
-  for (int k = 0; k < 7; ++k)
-    if (n > 0) HIPCHK(hipMemcpyAsync(s.a[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (h->prec == CYLPIC_FP64) {
+    for (int k = 0; k < 7; ++k)
+      if (n > 0) HIPCHK(hipMemcpyAsync(s.a[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  } else if (n > 0) {
+    double* tmp = nullptr;
+    HIPCHK(hipMalloc((void**)&tmp, 7 * n * sizeof(double)));
+    double* t7[7];
+    for (int k = 0; k < 7; ++k) {
+      t7[k] = tmp + k * n;
+      HIPCHK(hipMemcpyAsync(t7[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    launch_to_mixed(h->g, s.m, t7, viewm(s), n, h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(tmp));
+  }
   if (ids) {
     if (!s.id[0]) {
       HIPCHK(hipMalloc((void**)&s.id[0], (s.cap > 0 ? s.cap : 1) * sizeof(int64_t)));
@@ -565,7 +623,8 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
     s.ids = false;
   }
   s.n = n;
-  launch_keys_f64(h->g, h->tl, view64(s), n, h->stream);
+  if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, h->stream);
+  else launch_keys_mixed(h->g, h->tl, viewm(s), n, h->stream);
   int rc = sort_species(h, s);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -577,7 +636,7 @@ int cylpic_particle_count(cylpic_ctx* h, int species, int64_t* n_out) {
   if (species < 0 || species >= (int)h->sp.size() || !n_out) return fail(CYLPIC_EINVAL, "bad argument");
   HIPCHK(hipSetDevice(h->device));
   Species& s = h->sp[species];
-  int rc = sort_species(h, s);  // drop removed particles first
+  int rc = s.dirty ? sort_species(h, s) : CYLPIC_OK;  // drop removed particles first
   if (rc) return rc;
   rc = refresh_count(h, s);
   if (rc) return rc;
@@ -592,16 +651,28 @@ int cylpic_download_particles(cylpic_ctx* h, int species, int64_t capacity, int6
   if (species < 0 || species >= (int)h->sp.size()) return fail(CYLPIC_EINVAL, "unknown species");
   HIPCHK(hipSetDevice(h->device));
   Species& s = h->sp[species];
-  int rc = sort_species(h, s);
+  int rc = s.dirty ? sort_species(h, s) : CYLPIC_OK;  // drop removed particles first
   if (rc) return rc;
   rc = refresh_count(h, s);
   if (rc) return rc;
   if (n_out) *n_out = s.n;
   if (s.n > capacity) return fail(CYLPIC_EINVAL, "output capacity smaller than the particle count");
   double* dst[7] = {x, y, z, px, py, pz, w};
-  for (int k = 0; k < 7; ++k)
-    if (dst[k] && s.n > 0)
-      HIPCHK(hipMemcpyAsync(dst[k], s.a[k], s.n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (h->prec == CYLPIC_FP64) {
+    for (int k = 0; k < 7; ++k)
+      if (dst[k] && s.n > 0)
+        HIPCHK(hipMemcpyAsync(dst[k], s.a[k], s.n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  } else if (s.n > 0) {
+    double* tmp = nullptr;
+    HIPCHK(hipMalloc((void**)&tmp, 7 * s.n * sizeof(double)));
+    double* t7[7];
+    for (int k = 0; k < 7; ++k) t7[k] = tmp + k * s.n;
+    launch_from_mixed(h->g, s.m, viewm(s), t7, s.n, h->stream);
+    for (int k = 0; k < 7; ++k)
+      if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], t7[k], s.n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(tmp));
+  }
   if (ids && s.ids && s.n > 0)
     HIPCHK(hipMemcpyAsync(ids, s.id[0], s.n * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -612,7 +683,16 @@ int cylpic_upload_field(cylpic_ctx* h, int which, const double* f) {
   CHECK_H(h);
   if (which < 0 || which >= CYLPIC_NFIELDS || !f) return fail(CYLPIC_EINVAL, "bad field id or pointer");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemcpyAsync(h->f64.f[which], f, nfield(h) * sizeof(cplx<double>), hipMemcpyHostToDevice, h->stream));
+  if (h->prec == CYLPIC_FP64) {
+    HIPCHK(hipMemcpyAsync(h->f64.f[which], f, nfield(h) * sizeof(cplx<double>), hipMemcpyHostToDevice, h->stream));
+  } else {
+    cplx<double>* tmp = nullptr;
+    HIPCHK(hipMalloc((void**)&tmp, nfield(h) * sizeof(cplx<double>)));
+    HIPCHK(hipMemcpyAsync(tmp, f, nfield(h) * sizeof(cplx<double>), hipMemcpyHostToDevice, h->stream));
+    launch_d2f(tmp, h->f32.f[which], nfield(h), h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(tmp));
+  }
   HIPCHK(hipStreamSynchronize(h->stream));
   return CYLPIC_OK;
 }
@@ -621,7 +701,16 @@ int cylpic_download_field(cylpic_ctx* h, int which, double* f) {
   CHECK_H(h);
   if (which < 0 || which >= CYLPIC_NFIELDS || !f) return fail(CYLPIC_EINVAL, "bad field id or pointer");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemcpyAsync(f, h->f64.f[which], nfield(h) * sizeof(cplx<double>), hipMemcpyDeviceToHost, h->stream));
+  if (h->prec == CYLPIC_FP64) {
+    HIPCHK(hipMemcpyAsync(f, h->f64.f[which], nfield(h) * sizeof(cplx<double>), hipMemcpyDeviceToHost, h->stream));
+  } else {
+    cplx<double>* tmp = nullptr;
+    HIPCHK(hipMalloc((void**)&tmp, nfield(h) * sizeof(cplx<double>)));
+    launch_f2d(h->f32.f[which], tmp, nfield(h), h->stream);
+    HIPCHK(hipMemcpyAsync(f, tmp, nfield(h) * sizeof(cplx<double>), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(tmp));
+  }
   HIPCHK(hipStreamSynchronize(h->stream));
   return check_device_errors(h);
 }
@@ -629,7 +718,7 @@ int cylpic_download_field(cylpic_ctx* h, int which, double* f) {
 int cylpic_setup_field_boundaries(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
-  launch_setup_bnd(h->g, h->f64, h->stream);
+  with_fields(h, [&](auto& F) { launch_setup_bnd(h->g, F, h->stream); });
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;
 }
@@ -637,27 +726,27 @@ int cylpic_setup_field_boundaries(cylpic_ctx* h) {
 int cylpic_update_eb_fields_half(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
-  const size_t bytes = nfield(h) * sizeof(cplx<double>);
+  const size_t bytes = nfield(h) * cplx_bytes(h);
   {
     Scope s(h, "update_e");
-    launch_update_e(h->g, h->f64, h->stream);
+    with_fields(h, [&](auto& F) { launch_update_e(h->g, F, h->stream); });
   }
   {
     Scope s(h, "field_bcs");
-    launch_efield_bcs(h->g, h->f64, h->stream);
+    with_fields(h, [&](auto& F) { launch_efield_bcs(h->g, F, h->stream); });
   }
   {
     Scope s(h, "copy_old");
     for (int k = 0; k < 3; ++k)
-      HIPCHK(hipMemcpyAsync(h->f64.f[9 + k], h->f64.f[3 + k], bytes, hipMemcpyDeviceToDevice, h->stream));
+      HIPCHK(hipMemcpyAsync(field_ptr(h, 9 + k), field_ptr(h, 3 + k), bytes, hipMemcpyDeviceToDevice, h->stream));
   }
   {
     Scope s(h, "update_b");
-    launch_update_b(h->g, h->f64, h->stream);
+    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, h->stream); });
   }
   {
     Scope s(h, "field_bcs");
-    launch_bfield_bcs(h->g, h->f64, 1, h->stream);
+    with_fields(h, [&](auto& F) { launch_bfield_bcs(h->g, F, 1, h->stream); });
   }
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;
@@ -667,10 +756,12 @@ int cylpic_push_particles(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
   // jxm_old = jxm; jxm = 0 (particles.F90:163-169): swap roles, clear the new J
-  for (int k = 0; k < 3; ++k) std::swap(h->f64.f[6 + k], h->f64.f[12 + k]);
+  for (int k = 0; k < 3; ++k) swap_fields(h, 6 + k, 12 + k);
   {
     Scope s(h, "zero_j");
-    for (int k = 0; k < 3; ++k) launch_fill_zero(h->f64.f[6 + k], nfield(h), h->stream);
+    with_fields(h, [&](auto& F) {
+      for (int k = 0; k < 3; ++k) launch_fill_zero(F.f[6 + k], nfield(h), h->stream);
+    });
   }
   const double dt = h->g.dt;
   for (auto& s : h->sp) {
@@ -687,16 +778,17 @@ int cylpic_push_particles(cylpic_ctx* h) {
     pc.idt = 1.0 / dt;
     pc.dtco2 = C_LIGHT * (dt / 2.0);
     Scope sc(h, "push_deposit");
-    launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->d_err, h->stream);
+    s.dirty = true;
+    if (h->prec == CYLPIC_FP64) {
+      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->d_err, h->stream);
+    } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, h->f32, h->geof, h->ntab,
+                                         h->d_err, h->stream)) {
+      return fail(CYLPIC_EINVAL, "mixed push: unsupported mode count");
+    }
   }
   {
     Scope s(h, "current_axis");
-    launch_current_axis(h->g, h->f64, h->stream);
-  }
-  {
-    Scope sc(h, "particle_bcs");
-    for (auto& s : h->sp)
-      if (!s.immobile && s.n > 0) launch_particle_bcs_f64(h->g, h->tl, view64(s), s.n, s.d_count, h->stream);
+    with_fields(h, [&](auto& F) { launch_current_axis(h->g, F, h->stream); });
   }
   HIPCHK(hipGetLastError());
   h->push_count += 1;
@@ -724,7 +816,7 @@ int cylpic_current_finish(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
   Scope s(h, "current_finish");
-  launch_current_finish(h->g, h->f64, h->stream);
+  with_fields(h, [&](auto& F) { launch_current_finish(h->g, F, h->stream); });
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;
 }
@@ -746,19 +838,19 @@ int cylpic_update_eb_fields_final(cylpic_ctx* h, const double* src_xmin, const d
   }
   {
     Scope s(h, "update_b");
-    launch_update_b(h->g, h->f64, h->stream);
+    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, h->stream); });
   }
   {
     Scope s(h, "field_bcs");
-    launch_bfield_final_bcs(h->g, h->f64, dsrc[0], dsrc[1], h->stream);
+    with_fields(h, [&](auto& F) { launch_bfield_final_bcs(h->g, F, dsrc[0], dsrc[1], h->stream); });
   }
   {
     Scope s(h, "update_e");
-    launch_update_e(h->g, h->f64, h->stream);
+    with_fields(h, [&](auto& F) { launch_update_e(h->g, F, h->stream); });
   }
   {
     Scope s(h, "field_bcs");
-    launch_efield_bcs(h->g, h->f64, h->stream);
+    with_fields(h, [&](auto& F) { launch_efield_bcs(h->g, F, h->stream); });
   }
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;
@@ -781,7 +873,7 @@ int cylpic_mode_energy(cylpic_ctx* h, double* w_e, double* w_b) {
   HIPCHK(hipSetDevice(h->device));
   const int M = h->g.M;
   HIPCHK(hipMemsetAsync(h->d_energy, 0, 2 * M * sizeof(double), h->stream));
-  launch_energy(h->g, h->f64, h->d_energy, h->stream);
+  with_fields(h, [&](auto& F) { launch_energy(h->g, F, h->d_energy, h->stream); });
   std::vector<double> out(2 * M);
   HIPCHK(hipMemcpyAsync(out.data(), h->d_energy, 2 * M * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));

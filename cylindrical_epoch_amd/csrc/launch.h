@@ -80,15 +80,15 @@ void launch_f2d(const cplx<float>* src, cplx<double>* dst, long n, hipStream_t);
 void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const PartF64&,
                              const int* tile_start, FieldSet<double>&, const Geo&, int* err,
                              hipStream_t);
-void launch_particle_bcs_f64(const Grid&, const Tiles&, const PartF64&, long n, const int* count,
-                             hipStream_t);
 void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, hipStream_t);
-void launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, const PartMixed&,
-                               const int* tile_start, FieldSet<float>&, const Geo&, int* err,
-                               hipStream_t);
-void launch_particle_bcs_mixed(const Grid&, const Tiles&, const PartMixed&, long n, const int* count,
-                               hipStream_t);
+// push_mixed.hip
+int launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, double charge, const PartMixed&,
+                              const int* tile_start, FieldSet<float>&, const float* geo, int ntab, int* err,
+                              hipStream_t);
 void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, hipStream_t);
+void launch_to_mixed(const Grid&, double mass, const double* const* src, const PartMixed&, long n, hipStream_t);
+void launch_from_mixed(const Grid&, double mass, const PartMixed&, double* const* dst, long n, hipStream_t);
+size_t push_lds_bytes_mixed(const Grid&, const Tiles&);
 size_t push_lds_bytes(const Grid&, const Tiles&, int precision);
 
 // sort.hip
@@ -99,16 +99,18 @@ struct SortScratch {
   long cap;
 };
 size_t sort_scratch_words(long n);
-// Stable sort of int keys in [0, nkeys] (n elements); on return perm[i] is
-// the source index of sorted position i and keys_out the sorted keys.
-void radix_sort(const int* keys, long n, int nkeys, SortScratch& s, uint32_t** perm_out,
-                uint32_t** keys_out, hipStream_t);
-void launch_tile_bounds(const uint32_t* sorted_keys, long n, int ntiles, int* tile_start, hipStream_t);
+// Stable sort of the composite keys (tile << slot_bits) | (i mod 2^slot_bits)
+// of n particles with tile keys in [0, ntiles] (ntiles = removed).  The slot
+// spreads the particles of one cell over the tile's processing order.  On
+// return perm[i] is the source index of sorted position i and tiles_out[i]
+// its tile key.
+void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, SortScratch& s,
+                uint32_t* tiles_out, uint32_t** perm_out, hipStream_t);
+void launch_tile_bounds(const uint32_t* sorted_tiles, long n, int ntiles, int* tile_start, hipStream_t);
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst,
                        int narr, hipStream_t);
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst,
                       int narr, hipStream_t);
 void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, hipStream_t);
-void launch_copy_keys(const uint32_t* src, int* dst, long n, hipStream_t);
 
 }  // namespace cylpic

@@ -41,6 +41,12 @@ __device__ __forceinline__ cplx<T> patch(const Grid& g, const cplx<S>* __restric
   return acc;
 }
 
+// a[k + 2] for a run-time k in -2..2, without indexing a register array
+template <class T>
+__device__ __forceinline__ T sel5(const T* a, int k) {
+  return k == -2 ? a[0] : (k == -1 ? a[1] : (k == 0 ? a[2] : (k == 1 ? a[3] : a[4])));
+}
+
 struct LdsBox {
   int x0, r0, LX, LR, M;
 };
@@ -57,6 +63,85 @@ __device__ __forceinline__ void dep_add_f64(double* lds, const LdsBox& b, int co
     unsafeAtomicAdd(p, v.re);
     if (m) unsafeAtomicAdd(p + 1, v.im);
   }
+}
+
+
+// ------------------------------------------------------------------------
+// particle_bcs (one rank, boundary.F90:1541-1889) for particle ip at its new
+// position (x, y, z), already stored; returns the particle's tile key, or
+// ntiles if it left through an open boundary.
+// The tile key is the particle's cell (src/housekeeping/split_particle.F90:
+// 61-62, clamped into 1..nx, 1..ny) mapped to its tile.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int tile_of_cell(const Grid& g, const Tiles& tl, int cx, int cy) {
+  cx = cx < 1 ? 1 : (cx > g.nx ? g.nx : cx);
+  cy = cy < 1 ? 1 : (cy > g.ny ? g.ny : cy);
+  return ((cy - 1) / tl.tr) * tl.ntx + (cx - 1) / tl.tx;
+}
+
+__device__ __forceinline__ int key_f64(const Grid& g, const Tiles& tl, double x, double y, double z) {
+  const double r = sqrt(y * y + z * z);
+  const double fx = floor((x - g.x_grid_min) / g.dx + 1.5);
+  const double fy = floor((r - g.y_grid_min) / g.dy + 1.5);
+  const int cx = fx < -1.0e9 ? -1000000000 : (fx > 1.0e9 ? 1000000000 : (int)fx);
+  const int cy = fy < -1.0e9 ? -1000000000 : (fy > 1.0e9 ? 1000000000 : (int)fy);
+  return tile_of_cell(g, tl, cx, cy);
+}
+
+__device__ __forceinline__ int bcs_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long ip,
+                                       double x, double y, double z) {
+  const int png = 3;
+  const double bshift_x = g.dx * (double)((1 + png + 0) / 2);
+  const double x_min_outer = g.x_min - bshift_x, x_max_outer = g.x_max + bshift_x;
+  const double x_shift = g.x_max - g.x_min;
+  const double y_max_outer = g.y_max + g.dy * (double)((1 + png + 0) / 2);
+  bool out_of_bounds = false;
+  int bc = 0;
+  const double part_pos = x;
+  if (part_pos < g.x_min) {
+    bc = g.bc_particle[XMIN];
+    if (bc == BC_REFLECT) {
+      x = 2.0 * g.x_min - part_pos;
+      sp.x[ip] = x;
+      sp.px[ip] = -sp.px[ip];
+    } else if (bc == BC_PERIODIC) {
+      x = part_pos - (-1.0) * x_shift;
+      sp.x[ip] = x;
+    }
+    if (part_pos < x_min_outer && bc != BC_PERIODIC) out_of_bounds = true;
+  }
+  if (part_pos >= g.x_max) {
+    bc = g.bc_particle[XMAX];
+    if (bc == BC_REFLECT) {
+      x = 2.0 * g.x_max - part_pos;
+      sp.x[ip] = x;
+      sp.px[ip] = -sp.px[ip];
+    } else if (bc == BC_PERIODIC) {
+      x = part_pos - 1.0 * x_shift;
+      sp.x[ip] = x;
+    }
+    if (part_pos >= x_max_outer && bc != BC_PERIODIC) out_of_bounds = true;
+  }
+  const double r = sqrt(y * y + z * z);
+  if (r >= g.y_max) {
+    bc = g.bc_particle[YMAX];
+    if (bc == BC_REFLECT) {
+      const double radial_reduction = 2.0 * g.y_max / r - 1.0;
+      y = y * radial_reduction;
+      z = z * radial_reduction;
+      sp.y[ip] = y;
+      sp.z[ip] = z;
+      const double inv_final_r = 1.0 / sqrt(y * y + z * z);
+      const double cos_theta = y * inv_final_r, sin_theta = z * inv_final_r;
+      const double py = sp.py[ip], pz = sp.pz[ip];
+      const double part_pr = py * cos_theta + pz * sin_theta;
+      const double part_pt = -(py * sin_theta) + pz * cos_theta;
+      sp.py[ip] = -(part_pr * cos_theta) - part_pt * sin_theta;
+      sp.pz[ip] = -(part_pr * sin_theta) + part_pt * cos_theta;
+    }
+    if (r >= y_max_outer && bc != BC_PERIODIC) out_of_bounds = true;
+  }
+  return out_of_bounds ? tl.ntiles : key_f64(g, tl, x, y, z);
 }
 
 // ------------------------------------------------------------------------
@@ -237,15 +322,22 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
       atomicOr(err, ERR_DCELL);
       continue;
     }
-    for (int k = 0; k < 5; ++k) {
-      hx[k] = 0.0;
-      hy[k] = 0.0;
-    }
-    weights3(frac_x3, hx[1 + dcellx], hx[2 + dcellx], hx[3 + dcellx]);
-    weights3(frac_y3, hy[1 + dcelly], hy[2 + dcelly], hy[3 + dcelly]);
-    for (int k = 0; k < 5; ++k) {
-      hx[k] = hx[k] - gx[k];
-      hy[k] = hy[k] - gy[k];
+    {
+      double wm, w0, wp;
+      weights3(frac_x3, wm, w0, wp);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int o = k - 2 - dcellx;  // offset of slot k from the new centre
+        const double h = o == -1 ? wm : (o == 0 ? w0 : (o == 1 ? wp : 0.0));
+        hx[k] = h - gx[k];
+      }
+      weights3(frac_y3, wm, w0, wp);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int o = k - 2 - dcelly;
+        const double h = o == -1 ? wm : (o == 0 ? w0 : (o == 1 ? wp : 0.0));
+        hy[k] = h - gy[k];
+      }
     }
 
     const double q_weight_fac = (pc.part_q * part_weight) * fac;
@@ -284,11 +376,14 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
           m_fac_4 = cmul(m_fac_1, imul(inv_m2dth2, u));
         }
       }
+      // DO iy / DO ix of particles.F90:628-664, unrolled over the widest
+      // range -2..2 so the running sums stay in registers
       cplx<double> jyh[5];
+#pragma unroll
       for (int k = 0; k < 5; ++k) jyh[k] = mk(0.0, 0.0);
       for (int iy = ymin; iy <= ymax; ++iy) {
         const int cy = cell_y2 + iy;
-        const double gyv = gy[iy + 2], hyv = hy[iy + 2];
+        const double gyv = sel5(gy, iy), hyv = sel5(hy, iy);
         cplx<double> w_rt, ym_fac_1;
         if (im == 0) {
           w_rt = mk(gyv + 0.5 * hyv, 0.0);
@@ -302,7 +397,9 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
         const double fjz = fcz * geo.inv_volume[cy + jng];
         const double rat = geo.ratio_area_xt[cy + jng];
         cplx<double> jxh = mk(0.0, 0.0);
-        for (int ix = xmin; ix <= xmax; ++ix) {
+#pragma unroll
+        for (int ix = -2; ix <= 2; ++ix) {
+          if (ix < xmin || ix > xmax) continue;
           const int cx = cell_x2 + ix;
           const double gxv = gx[ix + 2], hxv = hx[ix + 2];
           cplx<double> w_xt;
@@ -318,6 +415,10 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
         }
       }
     }
+
+    // particle_bcs (boundary.F90:1541-1889) on the new position; it does not
+    // read the currents, so it runs here rather than after the axis fold
+    sp.key[ip] = bcs_f64(g, tl, sp, ip, part_x, part_y, part_z);
   }
   __syncthreads();
   // one pass of global atomics per tile
@@ -336,81 +437,6 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
     if (re != 0.0) unsafeAtomicAdd(p, re);
     if (imv != 0.0) unsafeAtomicAdd(p + 1, imv);
   }
-}
-
-// ------------------------------------------------------------------------
-// particle_bcs (one rank) and the tile key of the new position.
-// key = tile of the cell (src/housekeeping/split_particle.F90:61-62 with the
-// cell clamped into 1..nx, 1..ny), or ntiles once removed.
-// ------------------------------------------------------------------------
-__device__ __forceinline__ int tile_of_cell(const Grid& g, const Tiles& tl, int cx, int cy) {
-  cx = cx < 1 ? 1 : (cx > g.nx ? g.nx : cx);
-  cy = cy < 1 ? 1 : (cy > g.ny ? g.ny : cy);
-  return ((cy - 1) / tl.tr) * tl.ntx + (cx - 1) / tl.tx;
-}
-
-__device__ __forceinline__ int key_f64(const Grid& g, const Tiles& tl, double x, double y, double z) {
-  const double r = sqrt(y * y + z * z);
-  const double fx = floor((x - g.x_grid_min) / g.dx + 1.5);
-  const double fy = floor((r - g.y_grid_min) / g.dy + 1.5);
-  const int cx = fx < -1.0e9 ? -1000000000 : (fx > 1.0e9 ? 1000000000 : (int)fx);
-  const int cy = fy < -1.0e9 ? -1000000000 : (fy > 1.0e9 ? 1000000000 : (int)fy);
-  return tile_of_cell(g, tl, cx, cy);
-}
-
-__global__ void k_particle_bcs_f64(Grid g, Tiles tl, PartF64 sp, long n, const int* __restrict__ count) {
-  const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= n || ip >= (long)*count) return;
-  if (sp.key[ip] >= tl.ntiles) return;
-  const int png = 3;
-  const double bshift_x = g.dx * (double)((1 + png + 0) / 2);
-  const double x_min_outer = g.x_min - bshift_x, x_max_outer = g.x_max + bshift_x;
-  const double x_shift = g.x_max - g.x_min;
-  const double y_max_outer = g.y_max + g.dy * (double)((1 + png + 0) / 2);
-  bool out_of_bounds = false;
-  int bc = 0;
-  double part_pos = sp.x[ip];
-  if (part_pos < g.x_min) {
-    bc = g.bc_particle[XMIN];
-    if (bc == BC_REFLECT) {
-      sp.x[ip] = 2.0 * g.x_min - part_pos;
-      sp.px[ip] = -sp.px[ip];
-    } else if (bc == BC_PERIODIC) {
-      sp.x[ip] = part_pos - (-1.0) * x_shift;
-    }
-    if (part_pos < x_min_outer && bc != BC_PERIODIC) out_of_bounds = true;
-  }
-  if (part_pos >= g.x_max) {
-    bc = g.bc_particle[XMAX];
-    if (bc == BC_REFLECT) {
-      sp.x[ip] = 2.0 * g.x_max - part_pos;
-      sp.px[ip] = -sp.px[ip];
-    } else if (bc == BC_PERIODIC) {
-      sp.x[ip] = part_pos - 1.0 * x_shift;
-    }
-    if (part_pos >= x_max_outer && bc != BC_PERIODIC) out_of_bounds = true;
-  }
-  double y = sp.y[ip], z = sp.z[ip];
-  part_pos = sqrt(y * y + z * z);
-  if (part_pos >= g.y_max) {
-    bc = g.bc_particle[YMAX];
-    if (bc == BC_REFLECT) {
-      const double radial_reduction = 2.0 * g.y_max / part_pos - 1.0;
-      y = y * radial_reduction;
-      z = z * radial_reduction;
-      sp.y[ip] = y;
-      sp.z[ip] = z;
-      const double inv_final_r = 1.0 / sqrt(y * y + z * z);
-      const double cos_theta = y * inv_final_r, sin_theta = z * inv_final_r;
-      const double py = sp.py[ip], pz = sp.pz[ip];
-      const double part_pr = py * cos_theta + pz * sin_theta;
-      const double part_pt = -(py * sin_theta) + pz * cos_theta;
-      sp.py[ip] = -(part_pr * cos_theta) - part_pt * sin_theta;
-      sp.pz[ip] = -(part_pr * sin_theta) + part_pt * cos_theta;
-    }
-    if (part_pos >= y_max_outer && bc != BC_PERIODIC) out_of_bounds = true;
-  }
-  sp.key[ip] = out_of_bounds ? tl.ntiles : key_f64(g, tl, sp.x[ip], y, z);
 }
 
 __global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n) {
@@ -434,13 +460,6 @@ void launch_push_deposit_f64(const Grid& g, const Tiles& tl, const PushConst& pc
   size_t lds = push_lds_bytes(g, tl, 0);
   hipLaunchKernelGGL(k_push_deposit_f64, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
                      F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err);
-}
-
-void launch_particle_bcs_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, const int* count,
-                             hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_particle_bcs_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, sp, n,
-                     count);
 }
 
 void launch_keys_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, hipStream_t s) {

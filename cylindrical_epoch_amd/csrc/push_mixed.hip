@@ -1,0 +1,607 @@
+// Mixed-precision push + deposit on gfx950: the physics of push_particles
+// (src/particles.F90:268-665) with
+//   - particle x as an int32 cell (1..nx, nearest cell) plus an FP32 offset
+//     from the cell centre in cells, y and z in FP32 units of dy, momenta as
+//     FP32 u = p / (m c), FP32 weight;
+//   - FP32 fields staged per tile in LDS (6 components x modes, tile plus a
+//     2-cell border: the reach of the 3x3 gather stencils);
+//   - the current deposited into an FP64 LDS box (tile plus a 3-cell
+//     border), i.e. the per-tile sums carry no FP32 rounding, flushed once
+//     per tile to the FP32 global J with global atomics;
+//   - the deposit written as mode-independent real bases times per-mode
+//     complex factors (the same algebra as the reference's loops, fewer
+//     operations; the operation order differs, so results agree with the
+//     FP64 path to a tolerance, not bit for bit);
+//   - the m > 0 factors (particles.F90:589-626) evaluated by their power
+//     series to x^8 for |m dtheta| < 0.7 (the exact expressions cancel
+//     catastrophically in FP32 below that), by the exact expressions above;
+//   - dtheta = arg(exp(i theta_15) exp(-i theta_05)) plus the reference's
+//     +-2 pi jump when the two angles straddle the theta = pi cut (the
+//     reference subtracts two ATAN2 values without unwrapping,
+//     particles.F90:321-322, 522-524).
+// A particle whose stencils leave the tile's LDS regions (it drifted while
+// the sort ran less often than every step) gathers from and deposits to
+// global memory directly.
+#include "cplx_ops.h"
+#include "launch.h"
+
+namespace cylpic {
+
+constexpr int FIELD_HALO = 2;
+
+struct MixConst {
+  float kx, ky;           // c dt / (2 dx), c dt / (2 dy): half-step move in cells per unit u/gamma
+  float cmratio, ccmratio;
+  float q_fac_idt;        // q * fac / dt
+  float q_fac_c;          // q * fac * c
+  float yoff;             // y_grid_min / dy
+  float nyf;              // y_max / dy
+  int png_shift;          // (1 + png) / 2 outer-boundary margin in cells
+};
+
+struct GeoF {
+  const float* rt;
+  const float* xt;
+  const float* vol;
+  const float* rat;
+};
+
+__device__ __forceinline__ float fsel5(const float* a, int k) {
+  return k == -2 ? a[0] : (k == -1 ? a[1] : (k == 0 ? a[2] : (k == 1 ? a[3] : a[4])));
+}
+
+__device__ __forceinline__ void w3f(float frac, float& wm, float& w0, float& wp) {
+  const float cf2 = frac * frac;
+  wm = (0.25f + cf2) + frac;
+  w0 = 1.5f - 2.0f * cf2;
+  wp = (0.25f + cf2) - frac;
+}
+
+// f2, f3, f4 of the m-factors: m_fac_k = 2 exp(i m theta0) f_k(m dtheta).
+// f2 = (e^{ix}-1)/(ix), f3 = (e^{ix}(1-ix)-1)/x^2, f4 = i(e^{ix}(2-x^2-2ix)-2)/x^3
+__device__ __forceinline__ void mfactors(float x, cplx<float>& f2, cplx<float>& f3, cplx<float>& f4) {
+  const float ax = fabsf(x);
+  if (ax < 0.7f) {
+    const float s = x * x;
+    f2 = mk(1.0f + s * (-1.6666667e-1f + s * (8.3333333e-3f + s * (-1.9841270e-4f + s * 2.7557319e-6f))),
+            x * (0.5f + s * (-4.1666667e-2f + s * (1.3888889e-3f + s * -2.4801587e-5f))));
+    f3 = mk(0.5f + s * (-0.125f + s * (6.9444444e-3f + s * (-1.7361111e-4f + s * 2.4801587e-6f))),
+            x * (3.3333333e-1f + s * (-3.3333333e-2f + s * (1.1904762e-3f + s * -2.2045855e-5f))));
+    f4 = mk(3.3333333e-1f + s * (-0.1f + s * (5.9523810e-3f + s * (-1.5432099e-4f + s * 2.2546898e-6f))),
+            x * (0.25f + s * (-2.7777778e-2f + s * (1.0416667e-3f + s * -1.9841270e-5f))));
+  } else {
+    float sn, cs;
+    __sincosf(x, &sn, &cs);
+    const float ix = 1.0f / x, ix2 = ix * ix;
+    // e^{ix} - 1 = (cs - 1, sn) ; divide by i x
+    f2 = mk(sn * ix, -(cs - 1.0f) * ix);
+    // e^{ix}(1 - ix) - 1
+    const cplx<float> t3 = mk(cs + sn * x - 1.0f, sn - cs * x);
+    f3 = mk(t3.re * ix2, t3.im * ix2);
+    // i (e^{ix}(2 - x^2 - 2ix) - 2) / x^3
+    const float a = 2.0f - x * x, b = -2.0f * x;
+    const cplx<float> u = mk(cs * a - sn * b - 2.0f, cs * b + sn * a);
+    const float ix3 = ix2 * ix;
+    f4 = mk(-u.im * ix3, u.re * ix3);
+  }
+}
+
+template <int M>
+struct Boxes {
+  int fx0, fr0, FX, FR;  // field region origin (Fortran cell) and size
+  int jx0, jr0, LX, LR;  // current box origin and size
+};
+
+template <int M>
+__global__ __launch_bounds__(256) void k_push_deposit_mixed(
+    Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start,
+    const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
+    const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
+    cplx<float>* __restrict__ jx, cplx<float>* __restrict__ jr, cplx<float>* __restrict__ jt, GeoF geo,
+    int* __restrict__ err) {
+  extern __shared__ double lds_raw[];
+  const int tile = blockIdx.x;
+  const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
+  Boxes<M> b;
+  b.fx0 = ttx * tl.tx + 1 - FIELD_HALO;
+  b.fr0 = tty * tl.tr + 1 - FIELD_HALO;
+  b.FX = tl.tx + 2 * FIELD_HALO;
+  b.FR = tl.tr + 2 * FIELD_HALO;
+  b.jx0 = ttx * tl.tx + 1 - DEPOSIT_HALO;
+  b.jr0 = tty * tl.tr + 1 - DEPOSIT_HALO;
+  b.LX = tl.tx + 2 * DEPOSIT_HALO;
+  b.LR = tl.tr + 2 * DEPOSIT_HALO;
+  const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
+  const int nf = 6 * M * b.FR * b.FX;      // complex cells of the field region
+  double* jl = lds_raw;                    // [3][M][LR][LX] (re, im)
+  cplx<float>* fl = (cplx<float>*)(lds_raw + 2 * nj);  // [6][M][FR][FX]
+
+  for (int e = threadIdx.x; e < 2 * nj; e += blockDim.x) jl[e] = 0.0;
+  {
+    const cplx<float>* src[6] = {ex, er, et, bx, br, bt};
+    for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+      int rem = e;
+      const int fx = rem % b.FX;
+      rem /= b.FX;
+      const int fr = rem % b.FR;
+      rem /= b.FR;
+      const int m = rem % M;
+      const int c = rem / M;
+      const int cx = b.fx0 + fx, cy = b.fr0 + fr;
+      cplx<float> v = mk(0.0f, 0.0f);
+      if (cx >= 1 - g.ng && cx <= g.nx + g.ng && cy >= 1 - g.ng && cy <= g.ny + g.ng) v = src[c][g.idx(cx, cy, m)];
+      fl[e] = v;
+    }
+  }
+  __syncthreads();
+
+  const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
+  const int jng = g.ng;
+  const long fplane = (long)b.FR * b.FX;
+
+  for (int ip = p0 + (int)threadIdx.x; ip < p1; ip += blockDim.x) {
+    if (sp.key[ip] >= tl.ntiles) continue;
+    int cell = sp.cell[ip];
+    const float ox = sp.ox[ip], y = sp.y[ip], z = sp.z[ip];
+    float ux = sp.ux[ip], uy = sp.uy[ip], uz = sp.uz[ip];
+    const float wgt = sp.w[ip];
+
+    // half push to t + dt/2 (particles.F90:296-312), in cell units
+    float ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
+    const float oxh = ox + ux * (mc.kx * ig);
+    const float yh = y + uy * (mc.ky * ig);
+    const float zh = z + uz * (mc.ky * ig);
+    const float rh2 = yh * yh + zh * zh;
+    const float irh = __builtin_amdgcn_rsqf(rh2);
+    const float rh = rh2 * irh;
+    const cplx<float> emi = mk(yh * irh, -(zh * irh));  // exp(-i theta_05)
+
+    // weights at t + dt/2 (gx.inc, hx_dcell.inc) relative to the cell
+    const float fl1x = floorf(oxh + 0.5f), fl2x = floorf(oxh);
+    const int cx1 = cell + (int)fl1x, cx2 = cell + (int)fl2x;
+    const float cyr = rh - mc.yoff;
+    const float fl1y = floorf(cyr + 0.5f), fl2y = floorf(cyr);
+    const int cy1 = (int)fl1y + 1, cy2 = (int)fl2y + 1;
+    float gx[5], gy[5], hx[5], hy[5];
+    gx[0] = gx[4] = gy[0] = gy[4] = hx[0] = hx[4] = hy[0] = hy[4] = 0.0f;
+    w3f(fl1x - oxh, gx[1], gx[2], gx[3]);
+    w3f(fl1y - cyr, gy[1], gy[2], gy[3]);
+    w3f((fl2x - oxh) + 0.5f, hx[1], hx[2], hx[3]);
+    w3f((fl2y - cyr) + 0.5f, hy[1], hy[2], hy[3]);
+
+    if (cx1 < 2 - jng || cx1 > g.nx + jng - 1 || cy1 < 2 - jng || cy1 > g.ny + jng - 1 || cx2 < -2 ||
+        cx2 > g.nx + 2 || cy2 < -2 || cy2 > g.ny + 2) {
+      atomicOr(err, ERR_CELL_RANGE);
+      continue;
+    }
+
+    // gather (e_part.inc, b_part.inc): E and B as sum_m Re[e^{-im theta} S_m]
+    float ef[6];
+    {
+      // 2D weights of the four stencils
+      float Wgh[9], Whg[9], Whh[9], Wgg[9];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          Wgh[3 * j + i] = gx[1 + i] * hy[1 + j];
+          Whg[3 * j + i] = hx[1 + i] * gy[1 + j];
+          Whh[3 * j + i] = hx[1 + i] * hy[1 + j];
+          Wgg[3 * j + i] = gx[1 + i] * gy[1 + j];
+        }
+      const int lo_x = min(cx1, cx2) - 1, hi_x = max(cx1, cx2) + 1;
+      const int lo_y = min(cy1, cy2) - 1, hi_y = max(cy1, cy2) + 1;
+      const bool in_lds = lo_x >= b.fx0 && hi_x < b.fx0 + b.FX && lo_y >= b.fr0 && hi_y < b.fr0 + b.FR;
+      // stencil centre per component: ex (x1,y2,gh) er (x2,y1,hg) et (x2,y2,hh)
+      //                               bx (x2,y1,hg) br (x1,y2,gh) bt (x1,y1,gg)
+      const int ccx[6] = {cx1, cx2, cx2, cx2, cx1, cx1};
+      const int ccy[6] = {cy2, cy1, cy2, cy1, cy2, cy1};
+      const float* Ws[6] = {Wgh, Whg, Whh, Whg, Wgh, Wgg};
+      const cplx<float>* gsrc[6] = {ex, er, et, bx, br, bt};
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const float* W = Ws[c];
+        float acc = 0.0f;
+        cplx<float> e_m = mk(1.0f, 0.0f);
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          cplx<float> S = mk(0.0f, 0.0f);
+          if (in_lds) {
+            const cplx<float>* base =
+                fl + ((long)(c * M + m) * fplane + (long)(ccy[c] - 1 - b.fr0) * b.FX + (ccx[c] - 1 - b.fx0));
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                const cplx<float> f = base[j * b.FX + i];
+                S.re = fmaf(W[3 * j + i], f.re, S.re);
+                S.im = fmaf(W[3 * j + i], f.im, S.im);
+              }
+          } else {
+            const cplx<float>* base = gsrc[c] + g.idx(ccx[c] - 1, ccy[c] - 1, m);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                const cplx<float> f = base[j * g.nx2 + i];
+                S.re = fmaf(W[3 * j + i], f.re, S.re);
+                S.im = fmaf(W[3 * j + i], f.im, S.im);
+              }
+          }
+          acc = fmaf(e_m.re, S.re, fmaf(-e_m.im, S.im, acc));
+          if (m + 1 < M) e_m = cmul(e_m, emi);
+        }
+        ef[c] = acc;
+      }
+    }
+    const float ex_p = ef[0];
+    const float ey_p = ef[1] * emi.re + ef[2] * emi.im;
+    const float ez_p = -(ef[1] * emi.im) + ef[2] * emi.re;
+    const float bx_p = ef[3];
+    const float by_p = ef[4] * emi.re + ef[5] * emi.im;
+    const float bz_p = -(ef[4] * emi.im) + ef[5] * emi.re;
+
+    // Boris rotation (particles.F90:404-451)
+    const float uxm = fmaf(mc.cmratio, ex_p, ux), uym = fmaf(mc.cmratio, ey_p, uy), uzm = fmaf(mc.cmratio, ez_p, uz);
+    const float igm = __builtin_amdgcn_rsqf(1.0f + (uxm * uxm + uym * uym + uzm * uzm));
+    const float root = mc.ccmratio * igm;
+    const float tx = bx_p * root, ty = by_p * root, tz = bz_p * root;
+    const float tx2 = tx * tx, ty2 = ty * ty, tz2 = tz * tz;
+    const float tau = __builtin_amdgcn_rcpf(1.0f + tx2 + ty2 + tz2);
+    const float uxp = ((1.0f + tx2 - ty2 - tz2) * uxm + 2.0f * ((tx * ty + tz) * uym + (tx * tz - ty) * uzm)) * tau;
+    const float uyp = ((1.0f - tx2 + ty2 - tz2) * uym + 2.0f * ((ty * tz + tx) * uzm + (ty * tx - tz) * uxm)) * tau;
+    const float uzp = ((1.0f - tx2 - ty2 + tz2) * uzm + 2.0f * ((tz * tx + ty) * uxm + (tz * ty - tx) * uym)) * tau;
+    ux = fmaf(mc.cmratio, ex_p, uxp);
+    uy = fmaf(mc.cmratio, ey_p, uyp);
+    uz = fmaf(mc.cmratio, ez_p, uzp);
+    ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
+    const float dxn = ux * (mc.kx * ig), dyn = uy * (mc.ky * ig), dzn = uz * (mc.ky * ig);
+    float oxn = oxh + dxn;
+    float yn = yh + dyn, zn = zh + dzn;
+
+    // v_theta at t+dt (particles.F90:474-479): (-uy sin + uz cos) c / gamma
+    const float irn = __builtin_amdgcn_rsqf(yn * yn + zn * zn);
+    const float ut = (-(uy * zn) + uz * yn) * irn;
+    // position at t + 3dt/2, relative to the cell
+    const float ox15 = oxn + dxn;
+    const float y15 = yn + dyn, z15 = zn + dzn;
+    const float r152 = y15 * y15 + z15 * z15;
+    const float ir15 = __builtin_amdgcn_rsqf(r152);
+    const float r15 = r152 * ir15;
+    const float c15 = y15 * ir15, s15 = z15 * ir15;
+    const float c05 = emi.re, s05 = -emi.im;
+    float dth = atan2f(s15 * c05 - c15 * s05, c15 * c05 + s15 * s05);
+    {
+      // theta_15 - theta_05 of two ATAN2 values in (-pi, pi]: the short arc
+      // plus -+2 pi when it crosses the cut at theta = pi
+      const bool h05 = !signbit(s05), h15 = !signbit(s15);  // theta >= 0
+      const bool lt = (h15 != h05) ? !h15 : (c15 * s05 - s15 * c05 > 0.0f);
+      const bool gt = (h15 != h05) ? h15 : (c05 * s15 - s05 * c15 > 0.0f);
+      if (dth > 0.0f && lt) dth -= 6.2831853f;
+      else if (dth < 0.0f && gt) dth += 6.2831853f;
+    }
+
+    // weight changes t+dt/2 -> t+3dt/2 (hx_dcell.inc)
+    const float fl3x = floorf(ox15);
+    const int dcx = (cell + (int)fl3x) - cx2;
+    const float cyr15 = r15 - mc.yoff;
+    const float fl3y = floorf(cyr15);
+    const int dcy = ((int)fl3y + 1) - cy2;
+    if (dcx < -1 || dcx > 1 || dcy < -1 || dcy > 1) {
+      atomicOr(err, ERR_DCELL);
+      continue;
+    }
+    {
+      float wm, w0, wp;
+      w3f((fl3x - ox15) + 0.5f, wm, w0, wp);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int o = k - 2 - dcx;
+        const float h = o == -1 ? wm : (o == 0 ? w0 : (o == 1 ? wp : 0.0f));
+        const float gold = hx[k];
+        gx[k] = gold;
+        hx[k] = h - gold;
+      }
+      w3f((fl3y - cyr15) + 0.5f, wm, w0, wp);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int o = k - 2 - dcy;
+        const float h = o == -1 ? wm : (o == 0 ? w0 : (o == 1 ? wp : 0.0f));
+        const float gold = hy[k];
+        gy[k] = gold;
+        hy[k] = h - gold;
+      }
+    }
+    const int xmin = -1 + (dcx - 1) / 2, xmax = 1 + (dcx + 1) / 2;
+    const int ymin = -1 + (dcy - 1) / 2, ymax = 1 + (dcy + 1) / 2;
+
+    // per-mode factors m2, m3, m4 (m = 0: 1, 1/2, 1/3)
+    cplx<float> m2[M], m3[M], m4[M];
+    {
+      m2[0] = mk(1.0f, 0.0f);
+      m3[0] = mk(0.5f, 0.0f);
+      m4[0] = mk(0.33333333f, 0.0f);
+      const cplx<float> e05 = mk(c05, s05);
+      cplx<float> e_m = mk(1.0f, 0.0f);
+#pragma unroll
+      for (int m = 1; m < M; ++m) {
+        e_m = cmul(e_m, e05);
+        cplx<float> f2, f3, f4;
+        mfactors((float)m * dth, f2, f3, f4);
+        const cplx<float> two_e = mk(2.0f * e_m.re, 2.0f * e_m.im);
+        m2[m] = cmul(two_e, f2);
+        m3[m] = cmul(two_e, f3);
+        m4[m] = cmul(two_e, f4);
+      }
+    }
+
+    const float fcx = mc.q_fac_idt * wgt;
+    const float fcz = mc.q_fac_c * wgt * ut * ig;
+    // cumulative x sums of hx (jxh of the reference) are mode independent
+    float Hx[5];
+    {
+      float run = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const bool on = (k - 2) >= xmin && (k - 2) <= xmax;
+        run += on ? hx[k] : 0.0f;
+        Hx[k] = run;
+      }
+    }
+    // is the whole footprint inside the LDS current box?
+    const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
+    double* jbase = jl + 2 * ((long)(cy2 - b.jr0) * b.LX + (cx2 - b.jx0));
+    const long jcomp = (long)M * b.LR * b.LX * 2;  // doubles per component
+    const long jmode = (long)b.LR * b.LX * 2;
+    float R = 0.0f;
+    for (int iy = ymin; iy <= ymax; ++iy) {
+      const int cy = cy2 + iy;
+      const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
+      const float fjx = fcx * geo.rt[cy + jng];
+      R = R * geo.rat[cy + jng] + (fcx * hyv) * geo.xt[cy + jng];
+      const float fjz = fcz * geo.vol[cy + jng];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int ix = k - 2;
+        if (ix < xmin || ix > xmax) continue;
+        const float gxv = gx[k], hxv = hx[k];
+        const float A = fjz * (gxv * gyv), B = fjz * (gxv * hyv + hxv * gyv), C = fjz * (hxv * hyv);
+        const float PX = -fjx * Hx[k];
+        const int cx = cx2 + ix;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          // Jx(cx+1, cy) += -fjx w_rt Hx ; Jr(cx, cy+1) += -w_xt R ; Jt(cx, cy) += fjz w_xr
+          const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
+          const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
+          const cplx<float> vx = mk(PX * w_rt.re, PX * w_rt.im);
+          const cplx<float> vr = mk(-R * w_xt.re, -R * w_xt.im);
+          const cplx<float> vt = mk(m2[m].re * A + m3[m].re * B + m4[m].re * C,
+                                    m2[m].im * A + m3[m].im * B + m4[m].im * C);
+          if (jin) {
+            double* p = jbase + (long)m * jmode + (long)iy * b.LX * 2 + ix * 2;
+            atomicAdd(p + 2, (double)vx.re);                                  // (cx+1, cy)
+            atomicAdd(p + jcomp + (long)b.LX * 2, (double)vr.re);             // (cx, cy+1)
+            atomicAdd(p + 2 * jcomp, (double)vt.re);                          // (cx, cy)
+            if (m) {
+              atomicAdd(p + 3, (double)vx.im);
+              atomicAdd(p + jcomp + (long)b.LX * 2 + 1, (double)vr.im);
+              atomicAdd(p + 2 * jcomp + 1, (double)vt.im);
+            }
+          } else {
+            float* qx = (float*)(jx + g.idx(cx + 1, cy, m));
+            float* qr = (float*)(jr + g.idx(cx, cy + 1, m));
+            float* qt = (float*)(jt + g.idx(cx, cy, m));
+            unsafeAtomicAdd(qx, vx.re);
+            unsafeAtomicAdd(qr, vr.re);
+            unsafeAtomicAdd(qt, vt.re);
+            if (m) {
+              unsafeAtomicAdd(qx + 1, vx.im);
+              unsafeAtomicAdd(qr + 1, vr.im);
+              unsafeAtomicAdd(qt + 1, vt.im);
+            }
+          }
+        }
+      }
+    }
+
+    // new position relative to its nearest cell; particle_bcs (boundary.F90:
+    // 1541-1889) in cell units: periodic wraps are exact integer shifts
+    const float sh = floorf(oxn + 0.5f);
+    cell += (int)sh;
+    oxn -= sh;
+    bool out = false;
+    if (cell < 1) {
+      const int bc = g.bc_particle[XMIN];
+      const float xr = (float)(cell - 1) + oxn;  // (x - x_grid_min) / dx
+      if (bc == BC_PERIODIC) {
+        cell += g.nx;
+      } else {
+        if (bc == BC_REFLECT) {
+          ux = -ux;
+          const float xn = -1.0f - xr;            // 2 x_min - x
+          const float s2 = floorf(xn + 0.5f);
+          cell = (int)s2 + 1;
+          oxn = xn - s2;
+        }
+        if (xr < -0.5f - (float)mc.png_shift) out = true;
+      }
+    } else if (cell > g.nx) {
+      const int bc = g.bc_particle[XMAX];
+      const float xr = (float)(cell - 1) + oxn;
+      if (bc == BC_PERIODIC) {
+        cell -= g.nx;
+      } else {
+        if (bc == BC_REFLECT) {
+          ux = -ux;
+          const float xn = 2.0f * ((float)g.nx - 0.5f) - xr;
+          const float s2 = floorf(xn + 0.5f);
+          cell = (int)s2 + 1;
+          oxn = xn - s2;
+        }
+        if (xr >= (float)g.nx - 0.5f + (float)mc.png_shift) out = true;
+      }
+    }
+    const float rn = (yn * yn + zn * zn) * irn;
+    if (rn >= mc.nyf) {
+      const int bc = g.bc_particle[YMAX];
+      if (bc == BC_REFLECT) {
+        const float rr = 2.0f * mc.nyf / rn - 1.0f;
+        yn *= rr;
+        zn *= rr;
+        const float ir2 = __builtin_amdgcn_rsqf(yn * yn + zn * zn);
+        const float cs = yn * ir2, sn = zn * ir2;
+        const float pr = uy * cs + uz * sn, pt = -(uy * sn) + uz * cs;
+        uy = -(pr * cs) - pt * sn;
+        uz = -(pr * sn) + pt * cs;
+      }
+      if (rn >= mc.nyf + (float)mc.png_shift) out = true;
+    }
+    sp.cell[ip] = cell;
+    sp.ox[ip] = oxn;
+    sp.y[ip] = yn;
+    sp.z[ip] = zn;
+    sp.ux[ip] = ux;
+    sp.uy[ip] = uy;
+    sp.uz[ip] = uz;
+    int key = tl.ntiles;
+    if (!out) {
+      const float rk = sqrtf(yn * yn + zn * zn);
+      int cyk = (int)floorf(rk - mc.yoff + 1.5f);
+      int cxk = cell;
+      cxk = cxk < 1 ? 1 : (cxk > g.nx ? g.nx : cxk);
+      cyk = cyk < 1 ? 1 : (cyk > g.ny ? g.ny : cyk);
+      key = ((cyk - 1) / tl.tr) * tl.ntx + (cxk - 1) / tl.tx;
+    }
+    sp.key[ip] = key;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nj; e += blockDim.x) {
+    const double re = jl[2 * e], imv = jl[2 * e + 1];
+    if (re == 0.0 && imv == 0.0) continue;
+    int rem = e;
+    const int lx = rem % b.LX;
+    rem /= b.LX;
+    const int lr = rem % b.LR;
+    rem /= b.LR;
+    const int m = rem % M;
+    const int comp = rem / M;
+    const int cx = b.jx0 + lx, cy = b.jr0 + lr;
+    if (cx < 1 - g.ng || cx > g.nx + g.ng || cy < 1 - g.ng || cy > g.ny + g.ng) continue;
+    cplx<float>* J = comp == 0 ? jx : (comp == 1 ? jr : jt);
+    float* p = (float*)(J + g.idx(cx, cy, m));
+    if (re != 0.0) unsafeAtomicAdd(p, (float)re);
+    if (imv != 0.0) unsafeAtomicAdd(p + 1, (float)imv);
+  }
+}
+
+// ------------------------------------------------------------------------
+// key of an uploaded particle (no push)
+__global__ void k_keys_mixed(Grid g, Tiles tl, MixConst mc, PartMixed sp, long n) {
+  const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= n) return;
+  const float y = sp.y[ip], z = sp.z[ip];
+  int cyk = (int)floorf(sqrtf(y * y + z * z) - mc.yoff + 1.5f);
+  int cxk = sp.cell[ip];
+  cxk = cxk < 1 ? 1 : (cxk > g.nx ? g.nx : cxk);
+  cyk = cyk < 1 ? 1 : (cyk > g.ny ? g.ny : cyk);
+  sp.key[ip] = ((cyk - 1) / tl.tr) * tl.ntx + (cxk - 1) / tl.tx;
+}
+
+// FP64 SI particles -> mixed representation
+__global__ void k_to_mixed(Grid g, double mass, const double* x, const double* y, const double* z,
+                           const double* px, const double* py, const double* pz, const double* w,
+                           PartMixed sp, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xr = (x[i] - g.x_grid_min) / g.dx;
+  double c = floor(xr + 0.5);
+  c = c < -1.0e6 ? -1.0e6 : (c > 1.0e6 ? 1.0e6 : c);
+  sp.cell[i] = (int)c + 1;
+  sp.ox[i] = (float)(xr - c);
+  sp.y[i] = (float)(y[i] / g.dy);
+  sp.z[i] = (float)(z[i] / g.dy);
+  const double imc = 1.0 / (mass * C_LIGHT);
+  sp.ux[i] = (float)(px[i] * imc);
+  sp.uy[i] = (float)(py[i] * imc);
+  sp.uz[i] = (float)(pz[i] * imc);
+  sp.w[i] = (float)w[i];
+}
+
+__global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, double* y, double* z, double* px,
+                             double* py, double* pz, double* w, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = g.x_grid_min + ((double)(sp.cell[i] - 1) + (double)sp.ox[i]) * g.dx;
+  y[i] = (double)sp.y[i] * g.dy;
+  z[i] = (double)sp.z[i] * g.dy;
+  const double mc = mass * C_LIGHT;
+  px[i] = (double)sp.ux[i] * mc;
+  py[i] = (double)sp.uy[i] * mc;
+  pz[i] = (double)sp.uz[i] * mc;
+  w[i] = (double)sp.w[i];
+}
+
+// ------------------------------------------------------------------------
+size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl) {
+  const size_t nj = 3 * (size_t)g.M * (tl.tr + 2 * DEPOSIT_HALO) * (tl.tx + 2 * DEPOSIT_HALO);
+  const size_t nf = 6 * (size_t)g.M * (tl.tr + 2 * FIELD_HALO) * (tl.tx + 2 * FIELD_HALO);
+  return nj * 2 * sizeof(double) + nf * sizeof(cplx<float>);
+}
+
+MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
+  MixConst mc;
+  const double dt = g.dt;
+  mc.kx = (float)(C_LIGHT * dt / 2.0 / g.dx);
+  mc.ky = (float)(C_LIGHT * dt / 2.0 / g.dy);
+  mc.cmratio = (float)pc.cmratio;
+  mc.ccmratio = (float)pc.ccmratio;
+  mc.q_fac_idt = (float)(charge * 0.25 / dt);
+  mc.q_fac_c = (float)(charge * 0.25 * C_LIGHT);
+  mc.yoff = (float)(g.y_grid_min / g.dy);
+  mc.nyf = (float)(g.y_max / g.dy);
+  mc.png_shift = (1 + 3) / 2;
+  return mc;
+}
+
+int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& pc, double charge,
+                              const PartMixed& sp, const int* tile_start, FieldSet<float>& F,
+                              const float* geo, int ntab, int* err, hipStream_t s) {
+  const size_t lds = push_lds_bytes_mixed(g, tl);
+  MixConst mc = mix_const(g, pc, charge);
+  GeoF gf{geo, geo + ntab, geo + 2 * ntab, geo + 3 * ntab};
+#define LAUNCH(MM)                                                                                           \
+  hipLaunchKernelGGL(k_push_deposit_mixed<MM>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, mc, sp, tile_start, \
+                     F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), gf, err)
+  switch (g.M) {
+    case 1: LAUNCH(1); break;
+    case 2: LAUNCH(2); break;
+    case 3: LAUNCH(3); break;
+    case 4: LAUNCH(4); break;
+    default: return -1;
+  }
+#undef LAUNCH
+  return 0;
+}
+
+void launch_keys_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, long n, hipStream_t s) {
+  if (n <= 0) return;
+  PushConst pc{};
+  MixConst mc = mix_const(g, pc, 0.0);
+  hipLaunchKernelGGL(k_keys_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, mc, sp, n);
+}
+
+void launch_to_mixed(const Grid& g, double mass, const double* const* src, const PartMixed& sp, long n,
+                     hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, mass, src[0], src[1],
+                     src[2], src[3], src[4], src[5], src[6], sp, n);
+}
+
+void launch_from_mixed(const Grid& g, double mass, const PartMixed& sp, double* const* dst, long n,
+                       hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_from_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, mass, sp, dst[0], dst[1],
+                     dst[2], dst[3], dst[4], dst[5], dst[6], n);
+}
+
+}  // namespace cylpic

@@ -22,16 +22,22 @@ constexpr int SORT_TILE = SORT_BLOCK * IPT;   // 4096 keys per block
 constexpr int WAVES = SORT_BLOCK / 64;
 constexpr int WAVE_ITEMS = SORT_TILE / WAVES; // 1024 keys per wavefront
 
+// composite key of element i: pass 0 builds it from the tile key
+__device__ __forceinline__ uint32_t comp_key(const uint32_t* __restrict__ keys, long i, int compose_bits) {
+  const uint32_t k = keys[i];
+  return compose_bits >= 0 ? ((k << compose_bits) | ((uint32_t)i & ((1u << compose_bits) - 1u))) : k;
+}
+
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_count(const uint32_t* __restrict__ keys, long n,
-                                                             int shift, uint32_t* __restrict__ counts,
-                                                             int nb) {
+                                                             int shift, int compose_bits,
+                                                             uint32_t* __restrict__ counts, int nb) {
   __shared__ uint32_t hist[RADIX];
   for (int d = threadIdx.x; d < RADIX; d += SORT_BLOCK) hist[d] = 0;
   __syncthreads();
   const long base = (long)blockIdx.x * SORT_TILE;
   for (int r = 0; r < IPT; ++r) {
     const long i = base + (long)r * SORT_BLOCK + threadIdx.x;
-    if (i < n) atomicAdd(&hist[(keys[i] >> shift) & (RADIX - 1)], 1u);
+    if (i < n) atomicAdd(&hist[(comp_key(keys, i, compose_bits) >> shift) & (RADIX - 1)], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < RADIX; d += SORT_BLOCK) counts[(long)d * nb + blockIdx.x] = hist[d];
@@ -41,9 +47,12 @@ __device__ __forceinline__ uint32_t slot_get(int slot, uint32_t c0, uint32_t c1,
   return slot == 0 ? c0 : (slot == 1 ? c1 : (slot == 2 ? c2 : c3));
 }
 
+// compose_bits >= 0: first pass (keys are tile keys, values the index);
+// out_shift >= 0: last pass (write key >> out_shift, the tile key).
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                const uint32_t* __restrict__ vin, long n,
-                                                               int shift, const uint32_t* __restrict__ offs,
+                                                               int shift, int compose_bits, int out_shift,
+                                                               const uint32_t* __restrict__ offs,
                                                                int nb, uint32_t* __restrict__ kout,
                                                                uint32_t* __restrict__ vout) {
   __shared__ uint32_t wcnt[WAVES][RADIX];
@@ -56,7 +65,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __
   for (int r = 0; r < IPT; ++r) {
     const long i = base + (long)r * 64 + lane;
     const bool valid = i < n;
-    const uint32_t k = valid ? kin[i] : 0u;
+    const uint32_t k = valid ? comp_key(kin, i, compose_bits) : 0u;
     const uint32_t v = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     const uint32_t d = (k >> shift) & (RADIX - 1);
     unsigned long long active = __ballot(valid);
@@ -103,7 +112,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __
     if (i < n) {
       const uint32_t d = (key[r] >> shift) & (RADIX - 1);
       const uint32_t pos = offs[(long)d * nb + blockIdx.x] + wcnt[w][d] + rank[r];
-      kout[pos] = key[r];
+      kout[pos] = out_shift >= 0 ? (key[r] >> out_shift) : key[r];
       vout[pos] = val[r];
     }
   }
@@ -186,38 +195,30 @@ size_t sort_scratch_words(long n) {
   return (size_t)(4 * n + ncount + scan + 64);
 }
 
-__global__ void k_int_to_u32(const int* __restrict__ src, uint32_t* __restrict__ dst, long n) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (uint32_t)src[i];
-}
-
-void radix_sort(const int* keys, long n, int nkeys, SortScratch& sc, uint32_t** perm_out,
-                uint32_t** keys_out, hipStream_t s) {
-  int bits = 0;
-  while ((1l << bits) <= (long)nkeys) ++bits;
+void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, SortScratch& sc,
+                uint32_t* tiles_out, uint32_t** perm_out, hipStream_t s) {
+  int bits = slot_bits;
+  while ((1l << (bits - slot_bits)) <= (long)ntiles) ++bits;
   const int passes = (bits + 7) / 8;
   const int nb = (int)((n + SORT_TILE - 1) / SORT_TILE);
-  hipLaunchKernelGGL(k_int_to_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, sc.k[0], n);
-  int cur = 0;
+  const uint32_t* kin = tiles_in;
   const uint32_t* vin = nullptr;
+  int cur = 0;
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_count, dim3(nb), dim3(SORT_BLOCK), 0, s, sc.k[cur], n, shift, sc.counts, nb);
+    const int compose = p == 0 ? slot_bits : -1;
+    const bool last = p == passes - 1;
+    uint32_t* kout = last ? tiles_out : sc.k[cur];
+    uint32_t* vout = sc.v[cur];
+    hipLaunchKernelGGL(k_radix_count, dim3(nb), dim3(SORT_BLOCK), 0, s, kin, n, shift, compose, sc.counts, nb);
     exclusive_scan(sc.counts, (long)nb * RADIX, sc.scan_tmp, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(SORT_BLOCK), 0, s, sc.k[cur], vin, n, shift,
-                       sc.counts, nb, sc.k[cur ^ 1], sc.v[cur ^ 1]);
-    vin = sc.v[cur ^ 1];
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(SORT_BLOCK), 0, s, kin, vin, n, shift, compose,
+                       last ? slot_bits : -1, sc.counts, nb, kout, vout);
+    kin = kout;
+    vin = vout;
     cur ^= 1;
   }
-  if (passes == 0) {  // single key value: identity permutation
-    hipLaunchKernelGGL(k_radix_count, dim3(nb), dim3(SORT_BLOCK), 0, s, sc.k[0], n, 0, sc.counts, nb);
-    exclusive_scan(sc.counts, (long)nb * RADIX, sc.scan_tmp, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(SORT_BLOCK), 0, s, sc.k[0], (const uint32_t*)nullptr,
-                       n, 0, sc.counts, nb, sc.k[1], sc.v[1]);
-    cur = 1;
-  }
-  *perm_out = sc.v[cur];
-  *keys_out = sc.k[cur];
+  *perm_out = (uint32_t*)vin;
 }
 
 // tile_start[t] = first sorted index with key >= t, t in 0..ntiles
@@ -273,14 +274,6 @@ void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_
   p.dst[0] = dst;
   if (n > 0)
     hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, 1);
-}
-
-__global__ void k_copy_keys(const uint32_t* __restrict__ src, int* __restrict__ dst, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (int)src[i];
-}
-void launch_copy_keys(const uint32_t* src, int* dst, long n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_copy_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
 }
 
 }  // namespace cylpic

@@ -78,15 +78,18 @@ def test_current_finish_after_push(oracle):
 
 @pytest.mark.parametrize("tile", [(4, 4), (16, 8), (7, 3)])
 def test_sort_matches_stable_cpu_sort(oracle, tile):
-    """The upload sorts particles by tile key; the downloaded order must be
-    the stable sort of the uploaded order (bit-exact permutation)."""
+    """The upload sorts particles by the composite key (tile, i mod 64), i the
+    position in the uploaded arrays (the slot spreads one cell's particles over
+    the tile's processing order); the downloaded order must be the stable sort
+    of the uploaded order by that key (bit-exact permutation)."""
     log("sort tile", tile)
     nx, ny, dx, dy = 48, 20, 1e-7, 1e-7
     st, sim = make_pair(oracle, nx, ny, 1, dx, dy, tile=tile)
     p = add_plasma(st, sim, nx, ny, dx, dy, ppc=5, temp_ev=100.0, seed=11, shuffle=True)
     keys = oracle.cell_keys(st, p["x"], p["y"], p["z"], tile[0], tile[1]) // (tile[0] * tile[1])
     ntiles = ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
-    perm = oracle.stable_sort(keys, ntiles)
+    comp = keys * 64 + (np.arange(len(keys)) % 64)
+    perm = oracle.stable_sort(comp.astype(np.int32), ntiles * 64)
     got = sim.download_particles(0)
     assert np.array_equal(got["id"], p["id"][perm])
     sim.close()
