@@ -35,6 +35,7 @@ SIGNATURES = {
     "cylpic_sort_particles": [_p],
     "cylpic_set_sort_interval": [_p, _i],
     "cylpic_set_tile": [_p, _i, _i],
+    "cylpic_tune": [_p, _i, _i],
     "cylpic_mode_energy": [_p, _p, _p],
     "cylpic_synchronize": [_p],
     "cylpic_profile": [_p, _i],

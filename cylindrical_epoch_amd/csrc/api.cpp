@@ -78,7 +78,9 @@ struct cylpic_ctx {
   uint32_t* sc_mem = nullptr;
   long sc_cap = -1;
   int sort_interval = 1;
-  int slot_bits = 6;
+  int slot_bits = 0;
+  int acc_mode = 1;    // mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64)
+  int gather_lds = 1;  // mixed path: gather from the LDS-staged field region
   long push_count = 0;
   bool sorted_once = false;
   bool profiling = false;
@@ -239,7 +241,8 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   t.ntx = (h->g.nx + tx - 1) / tx;
   t.ntr = (h->g.ny + tr - 1) / tr;
   t.ntiles = t.ntx * t.ntr;
-  size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t) : push_lds_bytes(h->g, t, 0);
+  size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t, h->acc_mode, h->gather_lds)
+                                      : push_lds_bytes(h->g, t, 0);
   if (lds > 64 * 1024)
     return fail(CYLPIC_EINVAL, "tile too large: the push needs " + std::to_string(lds) +
                                    " bytes of LDS (limit 65536)");
@@ -298,7 +301,11 @@ PartF64 view64(Species& s) {
   return p;
 }
 
-int sort_species(cylpic_ctx* h, Species& s) {
+// Stable sort of a species by tile key; removed particles (key ntiles) go
+// last.  `interleave` additionally transposes each tile's particles so that
+// consecutive ones sit in different cells (done once, on upload: later
+// stable sorts keep that order).
+int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   Scope sc(h, "sort");
   s.dirty = false;
   if (s.n <= 0) {
@@ -310,18 +317,19 @@ int sort_species(cylpic_ctx* h, Species& s) {
   uint32_t* perm = nullptr;
   radix_sort(s.key[0], s.n, h->tl.ntiles, h->slot_bits, h->sc, s.key[1], &perm, h->stream);
   std::swap(s.key[0], s.key[1]);
+  launch_tile_bounds(s.key[0], s.n, h->tl.ntiles, s.tile_start, h->stream);
   if (h->prec == CYLPIC_FP64) {
-    launch_gather_f64(perm, s.n, s.a, s.b, 7, h->stream);
+    launch_gather_f64(perm, s.n, s.a, s.b, 7, s.key[0], s.tile_start, interleave, h->stream);
     for (int k = 0; k < 7; ++k) std::swap(s.a[k], s.b[k]);
   } else {
-    launch_gather_32(perm, s.n, s.ma, s.mb, 8, h->stream);
+    launch_gather_32(perm, s.n, s.ma, s.mb, 8, s.key[0], s.tile_start, interleave, h->stream);
     for (int k = 0; k < 8; ++k) std::swap(s.ma[k], s.mb[k]);
   }
   if (s.ids) {
-    launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], h->stream);
+    launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], s.key[0], s.tile_start, interleave,
+                     h->stream);
     std::swap(s.id[0], s.id[1]);
   }
-  launch_tile_bounds(s.key[0], s.n, h->tl.ntiles, s.tile_start, h->stream);
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;
 }
@@ -412,8 +420,8 @@ int alloc_species_arrays(cylpic_ctx* h, Species& s) {
   }
   HIPCHK(hipMalloc((void**)&s.key[0], cap * sizeof(uint32_t)));
   HIPCHK(hipMalloc((void**)&s.key[1], cap * sizeof(uint32_t)));
-  HIPCHK(hipMalloc((void**)&s.tile_start, (h->tl.ntiles + 1) * sizeof(int)));
-  HIPCHK(hipMemsetAsync(s.tile_start, 0, (h->tl.ntiles + 1) * sizeof(int), h->stream));
+  HIPCHK(hipMalloc((void**)&s.tile_start, (h->tl.ntiles + 2) * sizeof(int)));
+  HIPCHK(hipMemsetAsync(s.tile_start, 0, (h->tl.ntiles + 2) * sizeof(int), h->stream));
   s.d_count = s.tile_start + h->tl.ntiles;
   return CYLPIC_OK;
 }
@@ -564,6 +572,20 @@ int cylpic_set_tile(cylpic_ctx* h, int tile_x, int tile_r) {
   return set_tiles(h, tile_x, tile_r);
 }
 
+int cylpic_tune(cylpic_ctx* h, int acc_mode, int gather_lds) {
+  CHECK_H(h);
+  if (acc_mode < 0 || acc_mode > 2) return fail(CYLPIC_EINVAL, "acc_mode must be 0, 1 or 2");
+  const int old_acc = h->acc_mode, old_g = h->gather_lds;
+  h->acc_mode = acc_mode;
+  h->gather_lds = gather_lds ? 1 : 0;
+  int rc = set_tiles(h, h->tl.tx, h->tl.tr);
+  if (rc) {
+    h->acc_mode = old_acc;
+    h->gather_lds = old_g;
+  }
+  return rc;
+}
+
 int cylpic_set_sort_interval(cylpic_ctx* h, int interval) {
   CHECK_H(h);
   if (interval < 1) return fail(CYLPIC_EINVAL, "sort interval must be >= 1");
@@ -625,7 +647,7 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
   s.n = n;
   if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, h->stream);
   else launch_keys_mixed(h->g, h->tl, viewm(s), n, h->stream);
-  int rc = sort_species(h, s);
+  int rc = sort_species(h, s, 1);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return refresh_count(h, s);
@@ -782,7 +804,7 @@ int cylpic_push_particles(cylpic_ctx* h) {
     if (h->prec == CYLPIC_FP64) {
       launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->d_err, h->stream);
     } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, h->f32, h->geof, h->ntab,
-                                         h->d_err, h->stream)) {
+                                         h->acc_mode, h->gather_lds, h->d_err, h->stream)) {
       return fail(CYLPIC_EINVAL, "mixed push: unsupported mode count");
     }
   }

@@ -83,12 +83,12 @@ void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const 
 void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, hipStream_t);
 // push_mixed.hip
 int launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, double charge, const PartMixed&,
-                              const int* tile_start, FieldSet<float>&, const float* geo, int ntab, int* err,
-                              hipStream_t);
+                              const int* tile_start, FieldSet<float>&, const float* geo, int ntab, int acc,
+                              int ldsf, int* err, hipStream_t);
 void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, hipStream_t);
 void launch_to_mixed(const Grid&, double mass, const double* const* src, const PartMixed&, long n, hipStream_t);
 void launch_from_mixed(const Grid&, double mass, const PartMixed&, double* const* dst, long n, hipStream_t);
-size_t push_lds_bytes_mixed(const Grid&, const Tiles&);
+size_t push_lds_bytes_mixed(const Grid&, const Tiles&, int acc, int ldsf);
 size_t push_lds_bytes(const Grid&, const Tiles&, int precision);
 
 // sort.hip
@@ -100,17 +100,19 @@ struct SortScratch {
 };
 size_t sort_scratch_words(long n);
 // Stable sort of the composite keys (tile << slot_bits) | (i mod 2^slot_bits)
-// of n particles with tile keys in [0, ntiles] (ntiles = removed).  The slot
-// spreads the particles of one cell over the tile's processing order.  On
-// return perm[i] is the source index of sorted position i and tiles_out[i]
-// its tile key.
+// of n particles with tile keys in [0, ntiles] (ntiles = removed); slot_bits
+// = 0 sorts by tile alone.  On return perm[i] is the source index of sorted
+// position i and tiles_out[i] its tile key.
 void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, SortScratch& s,
                 uint32_t* tiles_out, uint32_t** perm_out, hipStream_t);
 void launch_tile_bounds(const uint32_t* sorted_tiles, long n, int ntiles, int* tile_start, hipStream_t);
+// Payload permutation dst[d(i)] = src[perm[i]]; d(i) = i, or with `inter`
+// the per-tile interleave of sort.hip (tile_start must describe the sorted keys).
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst,
-                       int narr, hipStream_t);
+                       int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst,
-                      int narr, hipStream_t);
-void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, hipStream_t);
+                      int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
+void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst,
+                      const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
 
 }  // namespace cylpic

@@ -92,7 +92,10 @@ struct Boxes {
   int jx0, jr0, LX, LR;  // current box origin and size
 };
 
-template <int M>
+// ACC: 0 = FP32 LDS atomics, 1 = FP32 with a 2Sum-compensated second word
+// (the rounding error of every add goes to a low word), 2 = FP64 LDS atomics.
+// LDSF: gather the fields from the LDS-staged tile region (1) or from L1/L2 (0).
+template <int M, int ACC, int LDSF>
 __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start,
     const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
@@ -112,12 +115,17 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   b.LX = tl.tx + 2 * DEPOSIT_HALO;
   b.LR = tl.tr + 2 * DEPOSIT_HALO;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
-  const int nf = 6 * M * b.FR * b.FX;      // complex cells of the field region
-  double* jl = lds_raw;                    // [3][M][LR][LX] (re, im)
-  cplx<float>* fl = (cplx<float>*)(lds_raw + 2 * nj);  // [6][M][FR][FX]
+  const int nf = LDSF ? 6 * M * b.FR * b.FX : 0;  // complex cells of the field region
+  // current box: [3][M][LR][LX] (re, im) in FP64 (ACC 2) or FP32 (ACC 0, 1;
+  // ACC 1 adds a second FP32 box for the low words)
+  double* jl = lds_raw;
+  float* jf = (float*)lds_raw;
+  float* jlo = jf + 2 * nj;
+  const int jwords = ACC == 2 ? 4 * nj : (ACC == 1 ? 4 * nj : 2 * nj);  // 32-bit words
+  cplx<float>* fl = (cplx<float>*)(((float*)lds_raw) + ((jwords + 1) & ~1));  // [6][M][FR][FX]
 
-  for (int e = threadIdx.x; e < 2 * nj; e += blockDim.x) jl[e] = 0.0;
-  {
+  for (int e = threadIdx.x; e < jwords; e += blockDim.x) jf[e] = 0.0f;
+  if (LDSF) {
     const cplx<float>* src[6] = {ex, er, et, bx, br, bt};
     for (int e = threadIdx.x; e < nf; e += blockDim.x) {
       int rem = e;
@@ -191,7 +199,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
         }
       const int lo_x = min(cx1, cx2) - 1, hi_x = max(cx1, cx2) + 1;
       const int lo_y = min(cy1, cy2) - 1, hi_y = max(cy1, cy2) + 1;
-      const bool in_lds = lo_x >= b.fx0 && hi_x < b.fx0 + b.FX && lo_y >= b.fr0 && hi_y < b.fr0 + b.FR;
+      const bool in_lds = LDSF && lo_x >= b.fx0 && hi_x < b.fx0 + b.FX && lo_y >= b.fr0 && hi_y < b.fr0 + b.FR;
       // stencil centre per component: ex (x1,y2,gh) er (x2,y1,hg) et (x2,y2,hh)
       //                               bx (x2,y1,hg) br (x1,y2,gh) bt (x1,y1,gg)
       const int ccx[6] = {cx1, cx2, cx2, cx2, cx1, cx1};
@@ -350,9 +358,23 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     }
     // is the whole footprint inside the LDS current box?
     const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
-    double* jbase = jl + 2 * ((long)(cy2 - b.jr0) * b.LX + (cx2 - b.jx0));
-    const long jcomp = (long)M * b.LR * b.LX * 2;  // doubles per component
+    const long jb = 2 * ((long)(cy2 - b.jr0) * b.LX + (cx2 - b.jx0));  // word offset of (cx2, cy2), comp 0, mode 0
+    const long jcomp = (long)M * b.LR * b.LX * 2;  // words per component
     const long jmode = (long)b.LR * b.LX * 2;
+    auto lds_add = [&](long off, float v) {
+      if (ACC == 2) {
+        atomicAdd(jl + off, (double)v);
+      } else if (ACC == 0) {
+        atomicAdd(jf + off, v);
+      } else {
+        // Knuth's TwoSum of the stored value and v; the error goes to the low word
+        const float old = atomicAdd(jf + off, v);
+        const float sum = old + v;
+        const float bp = sum - old;
+        const float err = (old - (sum - bp)) + (v - bp);
+        if (err != 0.0f) atomicAdd(jlo + off, err);
+      }
+    };
     float R = 0.0f;
     for (int iy = ymin; iy <= ymax; ++iy) {
       const int cy = cy2 + iy;
@@ -378,14 +400,14 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
           const cplx<float> vt = mk(m2[m].re * A + m3[m].re * B + m4[m].re * C,
                                     m2[m].im * A + m3[m].im * B + m4[m].im * C);
           if (jin) {
-            double* p = jbase + (long)m * jmode + (long)iy * b.LX * 2 + ix * 2;
-            atomicAdd(p + 2, (double)vx.re);                                  // (cx+1, cy)
-            atomicAdd(p + jcomp + (long)b.LX * 2, (double)vr.re);             // (cx, cy+1)
-            atomicAdd(p + 2 * jcomp, (double)vt.re);                          // (cx, cy)
+            const long p = jb + (long)m * jmode + (long)iy * b.LX * 2 + ix * 2;
+            lds_add(p + 2, vx.re);                                  // (cx+1, cy)
+            lds_add(p + jcomp + (long)b.LX * 2, vr.re);             // (cx, cy+1)
+            lds_add(p + 2 * jcomp, vt.re);                          // (cx, cy)
             if (m) {
-              atomicAdd(p + 3, (double)vx.im);
-              atomicAdd(p + jcomp + (long)b.LX * 2 + 1, (double)vr.im);
-              atomicAdd(p + 2 * jcomp + 1, (double)vt.im);
+              lds_add(p + 3, vx.im);
+              lds_add(p + jcomp + (long)b.LX * 2 + 1, vr.im);
+              lds_add(p + 2 * jcomp + 1, vt.im);
             }
           } else {
             float* qx = (float*)(jx + g.idx(cx + 1, cy, m));
@@ -476,7 +498,17 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   }
   __syncthreads();
   for (int e = threadIdx.x; e < nj; e += blockDim.x) {
-    const double re = jl[2 * e], imv = jl[2 * e + 1];
+    double re, imv;
+    if (ACC == 2) {
+      re = jl[2 * e];
+      imv = jl[2 * e + 1];
+    } else if (ACC == 0) {
+      re = jf[2 * e];
+      imv = jf[2 * e + 1];
+    } else {
+      re = (double)jf[2 * e] + (double)jlo[2 * e];
+      imv = (double)jf[2 * e + 1] + (double)jlo[2 * e + 1];
+    }
     if (re == 0.0 && imv == 0.0) continue;
     int rem = e;
     const int lx = rem % b.LX;
@@ -542,10 +574,11 @@ __global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, doubl
 }
 
 // ------------------------------------------------------------------------
-size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl) {
+size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl, int acc, int ldsf) {
   const size_t nj = 3 * (size_t)g.M * (tl.tr + 2 * DEPOSIT_HALO) * (tl.tx + 2 * DEPOSIT_HALO);
-  const size_t nf = 6 * (size_t)g.M * (tl.tr + 2 * FIELD_HALO) * (tl.tx + 2 * FIELD_HALO);
-  return nj * 2 * sizeof(double) + nf * sizeof(cplx<float>);
+  const size_t nf = ldsf ? 6 * (size_t)g.M * (tl.tr + 2 * FIELD_HALO) * (tl.tx + 2 * FIELD_HALO) : 0;
+  const size_t jwords = acc == 0 ? 2 * nj : 4 * nj;
+  return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(cplx<float>);
 }
 
 MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
@@ -565,20 +598,32 @@ MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
 
 int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& pc, double charge,
                               const PartMixed& sp, const int* tile_start, FieldSet<float>& F,
-                              const float* geo, int ntab, int* err, hipStream_t s) {
-  const size_t lds = push_lds_bytes_mixed(g, tl);
+                              const float* geo, int ntab, int acc, int ldsf, int* err, hipStream_t s) {
+  const size_t lds = push_lds_bytes_mixed(g, tl, acc, ldsf);
   MixConst mc = mix_const(g, pc, charge);
   GeoF gf{geo, geo + ntab, geo + 2 * ntab, geo + 3 * ntab};
-#define LAUNCH(MM)                                                                                           \
-  hipLaunchKernelGGL(k_push_deposit_mixed<MM>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, mc, sp, tile_start, \
-                     F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), gf, err)
-  switch (g.M) {
-    case 1: LAUNCH(1); break;
-    case 2: LAUNCH(2); break;
-    case 3: LAUNCH(3); break;
-    case 4: LAUNCH(4); break;
+#define LAUNCH(MM, AA, LL)                                                                                  \
+  hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL>), dim3(tl.ntiles), dim3(256), lds, s, g, tl, mc, sp, \
+                     tile_start, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), gf, err)
+#define BY_MODE(AA, LL)            \
+  switch (g.M) {                   \
+    case 1: LAUNCH(1, AA, LL); break; \
+    case 2: LAUNCH(2, AA, LL); break; \
+    case 3: LAUNCH(3, AA, LL); break; \
+    case 4: LAUNCH(4, AA, LL); break; \
+    default: return -1;            \
+  }
+  const int sel = acc * 2 + (ldsf ? 1 : 0);
+  switch (sel) {
+    case 0: BY_MODE(0, 0); break;
+    case 1: BY_MODE(0, 1); break;
+    case 2: BY_MODE(1, 0); break;
+    case 3: BY_MODE(1, 1); break;
+    case 4: BY_MODE(2, 0); break;
+    case 5: BY_MODE(2, 1); break;
     default: return -1;
   }
+#undef BY_MODE
 #undef LAUNCH
   return 0;
 }

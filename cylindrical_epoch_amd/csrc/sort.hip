@@ -21,6 +21,7 @@ constexpr int IPT = 16;                       // keys per thread
 constexpr int SORT_TILE = SORT_BLOCK * IPT;   // 4096 keys per block
 constexpr int WAVES = SORT_BLOCK / 64;
 constexpr int WAVE_ITEMS = SORT_TILE / WAVES; // 1024 keys per wavefront
+constexpr long INTERLEAVE_COLUMNS = 64;
 
 // composite key of element i: pass 0 builds it from the tile key
 __device__ __forceinline__ uint32_t comp_key(const uint32_t* __restrict__ keys, long i, int compose_bits) {
@@ -221,13 +222,15 @@ void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, Sor
   *perm_out = (uint32_t*)vin;
 }
 
-// tile_start[t] = first sorted index with key >= t, t in 0..ntiles
+// tile_start[t] = first sorted index with key >= t, t in 0..ntiles, and
+// tile_start[ntiles + 1] = n
 __global__ void k_tile_bounds(const uint32_t* __restrict__ key, long n, int ntiles, int* __restrict__ tile_start) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   const long lo = (i == 0) ? -1 : (long)key[i - 1];
   const long hi = (i == n) ? (long)ntiles : (long)key[i];
   for (long t = lo + 1; t <= hi && t <= ntiles; ++t) tile_start[t] = (int)i;
+  if (i == n) tile_start[ntiles + 1] = (int)n;  // sentinel past the removed ones
 }
 
 void launch_tile_bounds(const uint32_t* sorted_keys, long n, int ntiles, int* tile_start, hipStream_t s) {
@@ -240,40 +243,62 @@ struct Ptr8 {
   void* dst[8];
 };
 
+// Destination of sorted index i.  With `inter` set, the particles of each
+// tile segment [s, s+L) are transposed as an S-column table: the one at
+// segment offset j goes to column j mod S, row j / S, columns laid out one
+// after another (ragged: the first L mod S columns hold one more row).
+// Consecutive destinations then come from positions S apart in the stable
+// order, i.e. (for a cell-grouped load) from different cells.
+__device__ __forceinline__ long dest_of(long i, const uint32_t* __restrict__ key, const int* __restrict__ ts,
+                                        int inter) {
+  if (!inter) return i;
+  const uint32_t t = key[i];
+  const long s = ts[t], L = (long)ts[t + 1] - s, j = i - s;
+  const long S = INTERLEAVE_COLUMNS, A = L / S, R = L % S;
+  const long b = j % S, a = j / S;
+  return s + b * A + (b < R ? b : R) + a;
+}
+
 template <class W>
-__global__ void k_gather(const uint32_t* __restrict__ perm, long n, Ptr8 p, int narr) {
+__global__ void k_gather(const uint32_t* __restrict__ perm, long n, Ptr8 p, int narr,
+                         const uint32_t* __restrict__ key, const int* __restrict__ ts, int inter) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t j = perm[i];
-  for (int a = 0; a < narr; ++a) ((W*)p.dst[a])[i] = ((const W*)p.src[a])[j];
+  const long d = dest_of(i, key, ts, inter);
+  for (int a = 0; a < narr; ++a) ((W*)p.dst[a])[d] = ((const W*)p.src[a])[j];
 }
 
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst, int narr,
-                       hipStream_t s) {
+                       const uint32_t* key, const int* ts, int inter, hipStream_t s) {
   Ptr8 p;
   for (int a = 0; a < narr; ++a) {
     p.src[a] = src[a];
     p.dst[a] = dst[a];
   }
   if (n > 0)
-    hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr);
+    hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr,
+                       key, ts, inter);
 }
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst, int narr,
-                      hipStream_t s) {
+                      const uint32_t* key, const int* ts, int inter, hipStream_t s) {
   Ptr8 p;
   for (int a = 0; a < narr; ++a) {
     p.src[a] = src[a];
     p.dst[a] = dst[a];
   }
   if (n > 0)
-    hipLaunchKernelGGL(k_gather<uint32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr);
+    hipLaunchKernelGGL(k_gather<uint32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr,
+                       key, ts, inter);
 }
-void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, hipStream_t s) {
+void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, const uint32_t* key,
+                      const int* ts, int inter, hipStream_t s) {
   Ptr8 p;
   p.src[0] = src;
   p.dst[0] = dst;
   if (n > 0)
-    hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, 1);
+    hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, 1, key,
+                       ts, inter);
 }
 
 }  // namespace cylpic

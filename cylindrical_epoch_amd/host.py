@@ -40,7 +40,8 @@ class Simulation:
     def __init__(self, nx, ny, n_mode, dx, dy, dt, x_min=0.0, y_max=None,
                  bc_field=("periodic", "periodic", "other", "conduct"),
                  bc_particle=("periodic", "periodic", "other", "reflect"),
-                 precision="fp64", device=0, ng=5, tile=None, sort_interval=1):
+                 precision="fp64", device=0, ng=5, tile=None, sort_interval=1, acc_mode=None,
+                 gather_lds=None):
         self.nx, self.ny, self.n_mode, self.ng = nx, ny, n_mode, ng
         self.dx, self.dy, self.dt = dx, dy, dt
         self.x_min, self.x_max = x_min, x_min + nx * dx
@@ -56,6 +57,9 @@ class Simulation:
                                 self.y_min, self.y_max, C.cast(bf, C.c_void_p), C.cast(bp, C.c_void_p),
                                 PRECISION[precision], device))
         self.h = h
+        if acc_mode is not None or gather_lds is not None:
+            check(lib().cylpic_tune(self.h, 1 if acc_mode is None else int(acc_mode),
+                                    1 if gather_lds is None else int(gather_lds)))
         if tile is not None:
             check(lib().cylpic_set_tile(self.h, int(tile[0]), int(tile[1])))
         if sort_interval != 1:
@@ -150,6 +154,11 @@ class Simulation:
         b = None if src_xmax is None else np.ascontiguousarray(src_xmax, np.float64)
         check(lib().cylpic_step(self.h, _ptr(a), _ptr(b)))
         self.time += self.dt
+
+    def tune(self, acc_mode=1, gather_lds=1):
+        """Mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64)
+        and gather source (1 LDS-staged tile, 0 L1/L2)."""
+        check(lib().cylpic_tune(self.h, int(acc_mode), int(gather_lds)))
 
     def sort_particles(self):
         check(lib().cylpic_sort_particles(self.h))

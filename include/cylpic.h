@@ -133,6 +133,11 @@ int cylpic_sort_particles(cylpic_ctx *h);
 int cylpic_set_sort_interval(cylpic_ctx *h, int interval);
 /* Tile shape (cells) of the push/deposit work decomposition. */
 int cylpic_set_tile(cylpic_ctx *h, int tile_x, int tile_r);
+/* Mixed path only.  acc_mode: per-tile current sums in FP32 (0), FP32 with
+ * a 2Sum-compensated low word (1, default) or FP64 (2); gather_lds: stage the
+ * tile's fields in LDS (1, default) or gather from L1/L2 (0).  Has no
+ * counterpart in the reference (its deposit is serial FP64). */
+int cylpic_tune(cylpic_ctx *h, int acc_mode, int gather_lds);
 
 /* Per-mode field energies (FP64 reduction; the reference's Cartesian
  * energy sums are never updated in cylindrical mode, src/housekeeping/

@@ -28,15 +28,19 @@ def test_mixed_fields_half(oracle, M):
     sim.close()
 
 
-@pytest.mark.parametrize("M", [1, 2, 3])
-def test_mixed_push(oracle, M):
-    """One push of a warm plasma in given fields.  Positions and momenta
-    carry FP32 rounding (~1e-7 relative per operation); the current of a
-    particle is a difference of O(1) shape weights, so FP32 positions put
-    ~1e-5 relative error on slow particles' contributions."""
-    log("mixed push M", M)
+@pytest.mark.parametrize("M,acc,gather_lds", [(1, 1, 1), (2, 1, 1), (3, 1, 1), (3, 0, 1), (3, 2, 1),
+                                               (3, 1, 0), (2, 0, 0)])
+def test_mixed_push(oracle, M, acc, gather_lds):
+    """One push of a warm plasma in given fields, for each current
+    accumulator (FP32, FP32 + 2Sum, FP64) and gather source (LDS, L1/L2).
+    Positions and momenta carry FP32 rounding (~1e-7 relative per
+    operation); the current of a particle is a difference of O(1) shape
+    weights, so FP32 positions put ~1e-5 relative error on slow particles'
+    contributions."""
+    log("mixed push M", M, "acc", acc, "gather_lds", gather_lds)
     nx, ny, dx, dy = 40, 20, 1e-7, 1e-7
     st, sim = make_pair(oracle, nx, ny, M, dx, dy, tile=(8, 4), precision="mixed")
+    sim.tune(acc, gather_lds)
     rng = np.random.default_rng(M)
     f = random_fields(st.shape, rng, e_amp=3e10, b_amp=200.0, j_amp=0.0)
     load_fields(st, sim, f)

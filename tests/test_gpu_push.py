@@ -76,22 +76,47 @@ def test_current_finish_after_push(oracle):
     sim.close()
 
 
+def interleave_within_tiles(order, tiles, columns=64):
+    """The upload's per-tile interleave: the particles of a tile segment of
+    length L (stable order) are laid out as a table of `columns` columns,
+    offset j -> column j % columns, row j // columns, columns one after the
+    other (the first L % columns columns one row longer)."""
+    out = np.empty_like(order)
+    i = 0
+    n = len(order)
+    while i < n:
+        t = tiles[i]
+        e = i
+        while e < n and tiles[e] == t:
+            e += 1
+        L = e - i
+        A, R = divmod(L, columns)
+        for j in range(L):
+            b, a = j % columns, j // columns
+            out[i + b * A + min(b, R) + a] = order[i + j]
+        i = e
+    return out
+
+
 @pytest.mark.parametrize("tile", [(4, 4), (16, 8), (7, 3)])
 def test_sort_matches_stable_cpu_sort(oracle, tile):
-    """The upload sorts particles by the composite key (tile, i mod 64), i the
-    position in the uploaded arrays (the slot spreads one cell's particles over
-    the tile's processing order); the downloaded order must be the stable sort
-    of the uploaded order by that key (bit-exact permutation)."""
+    """The upload sorts particles by tile key (stable) and then interleaves
+    each tile's particles; the downloaded order must equal that permutation
+    of the uploaded order computed on the CPU (bit-exact)."""
     log("sort tile", tile)
     nx, ny, dx, dy = 48, 20, 1e-7, 1e-7
     st, sim = make_pair(oracle, nx, ny, 1, dx, dy, tile=tile)
-    p = add_plasma(st, sim, nx, ny, dx, dy, ppc=5, temp_ev=100.0, seed=11, shuffle=True)
+    p = add_plasma(st, sim, nx, ny, dx, dy, ppc=40, temp_ev=100.0, seed=11, shuffle=True)
     keys = oracle.cell_keys(st, p["x"], p["y"], p["z"], tile[0], tile[1]) // (tile[0] * tile[1])
     ntiles = ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
-    comp = keys * 64 + (np.arange(len(keys)) % 64)
-    perm = oracle.stable_sort(comp.astype(np.int32), ntiles * 64)
+    perm = oracle.stable_sort(keys, ntiles)
+    want = interleave_within_tiles(p["id"][perm], keys[perm])
     got = sim.download_particles(0)
-    assert np.array_equal(got["id"], p["id"][perm])
+    assert np.array_equal(got["id"], want)
+    # a second, plain sort keeps the order (no particle moved)
+    sim.sort_particles()
+    got2 = sim.download_particles(0)
+    assert np.array_equal(got2["id"], want)
     sim.close()
 
 
