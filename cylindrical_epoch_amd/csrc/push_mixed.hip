@@ -3,11 +3,13 @@
 //   - particle x as an int32 cell (1..nx, nearest cell) plus an FP32 offset
 //     from the cell centre in cells, y and z in FP32 units of dy, momenta as
 //     FP32 u = p / (m c), FP32 weight;
-//   - FP32 fields staged per tile in LDS (6 components x modes, tile plus a
-//     2-cell border: the reach of the 3x3 gather stencils);
-//   - the current deposited into an FP64 LDS box (tile plus a 3-cell
-//     border), i.e. the per-tile sums carry no FP32 rounding, flushed once
-//     per tile to the FP32 global J with global atomics;
+//   - FP32 fields gathered from L1/L2, or staged per tile in LDS (6
+//     components x modes, tile plus a 2-cell border: the reach of the 3x3
+//     gather stencils);
+//   - the current deposited into a per-tile LDS box (tile plus a 3-cell
+//     border) of int64 fixed-point words by default, i.e. exact,
+//     order-independent per-tile sums, flushed once per tile to the FP32
+//     global J with global atomics;
 //   - the deposit written as mode-independent real bases times per-mode
 //     complex factors (the same algebra as the reference's loops, fewer
 //     operations; the operation order differs, so results agree with the
@@ -28,6 +30,12 @@
 namespace cylpic {
 
 constexpr int FIELD_HALO = 2;
+
+// occupancy hint of the push kernel (register budget), e.g.
+// __attribute__((amdgpu_waves_per_eu(3)))
+#ifndef PUSH_MIXED_ATTR
+#define PUSH_MIXED_ATTR
+#endif
 
 struct MixConst {
   float kx, ky;           // c dt / (2 dx), c dt / (2 dy): half-step move in cells per unit u/gamma
@@ -92,17 +100,48 @@ struct Boxes {
   int jx0, jr0, LX, LR;  // current box origin and size
 };
 
-// ACC: 0 = FP32 LDS atomics, 1 = FP32 with a 2Sum-compensated second word
-// (the rounding error of every add goes to a low word), 2 = FP64 LDS atomics.
+// Per-tile current accumulators (ACC):
+//   0  FP32 LDS atomics;
+//   1  FP32 with a 2Sum-compensated second word (the rounding error of every
+//      add goes to a low word);
+//   2  FP64 LDS atomics;
+//   3  int64 fixed point per real word (ds_add_u64): every contribution is
+//      rounded once to an integer multiple of 1/S and summed exactly, so the
+//      tile sums do not depend on the order of the adds;
+//   4  one int64 per complex cell holding two 32-bit fixed-point halves
+//      (re * 2^32 + im, summed exactly as integers): half the atomics for
+//      m > 0, 32-bit sums (S is then sized for the tile's particle count).
+// The geometric factor of each current component depends only on the row it
+// lands in (Jx: inv_area_rt(cy), Jr: inv_area_xt(cy - 1), Jt: inv_volume(cy);
+// the reference's recursion jrh(iy) = jrh(iy-1) ratio_area_xt(iy) + ... is
+// inv_area_xt(iy) times a running sum because ratio_area_xt(iy) =
+// inv_area_xt(iy) / inv_area_xt(iy-1), particles.F90:209-216, 636-647), so it
+// is applied once per box cell when the box is flushed.
 // LDSF: gather the fields from the LDS-staged tile region (1) or from L1/L2 (0).
+template <int ACC>
+struct AccTraits {
+  static constexpr bool fixed = ACC >= 3;
+  // 32-bit LDS words per complex cell of the current box
+  static constexpr int words = ACC == 0 ? 2 : (ACC == 4 ? 2 : 4);
+};
+
+// scale S (a power of two) with S * bound <= 2^30
+__device__ __forceinline__ float fixed_scale(float bound) {
+  if (!(bound > 0.0f)) return 1.0f;
+  int e = ilogbf(1073741824.0f / bound);
+  e = e < -126 ? -126 : (e > 126 ? 126 : e);
+  return ldexpf(1.0f, e);
+}
+
 template <int M, int ACC, int LDSF>
-__global__ __launch_bounds__(256) void k_push_deposit_mixed(
+__global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
     Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start,
     const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
     const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
     cplx<float>* __restrict__ jx, cplx<float>* __restrict__ jr, cplx<float>* __restrict__ jt, GeoF geo,
     int* __restrict__ err) {
   extern __shared__ double lds_raw[];
+  __shared__ float wred[8];
   const int tile = blockIdx.x;
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   Boxes<M> b;
@@ -116,15 +155,35 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   b.LR = tl.tr + 2 * DEPOSIT_HALO;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
   const int nf = LDSF ? 6 * M * b.FR * b.FX : 0;  // complex cells of the field region
-  // current box: [3][M][LR][LX] (re, im) in FP64 (ACC 2) or FP32 (ACC 0, 1;
-  // ACC 1 adds a second FP32 box for the low words)
+  // current box: [3][M][LR][LX] complex cells, (re, im) words of FP64 (ACC 2),
+  // int64 (ACC 3), FP32 (ACC 0, 1; ACC 1 adds a second FP32 box for the low
+  // words) or one packed int64 per cell (ACC 4)
   double* jl = lds_raw;
   float* jf = (float*)lds_raw;
   float* jlo = jf + 2 * nj;
-  const int jwords = ACC == 2 ? 4 * nj : (ACC == 1 ? 4 * nj : 2 * nj);  // 32-bit words
+  unsigned long long* jq = (unsigned long long*)lds_raw;
+  const int jwords = AccTraits<ACC>::words * nj;  // 32-bit words
   cplx<float>* fl = (cplx<float>*)(((float*)lds_raw) + ((jwords + 1) & ~1));  // [6][M][FR][FX]
 
   for (int e = threadIdx.x; e < jwords; e += blockDim.x) jf[e] = 0.0f;
+  const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
+  // fixed-point scales: |Jx|, |Jr| words of one particle stay below
+  // 9 |q fac / dt| w and |Jt| words below 10.5 |q fac c| w (|Hx|, |sum hy| <= 2,
+  // |m2| <= 2, |m3| <= 1, |m4| <= 2/3, shape weights <= 1.5, |v_theta| <= c)
+  float sxr = 1.0f, st = 1.0f;
+  if (AccTraits<ACC>::fixed) {
+    float wm = 0.0f;
+    for (int ip = p0 + (int)threadIdx.x; ip < p1; ip += blockDim.x) wm = fmaxf(wm, fabsf(sp.w[ip]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o));
+    if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    wm = fmaxf(fmaxf(wred[0], wred[1]), fmaxf(wred[2], wred[3]));
+    // ACC 4 sums in 32 bits: one contribution per particle per word
+    const float cnt = ACC == 4 ? (float)(p1 - p0) : 1.0f;
+    sxr = fixed_scale(9.0f * fabsf(mc.q_fac_idt) * wm * cnt);
+    st = fixed_scale(10.5f * fabsf(mc.q_fac_c) * wm * cnt);
+  }
   if (LDSF) {
     const cplx<float>* src[6] = {ex, er, et, bx, br, bt};
     for (int e = threadIdx.x; e < nf; e += blockDim.x) {
@@ -143,7 +202,6 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   }
   __syncthreads();
 
-  const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
   const int jng = g.ng;
   const long fplane = (long)b.FR * b.FX;
 
@@ -343,8 +401,9 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
       }
     }
 
-    const float fcx = mc.q_fac_idt * wgt;
-    const float fcz = mc.q_fac_c * wgt * ut * ig;
+    // prefactors in box units (times S for the fixed-point boxes)
+    const float fcx = (mc.q_fac_idt * sxr) * wgt;
+    const float fcz = ((mc.q_fac_c * st) * wgt) * (ut * ig);
     // cumulative x sums of hx (jxh of the reference) are mode independent
     float Hx[5];
     {
@@ -358,11 +417,16 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     }
     // is the whole footprint inside the LDS current box?
     const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
-    const long jb = 2 * ((long)(cy2 - b.jr0) * b.LX + (cx2 - b.jx0));  // word offset of (cx2, cy2), comp 0, mode 0
-    const long jcomp = (long)M * b.LR * b.LX * 2;  // words per component
-    const long jmode = (long)b.LR * b.LX * 2;
-    auto lds_add = [&](long off, float v) {
-      if (ACC == 2) {
+    const int jb = (cy2 - b.jr0) * b.LX + (cx2 - b.jx0);  // box cell of (cx2, cy2), comp 0, mode 0
+    const int jcomp = M * b.LR * b.LX;                    // cells per component
+    const int jmode = b.LR * b.LX;
+    // one real word of box cell c (ACC 0..3)
+    auto lds_add = [&](int c, int part, float v) {
+      const int off = 2 * c + part;
+      if (ACC == 3) {
+        const long long q = (long long)__float2int_rn(v);
+        if (q != 0) atomicAdd(jq + off, (unsigned long long)q);
+      } else if (ACC == 2) {
         atomicAdd(jl + off, (double)v);
       } else if (ACC == 0) {
         atomicAdd(jf + off, v);
@@ -375,24 +439,34 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
         if (err != 0.0f) atomicAdd(jlo + off, err);
       }
     };
+    auto lds_add_c = [&](int c, float re, float im, bool has_im) {
+      if (ACC == 4) {
+        const int qr = __float2int_rn(re), qi = has_im ? __float2int_rn(im) : 0;
+        const unsigned long long q =
+            ((unsigned long long)(unsigned)qr << 32) + (unsigned long long)(long long)qi;
+        if (q != 0) atomicAdd(jq + c, q);
+      } else {
+        lds_add(c, 0, re);
+        if (has_im) lds_add(c, 1, im);
+      }
+    };
     float R = 0.0f;
     for (int iy = ymin; iy <= ymax; ++iy) {
       const int cy = cy2 + iy;
       const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
-      const float fjx = fcx * geo.rt[cy + jng];
-      R = R * geo.rat[cy + jng] + (fcx * hyv) * geo.xt[cy + jng];
-      const float fjz = fcz * geo.vol[cy + jng];
+      R = fmaf(fcx, hyv, R);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         const int ix = k - 2;
         if (ix < xmin || ix > xmax) continue;
         const float gxv = gx[k], hxv = hx[k];
-        const float A = fjz * (gxv * gyv), B = fjz * (gxv * hyv + hxv * gyv), C = fjz * (hxv * hyv);
-        const float PX = -fjx * Hx[k];
+        const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
+        const float PX = -fcx * Hx[k];
         const int cx = cx2 + ix;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-          // Jx(cx+1, cy) += -fjx w_rt Hx ; Jr(cx, cy+1) += -w_xt R ; Jt(cx, cy) += fjz w_xr
+          // Jx(cx+1, cy) += -fcx w_rt Hx ; Jr(cx, cy+1) += -w_xt R ; Jt(cx, cy) += fcz w_xr,
+          // each still to be multiplied by its row's geometric factor
           const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
           const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
           const cplx<float> vx = mk(PX * w_rt.re, PX * w_rt.im);
@@ -400,26 +474,22 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
           const cplx<float> vt = mk(m2[m].re * A + m3[m].re * B + m4[m].re * C,
                                     m2[m].im * A + m3[m].im * B + m4[m].im * C);
           if (jin) {
-            const long p = jb + (long)m * jmode + (long)iy * b.LX * 2 + ix * 2;
-            lds_add(p + 2, vx.re);                                  // (cx+1, cy)
-            lds_add(p + jcomp + (long)b.LX * 2, vr.re);             // (cx, cy+1)
-            lds_add(p + 2 * jcomp, vt.re);                          // (cx, cy)
-            if (m) {
-              lds_add(p + 3, vx.im);
-              lds_add(p + jcomp + (long)b.LX * 2 + 1, vr.im);
-              lds_add(p + 2 * jcomp + 1, vt.im);
-            }
+            const int c = jb + m * jmode + iy * b.LX + ix;
+            lds_add_c(c + 1, vx.re, vx.im, m != 0);              // (cx+1, cy)
+            lds_add_c(c + jcomp + b.LX, vr.re, vr.im, m != 0);   // (cx, cy+1)
+            lds_add_c(c + 2 * jcomp, vt.re, vt.im, m != 0);      // (cx, cy)
           } else {
+            const float gxf = geo.rt[cy + jng] / sxr, grf = geo.xt[cy + jng] / sxr, gtf = geo.vol[cy + jng] / st;
             float* qx = (float*)(jx + g.idx(cx + 1, cy, m));
             float* qr = (float*)(jr + g.idx(cx, cy + 1, m));
             float* qt = (float*)(jt + g.idx(cx, cy, m));
-            unsafeAtomicAdd(qx, vx.re);
-            unsafeAtomicAdd(qr, vr.re);
-            unsafeAtomicAdd(qt, vt.re);
+            unsafeAtomicAdd(qx, vx.re * gxf);
+            unsafeAtomicAdd(qr, vr.re * grf);
+            unsafeAtomicAdd(qt, vt.re * gtf);
             if (m) {
-              unsafeAtomicAdd(qx + 1, vx.im);
-              unsafeAtomicAdd(qr + 1, vr.im);
-              unsafeAtomicAdd(qt + 1, vt.im);
+              unsafeAtomicAdd(qx + 1, vx.im * gxf);
+              unsafeAtomicAdd(qr + 1, vr.im * grf);
+              unsafeAtomicAdd(qt + 1, vt.im * gtf);
             }
           }
         }
@@ -499,7 +569,15 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   __syncthreads();
   for (int e = threadIdx.x; e < nj; e += blockDim.x) {
     double re, imv;
-    if (ACC == 2) {
+    if (ACC == 4) {
+      const unsigned long long q = jq[e];
+      const int lo = (int)(unsigned)(q & 0xffffffffull);
+      re = (double)((long long)(q - (unsigned long long)(long long)lo) >> 32);
+      imv = (double)lo;
+    } else if (ACC == 3) {
+      re = (double)(long long)jq[2 * e];
+      imv = (double)(long long)jq[2 * e + 1];
+    } else if (ACC == 2) {
       re = jl[2 * e];
       imv = jl[2 * e + 1];
     } else if (ACC == 0) {
@@ -519,10 +597,15 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     const int comp = rem / M;
     const int cx = b.jx0 + lx, cy = b.jr0 + lr;
     if (cx < 1 - g.ng || cx > g.nx + g.ng || cy < 1 - g.ng || cy > g.ny + g.ng) continue;
+    // row factor: Jx inv_area_rt(cy), Jr inv_area_xt(cy - 1), Jt inv_volume(cy); undo S
+    double fac;
+    if (comp == 0) fac = (double)geo.rt[cy + jng] / (double)sxr;
+    else if (comp == 1) fac = (double)geo.xt[cy - 1 + jng] / (double)sxr;
+    else fac = (double)geo.vol[cy + jng] / (double)st;
     cplx<float>* J = comp == 0 ? jx : (comp == 1 ? jr : jt);
     float* p = (float*)(J + g.idx(cx, cy, m));
-    if (re != 0.0) unsafeAtomicAdd(p, (float)re);
-    if (imv != 0.0) unsafeAtomicAdd(p + 1, (float)imv);
+    if (re != 0.0) unsafeAtomicAdd(p, (float)(re * fac));
+    if (imv != 0.0) unsafeAtomicAdd(p + 1, (float)(imv * fac));
   }
 }
 
@@ -577,7 +660,7 @@ __global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, doubl
 size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl, int acc, int ldsf) {
   const size_t nj = 3 * (size_t)g.M * (tl.tr + 2 * DEPOSIT_HALO) * (tl.tx + 2 * DEPOSIT_HALO);
   const size_t nf = ldsf ? 6 * (size_t)g.M * (tl.tr + 2 * FIELD_HALO) * (tl.tx + 2 * FIELD_HALO) : 0;
-  const size_t jwords = acc == 0 ? 2 * nj : 4 * nj;
+  const size_t jwords = (acc == 0 || acc == 4) ? 2 * nj : 4 * nj;
   return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(cplx<float>);
 }
 
@@ -621,6 +704,10 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
     case 3: BY_MODE(1, 1); break;
     case 4: BY_MODE(2, 0); break;
     case 5: BY_MODE(2, 1); break;
+    case 6: BY_MODE(3, 0); break;
+    case 7: BY_MODE(3, 1); break;
+    case 8: BY_MODE(4, 0); break;
+    case 9: BY_MODE(4, 1); break;
     default: return -1;
   }
 #undef BY_MODE
