@@ -34,6 +34,8 @@ SIGNATURES = {
     "cylpic_step": [_p, _p, _p],
     "cylpic_sort_particles": [_p],
     "cylpic_set_sort_interval": [_p, _i],
+    "cylpic_set_sort_method": [_p, _i],
+    "cylpic_sort_counts": [_p, _p, _p],
     "cylpic_set_tile": [_p, _i, _i],
     "cylpic_tune": [_p, _i, _i],
     "cylpic_mode_energy": [_p, _p, _p],

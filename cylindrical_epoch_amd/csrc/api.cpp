@@ -45,9 +45,11 @@ struct Species {
   uint32_t* mb[8] = {};
   int64_t* id[2] = {};
   uint32_t* key[2] = {};      // tile key per particle (ntiles = removed), sort ping-pong
-  int* tile_start = nullptr;  // ntiles + 1
-  int* d_count = nullptr;     // == tile_start + ntiles
-  bool dirty = false;         // pushed since the last sort (removed particles may remain)
+  int* tile_start = nullptr;      // ntiles + 2
+  int* tile_start_alt = nullptr;  // ntiles + 2, migration sort output
+  int* d_count = nullptr;         // == tile_start + ntiles
+  bool dirty = false;             // pushed since the last sort (removed particles may remain)
+  bool ranged = false;            // tile_start describes the tile ranges of the keys' last sort
 };
 
 struct ProfEntry {
@@ -79,6 +81,11 @@ struct cylpic_ctx {
   long sc_cap = -1;
   int sort_interval = 1;
   int slot_bits = 0;
+  uint32_t* mig = nullptr;  // migration sort scratch (mig_scratch_words(ntiles))
+  int* mig_flag = nullptr;  // 2 ints: farther move seen, particles alive
+  int* mig_host = nullptr;  // pinned copy of mig_flag
+  bool migrate = true;      // use the neighbour migration sort when it applies
+  long sort_radix = 0, sort_migrate = 0;
   int acc_mode = 3;    // mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64, 3 int64, 4 2x32 packed)
   int gather_lds = 0;  // mixed path: gather from the LDS-staged field region (1) or L1/L2 (0)
   long push_count = 0;
@@ -306,13 +313,63 @@ PartF64 view64(Species& s) {
 // last.  `interleave` additionally transposes each tile's particles so that
 // consecutive ones sit in different cells (done once, on upload: later
 // stable sorts keep that order).
+// Neighbour migration sort of a species whose keys were tile-sorted before
+// the last push.  *done = false (nothing changed) when some particle moved
+// beyond the 3 x 3 tile neighbourhood.  Synchronises once (reads the flags).
+int migrate_species(cylpic_ctx* h, Species& s, bool* done) {
+  *done = false;
+  if (!h->mig) {
+    HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void**)&h->mig_flag, 2 * sizeof(int)));
+    HIPCHK(hipHostMalloc((void**)&h->mig_host, 2 * sizeof(int), 0));
+  }
+  HIPCHK(hipMemsetAsync(h->mig_flag, 0, 2 * sizeof(int), h->stream));
+  launch_mig_count(s.key[0], s.tile_start, h->tl, h->mig, h->mig_flag, h->stream);
+  HIPCHK(hipMemcpyAsync(h->mig_host, h->mig_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->mig_host[0]) return CYLPIC_OK;
+  const bool f64 = h->prec == CYLPIC_FP64;
+  const int narr = f64 ? 7 : 8;
+  const void* src[8];
+  void* dst[8];
+  for (int k = 0; k < narr; ++k) {
+    src[k] = f64 ? (const void*)s.a[k] : (const void*)s.ma[k];
+    dst[k] = f64 ? (void*)s.b[k] : (void*)s.mb[k];
+  }
+  launch_mig_scatter(s.key[0], s.tile_start, h->tl, h->mig, s.tile_start_alt, f64 ? 8 : 4, src, dst, narr, s.key[1],
+                     s.ids ? (const uint64_t*)s.id[0] : nullptr, s.ids ? (uint64_t*)s.id[1] : nullptr, h->stream);
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < narr; ++k) {
+    if (f64) std::swap(s.a[k], s.b[k]);
+    else std::swap(s.ma[k], s.mb[k]);
+  }
+  std::swap(s.key[0], s.key[1]);
+  if (s.ids) std::swap(s.id[0], s.id[1]);
+  std::swap(s.tile_start, s.tile_start_alt);
+  s.d_count = s.tile_start + h->tl.ntiles;
+  s.n = h->mig_host[1];
+  *done = true;
+  return CYLPIC_OK;
+}
+
 int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   Scope sc(h, "sort");
   s.dirty = false;
   if (s.n <= 0) {
     launch_tile_bounds(nullptr, 0, h->tl.ntiles, s.tile_start, h->stream);
+    s.ranged = true;
     return CYLPIC_OK;
   }
+  if (s.ranged && !interleave && h->migrate) {
+    bool done = false;
+    int rc = migrate_species(h, s, &done);
+    if (rc) return rc;
+    if (done) {
+      h->sort_migrate += 1;
+      return CYLPIC_OK;
+    }
+  }
+  h->sort_radix += 1;
   int rc = ensure_sort_scratch(h, s.n);
   if (rc) return rc;
   uint32_t* perm = nullptr;
@@ -332,6 +389,7 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
     std::swap(s.id[0], s.id[1]);
   }
   HIPCHK(hipGetLastError());
+  s.ranged = true;
   return CYLPIC_OK;
 }
 
@@ -384,6 +442,7 @@ void free_all(cylpic_ctx* h) {
     if (s.key[0]) (void)hipFree(s.key[0]);
     if (s.key[1]) (void)hipFree(s.key[1]);
     if (s.tile_start) (void)hipFree(s.tile_start);
+    if (s.tile_start_alt) (void)hipFree(s.tile_start_alt);
   }
   if (h->geo) (void)hipFree(h->geo);
   if (h->geof) (void)hipFree(h->geof);
@@ -392,6 +451,9 @@ void free_all(cylpic_ctx* h) {
     if (h->src_host[k]) (void)hipHostFree(h->src_host[k]);
   }
   if (h->d_err) (void)hipFree(h->d_err);
+  if (h->mig) (void)hipFree(h->mig);
+  if (h->mig_flag) (void)hipFree(h->mig_flag);
+  if (h->mig_host) (void)hipHostFree(h->mig_host);
   if (h->d_energy) (void)hipFree(h->d_energy);
   if (h->sc_mem) (void)hipFree(h->sc_mem);
   for (auto& kv : h->prof)
@@ -423,6 +485,7 @@ int alloc_species_arrays(cylpic_ctx* h, Species& s) {
   HIPCHK(hipMalloc((void**)&s.key[1], cap * sizeof(uint32_t)));
   HIPCHK(hipMalloc((void**)&s.tile_start, (h->tl.ntiles + 2) * sizeof(int)));
   HIPCHK(hipMemsetAsync(s.tile_start, 0, (h->tl.ntiles + 2) * sizeof(int), h->stream));
+  HIPCHK(hipMalloc((void**)&s.tile_start_alt, (h->tl.ntiles + 2) * sizeof(int)));
   s.d_count = s.tile_start + h->tl.ntiles;
   return CYLPIC_OK;
 }
@@ -589,6 +652,21 @@ int cylpic_tune(cylpic_ctx* h, int acc_mode, int gather_lds) {
   return rc;
 }
 
+int cylpic_set_sort_method(cylpic_ctx* h, int method) {
+  CHECK_H(h);
+  if (method != CYLPIC_SORT_RADIX && method != CYLPIC_SORT_MIGRATE)
+    return fail(CYLPIC_EINVAL, "sort method must be CYLPIC_SORT_RADIX or CYLPIC_SORT_MIGRATE");
+  h->migrate = method == CYLPIC_SORT_MIGRATE;
+  return CYLPIC_OK;
+}
+
+int cylpic_sort_counts(cylpic_ctx* h, int64_t* radix, int64_t* migrate) {
+  CHECK_H(h);
+  if (radix) *radix = h->sort_radix;
+  if (migrate) *migrate = h->sort_migrate;
+  return CYLPIC_OK;
+}
+
 int cylpic_set_sort_interval(cylpic_ctx* h, int interval) {
   CHECK_H(h);
   if (interval < 1) return fail(CYLPIC_EINVAL, "sort interval must be >= 1");
@@ -648,6 +726,7 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
     s.ids = false;
   }
   s.n = n;
+  s.ranged = false;
   if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, h->stream);
   else launch_keys_mixed(h->g, h->tl, viewm(s), n, h->stream);
   int rc = sort_species(h, s, 1);

@@ -115,4 +115,19 @@ void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, 
 void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst,
                       const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
 
+// Neighbour migration sort (keys tile-sorted before the last push).
+// scratch: mig_scratch_words(ntiles) words; flag: 2 ints, zeroed by the
+// caller: flag[0] != 0 when a particle's tile moved beyond the 3 x 3
+// neighbourhood (then sort with radix_sort), flag[1] = particles alive.
+constexpr int MIG_DIRS = 10;
+constexpr int MIG_REMOVED = 9;
+size_t mig_scratch_words(int ntiles);
+void launch_mig_count(const uint32_t* key, const int* tile_start, const Tiles& tl, uint32_t* scratch, int* flag,
+                      hipStream_t);
+// Sorted payload dst/key_out/id_out (ids optional) and tile_start_new
+// (ntiles + 2 ints) from the counts of launch_mig_count.
+void launch_mig_scatter(const uint32_t* key, const int* tile_start, const Tiles& tl, uint32_t* scratch,
+                        int* tile_start_new, int word_bytes, const void* const* src, void* const* dst, int narr,
+                        uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out, hipStream_t);
+
 }  // namespace cylpic

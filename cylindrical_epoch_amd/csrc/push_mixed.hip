@@ -36,6 +36,12 @@ constexpr int FIELD_HALO = 2;
 #ifndef PUSH_MIXED_ATTR
 #define PUSH_MIXED_ATTR
 #endif
+// work-items per tile workgroup: 512 (8 waves) gives one LDS current box twice
+// the waves of a 256 workgroup to hide the gather and atomic latency
+#ifndef PUSH_MIXED_THREADS
+#define PUSH_MIXED_THREADS 512
+#endif
+static_assert(PUSH_MIXED_THREADS % 64 == 0 && PUSH_MIXED_THREADS <= 512, "1..8 waves per tile workgroup");
 
 struct MixConst {
   float kx, ky;           // c dt / (2 dx), c dt / (2 dy): half-step move in cells per unit u/gamma
@@ -134,7 +140,7 @@ __device__ __forceinline__ float fixed_scale(float bound) {
 }
 
 template <int M, int ACC, int LDSF>
-__global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
+__global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_deposit_mixed(
     Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start,
     const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
     const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
@@ -178,7 +184,8 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
     for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o));
     if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = wm;
     __syncthreads();
-    wm = fmaxf(fmaxf(wred[0], wred[1]), fmaxf(wred[2], wred[3]));
+    wm = wred[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) wm = fmaxf(wm, wred[k]);
     // ACC 4 sums in 32 bits: one contribution per particle per word
     const float cnt = ACC == 4 ? (float)(p1 - p0) : 1.0f;
     sxr = fixed_scale(9.0f * fabsf(mc.q_fac_idt) * wm * cnt);
@@ -381,120 +388,9 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
     const int xmin = -1 + (dcx - 1) / 2, xmax = 1 + (dcx + 1) / 2;
     const int ymin = -1 + (dcy - 1) / 2, ymax = 1 + (dcy + 1) / 2;
 
-    // per-mode factors m2, m3, m4 (m = 0: 1, 1/2, 1/3)
-    cplx<float> m2[M], m3[M], m4[M];
-    {
-      m2[0] = mk(1.0f, 0.0f);
-      m3[0] = mk(0.5f, 0.0f);
-      m4[0] = mk(0.33333333f, 0.0f);
-      const cplx<float> e05 = mk(c05, s05);
-      cplx<float> e_m = mk(1.0f, 0.0f);
-#pragma unroll
-      for (int m = 1; m < M; ++m) {
-        e_m = cmul(e_m, e05);
-        cplx<float> f2, f3, f4;
-        mfactors((float)m * dth, f2, f3, f4);
-        const cplx<float> two_e = mk(2.0f * e_m.re, 2.0f * e_m.im);
-        m2[m] = cmul(two_e, f2);
-        m3[m] = cmul(two_e, f3);
-        m4[m] = cmul(two_e, f4);
-      }
-    }
-
     // prefactors in box units (times S for the fixed-point boxes)
     const float fcx = (mc.q_fac_idt * sxr) * wgt;
     const float fcz = ((mc.q_fac_c * st) * wgt) * (ut * ig);
-    // cumulative x sums of hx (jxh of the reference) are mode independent
-    float Hx[5];
-    {
-      float run = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        const bool on = (k - 2) >= xmin && (k - 2) <= xmax;
-        run += on ? hx[k] : 0.0f;
-        Hx[k] = run;
-      }
-    }
-    // is the whole footprint inside the LDS current box?
-    const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
-    const int jb = (cy2 - b.jr0) * b.LX + (cx2 - b.jx0);  // box cell of (cx2, cy2), comp 0, mode 0
-    const int jcomp = M * b.LR * b.LX;                    // cells per component
-    const int jmode = b.LR * b.LX;
-    // one real word of box cell c (ACC 0..3)
-    auto lds_add = [&](int c, int part, float v) {
-      const int off = 2 * c + part;
-      if (ACC == 3) {
-        const long long q = (long long)__float2int_rn(v);
-        if (q != 0) atomicAdd(jq + off, (unsigned long long)q);
-      } else if (ACC == 2) {
-        atomicAdd(jl + off, (double)v);
-      } else if (ACC == 0) {
-        atomicAdd(jf + off, v);
-      } else {
-        // Knuth's TwoSum of the stored value and v; the error goes to the low word
-        const float old = atomicAdd(jf + off, v);
-        const float sum = old + v;
-        const float bp = sum - old;
-        const float err = (old - (sum - bp)) + (v - bp);
-        if (err != 0.0f) atomicAdd(jlo + off, err);
-      }
-    };
-    auto lds_add_c = [&](int c, float re, float im, bool has_im) {
-      if (ACC == 4) {
-        const int qr = __float2int_rn(re), qi = has_im ? __float2int_rn(im) : 0;
-        const unsigned long long q =
-            ((unsigned long long)(unsigned)qr << 32) + (unsigned long long)(long long)qi;
-        if (q != 0) atomicAdd(jq + c, q);
-      } else {
-        lds_add(c, 0, re);
-        if (has_im) lds_add(c, 1, im);
-      }
-    };
-    float R = 0.0f;
-    for (int iy = ymin; iy <= ymax; ++iy) {
-      const int cy = cy2 + iy;
-      const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
-      R = fmaf(fcx, hyv, R);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        const int ix = k - 2;
-        if (ix < xmin || ix > xmax) continue;
-        const float gxv = gx[k], hxv = hx[k];
-        const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
-        const float PX = -fcx * Hx[k];
-        const int cx = cx2 + ix;
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-          // Jx(cx+1, cy) += -fcx w_rt Hx ; Jr(cx, cy+1) += -w_xt R ; Jt(cx, cy) += fcz w_xr,
-          // each still to be multiplied by its row's geometric factor
-          const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
-          const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
-          const cplx<float> vx = mk(PX * w_rt.re, PX * w_rt.im);
-          const cplx<float> vr = mk(-R * w_xt.re, -R * w_xt.im);
-          const cplx<float> vt = mk(m2[m].re * A + m3[m].re * B + m4[m].re * C,
-                                    m2[m].im * A + m3[m].im * B + m4[m].im * C);
-          if (jin) {
-            const int c = jb + m * jmode + iy * b.LX + ix;
-            lds_add_c(c + 1, vx.re, vx.im, m != 0);              // (cx+1, cy)
-            lds_add_c(c + jcomp + b.LX, vr.re, vr.im, m != 0);   // (cx, cy+1)
-            lds_add_c(c + 2 * jcomp, vt.re, vt.im, m != 0);      // (cx, cy)
-          } else {
-            const float gxf = geo.rt[cy + jng] / sxr, grf = geo.xt[cy + jng] / sxr, gtf = geo.vol[cy + jng] / st;
-            float* qx = (float*)(jx + g.idx(cx + 1, cy, m));
-            float* qr = (float*)(jr + g.idx(cx, cy + 1, m));
-            float* qt = (float*)(jt + g.idx(cx, cy, m));
-            unsafeAtomicAdd(qx, vx.re * gxf);
-            unsafeAtomicAdd(qr, vr.re * grf);
-            unsafeAtomicAdd(qt, vt.re * gtf);
-            if (m) {
-              unsafeAtomicAdd(qx + 1, vx.im * gxf);
-              unsafeAtomicAdd(qr + 1, vr.im * grf);
-              unsafeAtomicAdd(qt + 1, vt.im * gtf);
-            }
-          }
-        }
-      }
-    }
 
     // new position relative to its nearest cell; particle_bcs (boundary.F90:
     // 1541-1889) in cell units: periodic wraps are exact integer shifts
@@ -565,6 +461,152 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
       key = ((cyk - 1) / tl.tr) * tl.ntx + (cxk - 1) / tl.tx;
     }
     sp.key[ip] = key;
+
+    // per-mode factors m2, m3, m4 (m = 0: 1, 1/2, 1/3)
+    cplx<float> m2[M], m3[M], m4[M];
+    {
+      m2[0] = mk(1.0f, 0.0f);
+      m3[0] = mk(0.5f, 0.0f);
+      m4[0] = mk(0.33333333f, 0.0f);
+      const cplx<float> e05 = mk(c05, s05);
+      cplx<float> e_m = mk(1.0f, 0.0f);
+#pragma unroll
+      for (int m = 1; m < M; ++m) {
+        e_m = cmul(e_m, e05);
+        cplx<float> f2, f3, f4;
+        mfactors((float)m * dth, f2, f3, f4);
+        const cplx<float> two_e = mk(2.0f * e_m.re, 2.0f * e_m.im);
+        m2[m] = cmul(two_e, f2);
+        m3[m] = cmul(two_e, f3);
+        m4[m] = cmul(two_e, f4);
+      }
+    }
+
+    // cumulative x sums of hx (jxh of the reference) are mode independent
+    float Hx[5];
+    {
+      float run = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const bool on = (k - 2) >= xmin && (k - 2) <= xmax;
+        run += on ? hx[k] : 0.0f;
+        Hx[k] = run;
+      }
+    }
+    // is the whole footprint inside the LDS current box?
+    const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
+    const int jb = (cy2 - b.jr0) * b.LX + (cx2 - b.jx0);  // box cell of (cx2, cy2), comp 0, mode 0
+    const int jcomp = M * b.LR * b.LX;                    // cells per component
+    const int jmode = b.LR * b.LX;
+    // one real word of box cell c (ACC 0..3)
+    auto lds_add = [&](int c, int part, float v) {
+      const int off = 2 * c + part;
+      if (ACC == 3) {
+        atomicAdd(jq + off, (unsigned long long)(long long)__float2int_rn(v));
+      } else if (ACC == 2) {
+        atomicAdd(jl + off, (double)v);
+      } else if (ACC == 0) {
+        atomicAdd(jf + off, v);
+      } else {
+        // Knuth's TwoSum of the stored value and v; the error goes to the low word
+        const float old = atomicAdd(jf + off, v);
+        const float sum = old + v;
+        const float bp = sum - old;
+        const float err = (old - (sum - bp)) + (v - bp);
+        if (err != 0.0f) atomicAdd(jlo + off, err);
+      }
+    };
+    auto lds_add_c = [&](int c, float re, float im, bool has_im) {
+      if (ACC == 4) {
+        const int qr = __float2int_rn(re), qi = has_im ? __float2int_rn(im) : 0;
+        const unsigned long long q =
+            ((unsigned long long)(unsigned)qr << 32) + (unsigned long long)(long long)qi;
+        atomicAdd(jq + c, q);
+      } else {
+        lds_add(c, 0, re);
+        if (has_im) lds_add(c, 1, im);
+      }
+    };
+    if (jin) {
+      // footprint-aligned weights: column i of the footprint is ix = xmin + i
+      // (xmin is -2 or -1), row j is iy = ymin + j; 3 or 4 of each
+      const bool xs = xmin == -2, ys = ymin == -2;
+      float gxa[4], hxa[4], Hxa[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        gxa[i] = xs ? gx[i] : gx[i + 1];
+        hxa[i] = xs ? hx[i] : hx[i + 1];
+        Hxa[i] = xs ? Hx[i] : Hx[i + 1];
+      }
+      float gy0 = ys ? gy[0] : gy[1], gy1 = ys ? gy[1] : gy[2], gy2 = ys ? gy[2] : gy[3], gy3 = ys ? gy[3] : gy[4];
+      float hy0 = ys ? hy[0] : hy[1], hy1 = ys ? hy[1] : hy[2], hy2 = ys ? hy[2] : hy[3], hy3 = ys ? hy[3] : hy[4];
+      const bool four_x = xmax - xmin == 3;
+      const int nrow = ymax - ymin + 1;
+      float R = 0.0f;
+      int crow = jb + ymin * b.LX + xmin;  // box cell of the row's first column
+      for (int j = 0; j < nrow; ++j) {
+        const float gyv = gy0, hyv = hy0;
+        gy0 = gy1;
+        gy1 = gy2;
+        gy2 = gy3;
+        hy0 = hy1;
+        hy1 = hy2;
+        hy2 = hy3;
+        R = fmaf(fcx, hyv, R);
+        const float PR = -R;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (i == 3 && !four_x) break;
+          const float gxv = gxa[i], hxv = hxa[i];
+          const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
+          const float PX = -fcx * Hxa[i];
+          const int c = crow + i;
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            // Jx(cx+1, cy) += -fcx w_rt Hx ; Jr(cx, cy+1) += -w_xt R ; Jt(cx, cy) += fcz w_xr,
+            // each still to be multiplied by its row's geometric factor
+            const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
+            const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
+            const int cm = c + m * jmode;
+            lds_add_c(cm + 1, PX * w_rt.re, PX * w_rt.im, m != 0);                // (cx+1, cy)
+            lds_add_c(cm + jcomp + b.LX, PR * w_xt.re, PR * w_xt.im, m != 0);     // (cx, cy+1)
+            lds_add_c(cm + 2 * jcomp, m2[m].re * A + m3[m].re * B + m4[m].re * C, // (cx, cy)
+                      m2[m].im * A + m3[m].im * B + m4[m].im * C, m != 0);
+          }
+        }
+        crow += b.LX;
+      }
+    } else {
+      // footprint leaves the box: the same sums straight into global J
+      float R = 0.0f;
+      for (int iy = ymin; iy <= ymax; ++iy) {
+        const int cy = cy2 + iy;
+        const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
+        R = fmaf(fcx, hyv, R);
+        const float gxf = geo.rt[cy + jng] / sxr, grf = geo.xt[cy + jng] / sxr, gtf = geo.vol[cy + jng] / st;
+        for (int ix = xmin; ix <= xmax; ++ix) {
+          const float gxv = fsel5(gx, ix), hxv = fsel5(hx, ix);
+          const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
+          const float PX = -fcx * fsel5(Hx, ix);
+          const int cx = cx2 + ix;
+          for (int m = 0; m < M; ++m) {
+            const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
+            const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
+            float* qx = (float*)(jx + g.idx(cx + 1, cy, m));
+            float* qr = (float*)(jr + g.idx(cx, cy + 1, m));
+            float* qt = (float*)(jt + g.idx(cx, cy, m));
+            unsafeAtomicAdd(qx, PX * w_rt.re * gxf);
+            unsafeAtomicAdd(qr, -R * w_xt.re * grf);
+            unsafeAtomicAdd(qt, (m2[m].re * A + m3[m].re * B + m4[m].re * C) * gtf);
+            if (m) {
+              unsafeAtomicAdd(qx + 1, PX * w_rt.im * gxf);
+              unsafeAtomicAdd(qr + 1, -R * w_xt.im * grf);
+              unsafeAtomicAdd(qt + 1, (m2[m].im * A + m3[m].im * B + m4[m].im * C) * gtf);
+            }
+          }
+        }
+      }
+    }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < nj; e += blockDim.x) {
@@ -686,7 +728,7 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
   MixConst mc = mix_const(g, pc, charge);
   GeoF gf{geo, geo + ntab, geo + 2 * ntab, geo + 3 * ntab};
 #define LAUNCH(MM, AA, LL)                                                                                  \
-  hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL>), dim3(tl.ntiles), dim3(256), lds, s, g, tl, mc, sp, \
+  hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL>), dim3(tl.ntiles), dim3(PUSH_MIXED_THREADS), lds, s, g, tl, mc, sp, \
                      tile_start, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), gf, err)
 #define BY_MODE(AA, LL)            \
   switch (g.M) {                   \

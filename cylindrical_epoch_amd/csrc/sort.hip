@@ -301,4 +301,191 @@ void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_
                        ts, inter);
 }
 
+// ---------------- neighbour migration sort -------------------------------
+// Between two sorts a particle moves less than one cell in x and in r (the
+// CFL limit), so its new tile is its old tile or one of the 8 around it (x
+// taken modulo the tile columns, which covers periodic wraps).  If the keys
+// were sorted by tile before the push, the stable order by new key is: for
+// each destination tile, the particles arriving from its source tiles in
+// ascending source index, each source's particles in their current order.
+// That order is produced by one counting pass over the keys, an offsets pass
+// over the tiles and one scatter pass of the payload, instead of the radix
+// passes plus a gather.  A particle whose tile moved farther sets a flag and
+// the caller runs the radix sort instead.
+constexpr int MIG_BLOCK = 256;
+constexpr int MIG_WAVES = MIG_BLOCK / 64;
+
+// direction of the move from source tile S to key d: 0..8 = (dty+1)*3 + (dtx+1),
+// MIG_REMOVED for a removed particle, -1 for a farther move
+__device__ __forceinline__ int mig_dir(uint32_t d, int S, int ntx, uint32_t ntiles) {
+  if (d >= ntiles) return MIG_REMOVED;
+  const int dty = (int)(d / (uint32_t)ntx) - S / ntx;
+  int dtx = (int)(d % (uint32_t)ntx) - S % ntx;
+  if (dtx == ntx - 1) dtx = -1;
+  else if (dtx == 1 - ntx) dtx = 1;
+  if (dtx < -1 || dtx > 1 || dty < -1 || dty > 1) return -1;
+  return (dty + 1) * 3 + (dtx + 1);
+}
+
+// counts[S * MIG_DIRS + dir]; flag[0] |= 1 on a farther move, flag[1] += alive particles
+__global__ __launch_bounds__(MIG_BLOCK) void k_mig_count(const uint32_t* __restrict__ key,
+                                                         const int* __restrict__ ts, int ntx, int ntiles,
+                                                         uint32_t* __restrict__ counts, int* __restrict__ flag) {
+  __shared__ uint32_t c[MIG_DIRS];
+  if (threadIdx.x < MIG_DIRS) c[threadIdx.x] = 0;
+  __syncthreads();
+  const int S = blockIdx.x;
+  const int p0 = ts[S], p1 = ts[S + 1];
+  const int lane = threadIdx.x & 63;
+  bool far = false;
+  for (int base = p0; base < p1; base += MIG_BLOCK) {
+    const int i = base + (int)threadIdx.x;
+    const bool valid = i < p1;
+    const int dir = valid ? mig_dir(key[i], S, ntx, (uint32_t)ntiles) : -2;
+    far |= valid && dir < 0;
+#pragma unroll
+    for (int k = 0; k < MIG_DIRS; ++k) {
+      const unsigned long long m = __ballot(dir == k);
+      if (lane == 0 && m) atomicAdd(&c[k], (uint32_t)__popcll(m));
+    }
+  }
+  if (__ballot(far) && lane == 0) atomicOr(flag, 1);
+  __syncthreads();
+  if (threadIdx.x < MIG_DIRS) counts[(long)S * MIG_DIRS + threadIdx.x] = c[threadIdx.x];
+  if (threadIdx.x == 0) {
+    uint32_t alive = 0;
+    for (int k = 0; k < 9; ++k) alive += c[k];
+    if (alive) atomicAdd(flag + 1, (int)alive);
+  }
+}
+
+// per destination tile D: tot[D] = arrivals, pre[S * 9 + dir] = arrivals at D
+// from sources before S (ascending tile index); tot[ntiles] = 0
+__global__ void k_mig_offsets(const uint32_t* __restrict__ counts, int ntx, int ntr, int ntiles,
+                              uint32_t* __restrict__ tot, uint32_t* __restrict__ pre) {
+  const int D = blockIdx.x * blockDim.x + threadIdx.x;
+  if (D > ntiles) return;
+  if (D == ntiles) {
+    tot[D] = 0;
+    return;
+  }
+  const int tx = D % ntx, ty = D / ntx;
+  int src[9];
+  uint32_t c[9];
+#pragma unroll
+  for (int dir = 0; dir < 9; ++dir) {
+    const int dtx = dir % 3 - 1, dty = dir / 3 - 1;
+    const int sty = ty - dty;
+    src[dir] = -1;
+    c[dir] = 0;
+    if (sty >= 0 && sty < ntr) {
+      const int stx = ((tx - dtx) % ntx + ntx) % ntx;
+      src[dir] = sty * ntx + stx;
+      c[dir] = counts[(long)src[dir] * MIG_DIRS + dir];
+    }
+  }
+  uint32_t total = 0;
+#pragma unroll
+  for (int dir = 0; dir < 9; ++dir) {
+    total += c[dir];
+    if (src[dir] < 0) continue;
+    uint32_t p = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (src[k] >= 0 && (src[k] < src[dir] || (src[k] == src[dir] && k < dir))) p += c[k];
+    pre[(long)src[dir] * 9 + dir] = p;
+  }
+  tot[D] = total;
+}
+
+__global__ void k_mig_sentinel(int* ts, int ntiles) { ts[ntiles + 1] = ts[ntiles]; }
+
+// per source tile: stable rank of each particle among the tile's particles
+// with the same direction, then payload copy to its sorted position
+template <class W>
+__global__ __launch_bounds__(MIG_BLOCK) void k_mig_scatter(const uint32_t* __restrict__ key,
+                                                           const int* __restrict__ ts, int ntx, int ntiles,
+                                                           const uint32_t* __restrict__ pre,
+                                                           const int* __restrict__ ts_new, Ptr8 p, int narr,
+                                                           uint32_t* __restrict__ key_out,
+                                                           const uint64_t* __restrict__ id_in,
+                                                           uint64_t* __restrict__ id_out) {
+  __shared__ uint32_t run[9];
+  __shared__ uint32_t wc[MIG_WAVES][9];
+  const int S = blockIdx.x;
+  const int p0 = ts[S], p1 = ts[S + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  if (threadIdx.x < 9) run[threadIdx.x] = 0;
+  __syncthreads();
+  for (int base = p0; base < p1; base += MIG_BLOCK) {
+    const int i = base + (int)threadIdx.x;
+    const bool valid = i < p1;
+    const uint32_t d = valid ? key[i] : (uint32_t)ntiles;
+    const int dir = valid ? mig_dir(d, S, ntx, (uint32_t)ntiles) : -2;
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const unsigned long long m = __ballot(dir == k);
+      if (dir == k) r = (uint32_t)__popcll(m & lt);
+      if (lane == 0) wc[w][k] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (dir >= 0 && dir < 9) {
+      uint32_t off = run[dir];
+      for (int ww = 0; ww < w; ++ww) off += wc[ww][dir];
+      const long dst = (long)ts_new[d] + pre[(long)S * 9 + dir] + off + r;
+      for (int a = 0; a < narr; ++a) ((W*)p.dst[a])[dst] = ((const W*)p.src[a])[i];
+      key_out[dst] = d;
+      if (id_in) id_out[dst] = id_in[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+      uint32_t t = 0;
+      for (int ww = 0; ww < MIG_WAVES; ++ww) t += wc[ww][threadIdx.x];
+      run[threadIdx.x] += t;
+    }
+    __syncthreads();
+  }
+}
+
+size_t mig_scratch_words(int ntiles) {
+  long scan = 0, m = ntiles + 1;
+  while (m > 1) {
+    m = (m + SCAN_TILE - 1) / SCAN_TILE;
+    scan += m;
+  }
+  return (size_t)ntiles * (MIG_DIRS + 9) + (size_t)scan + 64;
+}
+
+void launch_mig_count(const uint32_t* key, const int* ts, const Tiles& tl, uint32_t* scratch, int* flag,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_mig_count, dim3(tl.ntiles), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, tl.ntiles, scratch,
+                     flag);
+}
+
+void launch_mig_scatter(const uint32_t* key, const int* ts, const Tiles& tl, uint32_t* scratch, int* ts_new,
+                        int word_bytes, const void* const* src, void* const* dst, int narr, uint32_t* key_out,
+                        const uint64_t* id_in, uint64_t* id_out, hipStream_t s) {
+  const int nt = tl.ntiles;
+  uint32_t* counts = scratch;
+  uint32_t* pre = scratch + (size_t)nt * MIG_DIRS;
+  uint32_t* tmp = pre + (size_t)nt * 9;
+  hipLaunchKernelGGL(k_mig_offsets, dim3((nt + 1 + 255) / 256), dim3(256), 0, s, counts, tl.ntx, tl.ntr, nt,
+                     (uint32_t*)ts_new, pre);
+  exclusive_scan((uint32_t*)ts_new, nt + 1, tmp, s);
+  hipLaunchKernelGGL(k_mig_sentinel, dim3(1), dim3(1), 0, s, ts_new, nt);
+  Ptr8 p;
+  for (int a = 0; a < narr; ++a) {
+    p.src[a] = src[a];
+    p.dst[a] = dst[a];
+  }
+  if (word_bytes == 8)
+    hipLaunchKernelGGL(k_mig_scatter<uint64_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, nt, pre, ts_new,
+                       p, narr, key_out, id_in, id_out);
+  else
+    hipLaunchKernelGGL(k_mig_scatter<uint32_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, nt, pre, ts_new,
+                       p, narr, key_out, id_in, id_out);
+}
+
 }  // namespace cylpic

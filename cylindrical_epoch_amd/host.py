@@ -164,6 +164,16 @@ class Simulation:
     def sort_particles(self):
         check(lib().cylpic_sort_particles(self.h))
 
+    def set_sort_method(self, method):
+        """"radix" (full LSD radix sort every time) or "migrate" (neighbour
+        migration when it applies, radix otherwise)."""
+        check(lib().cylpic_set_sort_method(self.h, {"radix": 0, "migrate": 1}[method]))
+
+    def sort_counts(self):
+        r, m = C.c_int64(), C.c_int64()
+        check(lib().cylpic_sort_counts(self.h, C.byref(r), C.byref(m)))
+        return {"radix": r.value, "migrate": m.value}
+
     def synchronize(self):
         check(lib().cylpic_synchronize(self.h))
 

@@ -11,6 +11,7 @@ MODULE cylpic
   IMPLICIT NONE
 
   INTEGER(C_INT), PARAMETER :: CYLPIC_FP64 = 0, CYLPIC_MIXED = 1
+  INTEGER(C_INT), PARAMETER :: CYLPIC_SORT_RADIX = 0, CYLPIC_SORT_MIGRATE = 1
 
   ! field identifiers, in the order of src/shared_data.F90:474-478
   INTEGER(C_INT), PARAMETER :: CYLPIC_EXM = 0, CYLPIC_ERM = 1, CYLPIC_ETM = 2
@@ -182,6 +183,22 @@ MODULE cylpic
       INTEGER(C_INT), VALUE :: interval
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_set_sort_interval
+
+    FUNCTION cylpic_set_sort_method(h, method) &
+        BIND(C, NAME='cylpic_set_sort_method') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: method
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_set_sort_method
+
+    FUNCTION cylpic_sort_counts(h, radix, migrate) &
+        BIND(C, NAME='cylpic_sort_counts') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_INT64_T
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT64_T), INTENT(OUT) :: radix, migrate
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_sort_counts
 
     FUNCTION cylpic_set_tile(h, tile_x, tile_r) BIND(C, NAME='cylpic_set_tile') &
         RESULT(ierr)

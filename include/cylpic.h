@@ -131,6 +131,17 @@ int cylpic_step(cylpic_ctx *h, const double *src_xmin, const double *src_xmax);
  * every `interval` steps (default 1); this forces one now. */
 int cylpic_sort_particles(cylpic_ctx *h);
 int cylpic_set_sort_interval(cylpic_ctx *h, int interval);
+/* Sort method (no counterpart in the reference, which keeps no order):
+ * CYLPIC_SORT_RADIX always runs the full LSD radix sort; CYLPIC_SORT_MIGRATE
+ * (default) moves particles by one counting and one scatter pass when every
+ * particle's tile moved at most one tile in x and r since the last sort
+ * (always true for one push under the CFL limit), and falls back to the radix
+ * sort otherwise.  Both give the stable order by tile key. */
+#define CYLPIC_SORT_RADIX 0
+#define CYLPIC_SORT_MIGRATE 1
+int cylpic_set_sort_method(cylpic_ctx *h, int method);
+/* Number of sorts done so far by each method. */
+int cylpic_sort_counts(cylpic_ctx *h, int64_t *radix, int64_t *migrate);
 /* Tile shape (cells) of the push/deposit work decomposition. */
 int cylpic_set_tile(cylpic_ctx *h, int tile_x, int tile_r);
 /* Mixed path only.  acc_mode: per-tile current sums in FP32 (0), FP32 with

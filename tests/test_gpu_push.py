@@ -15,7 +15,8 @@ this on the CPU); the tolerance there is 1e-7 * max|J_m|."""
 import numpy as np
 import pytest
 
-from gpu_helpers import (add_plasma, by_id, load_fields, log, make_pair, max_rel,
+from cylindrical_epoch_amd.host import Simulation
+from gpu_helpers import (M0, Q0, add_plasma, by_id, load_fields, log, make_pair, max_rel,
                          random_fields)
 
 pytestmark = pytest.mark.gpu
@@ -139,4 +140,70 @@ def test_multi_step_thermal(oracle):
     want = by_id({k: st.species[0][k] for k in ("x", "y", "z", "px", "py", "pz", "w", "id")})
     for k in ("x", "y", "z", "px", "py", "pz"):
         assert max_rel(got[k], want[k]) < 1e-9, k
+    sim.close()
+
+
+@pytest.mark.parametrize("precision,tile,bc,temp_ev,pushes,method", [
+    ("fp64", (8, 4), "periodic", 100.0, 1, "migrate"),
+    ("fp64", (1, 1), "open", 2.0e5, 1, "migrate"),
+    ("fp64", (16, 8), "periodic", 2.0e5, 1, "migrate"),
+    ("fp64", (2, 1), "periodic", 2.0e5, 1, "migrate"),
+    ("fp64", (1, 1), "periodic", 2.0e5, 5, "radix"),
+    ("mixed", (8, 4), "periodic", 2.0e5, 1, "migrate"),
+    ("mixed", (4, 2), "open", 2.0e5, 1, "migrate"),
+])
+def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, temp_ev, pushes, method):
+    """After `pushes` pushes without a sort, the next sort puts the particles
+    in the stable order of their new tile keys, removed ones dropped
+    (bit-exact against the CPU stable sort of the keys).  One push moves a
+    particle at most one tile, so the neighbour migration sort applies; five
+    pushes of a hot plasma on 1 x 1 tiles move some farther, and the sort
+    falls back to the radix passes."""
+    log("sort after push", precision, tile, bc, temp_ev, pushes)
+    nx, ny, dx, dy = 24, 12, 1e-7, 1e-7
+    bcf = (bc, bc, "other", "conduct") if bc == "periodic" else ("simple_outflow", "simple_outflow", "other",
+                                                                   "conduct")
+    bcp = (bc, bc, "other", "reflect") if bc == "periodic" else ("open", "open", "other", "open")
+    st, sim = make_pair(oracle, nx, ny, 1, dx, dy, bc_field=bcf, bc_particle=bcp, precision=precision,
+                        tile=tile, sort_interval=1000)
+    sim_inputs = add_plasma(st, sim, nx, ny, dx, dy, ppc=6, temp_ev=temp_ev, seed=31, shuffle=True)
+    before = sim.download_particles(0)["id"]
+    c0 = sim.sort_counts()
+    for _ in range(pushes):
+        sim.push_particles()
+        if precision == "fp64":
+            st.push_particles()
+    got = sim.download_particles(0)
+    c1 = sim.sort_counts()
+    assert c1[method] == c0[method] + 1, (c0, c1)
+    ntiles = ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
+    if precision == "fp64":
+        # positions are bit-exact with the oracle: keys from the oracle's particles
+        sp = st.species[0]
+        pos = {i: k for k, i in enumerate(sp["id"])}
+        keys_alive = oracle.cell_keys(st, sp["x"], sp["y"], sp["z"], tile[0], tile[1]) // (tile[0] * tile[1])
+        keys = np.array([keys_alive[pos[i]] if i in pos else ntiles for i in before], np.int32)
+    else:
+        # the mixed path's keys come from its FP32 state: compare with the same
+        # run sorted by the radix passes (the push of a particle does not
+        # depend on the others, so both runs hold identical keys)
+        ref = Simulation(nx, ny, 1, dx, dy, st.g.dt, bc_field=bcf, bc_particle=bcp, precision=precision,
+                         tile=tile, sort_interval=1000)
+        ref.set_sort_method("radix")
+        sid = ref.set_species(-Q0, M0, False, capacity=len(before))
+        p = sim_inputs
+        ref.upload_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"], ids=p["id"])
+        for _ in range(pushes):
+            ref.push_particles()
+        want = ref.download_particles(0)
+        assert ref.sort_counts()["migrate"] == 0
+        ref.close()
+        for k in ("id", "x", "y", "z", "px", "py", "pz", "w"):
+            assert np.array_equal(got[k], want[k]), k
+        sim.close()
+        return
+    perm = oracle.stable_sort(keys, ntiles + 1)
+    want = before[perm][: int(np.sum(keys < ntiles))]
+    log(f"  {len(before)} -> {len(want)} particles, sort counts {c1}")
+    assert np.array_equal(got["id"], want)
     sim.close()
