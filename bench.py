@@ -58,14 +58,21 @@ class Laser:
         return np.concatenate([base * math.cos(self.pol), base * math.sin(self.pol)])
 
 
-def make_sim(w, precision, particles):
+def make_sim(w, precision, particles, tune=None):
     dx, dy = w["dx"], w["dy"]
     laser = Laser(w["ny"], dy)
     dt = deck.set_dt(dx, dy, 0.95, plasma=[(w["density"], 100.0 * deck.EV / deck.KB, deck.M0)],
                      laser_omegas=[laser.omega])
+    tune = tune or {}
+    mixed = precision == "mixed"
     sim = Simulation(w["nx"], w["ny"], w["n_mode"], dx, dy, dt,
                      bc_field=("simple_laser", "simple_outflow", "other", "open"),
-                     bc_particle=("open", "open", "other", "open"), precision=precision)
+                     bc_particle=("open", "open", "other", "open"), precision=precision,
+                     tile=tune.get("tile_" + precision),
+                     acc_mode=tune.get("acc_mode") if mixed else None,
+                     gather_lds=tune.get("gather_lds") if mixed else None)
+    if tune.get("sort_method"):
+        sim.set_sort_method(tune["sort_method"])
     n = len(particles["x"])
     sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=n)
     sim.upload_particles(sid, particles["x"], particles["y"], particles["z"], particles["px"],
@@ -94,6 +101,18 @@ def run(sim, laser, dt, steps, warmup):
     return (t1 - t0) / steps, prof
 
 
+def measured_traffic(precision):
+    """HBM bytes per push_deposit launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary for this kernel and workload exists."""
+    path = os.path.join(ROOT, "profiles", f"push_deposit_{precision}_{WORKLOAD}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def roofline(w, n, prof_push, precision):
     """Algorithmic bytes of one push_deposit launch / its average duration."""
     ms, cnt = prof_push
@@ -110,7 +129,7 @@ def roofline(w, n, prof_push, precision):
     avg_s = ms / cnt / 1e3
     achieved = bytes_ / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(precision),
             "kernel": "push_deposit", "avg_launch_ms": round(ms / cnt, 4),
             "algorithmic_bytes_per_launch": bytes_}
 
@@ -161,6 +180,11 @@ def main():
     ap.add_argument("--workload", default=WORKLOAD, choices=sorted(deck.WORKLOADS))
     ap.add_argument("--cpu-sample-cols", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-mixed", default=None, help="tile of the mixed path, e.g. 32x4 (library default)")
+    ap.add_argument("--tile-fp64", default=None, help="tile of the FP64 path, e.g. 16x8 (library default)")
+    ap.add_argument("--acc-mode", type=int, default=None, help="mixed current accumulator 0..4")
+    ap.add_argument("--gather-lds", type=int, default=None, help="mixed gather from LDS (1) or L1/L2 (0)")
+    ap.add_argument("--sort-method", default=None, choices=["radix", "migrate"])
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no "
@@ -173,13 +197,18 @@ def main():
     n = len(parts["x"])
     log(f"{n} particles in {time.perf_counter() - t0:.1f} s")
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    tune = {"acc_mode": a.acc_mode, "gather_lds": a.gather_lds, "sort_method": a.sort_method}
+    for k in ("mixed", "fp64"):
+        t = getattr(a, "tile_" + k)
+        tune["tile_" + k] = tuple(int(v) for v in t.split("x")) if t else None
     results = {}
     for prec in precs:
-        sim, laser, dt = make_sim(w, prec, parts)
+        sim, laser, dt = make_sim(w, prec, parts, tune)
         log(f"{prec}: uploaded, dt = {dt:.4e}")
         ms, prof = run(sim, laser, dt, a.steps, a.warmup)
+        sorts = sim.sort_counts()
         sim.close()
-        results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof)
+        results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof, sorts=sorts)
         log(f"{prec}: {ms * 1e3:.3f} ms/step, {n / ms:.4e} particle-steps/s")
         for k, (kms, cnt) in prof.items():
             if cnt:
@@ -198,7 +227,10 @@ def main():
                    "ppc": w["ppc"], "particles": n, "precision": main_prec, "parallelism": "single-gpu",
                    "laser": "x_min, a0=3, lambda 0.8 um, waist 15 um, 25 fs", "moving_window": False},
         "field_cell_updates_per_s": round(cells / (r["ms"] / 1e3), 1),
-        "per_precision": {k: {"ms_per_step": round(v["ms"], 4), "particle_steps_per_s": round(v["value"], 1)}
+        "per_precision": {k: {"ms_per_step": round(v["ms"], 4), "particle_steps_per_s": round(v["value"], 1),
+                              "kernel_ms_per_step": {kk: round(kms / a.steps, 4) for kk, (kms, cnt) in
+                                                     v["prof"].items() if cnt},
+                              "sorts": v["sorts"]}
                           for k, v in results.items()},
         "roofline": roofline(w, n, r["prof"]["push_deposit"], main_prec),
     }

@@ -309,10 +309,6 @@ PartF64 view64(Species& s) {
   return p;
 }
 
-// Stable sort of a species by tile key; removed particles (key ntiles) go
-// last.  `interleave` additionally transposes each tile's particles so that
-// consecutive ones sit in different cells (done once, on upload: later
-// stable sorts keep that order).
 // Neighbour migration sort of a species whose keys were tile-sorted before
 // the last push.  *done = false (nothing changed) when some particle moved
 // beyond the 3 x 3 tile neighbourhood.  Synchronises once (reads the flags).
@@ -352,6 +348,11 @@ int migrate_species(cylpic_ctx* h, Species& s, bool* done) {
   return CYLPIC_OK;
 }
 
+// Stable sort of a species by tile key; removed particles (key ntiles) go
+// last.  With `interleave` (on upload) the keys are tile-major cell keys: the
+// sort groups each tile's particles by cell, and the gather then transposes
+// each tile so that consecutive particles sit in consecutive cells; later
+// stable sorts by tile keep that order.
 int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   Scope sc(h, "sort");
   s.dirty = false;
@@ -373,18 +374,22 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   int rc = ensure_sort_scratch(h, s.n);
   if (rc) return rc;
   uint32_t* perm = nullptr;
-  radix_sort(s.key[0], s.n, h->tl.ntiles, h->slot_bits, h->sc, s.key[1], &perm, h->stream);
+  // with `interleave` the keys are tile-major cell keys (upload)
+  const int cells = interleave ? h->tl.tx * h->tl.tr : 0;
+  radix_sort(s.key[0], s.n, interleave ? h->tl.ntiles * cells : h->tl.ntiles, h->slot_bits, h->sc, s.key[1], &perm,
+             h->stream);
   std::swap(s.key[0], s.key[1]);
+  if (interleave) launch_div_keys(s.key[0], s.n, (uint32_t)cells, h->stream);
   launch_tile_bounds(s.key[0], s.n, h->tl.ntiles, s.tile_start, h->stream);
   if (h->prec == CYLPIC_FP64) {
-    launch_gather_f64(perm, s.n, s.a, s.b, 7, s.key[0], s.tile_start, interleave, h->stream);
+    launch_gather_f64(perm, s.n, s.a, s.b, 7, s.key[0], s.tile_start, cells, h->stream);
     for (int k = 0; k < 7; ++k) std::swap(s.a[k], s.b[k]);
   } else {
-    launch_gather_32(perm, s.n, s.ma, s.mb, 8, s.key[0], s.tile_start, interleave, h->stream);
+    launch_gather_32(perm, s.n, s.ma, s.mb, 8, s.key[0], s.tile_start, cells, h->stream);
     for (int k = 0; k < 8; ++k) std::swap(s.ma[k], s.mb[k]);
   }
   if (s.ids) {
-    launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], s.key[0], s.tile_start, interleave,
+    launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], s.key[0], s.tile_start, cells,
                      h->stream);
     std::swap(s.id[0], s.id[1]);
   }
@@ -727,8 +732,9 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
   }
   s.n = n;
   s.ranged = false;
-  if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, h->stream);
-  else launch_keys_mixed(h->g, h->tl, viewm(s), n, h->stream);
+  // tile-major cell keys: the upload sort groups each tile's particles by cell
+  if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, 1, h->stream);
+  else launch_keys_mixed(h->g, h->tl, viewm(s), n, 1, h->stream);
   int rc = sort_species(h, s, 1);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));

@@ -53,6 +53,14 @@ struct Grid {
 // numbered row-major in x; H halo cells around each tile in the LDS current box.
 struct Tiles {
   int tx, tr, ntx, ntr, ntiles;
+  // Key of a (clamped) cell: its tile (cell_key = false), or the tile-major
+  // cell number tile * tx * tr + row-in-tile * tx + column-in-tile.
+  __host__ __device__ __forceinline__ int key(int cx, int cy, int nx, int ny, bool cell_key) const {
+    cx = cx < 1 ? 1 : (cx > nx ? nx : cx);
+    cy = cy < 1 ? 1 : (cy > ny ? ny : cy);
+    const int lx = cx - 1, ly = cy - 1, t = (ly / tr) * ntx + lx / tx;
+    return cell_key ? (t * tr + ly % tr) * tx + lx % tx : t;
+  }
 };
 constexpr int DEPOSIT_HALO = 3;
 

@@ -80,12 +80,13 @@ void launch_f2d(const cplx<float>* src, cplx<double>* dst, long n, hipStream_t);
 void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const PartF64&,
                              const int* tile_start, FieldSet<double>&, const Geo&, int* err,
                              hipStream_t);
-void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, hipStream_t);
+// key of every particle: its tile, or with cell_key its tile-major cell (Tiles::key)
+void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, int cell_key, hipStream_t);
 // push_mixed.hip
 int launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, double charge, const PartMixed&,
                               const int* tile_start, FieldSet<float>&, const float* geo, int ntab, int acc,
                               int ldsf, int* err, hipStream_t);
-void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, hipStream_t);
+void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, int cell_key, hipStream_t);
 void launch_to_mixed(const Grid&, double mass, const double* const* src, const PartMixed&, long n, hipStream_t);
 void launch_from_mixed(const Grid&, double mass, const PartMixed&, double* const* dst, long n, hipStream_t);
 size_t push_lds_bytes_mixed(const Grid&, const Tiles&, int acc, int ldsf);
@@ -106,8 +107,11 @@ size_t sort_scratch_words(long n);
 void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, SortScratch& s,
                 uint32_t* tiles_out, uint32_t** perm_out, hipStream_t);
 void launch_tile_bounds(const uint32_t* sorted_tiles, long n, int ntiles, int* tile_start, hipStream_t);
+// key[i] /= div (tile-major cell keys -> tile keys)
+void launch_div_keys(uint32_t* key, long n, uint32_t div, hipStream_t);
 // Payload permutation dst[d(i)] = src[perm[i]]; d(i) = i, or with `inter`
-// the per-tile interleave of sort.hip (tile_start must describe the sorted keys).
+// = the cells of a tile (> 0) the per-tile interleave of sort.hip
+// (tile_start must describe the sorted keys).
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst,
                        int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst,

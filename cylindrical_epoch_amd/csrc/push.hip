@@ -55,9 +55,12 @@ __device__ __forceinline__ void dep_add_f64(double* lds, const LdsBox& b, int co
                                             cplx<double> v, cplx<double>* __restrict__ gJ, const Grid& g) {
   const int lx = cx - b.x0, lr = cy - b.r0;
   if ((unsigned)lx < (unsigned)b.LX && (unsigned)lr < (unsigned)b.LR) {
-    double* p = lds + 2 * (((comp * b.M + m) * b.LR + lr) * b.LX + lx);
+    // planes [comp][m], real words then imaginary words: lanes at
+    // consecutive cells touch consecutive words (distinct banks)
+    const int ncell = b.LR * b.LX;
+    double* p = lds + 2 * (comp * b.M + m) * ncell + lr * b.LX + lx;
     atomicAdd(p, v.re);
-    if (m) atomicAdd(p + 1, v.im);
+    if (m) atomicAdd(p + ncell, v.im);
   } else {
     double* p = (double*)(gJ + g.idx(cx, cy, m));
     unsafeAtomicAdd(p, v.re);
@@ -73,19 +76,14 @@ __device__ __forceinline__ void dep_add_f64(double* lds, const LdsBox& b, int co
 // The tile key is the particle's cell (src/housekeeping/split_particle.F90:
 // 61-62, clamped into 1..nx, 1..ny) mapped to its tile.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ int tile_of_cell(const Grid& g, const Tiles& tl, int cx, int cy) {
-  cx = cx < 1 ? 1 : (cx > g.nx ? g.nx : cx);
-  cy = cy < 1 ? 1 : (cy > g.ny ? g.ny : cy);
-  return ((cy - 1) / tl.tr) * tl.ntx + (cx - 1) / tl.tx;
-}
-
-__device__ __forceinline__ int key_f64(const Grid& g, const Tiles& tl, double x, double y, double z) {
+__device__ __forceinline__ int key_f64(const Grid& g, const Tiles& tl, double x, double y, double z,
+                                       bool cell_key = false) {
   const double r = sqrt(y * y + z * z);
   const double fx = floor((x - g.x_grid_min) / g.dx + 1.5);
   const double fy = floor((r - g.y_grid_min) / g.dy + 1.5);
   const int cx = fx < -1.0e9 ? -1000000000 : (fx > 1.0e9 ? 1000000000 : (int)fx);
   const int cy = fy < -1.0e9 ? -1000000000 : (fy > 1.0e9 ? 1000000000 : (int)fy);
-  return tile_of_cell(g, tl, cx, cy);
+  return tl.key(cx, cy, g.nx, g.ny, cell_key);
 }
 
 __device__ __forceinline__ int bcs_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long ip,
@@ -422,8 +420,10 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
   }
   __syncthreads();
   // one pass of global atomics per tile
+  const int ncell = box.LR * box.LX;
   for (int e = threadIdx.x; e < nl; e += blockDim.x) {
-    const double re = lds[2 * e], imv = lds[2 * e + 1];
+    const int wr = e + (e / ncell) * ncell;  // real word of complex cell e
+    const double re = lds[wr], imv = lds[wr + ncell];
     if (re == 0.0 && imv == 0.0) continue;
     int rem = e;
     const int lx = rem % box.LX;
@@ -432,17 +432,19 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
     rem /= box.LR;
     const int m = rem % M;
     const int comp = rem / M;
+    const int cx = box.x0 + lx, cy = box.r0 + lr;
+    if (cx < 1 - g.ng || cx > g.nx + g.ng || cy < 1 - g.ng || cy > g.ny + g.ng) continue;
     cplx<double>* J = comp == 0 ? jx : (comp == 1 ? jr : jt);
-    double* p = (double*)(J + g.idx(box.x0 + lx, box.r0 + lr, m));
+    double* p = (double*)(J + g.idx(cx, cy, m));
     if (re != 0.0) unsafeAtomicAdd(p, re);
     if (imv != 0.0) unsafeAtomicAdd(p + 1, imv);
   }
 }
 
-__global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n) {
+__global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n, int cell_key) {
   const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= n) return;
-  sp.key[ip] = key_f64(g, tl, sp.x[ip], sp.y[ip], sp.z[ip]);
+  sp.key[ip] = key_f64(g, tl, sp.x[ip], sp.y[ip], sp.z[ip], cell_key != 0);
 }
 
 // ------------------------------------------------------------------------
@@ -462,9 +464,9 @@ void launch_push_deposit_f64(const Grid& g, const Tiles& tl, const PushConst& pc
                      F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err);
 }
 
-void launch_keys_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, hipStream_t s) {
+void launch_keys_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, int cell_key, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_keys_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, sp, n);
+  hipLaunchKernelGGL(k_keys_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, sp, n, cell_key);
 }
 
 }  // namespace cylpic

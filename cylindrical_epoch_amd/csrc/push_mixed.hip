@@ -36,12 +36,6 @@ constexpr int FIELD_HALO = 2;
 #ifndef PUSH_MIXED_ATTR
 #define PUSH_MIXED_ATTR
 #endif
-// work-items per tile workgroup: 512 (8 waves) gives one LDS current box twice
-// the waves of a 256 workgroup to hide the gather and atomic latency
-#ifndef PUSH_MIXED_THREADS
-#define PUSH_MIXED_THREADS 512
-#endif
-static_assert(PUSH_MIXED_THREADS % 64 == 0 && PUSH_MIXED_THREADS <= 512, "1..8 waves per tile workgroup");
 
 struct MixConst {
   float kx, ky;           // c dt / (2 dx), c dt / (2 dy): half-step move in cells per unit u/gamma
@@ -112,7 +106,7 @@ struct Boxes {
 //      add goes to a low word);
 //   2  FP64 LDS atomics;
 //   3  int64 fixed point per real word (ds_add_u64): every contribution is
-//      rounded once to an integer multiple of 1/S and summed exactly, so the
+//      truncated once to an integer multiple of 1/S and summed exactly, so the
 //      tile sums do not depend on the order of the adds;
 //   4  one int64 per complex cell holding two 32-bit fixed-point halves
 //      (re * 2^32 + im, summed exactly as integers): half the atomics for
@@ -139,8 +133,11 @@ __device__ __forceinline__ float fixed_scale(float bound) {
   return ldexpf(1.0f, e);
 }
 
-template <int M, int ACC, int LDSF>
-__global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_deposit_mixed(
+// TX, TR > 0: the tile shape fixed at compile time, so that the current box's
+// pitch and plane strides are constants and every LDS atomic addresses one
+// per-row base register plus an immediate offset; 0: the shape of `tl`.
+template <int M, int ACC, int LDSF, int TX = 0, int TR = 0>
+__global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
     Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start,
     const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
     const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
@@ -151,19 +148,22 @@ __global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_dep
   const int tile = blockIdx.x;
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   Boxes<M> b;
-  b.fx0 = ttx * tl.tx + 1 - FIELD_HALO;
-  b.fr0 = tty * tl.tr + 1 - FIELD_HALO;
-  b.FX = tl.tx + 2 * FIELD_HALO;
-  b.FR = tl.tr + 2 * FIELD_HALO;
-  b.jx0 = ttx * tl.tx + 1 - DEPOSIT_HALO;
-  b.jr0 = tty * tl.tr + 1 - DEPOSIT_HALO;
-  b.LX = tl.tx + 2 * DEPOSIT_HALO;
-  b.LR = tl.tr + 2 * DEPOSIT_HALO;
+  const int tx = TX > 0 ? TX : tl.tx, tr = TR > 0 ? TR : tl.tr;
+  b.fx0 = ttx * tx + 1 - FIELD_HALO;
+  b.fr0 = tty * tr + 1 - FIELD_HALO;
+  b.FX = tx + 2 * FIELD_HALO;
+  b.FR = tr + 2 * FIELD_HALO;
+  b.jx0 = ttx * tx + 1 - DEPOSIT_HALO;
+  b.jr0 = tty * tr + 1 - DEPOSIT_HALO;
+  b.LX = tx + 2 * DEPOSIT_HALO;
+  b.LR = tr + 2 * DEPOSIT_HALO;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
   const int nf = LDSF ? 6 * M * b.FR * b.FX : 0;  // complex cells of the field region
-  // current box: [3][M][LR][LX] complex cells, (re, im) words of FP64 (ACC 2),
-  // int64 (ACC 3), FP32 (ACC 0, 1; ACC 1 adds a second FP32 box for the low
-  // words) or one packed int64 per cell (ACC 4)
+  // current box: planes [3][M] of LR x LX cells; each plane holds the real
+  // words then the imaginary words (FP64 for ACC 2, int64 for ACC 3, FP32 for
+  // ACC 0, 1, where ACC 1 adds a second FP32 box for the low words), or one
+  // packed int64 per cell (ACC 4).  Consecutive cells are consecutive words,
+  // so lanes at consecutive cells hit distinct LDS banks.
   double* jl = lds_raw;
   float* jf = (float*)lds_raw;
   float* jlo = jf + 2 * nj;
@@ -184,13 +184,13 @@ __global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_dep
     for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o));
     if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = wm;
     __syncthreads();
-    wm = wred[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) wm = fmaxf(wm, wred[k]);
+    wm = fmaxf(fmaxf(wred[0], wred[1]), fmaxf(wred[2], wred[3]));
     // ACC 4 sums in 32 bits: one contribution per particle per word
     const float cnt = ACC == 4 ? (float)(p1 - p0) : 1.0f;
     sxr = fixed_scale(9.0f * fabsf(mc.q_fac_idt) * wm * cnt);
     st = fixed_scale(10.5f * fabsf(mc.q_fac_c) * wm * cnt);
   }
+  const float isxr = 1.0f / sxr, ist = 1.0f / st;  // exact: powers of two
   if (LDSF) {
     const cplx<float>* src[6] = {ex, er, et, bx, br, bt};
     for (int e = threadIdx.x; e < nf; e += blockDim.x) {
@@ -388,9 +388,145 @@ __global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_dep
     const int xmin = -1 + (dcx - 1) / 2, xmax = 1 + (dcx + 1) / 2;
     const int ymin = -1 + (dcy - 1) / 2, ymax = 1 + (dcy + 1) / 2;
 
+    // per-mode factors m2, m3, m4 (m = 0: 1, 1/2, 1/3)
+    cplx<float> m2[M], m3[M], m4[M];
+    {
+      m2[0] = mk(1.0f, 0.0f);
+      m3[0] = mk(0.5f, 0.0f);
+      m4[0] = mk(0.33333333f, 0.0f);
+      const cplx<float> e05 = mk(c05, s05);
+      cplx<float> e_m = mk(1.0f, 0.0f);
+#pragma unroll
+      for (int m = 1; m < M; ++m) {
+        e_m = cmul(e_m, e05);
+        cplx<float> f2, f3, f4;
+        mfactors((float)m * dth, f2, f3, f4);
+        const cplx<float> two_e = mk(2.0f * e_m.re, 2.0f * e_m.im);
+        m2[m] = cmul(two_e, f2);
+        m3[m] = cmul(two_e, f3);
+        m4[m] = cmul(two_e, f4);
+      }
+    }
+
     // prefactors in box units (times S for the fixed-point boxes)
     const float fcx = (mc.q_fac_idt * sxr) * wgt;
     const float fcz = ((mc.q_fac_c * st) * wgt) * (ut * ig);
+    // cumulative x sums of hx (jxh of the reference) are mode independent
+    float Hx[5];
+    {
+      float run = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const bool on = (k - 2) >= xmin && (k - 2) <= xmax;
+        run += on ? hx[k] : 0.0f;
+        Hx[k] = run;
+      }
+    }
+    // is the whole footprint inside the LDS current box?
+    const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
+    const int ncell = b.LR * b.LX;
+    const int jb = (cy2 - b.jr0) * b.LX + (cx2 - b.jx0);      // box cell of (cx2, cy2), comp 0, mode 0
+    const int jmode = ACC == 4 ? ncell : 2 * ncell;            // words per mode plane
+    const int jcomp = M * jmode;                               // words per component
+    // real (part 0) or imaginary (part 1) word of the cell at word index c (ACC 0..3)
+    auto lds_add = [&](int c, int part, float v) {
+      const int off = c + part * ncell;
+      if (ACC == 3) {
+        // truncation toward zero: the bias is below 2^-30 of the bound per add
+        atomicAdd(jq + off, (unsigned long long)(long long)(int)v);
+      } else if (ACC == 2) {
+        atomicAdd(jl + off, (double)v);
+      } else if (ACC == 0) {
+        atomicAdd(jf + off, v);
+      } else {
+        // Knuth's TwoSum of the stored value and v; the error goes to the low word
+        const float old = atomicAdd(jf + off, v);
+        const float sum = old + v;
+        const float bp = sum - old;
+        const float err = (old - (sum - bp)) + (v - bp);
+        if (err != 0.0f) atomicAdd(jlo + off, err);
+      }
+    };
+    auto lds_add_c = [&](int c, float re, float im, bool has_im) {
+      if (ACC == 4) {
+        const int qr = (int)re, qi = has_im ? (int)im : 0;
+        atomicAdd(jq + c, ((unsigned long long)(unsigned)qr << 32) + (unsigned long long)(long long)qi);
+      } else {
+        lds_add(c, 0, re);
+        if (has_im) lds_add(c, 1, im);
+      }
+    };
+    // Jx(cx+1, cy) += -fcx w_rt Hx ; Jr(cx, cy+1) += -w_xt R ; Jt(cx, cy) += fcz w_xr,
+    // each still to be multiplied by its row's geometric factor.  The last
+    // column's Jx and the last row's Jr are sums of all the weight changes,
+    // zero by charge conservation (both shapes sum to 2): skipped, as FP32
+    // would only add their round-off.
+    if (jin) {
+      float R = 0.0f;
+      for (int iy = ymin; iy <= ymax; ++iy) {
+        const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
+        R = fmaf(fcx, hyv, R);
+        // row factors: w_rt = m2 gy + m3 hy per mode; fcz gy, fcz hy
+        cplx<float> wrt[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+          wrt[m] = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
+        const float ay = fcz * gyv, by = fcz * hyv;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const int ix = k - 2;
+          if (ix < xmin || ix > xmax) continue;
+          const float gxv = gx[k], hxv = hx[k];
+          const float A = ay * gxv, B = fmaf(by, gxv, ay * hxv), C = by * hxv;
+          const float PX = -fcx * Hx[k];
+          const int c = jb + iy * b.LX + ix;
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
+            const int cm = c + m * jmode;
+            if (ix != xmax) lds_add_c(cm + 1, PX * wrt[m].re, PX * wrt[m].im, m != 0);      // (cx+1, cy)
+            if (iy != ymax) lds_add_c(cm + jcomp + b.LX, -R * w_xt.re, -R * w_xt.im, m != 0);  // (cx, cy+1)
+            lds_add_c(cm + 2 * jcomp, m2[m].re * A + m3[m].re * B + m4[m].re * C,            // (cx, cy)
+                      m2[m].im * A + m3[m].im * B + m4[m].im * C, m != 0);
+          }
+        }
+      }
+    } else {
+      // the footprint leaves the box (the particle drifted while sorts were
+      // skipped): the same sums straight into the global FP32 J
+      float R = 0.0f;
+      for (int iy = ymin; iy <= ymax; ++iy) {
+        const int cy = cy2 + iy;
+        const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
+        R = fmaf(fcx, hyv, R);
+        const float gxf = geo.rt[cy + jng] * isxr, grf = geo.xt[cy + jng] * isxr, gtf = geo.vol[cy + jng] * ist;
+#pragma unroll 1
+        for (int ix = xmin; ix <= xmax; ++ix) {
+          const float gxv = fsel5(gx, ix), hxv = fsel5(hx, ix);
+          const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
+          const float PX = -fcx * fsel5(Hx, ix);
+          const int cx = cx2 + ix;
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
+            const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
+            const cplx<float> vt = mk(m2[m].re * A + m3[m].re * B + m4[m].re * C,
+                                      m2[m].im * A + m3[m].im * B + m4[m].im * C);
+            float* qx = (float*)(jx + g.idx(cx + 1, cy, m));
+            float* qr = (float*)(jr + g.idx(cx, cy + 1, m));
+            float* qt = (float*)(jt + g.idx(cx, cy, m));
+            if (ix != xmax) unsafeAtomicAdd(qx, PX * w_rt.re * gxf);
+            if (iy != ymax) unsafeAtomicAdd(qr, -R * w_xt.re * grf);
+            unsafeAtomicAdd(qt, vt.re * gtf);
+            if (m) {
+              if (ix != xmax) unsafeAtomicAdd(qx + 1, PX * w_rt.im * gxf);
+              if (iy != ymax) unsafeAtomicAdd(qr + 1, -R * w_xt.im * grf);
+              unsafeAtomicAdd(qt + 1, vt.im * gtf);
+            }
+          }
+        }
+      }
+    }
 
     // new position relative to its nearest cell; particle_bcs (boundary.F90:
     // 1541-1889) in cell units: periodic wraps are exact integer shifts
@@ -461,155 +597,12 @@ __global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_dep
       key = ((cyk - 1) / tl.tr) * tl.ntx + (cxk - 1) / tl.tx;
     }
     sp.key[ip] = key;
-
-    // per-mode factors m2, m3, m4 (m = 0: 1, 1/2, 1/3)
-    cplx<float> m2[M], m3[M], m4[M];
-    {
-      m2[0] = mk(1.0f, 0.0f);
-      m3[0] = mk(0.5f, 0.0f);
-      m4[0] = mk(0.33333333f, 0.0f);
-      const cplx<float> e05 = mk(c05, s05);
-      cplx<float> e_m = mk(1.0f, 0.0f);
-#pragma unroll
-      for (int m = 1; m < M; ++m) {
-        e_m = cmul(e_m, e05);
-        cplx<float> f2, f3, f4;
-        mfactors((float)m * dth, f2, f3, f4);
-        const cplx<float> two_e = mk(2.0f * e_m.re, 2.0f * e_m.im);
-        m2[m] = cmul(two_e, f2);
-        m3[m] = cmul(two_e, f3);
-        m4[m] = cmul(two_e, f4);
-      }
-    }
-
-    // cumulative x sums of hx (jxh of the reference) are mode independent
-    float Hx[5];
-    {
-      float run = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        const bool on = (k - 2) >= xmin && (k - 2) <= xmax;
-        run += on ? hx[k] : 0.0f;
-        Hx[k] = run;
-      }
-    }
-    // is the whole footprint inside the LDS current box?
-    const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
-    const int jb = (cy2 - b.jr0) * b.LX + (cx2 - b.jx0);  // box cell of (cx2, cy2), comp 0, mode 0
-    const int jcomp = M * b.LR * b.LX;                    // cells per component
-    const int jmode = b.LR * b.LX;
-    // one real word of box cell c (ACC 0..3)
-    auto lds_add = [&](int c, int part, float v) {
-      const int off = 2 * c + part;
-      if (ACC == 3) {
-        atomicAdd(jq + off, (unsigned long long)(long long)__float2int_rn(v));
-      } else if (ACC == 2) {
-        atomicAdd(jl + off, (double)v);
-      } else if (ACC == 0) {
-        atomicAdd(jf + off, v);
-      } else {
-        // Knuth's TwoSum of the stored value and v; the error goes to the low word
-        const float old = atomicAdd(jf + off, v);
-        const float sum = old + v;
-        const float bp = sum - old;
-        const float err = (old - (sum - bp)) + (v - bp);
-        if (err != 0.0f) atomicAdd(jlo + off, err);
-      }
-    };
-    auto lds_add_c = [&](int c, float re, float im, bool has_im) {
-      if (ACC == 4) {
-        const int qr = __float2int_rn(re), qi = has_im ? __float2int_rn(im) : 0;
-        const unsigned long long q =
-            ((unsigned long long)(unsigned)qr << 32) + (unsigned long long)(long long)qi;
-        atomicAdd(jq + c, q);
-      } else {
-        lds_add(c, 0, re);
-        if (has_im) lds_add(c, 1, im);
-      }
-    };
-    if (jin) {
-      // footprint-aligned weights: column i of the footprint is ix = xmin + i
-      // (xmin is -2 or -1), row j is iy = ymin + j; 3 or 4 of each
-      const bool xs = xmin == -2, ys = ymin == -2;
-      float gxa[4], hxa[4], Hxa[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        gxa[i] = xs ? gx[i] : gx[i + 1];
-        hxa[i] = xs ? hx[i] : hx[i + 1];
-        Hxa[i] = xs ? Hx[i] : Hx[i + 1];
-      }
-      float gy0 = ys ? gy[0] : gy[1], gy1 = ys ? gy[1] : gy[2], gy2 = ys ? gy[2] : gy[3], gy3 = ys ? gy[3] : gy[4];
-      float hy0 = ys ? hy[0] : hy[1], hy1 = ys ? hy[1] : hy[2], hy2 = ys ? hy[2] : hy[3], hy3 = ys ? hy[3] : hy[4];
-      const bool four_x = xmax - xmin == 3;
-      const int nrow = ymax - ymin + 1;
-      float R = 0.0f;
-      int crow = jb + ymin * b.LX + xmin;  // box cell of the row's first column
-      for (int j = 0; j < nrow; ++j) {
-        const float gyv = gy0, hyv = hy0;
-        gy0 = gy1;
-        gy1 = gy2;
-        gy2 = gy3;
-        hy0 = hy1;
-        hy1 = hy2;
-        hy2 = hy3;
-        R = fmaf(fcx, hyv, R);
-        const float PR = -R;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (i == 3 && !four_x) break;
-          const float gxv = gxa[i], hxv = hxa[i];
-          const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
-          const float PX = -fcx * Hxa[i];
-          const int c = crow + i;
-#pragma unroll
-          for (int m = 0; m < M; ++m) {
-            // Jx(cx+1, cy) += -fcx w_rt Hx ; Jr(cx, cy+1) += -w_xt R ; Jt(cx, cy) += fcz w_xr,
-            // each still to be multiplied by its row's geometric factor
-            const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
-            const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
-            const int cm = c + m * jmode;
-            lds_add_c(cm + 1, PX * w_rt.re, PX * w_rt.im, m != 0);                // (cx+1, cy)
-            lds_add_c(cm + jcomp + b.LX, PR * w_xt.re, PR * w_xt.im, m != 0);     // (cx, cy+1)
-            lds_add_c(cm + 2 * jcomp, m2[m].re * A + m3[m].re * B + m4[m].re * C, // (cx, cy)
-                      m2[m].im * A + m3[m].im * B + m4[m].im * C, m != 0);
-          }
-        }
-        crow += b.LX;
-      }
-    } else {
-      // footprint leaves the box: the same sums straight into global J
-      float R = 0.0f;
-      for (int iy = ymin; iy <= ymax; ++iy) {
-        const int cy = cy2 + iy;
-        const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
-        R = fmaf(fcx, hyv, R);
-        const float gxf = geo.rt[cy + jng] / sxr, grf = geo.xt[cy + jng] / sxr, gtf = geo.vol[cy + jng] / st;
-        for (int ix = xmin; ix <= xmax; ++ix) {
-          const float gxv = fsel5(gx, ix), hxv = fsel5(hx, ix);
-          const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
-          const float PX = -fcx * fsel5(Hx, ix);
-          const int cx = cx2 + ix;
-          for (int m = 0; m < M; ++m) {
-            const cplx<float> w_rt = mk(m2[m].re * gyv + m3[m].re * hyv, m2[m].im * gyv + m3[m].im * hyv);
-            const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
-            float* qx = (float*)(jx + g.idx(cx + 1, cy, m));
-            float* qr = (float*)(jr + g.idx(cx, cy + 1, m));
-            float* qt = (float*)(jt + g.idx(cx, cy, m));
-            unsafeAtomicAdd(qx, PX * w_rt.re * gxf);
-            unsafeAtomicAdd(qr, -R * w_xt.re * grf);
-            unsafeAtomicAdd(qt, (m2[m].re * A + m3[m].re * B + m4[m].re * C) * gtf);
-            if (m) {
-              unsafeAtomicAdd(qx + 1, PX * w_rt.im * gxf);
-              unsafeAtomicAdd(qr + 1, -R * w_xt.im * grf);
-              unsafeAtomicAdd(qt + 1, (m2[m].im * A + m3[m].im * B + m4[m].im * C) * gtf);
-            }
-          }
-        }
-      }
-    }
   }
   __syncthreads();
+  const int ncell = b.LR * b.LX;
   for (int e = threadIdx.x; e < nj; e += blockDim.x) {
+    // complex cell e = plane * ncell + cell; real and imaginary words
+    const int wr = ACC == 4 ? e : e + (e / ncell) * ncell, wi = wr + ncell;
     double re, imv;
     if (ACC == 4) {
       const unsigned long long q = jq[e];
@@ -617,17 +610,17 @@ __global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_dep
       re = (double)((long long)(q - (unsigned long long)(long long)lo) >> 32);
       imv = (double)lo;
     } else if (ACC == 3) {
-      re = (double)(long long)jq[2 * e];
-      imv = (double)(long long)jq[2 * e + 1];
+      re = (double)(long long)jq[wr];
+      imv = (double)(long long)jq[wi];
     } else if (ACC == 2) {
-      re = jl[2 * e];
-      imv = jl[2 * e + 1];
+      re = jl[wr];
+      imv = jl[wi];
     } else if (ACC == 0) {
-      re = jf[2 * e];
-      imv = jf[2 * e + 1];
+      re = jf[wr];
+      imv = jf[wi];
     } else {
-      re = (double)jf[2 * e] + (double)jlo[2 * e];
-      imv = (double)jf[2 * e + 1] + (double)jlo[2 * e + 1];
+      re = (double)jf[wr] + (double)jlo[wr];
+      imv = (double)jf[wi] + (double)jlo[wi];
     }
     if (re == 0.0 && imv == 0.0) continue;
     int rem = e;
@@ -653,15 +646,12 @@ __global__ __launch_bounds__(PUSH_MIXED_THREADS) PUSH_MIXED_ATTR void k_push_dep
 
 // ------------------------------------------------------------------------
 // key of an uploaded particle (no push)
-__global__ void k_keys_mixed(Grid g, Tiles tl, MixConst mc, PartMixed sp, long n) {
+__global__ void k_keys_mixed(Grid g, Tiles tl, MixConst mc, PartMixed sp, long n, int cell_key) {
   const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= n) return;
   const float y = sp.y[ip], z = sp.z[ip];
-  int cyk = (int)floorf(sqrtf(y * y + z * z) - mc.yoff + 1.5f);
-  int cxk = sp.cell[ip];
-  cxk = cxk < 1 ? 1 : (cxk > g.nx ? g.nx : cxk);
-  cyk = cyk < 1 ? 1 : (cyk > g.ny ? g.ny : cyk);
-  sp.key[ip] = ((cyk - 1) / tl.tr) * tl.ntx + (cxk - 1) / tl.tx;
+  const int cyk = (int)floorf(sqrtf(y * y + z * z) - mc.yoff + 1.5f);
+  sp.key[ip] = tl.key(sp.cell[ip], cyk, g.nx, g.ny, cell_key != 0);
 }
 
 // FP64 SI particles -> mixed representation
@@ -727,9 +717,10 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
   const size_t lds = push_lds_bytes_mixed(g, tl, acc, ldsf);
   MixConst mc = mix_const(g, pc, charge);
   GeoF gf{geo, geo + ntab, geo + 2 * ntab, geo + 3 * ntab};
-#define LAUNCH(MM, AA, LL)                                                                                  \
-  hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL>), dim3(tl.ntiles), dim3(PUSH_MIXED_THREADS), lds, s, g, tl, mc, sp, \
-                     tile_start, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), gf, err)
+#define LAUNCH_T(MM, AA, LL, XX, RR)                                                                    \
+  hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL, XX, RR>), dim3(tl.ntiles), dim3(256), lds, s, g, tl, mc, \
+                     sp, tile_start, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), gf, err)
+#define LAUNCH(MM, AA, LL) LAUNCH_T(MM, AA, LL, 0, 0)
 #define BY_MODE(AA, LL)            \
   switch (g.M) {                   \
     case 1: LAUNCH(1, AA, LL); break; \
@@ -737,6 +728,22 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
     case 3: LAUNCH(3, AA, LL); break; \
     case 4: LAUNCH(4, AA, LL); break; \
     default: return -1;            \
+  }
+#define BY_MODE_T(AA, XX, RR)                     \
+  switch (g.M) {                                  \
+    case 1: LAUNCH_T(1, AA, 0, XX, RR); break;    \
+    case 2: LAUNCH_T(2, AA, 0, XX, RR); break;    \
+    case 3: LAUNCH_T(3, AA, 0, XX, RR); break;    \
+    case 4: LAUNCH_T(4, AA, 0, XX, RR); break;    \
+    default: return -1;                           \
+  }
+  // compile-time tile shapes for the fixed-point accumulators with L1/L2 gather
+  if ((acc == 3 || acc == 4) && !ldsf) {
+    const int shape = tl.tx == 32 && tl.tr == 4 ? 1 : (tl.tx == 16 && tl.tr == 8 ? 2 : 0);
+    if (shape == 1 && acc == 3) { BY_MODE_T(3, 32, 4); return 0; }
+    if (shape == 1 && acc == 4) { BY_MODE_T(4, 32, 4); return 0; }
+    if (shape == 2 && acc == 3) { BY_MODE_T(3, 16, 8); return 0; }
+    if (shape == 2 && acc == 4) { BY_MODE_T(4, 16, 8); return 0; }
   }
   const int sel = acc * 2 + (ldsf ? 1 : 0);
   switch (sel) {
@@ -752,16 +759,19 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
     case 9: BY_MODE(4, 1); break;
     default: return -1;
   }
+#undef BY_MODE_T
 #undef BY_MODE
 #undef LAUNCH
+#undef LAUNCH_T
   return 0;
 }
 
-void launch_keys_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, long n, hipStream_t s) {
+void launch_keys_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, long n, int cell_key, hipStream_t s) {
   if (n <= 0) return;
   PushConst pc{};
   MixConst mc = mix_const(g, pc, 0.0);
-  hipLaunchKernelGGL(k_keys_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, mc, sp, n);
+  hipLaunchKernelGGL(k_keys_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, tl, mc, sp, n,
+                     cell_key);
 }
 
 void launch_to_mixed(const Grid& g, double mass, const double* const* src, const PartMixed& sp, long n,

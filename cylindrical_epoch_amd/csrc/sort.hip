@@ -21,7 +21,6 @@ constexpr int IPT = 16;                       // keys per thread
 constexpr int SORT_TILE = SORT_BLOCK * IPT;   // 4096 keys per block
 constexpr int WAVES = SORT_BLOCK / 64;
 constexpr int WAVE_ITEMS = SORT_TILE / WAVES; // 1024 keys per wavefront
-constexpr long INTERLEAVE_COLUMNS = 64;
 
 // composite key of element i: pass 0 builds it from the tile key
 __device__ __forceinline__ uint32_t comp_key(const uint32_t* __restrict__ keys, long i, int compose_bits) {
@@ -238,23 +237,35 @@ void launch_tile_bounds(const uint32_t* sorted_keys, long n, int ntiles, int* ti
                      ntiles, tile_start);
 }
 
+__global__ void k_div_keys(uint32_t* __restrict__ key, long n, uint32_t div) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] /= div;
+}
+
+void launch_div_keys(uint32_t* key, long n, uint32_t div, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_div_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, key, n, div);
+}
+
 struct Ptr8 {
   const void* src[8];
   void* dst[8];
 };
 
-// Destination of sorted index i.  With `inter` set, the particles of each
-// tile segment [s, s+L) are transposed as an S-column table: the one at
+// Destination of sorted index i.  With `cells` > 0 (the cells of a tile),
+// the particles of each tile segment [s, s+L) are transposed as an S-column
+// table, S = max(1, L / cells) the tile's mean particles per cell: the one at
 // segment offset j goes to column j mod S, row j / S, columns laid out one
 // after another (ragged: the first L mod S columns hold one more row).
 // Consecutive destinations then come from positions S apart in the stable
-// order, i.e. (for a cell-grouped load) from different cells.
+// order, i.e. (for a cell-grouped load) from consecutive cells, which puts
+// the 64 lanes of a wavefront on 64 different cells whose current-box words
+// are consecutive.
 __device__ __forceinline__ long dest_of(long i, const uint32_t* __restrict__ key, const int* __restrict__ ts,
-                                        int inter) {
-  if (!inter) return i;
+                                        int cells) {
+  if (cells <= 0) return i;
   const uint32_t t = key[i];
   const long s = ts[t], L = (long)ts[t + 1] - s, j = i - s;
-  const long S = INTERLEAVE_COLUMNS, A = L / S, R = L % S;
+  const long S = L / cells > 1 ? L / cells : 1, A = L / S, R = L % S;
   const long b = j % S, a = j / S;
   return s + b * A + (b < R ? b : R) + a;
 }

@@ -77,11 +77,11 @@ def test_current_finish_after_push(oracle):
     sim.close()
 
 
-def interleave_within_tiles(order, tiles, columns=64):
+def interleave_within_tiles(order, tiles, cells):
     """The upload's per-tile interleave: the particles of a tile segment of
-    length L (stable order) are laid out as a table of `columns` columns,
-    offset j -> column j % columns, row j // columns, columns one after the
-    other (the first L % columns columns one row longer)."""
+    length L (stable order) are laid out as a table of S = max(1, L // cells)
+    columns (`cells` = cells per tile), offset j -> column j % S, row j // S,
+    columns one after the other (the first L % S columns one row longer)."""
     out = np.empty_like(order)
     i = 0
     n = len(order)
@@ -91,6 +91,7 @@ def interleave_within_tiles(order, tiles, columns=64):
         while e < n and tiles[e] == t:
             e += 1
         L = e - i
+        columns = max(1, L // cells)
         A, R = divmod(L, columns)
         for j in range(L):
             b, a = j % columns, j // columns
@@ -101,17 +102,18 @@ def interleave_within_tiles(order, tiles, columns=64):
 
 @pytest.mark.parametrize("tile", [(4, 4), (16, 8), (7, 3)])
 def test_sort_matches_stable_cpu_sort(oracle, tile):
-    """The upload sorts particles by tile key (stable) and then interleaves
-    each tile's particles; the downloaded order must equal that permutation
-    of the uploaded order computed on the CPU (bit-exact)."""
+    """The upload sorts particles by their tile-major cell key (stable) and
+    then interleaves each tile's particles; the downloaded order must equal
+    that permutation of the uploaded order computed on the CPU (bit-exact)."""
     log("sort tile", tile)
     nx, ny, dx, dy = 48, 20, 1e-7, 1e-7
     st, sim = make_pair(oracle, nx, ny, 1, dx, dy, tile=tile)
     p = add_plasma(st, sim, nx, ny, dx, dy, ppc=40, temp_ev=100.0, seed=11, shuffle=True)
-    keys = oracle.cell_keys(st, p["x"], p["y"], p["z"], tile[0], tile[1]) // (tile[0] * tile[1])
+    cells = tile[0] * tile[1]
+    keys = oracle.cell_keys(st, p["x"], p["y"], p["z"], tile[0], tile[1])
     ntiles = ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
-    perm = oracle.stable_sort(keys, ntiles)
-    want = interleave_within_tiles(p["id"][perm], keys[perm])
+    perm = oracle.stable_sort(keys, ntiles * cells)
+    want = interleave_within_tiles(p["id"][perm], keys[perm] // cells, cells)
     got = sim.download_particles(0)
     assert np.array_equal(got["id"], want)
     # a second, plain sort keeps the order (no particle moved)
