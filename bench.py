@@ -171,6 +171,20 @@ def cpu_baseline(w, sample_cols):
             "push_particle_steps_per_s": round(ns / (t2 - t1), 1)}
 
 
+def copy_calibration():
+    """A plain device copy of 2^28 float32 (1 GiB read + 1 GiB written per
+    launch), three launches."""
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    x.fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 1 << 30, "copy_bytes_written_per_launch": 1 << 30,
+                      "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,14 +192,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--workload", default=WORKLOAD, choices=sorted(deck.WORKLOADS))
-    ap.add_argument("--cpu-sample-cols", type=int, default=64)
+    ap.add_argument("--cpu-sample-cols", type=int, default=2048,
+                    help="x-columns of particles the CPU baseline pushes (2048: ~33.5M particles, ~10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-mixed", default=None, help="tile of the mixed path, e.g. 32x4 (library default)")
     ap.add_argument("--tile-fp64", default=None, help="tile of the FP64 path, e.g. 16x8 (library default)")
     ap.add_argument("--acc-mode", type=int, default=None, help="mixed current accumulator 0..4")
     ap.add_argument("--gather-lds", type=int, default=None, help="mixed gather from LDS (1) or L1/L2 (0)")
     ap.add_argument("--sort-method", default=None, choices=["radix", "migrate"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (the reference kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)

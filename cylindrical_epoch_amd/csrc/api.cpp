@@ -50,6 +50,9 @@ struct Species {
   int* d_count = nullptr;         // == tile_start + ntiles
   bool dirty = false;             // pushed since the last sort (removed particles may remain)
   bool ranged = false;            // tile_start describes the tile ranges of the keys' last sort
+  uint32_t* mig_counts = nullptr; // [ntiles][MIG_DIRS] direction counts for the migration sort
+  int* mig_flag = nullptr;        // 2 ints: farther move seen, particles kept
+  bool counted = false;           // the last push left fresh direction counts
 };
 
 struct ProfEntry {
@@ -82,12 +85,12 @@ struct cylpic_ctx {
   int sort_interval = 1;
   int slot_bits = 0;
   uint32_t* mig = nullptr;  // migration sort scratch (mig_scratch_words(ntiles))
-  int* mig_flag = nullptr;  // 2 ints: farther move seen, particles alive
-  int* mig_host = nullptr;  // pinned copy of mig_flag
+  int* mig_host = nullptr;  // pinned copy of a species' mig_flag
   bool migrate = true;      // use the neighbour migration sort when it applies
   long sort_radix = 0, sort_migrate = 0;
   int acc_mode = 3;    // mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64, 3 int64, 4 2x32 packed)
-  int gather_lds = 0;  // mixed path: gather from the LDS-staged field region (1) or L1/L2 (0)
+  int gather_lds = 1;  // mixed path: gather from the LDS-staged field region (1) or L1/L2 (0)
+  int lds_max = 65536;  // LDS bytes per workgroup on the device
   long push_count = 0;
   bool sorted_once = false;
   bool profiling = false;
@@ -240,9 +243,10 @@ int alloc_fields(cylpic_ctx* h) {
   return CYLPIC_OK;
 }
 
+// Tile shape of the push; the handle keeps its previous shape on failure.
 int set_tiles(cylpic_ctx* h, int tx, int tr) {
   if (tx < 1 || tr < 1) return fail(CYLPIC_EINVAL, "tile sizes must be >= 1");
-  Tiles& t = h->tl;
+  Tiles t;
   t.tx = tx;
   t.tr = tr;
   t.ntx = (h->g.nx + tx - 1) / tx;
@@ -250,10 +254,12 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   t.ntiles = t.ntx * t.ntr;
   size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t, h->acc_mode, h->gather_lds)
                                       : push_lds_bytes(h->g, t, 0);
-  // 64 KB per workgroup, less the push kernels' 64 bytes of static LDS
-  if (lds > 64 * 1024 - 64)
-    return fail(CYLPIC_EINVAL, "tile too large: the push needs " + std::to_string(lds) +
-                                   " bytes of LDS (limit 65472)");
+  // the device's LDS per workgroup, less the push kernels' 64 bytes of static LDS
+  const size_t limit = (size_t)h->lds_max - 64;
+  if (lds > limit)
+    return fail(CYLPIC_EINVAL, "tile too large: the push needs " + std::to_string(lds) + " bytes of LDS (limit " +
+                                   std::to_string(limit) + ")");
+  h->tl = t;
   return CYLPIC_OK;
 }
 
@@ -310,18 +316,21 @@ PartF64 view64(Species& s) {
 }
 
 // Neighbour migration sort of a species whose keys were tile-sorted before
-// the last push.  *done = false (nothing changed) when some particle moved
-// beyond the 3 x 3 tile neighbourhood.  Synchronises once (reads the flags).
-int migrate_species(cylpic_ctx* h, Species& s, bool* done) {
+// the last push.  `counted`: that push left the direction counts (else they
+// are counted here).  *done = false (nothing changed) when some particle
+// moved beyond the 3 x 3 tile neighbourhood.  Synchronises once (reads the
+// flags).
+int migrate_species(cylpic_ctx* h, Species& s, bool counted, bool* done) {
   *done = false;
   if (!h->mig) {
     HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
-    HIPCHK(hipMalloc((void**)&h->mig_flag, 2 * sizeof(int)));
     HIPCHK(hipHostMalloc((void**)&h->mig_host, 2 * sizeof(int), 0));
   }
-  HIPCHK(hipMemsetAsync(h->mig_flag, 0, 2 * sizeof(int), h->stream));
-  launch_mig_count(s.key[0], s.tile_start, h->tl, h->mig, h->mig_flag, h->stream);
-  HIPCHK(hipMemcpyAsync(h->mig_host, h->mig_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (!counted) {
+    HIPCHK(hipMemsetAsync(s.mig_flag, 0, 2 * sizeof(int), h->stream));
+    launch_mig_count(s.key[0], s.tile_start, h->tl, MigCount{s.mig_counts, s.mig_flag}, h->stream);
+  }
+  HIPCHK(hipMemcpyAsync(h->mig_host, s.mig_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   if (h->mig_host[0]) return CYLPIC_OK;
   const bool f64 = h->prec == CYLPIC_FP64;
@@ -332,7 +341,8 @@ int migrate_species(cylpic_ctx* h, Species& s, bool* done) {
     src[k] = f64 ? (const void*)s.a[k] : (const void*)s.ma[k];
     dst[k] = f64 ? (void*)s.b[k] : (void*)s.mb[k];
   }
-  launch_mig_scatter(s.key[0], s.tile_start, h->tl, h->mig, s.tile_start_alt, f64 ? 8 : 4, src, dst, narr, s.key[1],
+  launch_mig_scatter(s.key[0], s.tile_start, h->tl, s.mig_counts, h->mig, s.tile_start_alt, f64 ? 8 : 4, src, dst,
+                     narr, s.key[1],
                      s.ids ? (const uint64_t*)s.id[0] : nullptr, s.ids ? (uint64_t*)s.id[1] : nullptr, h->stream);
   HIPCHK(hipGetLastError());
   for (int k = 0; k < narr; ++k) {
@@ -356,6 +366,8 @@ int migrate_species(cylpic_ctx* h, Species& s, bool* done) {
 int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   Scope sc(h, "sort");
   s.dirty = false;
+  const bool counted = s.counted;
+  s.counted = false;  // the counts describe the keys of the tile ranges before this sort
   if (s.n <= 0) {
     launch_tile_bounds(nullptr, 0, h->tl.ntiles, s.tile_start, h->stream);
     s.ranged = true;
@@ -363,7 +375,7 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   }
   if (s.ranged && !interleave && h->migrate) {
     bool done = false;
-    int rc = migrate_species(h, s, &done);
+    int rc = migrate_species(h, s, counted, &done);
     if (rc) return rc;
     if (done) {
       h->sort_migrate += 1;
@@ -448,6 +460,8 @@ void free_all(cylpic_ctx* h) {
     if (s.key[1]) (void)hipFree(s.key[1]);
     if (s.tile_start) (void)hipFree(s.tile_start);
     if (s.tile_start_alt) (void)hipFree(s.tile_start_alt);
+    if (s.mig_counts) (void)hipFree(s.mig_counts);
+    if (s.mig_flag) (void)hipFree(s.mig_flag);
   }
   if (h->geo) (void)hipFree(h->geo);
   if (h->geof) (void)hipFree(h->geof);
@@ -457,7 +471,6 @@ void free_all(cylpic_ctx* h) {
   }
   if (h->d_err) (void)hipFree(h->d_err);
   if (h->mig) (void)hipFree(h->mig);
-  if (h->mig_flag) (void)hipFree(h->mig_flag);
   if (h->mig_host) (void)hipHostFree(h->mig_host);
   if (h->d_energy) (void)hipFree(h->d_energy);
   if (h->sc_mem) (void)hipFree(h->sc_mem);
@@ -491,6 +504,8 @@ int alloc_species_arrays(cylpic_ctx* h, Species& s) {
   HIPCHK(hipMalloc((void**)&s.tile_start, (h->tl.ntiles + 2) * sizeof(int)));
   HIPCHK(hipMemsetAsync(s.tile_start, 0, (h->tl.ntiles + 2) * sizeof(int), h->stream));
   HIPCHK(hipMalloc((void**)&s.tile_start_alt, (h->tl.ntiles + 2) * sizeof(int)));
+  HIPCHK(hipMalloc((void**)&s.mig_counts, (size_t)h->tl.ntiles * MIG_DIRS * sizeof(uint32_t)));
+  HIPCHK(hipMalloc((void**)&s.mig_flag, 2 * sizeof(int)));
   s.d_count = s.tile_start + h->tl.ntiles;
   return CYLPIC_OK;
 }
@@ -572,12 +587,14 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
   if (hipSetDevice(device) != hipSuccess) return bail(fail(CYLPIC_EHIP, "hipSetDevice failed"));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(CYLPIC_EHIP, "stream creation failed"));
+  if (hipDeviceGetAttribute(&h->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+    return bail(fail(CYLPIC_EHIP, "LDS size query failed"));
   int rc = alloc_fields(h);
   if (rc) return bail(rc);
   {
-    // default tile: the first of these whose LDS needs fit in 64 KB (the
-    // fastest measured first: 32 x 4 for the mixed path, 16 x 8 for FP64)
-    const int cand_m[][2] = {{32, 4}, {16, 8}, {16, 4}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}};
+    // default tile: the first of these whose LDS needs fit (the fastest
+    // measured first: 16 x 4 for the mixed path, 16 x 8 for FP64)
+    const int cand_m[][2] = {{16, 4}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}, {1, 1}, {1, 1}};
     const int cand_d[][2] = {{16, 8}, {16, 4}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}, {1, 1}};
     rc = CYLPIC_EINVAL;
     for (auto& c : (precision == CYLPIC_MIXED ? cand_m : cand_d)) {
@@ -732,6 +749,7 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
   }
   s.n = n;
   s.ranged = false;
+  s.counted = false;
   // tile-major cell keys: the upload sort groups each tile's particles by cell
   if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, 1, h->stream);
   else launch_keys_mixed(h->g, h->tl, viewm(s), n, 1, h->stream);
@@ -887,12 +905,21 @@ int cylpic_push_particles(cylpic_ctx* h) {
     pc.idy = 1.0 / h->g.dy;
     pc.idt = 1.0 / dt;
     pc.dtco2 = C_LIGHT * (dt / 2.0);
+    // the push counts its tiles' new keys by direction for the next
+    // migration sort (the tile ranges are those of the last sort)
+    MigCount migc{nullptr, nullptr};
+    if (h->migrate && s.ranged) {
+      HIPCHK(hipMemsetAsync(s.mig_flag, 0, 2 * sizeof(int), h->stream));
+      migc = MigCount{s.mig_counts, s.mig_flag};
+    }
     Scope sc(h, "push_deposit");
     s.dirty = true;
+    s.counted = migc.counts != nullptr;
     if (h->prec == CYLPIC_FP64) {
-      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->d_err, h->stream);
+      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->d_err, migc,
+                              h->stream);
     } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, h->f32, h->geof, h->ntab,
-                                         h->acc_mode, h->gather_lds, h->d_err, h->stream)) {
+                                         h->acc_mode, h->gather_lds, h->d_err, migc, h->stream)) {
       return fail(CYLPIC_EINVAL, "mixed push: unsupported mode count");
     }
   }

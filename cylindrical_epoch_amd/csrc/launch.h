@@ -77,15 +77,24 @@ void launch_d2f(const cplx<double>* src, cplx<float>* dst, long n, hipStream_t);
 void launch_f2d(const cplx<float>* src, cplx<double>* dst, long n, hipStream_t);
 
 // push.hip
+// Neighbour-migration direction counts a push leaves for the next sort
+// (counts: [ntiles][MIG_DIRS]; flag: 2 ints zeroed before the push); a null
+// counts pointer skips the counting.
+constexpr int MIG_DIRS = 10;
+constexpr int MIG_REMOVED = 9;
+struct MigCount {
+  uint32_t* counts;
+  int* flag;
+};
 void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const PartF64&,
-                             const int* tile_start, FieldSet<double>&, const Geo&, int* err,
+                             const int* tile_start, FieldSet<double>&, const Geo&, int* err, MigCount,
                              hipStream_t);
 // key of every particle: its tile, or with cell_key its tile-major cell (Tiles::key)
 void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, int cell_key, hipStream_t);
 // push_mixed.hip
 int launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, double charge, const PartMixed&,
                               const int* tile_start, FieldSet<float>&, const float* geo, int ntab, int acc,
-                              int ldsf, int* err, hipStream_t);
+                              int ldsf, int* err, MigCount, hipStream_t);
 void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, int cell_key, hipStream_t);
 void launch_to_mixed(const Grid&, double mass, const double* const* src, const PartMixed&, long n, hipStream_t);
 void launch_from_mixed(const Grid&, double mass, const PartMixed&, double* const* dst, long n, hipStream_t);
@@ -120,18 +129,17 @@ void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_
                       const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
 
 // Neighbour migration sort (keys tile-sorted before the last push).
-// scratch: mig_scratch_words(ntiles) words; flag: 2 ints, zeroed by the
-// caller: flag[0] != 0 when a particle's tile moved beyond the 3 x 3
-// neighbourhood (then sort with radix_sort), flag[1] = particles alive.
-constexpr int MIG_DIRS = 10;
-constexpr int MIG_REMOVED = 9;
+// launch_mig_count fills the direction counts (as the push kernels do);
+// mc.flag must be zeroed first.  mc.flag[0] != 0: a particle's tile moved
+// beyond the 3 x 3 neighbourhood (sort with radix_sort instead).
+// launch_mig_scatter then writes the sorted payload dst/key_out/id_out (ids
+// optional) and tile_start_new (ntiles + 2 ints); scratch holds
+// mig_scratch_words(ntiles) words.
 size_t mig_scratch_words(int ntiles);
-void launch_mig_count(const uint32_t* key, const int* tile_start, const Tiles& tl, uint32_t* scratch, int* flag,
-                      hipStream_t);
-// Sorted payload dst/key_out/id_out (ids optional) and tile_start_new
-// (ntiles + 2 ints) from the counts of launch_mig_count.
-void launch_mig_scatter(const uint32_t* key, const int* tile_start, const Tiles& tl, uint32_t* scratch,
-                        int* tile_start_new, int word_bytes, const void* const* src, void* const* dst, int narr,
-                        uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out, hipStream_t);
+void launch_mig_count(const uint32_t* key, const int* tile_start, const Tiles& tl, MigCount mc, hipStream_t);
+void launch_mig_scatter(const uint32_t* key, const int* tile_start, const Tiles& tl, const uint32_t* counts,
+                        uint32_t* scratch, int* tile_start_new, int word_bytes, const void* const* src,
+                        void* const* dst, int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out,
+                        hipStream_t);
 
 }  // namespace cylpic

@@ -14,6 +14,7 @@
 // and stays in L1/L2 across the tile's particles.
 #include "cplx_ops.h"
 #include "launch.h"
+#include "migrate.h"
 
 namespace cylpic {
 
@@ -152,8 +153,9 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
     const cplx<double>* __restrict__ et, const cplx<double>* __restrict__ bx,
     const cplx<double>* __restrict__ br, const cplx<double>* __restrict__ bt,
     cplx<double>* __restrict__ jx, cplx<double>* __restrict__ jr, cplx<double>* __restrict__ jt, Geo geo,
-    int* __restrict__ err) {
+    int* __restrict__ err, MigCount migc) {
   extern __shared__ double lds[];
+  __shared__ uint32_t migs[MIG_DIRS];
   const int tile = blockIdx.x;
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   LdsBox box;
@@ -439,6 +441,8 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
     if (re != 0.0) unsafeAtomicAdd(p, re);
     if (imv != 0.0) unsafeAtomicAdd(p + 1, imv);
   }
+  // direction counts of the tile's new keys for the next migration sort
+  if (migc.counts) mig_count_tile((const uint32_t*)sp.key, tile, p0, p1, tl.ntx, tl.ntiles, migc, migs);
 }
 
 __global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n, int cell_key) {
@@ -458,10 +462,10 @@ size_t push_lds_bytes(const Grid& g, const Tiles& tl, int precision) {
 
 void launch_push_deposit_f64(const Grid& g, const Tiles& tl, const PushConst& pc, const PartF64& sp,
                              const int* tile_start, FieldSet<double>& F, const Geo& geo, int* err,
-                             hipStream_t s) {
+                             MigCount migc, hipStream_t s) {
   size_t lds = push_lds_bytes(g, tl, 0);
   hipLaunchKernelGGL(k_push_deposit_f64, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
-                     F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err);
+                     F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err, migc);
 }
 
 void launch_keys_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, int cell_key, hipStream_t s) {

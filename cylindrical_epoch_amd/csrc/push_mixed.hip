@@ -26,16 +26,17 @@
 // global memory directly.
 #include "cplx_ops.h"
 #include "launch.h"
+#include "migrate.h"
 
 namespace cylpic {
 
 constexpr int FIELD_HALO = 2;
+// Current box border: a particle whose nearest cell c is in the tile deposits
+// into cells c-3 .. c+2 (the half-step cell is c-1 or c, the footprint spans
+// -2..+2 around it, and the always-zero last Jx column and Jr row are not
+// added), so the box reaches 3 cells below the tile and 2 above.
+constexpr int BOX_LO = 3, BOX_HI = 2;
 
-// occupancy hint of the push kernel (register budget), e.g.
-// __attribute__((amdgpu_waves_per_eu(3)))
-#ifndef PUSH_MIXED_ATTR
-#define PUSH_MIXED_ATTR
-#endif
 
 struct MixConst {
   float kx, ky;           // c dt / (2 dx), c dt / (2 dy): half-step move in cells per unit u/gamma
@@ -137,14 +138,15 @@ __device__ __forceinline__ float fixed_scale(float bound) {
 // pitch and plane strides are constants and every LDS atomic addresses one
 // per-row base register plus an immediate offset; 0: the shape of `tl`.
 template <int M, int ACC, int LDSF, int TX = 0, int TR = 0>
-__global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
+__global__ __launch_bounds__(256) void k_push_deposit_mixed(
     Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start,
     const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
     const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
     cplx<float>* __restrict__ jx, cplx<float>* __restrict__ jr, cplx<float>* __restrict__ jt, GeoF geo,
-    int* __restrict__ err) {
+    int* __restrict__ err, MigCount migc) {
   extern __shared__ double lds_raw[];
   __shared__ float wred[8];
+  __shared__ uint32_t migs[MIG_DIRS];
   const int tile = blockIdx.x;
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   Boxes<M> b;
@@ -153,10 +155,10 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
   b.fr0 = tty * tr + 1 - FIELD_HALO;
   b.FX = tx + 2 * FIELD_HALO;
   b.FR = tr + 2 * FIELD_HALO;
-  b.jx0 = ttx * tx + 1 - DEPOSIT_HALO;
-  b.jr0 = tty * tr + 1 - DEPOSIT_HALO;
-  b.LX = tx + 2 * DEPOSIT_HALO;
-  b.LR = tr + 2 * DEPOSIT_HALO;
+  b.jx0 = ttx * tx + 1 - BOX_LO;
+  b.jr0 = tty * tr + 1 - BOX_LO;
+  b.LX = tx + BOX_LO + BOX_HI;
+  b.LR = tr + BOX_LO + BOX_HI;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
   const int nf = LDSF ? 6 * M * b.FR * b.FX : 0;  // complex cells of the field region
   // current box: planes [3][M] of LR x LX cells; each plane holds the real
@@ -192,7 +194,6 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
   }
   const float isxr = 1.0f / sxr, ist = 1.0f / st;  // exact: powers of two
   if (LDSF) {
-    const cplx<float>* src[6] = {ex, er, et, bx, br, bt};
     for (int e = threadIdx.x; e < nf; e += blockDim.x) {
       int rem = e;
       const int fx = rem % b.FX;
@@ -203,7 +204,8 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
       const int c = rem / M;
       const int cx = b.fx0 + fx, cy = b.fr0 + fr;
       cplx<float> v = mk(0.0f, 0.0f);
-      if (cx >= 1 - g.ng && cx <= g.nx + g.ng && cy >= 1 - g.ng && cy <= g.ny + g.ng) v = src[c][g.idx(cx, cy, m)];
+      const cplx<float>* src = c == 0 ? ex : (c == 1 ? er : (c == 2 ? et : (c == 3 ? bx : (c == 4 ? br : bt))));
+      if (cx >= 1 - g.ng && cx <= g.nx + g.ng && cy >= 1 - g.ng && cy <= g.ny + g.ng) v = src[g.idx(cx, cy, m)];
       fl[e] = v;
     }
   }
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
       }
     }
     // is the whole footprint inside the LDS current box?
-    const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 3) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 3) < b.jr0 + b.LR;
+    const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 2) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 2) < b.jr0 + b.LR;
     const int ncell = b.LR * b.LX;
     const int jb = (cy2 - b.jr0) * b.LX + (cx2 - b.jx0);      // box cell of (cx2, cy2), comp 0, mode 0
     const int jmode = ACC == 4 ? ncell : 2 * ncell;            // words per mode plane
@@ -500,11 +502,13 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
         const float gyv = fsel5(gy, iy), hyv = fsel5(hy, iy);
         R = fmaf(fcx, hyv, R);
         const float gxf = geo.rt[cy + jng] * isxr, grf = geo.xt[cy + jng] * isxr, gtf = geo.vol[cy + jng] * ist;
+        float hrun = 0.0f;  // Hx at column ix: running sum of hx over the footprint
 #pragma unroll 1
         for (int ix = xmin; ix <= xmax; ++ix) {
           const float gxv = fsel5(gx, ix), hxv = fsel5(hx, ix);
+          hrun += hxv;
           const float A = fcz * (gxv * gyv), B = fcz * (gxv * hyv + hxv * gyv), C = fcz * (hxv * hyv);
-          const float PX = -fcx * fsel5(Hx, ix);
+          const float PX = -fcx * hrun;
           const int cx = cx2 + ix;
 #pragma unroll
           for (int m = 0; m < M; ++m) {
@@ -642,6 +646,8 @@ __global__ __launch_bounds__(256) PUSH_MIXED_ATTR void k_push_deposit_mixed(
     if (re != 0.0) unsafeAtomicAdd(p, (float)(re * fac));
     if (imv != 0.0) unsafeAtomicAdd(p + 1, (float)(imv * fac));
   }
+  // direction counts of the tile's new keys for the next migration sort
+  if (migc.counts) mig_count_tile((const uint32_t*)sp.key, tile, p0, p1, tl.ntx, tl.ntiles, migc, migs);
 }
 
 // ------------------------------------------------------------------------
@@ -690,7 +696,7 @@ __global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, doubl
 
 // ------------------------------------------------------------------------
 size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl, int acc, int ldsf) {
-  const size_t nj = 3 * (size_t)g.M * (tl.tr + 2 * DEPOSIT_HALO) * (tl.tx + 2 * DEPOSIT_HALO);
+  const size_t nj = 3 * (size_t)g.M * (tl.tr + BOX_LO + BOX_HI) * (tl.tx + BOX_LO + BOX_HI);
   const size_t nf = ldsf ? 6 * (size_t)g.M * (tl.tr + 2 * FIELD_HALO) * (tl.tx + 2 * FIELD_HALO) : 0;
   const size_t jwords = (acc == 0 || acc == 4) ? 2 * nj : 4 * nj;
   return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(cplx<float>);
@@ -713,13 +719,20 @@ MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
 
 int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& pc, double charge,
                               const PartMixed& sp, const int* tile_start, FieldSet<float>& F,
-                              const float* geo, int ntab, int acc, int ldsf, int* err, hipStream_t s) {
+                              const float* geo, int ntab, int acc, int ldsf, int* err, MigCount migc,
+                              hipStream_t s) {
   const size_t lds = push_lds_bytes_mixed(g, tl, acc, ldsf);
   MixConst mc = mix_const(g, pc, charge);
   GeoF gf{geo, geo + ntab, geo + 2 * ntab, geo + 3 * ntab};
 #define LAUNCH_T(MM, AA, LL, XX, RR)                                                                    \
-  hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL, XX, RR>), dim3(tl.ntiles), dim3(256), lds, s, g, tl, mc, \
-                     sp, tile_start, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), gf, err)
+  do {                                                                                                  \
+    if (lds > 65536)                                                                                    \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_mixed<MM, AA, LL, XX, RR>),  \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \
+    hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL, XX, RR>), dim3(tl.ntiles), dim3(256), lds, s, g, tl, \
+                       mc, sp, tile_start, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), \
+                       gf, err, migc);                                                                  \
+  } while (0)
 #define LAUNCH(MM, AA, LL) LAUNCH_T(MM, AA, LL, 0, 0)
 #define BY_MODE(AA, LL)            \
   switch (g.M) {                   \
@@ -729,22 +742,25 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
     case 4: LAUNCH(4, AA, LL); break; \
     default: return -1;            \
   }
-#define BY_MODE_T(AA, XX, RR)                     \
+#define BY_MODE_T(AA, LL, XX, RR)                 \
   switch (g.M) {                                  \
-    case 1: LAUNCH_T(1, AA, 0, XX, RR); break;    \
-    case 2: LAUNCH_T(2, AA, 0, XX, RR); break;    \
-    case 3: LAUNCH_T(3, AA, 0, XX, RR); break;    \
-    case 4: LAUNCH_T(4, AA, 0, XX, RR); break;    \
+    case 1: LAUNCH_T(1, AA, LL, XX, RR); break;   \
+    case 2: LAUNCH_T(2, AA, LL, XX, RR); break;   \
+    case 3: LAUNCH_T(3, AA, LL, XX, RR); break;   \
+    case 4: LAUNCH_T(4, AA, LL, XX, RR); break;   \
     default: return -1;                           \
   }
-  // compile-time tile shapes for the fixed-point accumulators with L1/L2 gather
-  if ((acc == 3 || acc == 4) && !ldsf) {
-    const int shape = tl.tx == 32 && tl.tr == 4 ? 1 : (tl.tx == 16 && tl.tr == 8 ? 2 : 0);
-    if (shape == 1 && acc == 3) { BY_MODE_T(3, 32, 4); return 0; }
-    if (shape == 1 && acc == 4) { BY_MODE_T(4, 32, 4); return 0; }
-    if (shape == 2 && acc == 3) { BY_MODE_T(3, 16, 8); return 0; }
-    if (shape == 2 && acc == 4) { BY_MODE_T(4, 16, 8); return 0; }
-  }
+#define BY_SHAPE_T(AA, LL)                                                   \
+  if (tl.tx == 32 && tl.tr == 4) { BY_MODE_T(AA, LL, 32, 4); return 0; }     \
+  if (tl.tx == 16 && tl.tr == 8) { BY_MODE_T(AA, LL, 16, 8); return 0; }     \
+  if (tl.tx == 16 && tl.tr == 4) { BY_MODE_T(AA, LL, 16, 4); return 0; }     \
+  if (tl.tx == 8 && tl.tr == 8) { BY_MODE_T(AA, LL, 8, 8); return 0; }       \
+  if (tl.tx == 8 && tl.tr == 4) { BY_MODE_T(AA, LL, 8, 4); return 0; }
+  // compile-time tile shapes for the fixed-point accumulators
+  if (acc == 3 && !ldsf) { BY_SHAPE_T(3, 0) }
+  if (acc == 3 && ldsf) { BY_SHAPE_T(3, 1) }
+  if (acc == 4 && !ldsf) { BY_SHAPE_T(4, 0) }
+  if (acc == 4 && ldsf) { BY_SHAPE_T(4, 1) }
   const int sel = acc * 2 + (ldsf ? 1 : 0);
   switch (sel) {
     case 0: BY_MODE(0, 0); break;
@@ -759,6 +775,7 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
     case 9: BY_MODE(4, 1); break;
     default: return -1;
   }
+#undef BY_SHAPE_T
 #undef BY_MODE_T
 #undef BY_MODE
 #undef LAUNCH

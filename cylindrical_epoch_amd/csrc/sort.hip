@@ -12,6 +12,7 @@
 //             keys), and the per-digit running counts live in registers
 //             spread across the wavefront's lanes.
 #include "launch.h"
+#include "migrate.h"
 
 namespace cylpic {
 
@@ -326,48 +327,13 @@ void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_
 constexpr int MIG_BLOCK = 256;
 constexpr int MIG_WAVES = MIG_BLOCK / 64;
 
-// direction of the move from source tile S to key d: 0..8 = (dty+1)*3 + (dtx+1),
-// MIG_REMOVED for a removed particle, -1 for a farther move
-__device__ __forceinline__ int mig_dir(uint32_t d, int S, int ntx, uint32_t ntiles) {
-  if (d >= ntiles) return MIG_REMOVED;
-  const int dty = (int)(d / (uint32_t)ntx) - S / ntx;
-  int dtx = (int)(d % (uint32_t)ntx) - S % ntx;
-  if (dtx == ntx - 1) dtx = -1;
-  else if (dtx == 1 - ntx) dtx = 1;
-  if (dtx < -1 || dtx > 1 || dty < -1 || dty > 1) return -1;
-  return (dty + 1) * 3 + (dtx + 1);
-}
-
 // counts[S * MIG_DIRS + dir]; flag[0] |= 1 on a farther move, flag[1] += alive particles
 __global__ __launch_bounds__(MIG_BLOCK) void k_mig_count(const uint32_t* __restrict__ key,
                                                          const int* __restrict__ ts, int ntx, int ntiles,
-                                                         uint32_t* __restrict__ counts, int* __restrict__ flag) {
+                                                         MigCount mc) {
   __shared__ uint32_t c[MIG_DIRS];
-  if (threadIdx.x < MIG_DIRS) c[threadIdx.x] = 0;
-  __syncthreads();
   const int S = blockIdx.x;
-  const int p0 = ts[S], p1 = ts[S + 1];
-  const int lane = threadIdx.x & 63;
-  bool far = false;
-  for (int base = p0; base < p1; base += MIG_BLOCK) {
-    const int i = base + (int)threadIdx.x;
-    const bool valid = i < p1;
-    const int dir = valid ? mig_dir(key[i], S, ntx, (uint32_t)ntiles) : -2;
-    far |= valid && dir < 0;
-#pragma unroll
-    for (int k = 0; k < MIG_DIRS; ++k) {
-      const unsigned long long m = __ballot(dir == k);
-      if (lane == 0 && m) atomicAdd(&c[k], (uint32_t)__popcll(m));
-    }
-  }
-  if (__ballot(far) && lane == 0) atomicOr(flag, 1);
-  __syncthreads();
-  if (threadIdx.x < MIG_DIRS) counts[(long)S * MIG_DIRS + threadIdx.x] = c[threadIdx.x];
-  if (threadIdx.x == 0) {
-    uint32_t alive = 0;
-    for (int k = 0; k < 9; ++k) alive += c[k];
-    if (alive) atomicAdd(flag + 1, (int)alive);
-  }
+  mig_count_tile(key, S, ts[S], ts[S + 1], ntx, ntiles, mc, c);
 }
 
 // per destination tile D: tot[D] = arrivals, pre[S * 9 + dir] = arrivals at D
@@ -466,21 +432,18 @@ size_t mig_scratch_words(int ntiles) {
     m = (m + SCAN_TILE - 1) / SCAN_TILE;
     scan += m;
   }
-  return (size_t)ntiles * (MIG_DIRS + 9) + (size_t)scan + 64;
+  return (size_t)ntiles * 9 + (size_t)scan + 64;
 }
 
-void launch_mig_count(const uint32_t* key, const int* ts, const Tiles& tl, uint32_t* scratch, int* flag,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(k_mig_count, dim3(tl.ntiles), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, tl.ntiles, scratch,
-                     flag);
+void launch_mig_count(const uint32_t* key, const int* ts, const Tiles& tl, MigCount mc, hipStream_t s) {
+  hipLaunchKernelGGL(k_mig_count, dim3(tl.ntiles), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, tl.ntiles, mc);
 }
 
-void launch_mig_scatter(const uint32_t* key, const int* ts, const Tiles& tl, uint32_t* scratch, int* ts_new,
-                        int word_bytes, const void* const* src, void* const* dst, int narr, uint32_t* key_out,
-                        const uint64_t* id_in, uint64_t* id_out, hipStream_t s) {
+void launch_mig_scatter(const uint32_t* key, const int* ts, const Tiles& tl, const uint32_t* counts,
+                        uint32_t* scratch, int* ts_new, int word_bytes, const void* const* src, void* const* dst,
+                        int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out, hipStream_t s) {
   const int nt = tl.ntiles;
-  uint32_t* counts = scratch;
-  uint32_t* pre = scratch + (size_t)nt * MIG_DIRS;
+  uint32_t* pre = scratch;
   uint32_t* tmp = pre + (size_t)nt * 9;
   hipLaunchKernelGGL(k_mig_offsets, dim3((nt + 1 + 255) / 256), dim3(256), 0, s, counts, tl.ntx, tl.ntr, nt,
                      (uint32_t*)ts_new, pre);

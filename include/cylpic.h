@@ -148,8 +148,8 @@ int cylpic_set_tile(cylpic_ctx *h, int tile_x, int tile_r);
  * a 2Sum-compensated low word (1), FP64 (2), int64 fixed point (3, default:
  * exact, order-independent tile sums) or two 32-bit fixed-point halves packed
  * in one int64 per complex cell (4: half the atomics, ~1e-5 relative
- * resolution); gather_lds: stage the tile's fields in LDS (1) or gather from
- * L1/L2 (0, default).  Has no counterpart in the reference (its deposit is
+ * resolution); gather_lds: stage the tile's fields in LDS (1, default) or
+ * gather from L1/L2 (0).  Has no counterpart in the reference (its deposit is
  * serial FP64). */
 int cylpic_tune(cylpic_ctx *h, int acc_mode, int gather_lds);
 
