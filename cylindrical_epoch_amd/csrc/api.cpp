@@ -89,7 +89,7 @@ struct cylpic_ctx {
   bool migrate = true;      // use the neighbour migration sort when it applies
   long sort_radix = 0, sort_migrate = 0;
   int acc_mode = 3;    // mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64, 3 int64, 4 2x32 packed)
-  int gather_lds = 1;  // mixed path: gather from the LDS-staged field region (1) or L1/L2 (0)
+  int gather_lds = 1;  // gather from the LDS-staged field region (1) or L1/L2 (0); init sets it per precision
   int lds_max = 65536;  // LDS bytes per workgroup on the device
   long push_count = 0;
   bool sorted_once = false;
@@ -253,7 +253,7 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   t.ntr = (h->g.ny + tr - 1) / tr;
   t.ntiles = t.ntx * t.ntr;
   size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t, h->acc_mode, h->gather_lds)
-                                      : push_lds_bytes(h->g, t, 0);
+                                      : push_lds_bytes(h->g, t, h->gather_lds);
   // the device's LDS per workgroup, less the push kernels' 64 bytes of static LDS
   const size_t limit = (size_t)h->lds_max - 64;
   if (lds > limit)
@@ -592,6 +592,9 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
   int rc = alloc_fields(h);
   if (rc) return bail(rc);
   {
+    // measured defaults: the mixed path gathers from LDS-staged fields, the
+    // VALU-bound FP64 path from L1/L2
+    h->gather_lds = precision == CYLPIC_MIXED ? 1 : 0;
     // default tile: the first of these whose LDS needs fit (the fastest
     // measured first: 16 x 4 for the mixed path, 16 x 8 for FP64)
     const int cand_m[][2] = {{16, 4}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}, {1, 1}, {1, 1}};
@@ -916,8 +919,8 @@ int cylpic_push_particles(cylpic_ctx* h) {
     s.dirty = true;
     s.counted = migc.counts != nullptr;
     if (h->prec == CYLPIC_FP64) {
-      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->d_err, migc,
-                              h->stream);
+      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->gather_lds,
+                              h->d_err, migc, h->stream);
     } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, h->f32, h->geof, h->ntab,
                                          h->acc_mode, h->gather_lds, h->d_err, migc, h->stream)) {
       return fail(CYLPIC_EINVAL, "mixed push: unsupported mode count");

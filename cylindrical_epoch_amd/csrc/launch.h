@@ -87,7 +87,7 @@ struct MigCount {
   int* flag;
 };
 void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const PartF64&,
-                             const int* tile_start, FieldSet<double>&, const Geo&, int* err, MigCount,
+                             const int* tile_start, FieldSet<double>&, const Geo&, int ldsf, int* err, MigCount,
                              hipStream_t);
 // key of every particle: its tile, or with cell_key its tile-major cell (Tiles::key)
 void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, int cell_key, hipStream_t);
@@ -99,7 +99,7 @@ void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, int 
 void launch_to_mixed(const Grid&, double mass, const double* const* src, const PartMixed&, long n, hipStream_t);
 void launch_from_mixed(const Grid&, double mass, const PartMixed&, double* const* dst, long n, hipStream_t);
 size_t push_lds_bytes_mixed(const Grid&, const Tiles&, int acc, int ldsf);
-size_t push_lds_bytes(const Grid&, const Tiles&, int precision);
+size_t push_lds_bytes(const Grid&, const Tiles&, int ldsf);
 
 // sort.hip
 struct SortScratch {

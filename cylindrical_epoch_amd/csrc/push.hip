@@ -27,14 +27,15 @@ __device__ __forceinline__ void weights3(T frac, T& wm, T& w0, T& wp) {
   wp = ((T)0.25 + cf2) - frac;
 }
 
-// One 3x3 patch of mode m weighted by wx (x) and wy (r), e_part.inc order.
+// One 3x3 patch weighted by wx (x) and wy (r), e_part.inc order: F[k0] is
+// the centre, rows `stride` apart (global mode plane or LDS field region).
 template <class T, class S>
-__device__ __forceinline__ cplx<T> patch(const Grid& g, const cplx<S>* __restrict__ F, int cx, int cy, int m,
-                                         const T* wx, const T* wy) {
+__device__ __forceinline__ cplx<T> patch(const cplx<S>* __restrict__ F, long k0, long stride, const T* wx,
+                                         const T* wy) {
   cplx<T> acc;
 #pragma unroll
   for (int jy = -1; jy <= 1; ++jy) {
-    const long k = g.idx(cx, cy + jy, m);
+    const long k = k0 + jy * stride;
     cplx<T> a = conv<T>(F[k - 1]), b = conv<T>(F[k]), c = conv<T>(F[k + 1]);
     cplx<T> row = cadd(cadd(rmul(wx[0], a), rmul(wx[1], b)), rmul(wx[2], c));
     acc = (jy == -1) ? rmul(wy[0], row) : cadd(acc, rmul(wy[jy + 1], row));
@@ -147,6 +148,10 @@ __device__ __forceinline__ int bcs_f64(const Grid& g, const Tiles& tl, const Par
 // FP64 push + deposit: the arithmetic of particles.F90:268-665 in the same
 // operation order (this file is compiled without FP contraction).
 // ------------------------------------------------------------------------
+// LDSF: gather from the tile's fields staged in LDS (1) or from L1/L2 (0);
+// the arithmetic is the same either way.
+constexpr int FIELD_HALO_F64 = 2;
+template <int LDSF>
 __global__ __launch_bounds__(256) void k_push_deposit_f64(
     Grid g, Tiles tl, PushConst pc, PartF64 sp, const int* __restrict__ tile_start,
     const cplx<double>* __restrict__ ex, const cplx<double>* __restrict__ er,
@@ -166,6 +171,29 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
   box.M = g.M;
   const int nl = 3 * g.M * box.LR * box.LX;
   for (int e = threadIdx.x; e < 2 * nl; e += blockDim.x) lds[e] = 0.0;
+  // field region [6][M][FR][FX] after the current box: the tile plus the
+  // 2-cell reach of the gather stencils
+  cplx<double>* fl = (cplx<double>*)(lds + 2 * nl);
+  const int fx0 = ttx * tl.tx + 1 - FIELD_HALO_F64, fr0 = tty * tl.tr + 1 - FIELD_HALO_F64;
+  const int FX = tl.tx + 2 * FIELD_HALO_F64, FR = tl.tr + 2 * FIELD_HALO_F64;
+  const long fplane = (long)FX * FR;
+  if (LDSF) {
+    const int nf = 6 * g.M * FR * FX;
+    for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+      int rem = e;
+      const int fx = rem % FX;
+      rem /= FX;
+      const int fr = rem % FR;
+      rem /= FR;
+      const int m = rem % g.M;
+      const int c = rem / g.M;
+      const int cx = fx0 + fx, cy = fr0 + fr;
+      const cplx<double>* src = c == 0 ? ex : (c == 1 ? er : (c == 2 ? et : (c == 3 ? bx : (c == 4 ? br : bt))));
+      cplx<double> v = mk(0.0, 0.0);
+      if (cx >= 1 - g.ng && cx <= g.nx + g.ng && cy >= 1 - g.ng && cy <= g.ny + g.ng) v = src[g.idx(cx, cy, m)];
+      fl[e] = v;
+    }
+  }
   __syncthreads();
 
   const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
@@ -236,18 +264,27 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
     double ex_part = 0.0, er_part = 0.0, et_part = 0.0;
     double bx_part = 0.0, br_part = 0.0, bt_part = 0.0;
     {
+      const bool in_lds = LDSF && min(cell_x1, cell_x2) - 1 >= fx0 && max(cell_x1, cell_x2) + 1 < fx0 + FX &&
+                          min(cell_y1, cell_y2) - 1 >= fr0 && max(cell_y1, cell_y2) + 1 < fr0 + FR;
+      // patch of component c (ex, er, et, bx, br, bt) of mode im centred on (cx, cy)
+      auto P = [&](int c, const cplx<double>* __restrict__ F, int cx, int cy, int im, const double* wx,
+                   const double* wy) {
+        if (in_lds)
+          return patch<double>(fl + (long)(c * M + im) * fplane, (long)(cy - fr0) * FX + (cx - fx0), FX, wx, wy);
+        return patch<double>(F, g.idx(cx, cy, im), g.nx2, wx, wy);
+      };
       cplx<double> e_m = mk(1.0, 0.0);
       for (int im = 0; im < M; ++im) {
-        ex_part = ex_part + re_mul(e_m, patch<double>(g, ex, cell_x1, cell_y2, im, gx + 1, hy + 1));
-        er_part = er_part + re_mul(e_m, patch<double>(g, er, cell_x2, cell_y1, im, hx + 1, gy + 1));
-        et_part = et_part + re_mul(e_m, patch<double>(g, et, cell_x2, cell_y2, im, hx + 1, hy + 1));
+        ex_part = ex_part + re_mul(e_m, P(0, ex, cell_x1, cell_y2, im, gx + 1, hy + 1));
+        er_part = er_part + re_mul(e_m, P(1, er, cell_x2, cell_y1, im, hx + 1, gy + 1));
+        et_part = et_part + re_mul(e_m, P(2, et, cell_x2, cell_y2, im, hx + 1, hy + 1));
         e_m = cmul(e_m, emi);
       }
       e_m = mk(1.0, 0.0);
       for (int im = 0; im < M; ++im) {
-        bx_part = bx_part + re_mul(e_m, patch<double>(g, bx, cell_x2, cell_y1, im, hx + 1, gy + 1));
-        br_part = br_part + re_mul(e_m, patch<double>(g, br, cell_x1, cell_y2, im, gx + 1, hy + 1));
-        bt_part = bt_part + re_mul(e_m, patch<double>(g, bt, cell_x1, cell_y1, im, gx + 1, gy + 1));
+        bx_part = bx_part + re_mul(e_m, P(3, bx, cell_x2, cell_y1, im, hx + 1, gy + 1));
+        br_part = br_part + re_mul(e_m, P(4, br, cell_x1, cell_y2, im, gx + 1, hy + 1));
+        bt_part = bt_part + re_mul(e_m, P(5, bt, cell_x1, cell_y1, im, gx + 1, gy + 1));
         e_m = cmul(e_m, emi);
       }
     }
@@ -454,18 +491,26 @@ __global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n, int cell_key) {
 // ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
-size_t push_lds_bytes(const Grid& g, const Tiles& tl, int precision) {
-  size_t cells = (size_t)(tl.tx + 2 * DEPOSIT_HALO) * (size_t)(tl.tr + 2 * DEPOSIT_HALO);
-  size_t per = precision == 0 ? 2 * sizeof(double) : 4 * sizeof(float);  // mixed: hi + lo pairs
-  return 3 * (size_t)g.M * cells * per;
+size_t push_lds_bytes(const Grid& g, const Tiles& tl, int ldsf) {
+  const size_t cells = (size_t)(tl.tx + 2 * DEPOSIT_HALO) * (size_t)(tl.tr + 2 * DEPOSIT_HALO);
+  const size_t fcells = ldsf ? (size_t)(tl.tx + 2 * FIELD_HALO_F64) * (tl.tr + 2 * FIELD_HALO_F64) : 0;
+  return (3 * cells + 6 * fcells) * (size_t)g.M * sizeof(cplx<double>);
 }
 
 void launch_push_deposit_f64(const Grid& g, const Tiles& tl, const PushConst& pc, const PartF64& sp,
-                             const int* tile_start, FieldSet<double>& F, const Geo& geo, int* err,
+                             const int* tile_start, FieldSet<double>& F, const Geo& geo, int ldsf, int* err,
                              MigCount migc, hipStream_t s) {
-  size_t lds = push_lds_bytes(g, tl, 0);
-  hipLaunchKernelGGL(k_push_deposit_f64, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
-                     F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err, migc);
+  const size_t lds = push_lds_bytes(g, tl, ldsf);
+  if (ldsf) {
+    if (lds > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_f64<1>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_push_deposit_f64<1>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
+                       F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err, migc);
+  } else {
+    hipLaunchKernelGGL(k_push_deposit_f64<0>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
+                       F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err, migc);
+  }
 }
 
 void launch_keys_f64(const Grid& g, const Tiles& tl, const PartF64& sp, long n, int cell_key, hipStream_t s) {

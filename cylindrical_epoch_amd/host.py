@@ -58,8 +58,7 @@ class Simulation:
                                 PRECISION[precision], device))
         self.h = h
         if acc_mode is not None or gather_lds is not None:
-            check(lib().cylpic_tune(self.h, 3 if acc_mode is None else int(acc_mode),
-                                    1 if gather_lds is None else int(gather_lds)))
+            self.tune(3 if acc_mode is None else int(acc_mode), gather_lds)
         if tile is not None:
             check(lib().cylpic_set_tile(self.h, int(tile[0]), int(tile[1])))
         if sort_interval != 1:
@@ -155,10 +154,13 @@ class Simulation:
         check(lib().cylpic_step(self.h, _ptr(a), _ptr(b)))
         self.time += self.dt
 
-    def tune(self, acc_mode=3, gather_lds=1):
-        """Mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64,
-        3 int64 fixed point, 4 packed 2 x 32-bit fixed point) and gather
-        source (1 LDS-staged tile, 0 L1/L2)."""
+    def tune(self, acc_mode=3, gather_lds=None):
+        """Current accumulator of the mixed path (0 FP32, 1 FP32 + 2Sum,
+        2 FP64, 3 int64 fixed point, 4 packed 2 x 32-bit fixed point) and
+        gather source of both paths (1 LDS-staged tile, 0 L1/L2; default:
+        1 for mixed, 0 for FP64)."""
+        if gather_lds is None:
+            gather_lds = 1 if self.precision == "mixed" else 0
         check(lib().cylpic_tune(self.h, int(acc_mode), int(gather_lds)))
 
     def sort_particles(self):

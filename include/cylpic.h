@@ -144,13 +144,13 @@ int cylpic_set_sort_method(cylpic_ctx *h, int method);
 int cylpic_sort_counts(cylpic_ctx *h, int64_t *radix, int64_t *migrate);
 /* Tile shape (cells) of the push/deposit work decomposition. */
 int cylpic_set_tile(cylpic_ctx *h, int tile_x, int tile_r);
-/* Mixed path only.  acc_mode: per-tile current sums in FP32 (0), FP32 with
+/* acc_mode (mixed path): per-tile current sums in FP32 (0), FP32 with
  * a 2Sum-compensated low word (1), FP64 (2), int64 fixed point (3, default:
  * exact, order-independent tile sums) or two 32-bit fixed-point halves packed
  * in one int64 per complex cell (4: half the atomics, ~1e-5 relative
- * resolution); gather_lds: stage the tile's fields in LDS (1, default) or
- * gather from L1/L2 (0).  Has no counterpart in the reference (its deposit is
- * serial FP64). */
+ * resolution); gather_lds (both paths): stage the tile's fields in LDS (1,
+ * the mixed path's default) or gather from L1/L2 (0, the FP64 default).  Has
+ * no counterpart in the reference (its deposit is serial FP64). */
 int cylpic_tune(cylpic_ctx *h, int acc_mode, int gather_lds);
 
 /* Per-mode field energies (FP64 reduction; the reference's Cartesian

@@ -40,10 +40,12 @@ def _compare_currents(st, sim):
             assert err <= (J_TOL_M0 if m == 0 else J_TOL_M), (n, m, err)
 
 
-@pytest.mark.parametrize("M", [1, 2, 3])
+@pytest.mark.parametrize("M,gather_lds", [(1, 0), (2, 0), (3, 0), (2, 1), (3, 1)])
 @pytest.mark.parametrize("bcs", ["periodic", "open"])
-def test_push_particles_parity(oracle, M, bcs):
-    log("push parity M", M, bcs)
+def test_push_particles_parity(oracle, M, gather_lds, bcs):
+    """Particles bit-exact and currents within the stated tolerances, with
+    the fields gathered from L1/L2 or from the LDS-staged tile region."""
+    log("push parity M", M, "gather_lds", gather_lds, bcs)
     nx, ny, dx, dy = 40, 20, 1e-7, 1e-7
     if bcs == "periodic":
         bf = ("periodic", "periodic", "other", "conduct")
@@ -52,6 +54,7 @@ def test_push_particles_parity(oracle, M, bcs):
         bf = ("simple_outflow", "simple_outflow", "other", "open")
         bp = ("open", "open", "other", "open")
     st, sim = make_pair(oracle, nx, ny, M, dx, dy, bc_field=bf, bc_particle=bp, tile=(8, 4))
+    sim.tune(gather_lds=gather_lds)
     rng = np.random.default_rng(M)
     f = random_fields(st.shape, rng, e_amp=3e10, b_amp=200.0, j_amp=0.0)
     load_fields(st, sim, f)
