@@ -24,6 +24,8 @@ SIGNATURES = {
     "cylpic_upload_particles": [_p, _i, _l, _p, _p, _p, _p, _p, _p, _p, _p],
     "cylpic_download_particles": [_p, _i, _l, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "cylpic_particle_count": [_p, _i, _p],
+    "cylpic_inject_particles": [_p, _i, _l, _p, _p, _p, _p, _p, _p, _p, _p],
+    "cylpic_move_window": [_p, _i],
     "cylpic_upload_field": [_p, _i, _p],
     "cylpic_download_field": [_p, _i, _p],
     "cylpic_setup_field_boundaries": [_p],

@@ -51,6 +51,9 @@ struct Species {
   bool dirty = false;             // pushed since the last sort (removed particles may remain)
   bool ranged = false;            // tile_start describes the tile ranges of the keys' last sort
   uint32_t* mig_counts = nullptr; // [ntiles][MIG_DIRS] direction counts for the migration sort
+  uint32_t* mig_counts_x = nullptr;  // the same for the extra ranges
+  int* extra_start = nullptr;     // ntiles + 2: tile ranges of the appended (injected) particles
+  long n_extra = 0;               // particles appended after the main ranges since the last sort
   int* mig_flag = nullptr;        // 2 ints: farther move seen, particles kept
   bool counted = false;           // the last push left fresh direction counts
 };
@@ -87,6 +90,10 @@ struct cylpic_ctx {
   uint32_t* mig = nullptr;  // migration sort scratch (mig_scratch_words(ntiles))
   int* mig_host = nullptr;  // pinned copy of a species' mig_flag
   bool migrate = true;      // use the neighbour migration sort when it applies
+  double* stage = nullptr;  // FP64 staging of host particles for the mixed path
+  long stage_cap = 0;
+  cplx<double>* spare64 = nullptr;  // field-sized buffer of the moving window's shifts
+  cplx<float>* spare32 = nullptr;
   long sort_radix = 0, sort_migrate = 0;
   int acc_mode = 3;    // mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64, 3 int64, 4 2x32 packed)
   int gather_lds = 1;  // gather from the LDS-staged field region (1) or L1/L2 (0); init sets it per precision
@@ -288,31 +295,60 @@ int ensure_sort_scratch(cylpic_ctx* h, long n) {
 }
 
 // FP64 / mixed particle views
-PartMixed viewm(Species& s) {
+// the species' current arrays from particle `off` on
+PartMixed viewm(Species& s, long off = 0) {
   PartMixed p;
-  p.cell = (int*)s.ma[0];
-  p.ox = (float*)s.ma[1];
-  p.y = (float*)s.ma[2];
-  p.z = (float*)s.ma[3];
-  p.ux = (float*)s.ma[4];
-  p.uy = (float*)s.ma[5];
-  p.uz = (float*)s.ma[6];
-  p.w = (float*)s.ma[7];
-  p.key = (int*)s.key[0];
+  p.cell = (int*)s.ma[0] + off;
+  p.ox = (float*)s.ma[1] + off;
+  p.y = (float*)s.ma[2] + off;
+  p.z = (float*)s.ma[3] + off;
+  p.ux = (float*)s.ma[4] + off;
+  p.uy = (float*)s.ma[5] + off;
+  p.uz = (float*)s.ma[6] + off;
+  p.w = (float*)s.ma[7] + off;
+  p.key = (int*)s.key[0] + off;
   return p;
 }
 
-PartF64 view64(Species& s) {
+PartF64 view64(Species& s, long off = 0) {
   PartF64 p;
-  p.x = s.a[0];
-  p.y = s.a[1];
-  p.z = s.a[2];
-  p.px = s.a[3];
-  p.py = s.a[4];
-  p.pz = s.a[5];
-  p.w = s.a[6];
-  p.key = (int*)s.key[0];
+  p.x = s.a[0] + off;
+  p.y = s.a[1] + off;
+  p.z = s.a[2] + off;
+  p.px = s.a[3] + off;
+  p.py = s.a[4] + off;
+  p.pz = s.a[5] + off;
+  p.w = s.a[6] + off;
+  p.key = (int*)s.key[0] + off;
   return p;
+}
+
+// Host particles (SI, FP64) into the species' arrays from index base on.
+int stage_in(cylpic_ctx* h, Species& s, long base, long n, const double* const src[7]) {
+  if (n <= 0) return CYLPIC_OK;
+  if (h->prec == CYLPIC_FP64) {
+    for (int k = 0; k < 7; ++k)
+      HIPCHK(hipMemcpyAsync(s.a[k] + base, src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    return CYLPIC_OK;
+  }
+  if (7 * n > h->stage_cap) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->stage) HIPCHK(hipFree(h->stage));
+    h->stage = nullptr;
+    HIPCHK(hipMalloc((void**)&h->stage, 7 * n * sizeof(double)));
+    h->stage_cap = 7 * n;
+  }
+  double* t7[7];
+  for (int k = 0; k < 7; ++k) {
+    t7[k] = h->stage + k * n;
+    HIPCHK(hipMemcpyAsync(t7[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  }
+  launch_to_mixed(h->g, s.m, t7, viewm(s, base), n, h->stream);
+  return CYLPIC_OK;
+}
+
+ExtraRange extra_of(const Species& s) {
+  return ExtraRange{s.n_extra > 0 ? s.extra_start : nullptr, (int)(s.n - s.n_extra)};
 }
 
 // Neighbour migration sort of a species whose keys were tile-sorted before
@@ -326,9 +362,10 @@ int migrate_species(cylpic_ctx* h, Species& s, bool counted, bool* done) {
     HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
     HIPCHK(hipHostMalloc((void**)&h->mig_host, 2 * sizeof(int), 0));
   }
+  const MigCount mc{s.mig_counts, s.mig_counts_x, s.mig_flag};
   if (!counted) {
     HIPCHK(hipMemsetAsync(s.mig_flag, 0, 2 * sizeof(int), h->stream));
-    launch_mig_count(s.key[0], s.tile_start, h->tl, MigCount{s.mig_counts, s.mig_flag}, h->stream);
+    launch_mig_count(s.key[0], s.tile_start, extra_of(s), h->tl, mc, h->stream);
   }
   HIPCHK(hipMemcpyAsync(h->mig_host, s.mig_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -341,8 +378,8 @@ int migrate_species(cylpic_ctx* h, Species& s, bool counted, bool* done) {
     src[k] = f64 ? (const void*)s.a[k] : (const void*)s.ma[k];
     dst[k] = f64 ? (void*)s.b[k] : (void*)s.mb[k];
   }
-  launch_mig_scatter(s.key[0], s.tile_start, h->tl, s.mig_counts, h->mig, s.tile_start_alt, f64 ? 8 : 4, src, dst,
-                     narr, s.key[1],
+  launch_mig_scatter(s.key[0], s.tile_start, extra_of(s), h->tl, mc, h->mig, s.tile_start_alt, f64 ? 8 : 4, src,
+                     dst, narr, s.key[1],
                      s.ids ? (const uint64_t*)s.id[0] : nullptr, s.ids ? (uint64_t*)s.id[1] : nullptr, h->stream);
   HIPCHK(hipGetLastError());
   for (int k = 0; k < narr; ++k) {
@@ -354,6 +391,7 @@ int migrate_species(cylpic_ctx* h, Species& s, bool counted, bool* done) {
   std::swap(s.tile_start, s.tile_start_alt);
   s.d_count = s.tile_start + h->tl.ntiles;
   s.n = h->mig_host[1];
+  s.n_extra = 0;
   *done = true;
   return CYLPIC_OK;
 }
@@ -407,6 +445,7 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   }
   HIPCHK(hipGetLastError());
   s.ranged = true;
+  s.n_extra = 0;
   return CYLPIC_OK;
 }
 
@@ -461,6 +500,8 @@ void free_all(cylpic_ctx* h) {
     if (s.tile_start) (void)hipFree(s.tile_start);
     if (s.tile_start_alt) (void)hipFree(s.tile_start_alt);
     if (s.mig_counts) (void)hipFree(s.mig_counts);
+    if (s.mig_counts_x) (void)hipFree(s.mig_counts_x);
+    if (s.extra_start) (void)hipFree(s.extra_start);
     if (s.mig_flag) (void)hipFree(s.mig_flag);
   }
   if (h->geo) (void)hipFree(h->geo);
@@ -471,6 +512,9 @@ void free_all(cylpic_ctx* h) {
   }
   if (h->d_err) (void)hipFree(h->d_err);
   if (h->mig) (void)hipFree(h->mig);
+  if (h->stage) (void)hipFree(h->stage);
+  if (h->spare64) (void)hipFree(h->spare64);
+  if (h->spare32) (void)hipFree(h->spare32);
   if (h->mig_host) (void)hipHostFree(h->mig_host);
   if (h->d_energy) (void)hipFree(h->d_energy);
   if (h->sc_mem) (void)hipFree(h->sc_mem);
@@ -505,6 +549,8 @@ int alloc_species_arrays(cylpic_ctx* h, Species& s) {
   HIPCHK(hipMemsetAsync(s.tile_start, 0, (h->tl.ntiles + 2) * sizeof(int), h->stream));
   HIPCHK(hipMalloc((void**)&s.tile_start_alt, (h->tl.ntiles + 2) * sizeof(int)));
   HIPCHK(hipMalloc((void**)&s.mig_counts, (size_t)h->tl.ntiles * MIG_DIRS * sizeof(uint32_t)));
+  HIPCHK(hipMalloc((void**)&s.mig_counts_x, (size_t)h->tl.ntiles * MIG_DIRS * sizeof(uint32_t)));
+  HIPCHK(hipMalloc((void**)&s.extra_start, (h->tl.ntiles + 2) * sizeof(int)));
   HIPCHK(hipMalloc((void**)&s.mig_flag, 2 * sizeof(int)));
   s.d_count = s.tile_start + h->tl.ntiles;
   return CYLPIC_OK;
@@ -725,21 +771,8 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
   if (n > 0 && (!x || !y || !z || !px || !py || !pz || !w)) return fail(CYLPIC_EINVAL, "null particle array");
   HIPCHK(hipSetDevice(h->device));
   const double* src[7] = {x, y, z, px, py, pz, w};
-  if (h->prec == CYLPIC_FP64) {
-    for (int k = 0; k < 7; ++k)
-      if (n > 0) HIPCHK(hipMemcpyAsync(s.a[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  } else if (n > 0) {
-    double* tmp = nullptr;
-    HIPCHK(hipMalloc((void**)&tmp, 7 * n * sizeof(double)));
-    double* t7[7];
-    for (int k = 0; k < 7; ++k) {
-      t7[k] = tmp + k * n;
-      HIPCHK(hipMemcpyAsync(t7[k], src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    }
-    launch_to_mixed(h->g, s.m, t7, viewm(s), n, h->stream);
-    HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipFree(tmp));
-  }
+  int rc = stage_in(h, s, 0, n, src);
+  if (rc) return rc;
   if (ids) {
     if (!s.id[0]) {
       HIPCHK(hipMalloc((void**)&s.id[0], (s.cap > 0 ? s.cap : 1) * sizeof(int64_t)));
@@ -751,15 +784,120 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
     s.ids = false;
   }
   s.n = n;
+  s.n_extra = 0;
   s.ranged = false;
   s.counted = false;
   // tile-major cell keys: the upload sort groups each tile's particles by cell
   if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, 1, h->stream);
   else launch_keys_mixed(h->g, h->tl, viewm(s), n, 1, h->stream);
-  int rc = sort_species(h, s, 1);
+  rc = sort_species(h, s, 1);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return refresh_count(h, s);
+}
+
+int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double* x, const double* y,
+                            const double* z, const double* px, const double* py, const double* pz,
+                            const double* w, const int64_t* ids) {
+  CHECK_H(h);
+  if (species < 0 || species >= (int)h->sp.size()) return fail(CYLPIC_EINVAL, "unknown species");
+  Species& s = h->sp[species];
+  if (n < 0) return fail(CYLPIC_EINVAL, "negative particle count");
+  if (n == 0) return CYLPIC_OK;
+  if (!x || !y || !z || !px || !py || !pz || !w) return fail(CYLPIC_EINVAL, "null particle array");
+  if (!s.ranged) return fail(CYLPIC_ESTATE, "upload the species' particles before injecting");
+  if ((ids != nullptr) != s.ids) return fail(CYLPIC_EINVAL, "ids must be given exactly when the upload gave them");
+  HIPCHK(hipSetDevice(h->device));
+  if (s.n_extra > 0) {  // merge the previous batch first
+    int rc = sort_species(h, s);
+    if (rc) return rc;
+  }
+  const long base = s.n;
+  if (base + n > s.cap) return fail(CYLPIC_EINVAL, "injection exceeds the species capacity");
+  const double* src[7] = {x, y, z, px, py, pz, w};
+  int rc = stage_in(h, s, base, n, src);
+  if (rc) return rc;
+  if (ids) HIPCHK(hipMemcpyAsync(s.id[0] + base, ids, n * sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+  // tile keys of the batch, a stable sort of the batch by key in place, and
+  // its tile ranges: the push then treats it as each tile's extra range
+  if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s, base), n, 0, h->stream);
+  else launch_keys_mixed(h->g, h->tl, viewm(s, base), n, 0, h->stream);
+  rc = ensure_sort_scratch(h, n);
+  if (rc) return rc;
+  uint32_t* perm = nullptr;
+  radix_sort(s.key[0] + base, n, h->tl.ntiles, 0, h->sc, s.key[1] + base, &perm, h->stream);
+  HIPCHK(hipMemcpyAsync(s.key[0] + base, s.key[1] + base, n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                        h->stream));
+  const bool f64 = h->prec == CYLPIC_FP64;
+  const int narr = f64 ? 7 : 8;
+  const size_t wb = f64 ? sizeof(double) : sizeof(uint32_t);
+  if (f64) {
+    double* sa[7];
+    double* sb[7];
+    for (int k = 0; k < 7; ++k) {
+      sa[k] = s.a[k] + base;
+      sb[k] = s.b[k] + base;
+    }
+    launch_gather_f64(perm, n, sa, sb, 7, nullptr, nullptr, 0, h->stream);
+  } else {
+    uint32_t* sa[8];
+    uint32_t* sb[8];
+    for (int k = 0; k < 8; ++k) {
+      sa[k] = s.ma[k] + base;
+      sb[k] = s.mb[k] + base;
+    }
+    launch_gather_32(perm, n, sa, sb, 8, nullptr, nullptr, 0, h->stream);
+  }
+  for (int k = 0; k < narr; ++k) {
+    void* dst = f64 ? (void*)(s.a[k] + base) : (void*)(s.ma[k] + base);
+    const void* from = f64 ? (const void*)(s.b[k] + base) : (const void*)(s.mb[k] + base);
+    HIPCHK(hipMemcpyAsync(dst, from, n * wb, hipMemcpyDeviceToDevice, h->stream));
+  }
+  if (ids) {
+    launch_gather_64(perm, n, (const uint64_t*)s.id[0] + base, (uint64_t*)s.id[1] + base, nullptr, nullptr, 0,
+                     h->stream);
+    HIPCHK(hipMemcpyAsync(s.id[0] + base, s.id[1] + base, n * sizeof(int64_t), hipMemcpyDeviceToDevice, h->stream));
+  }
+  launch_tile_bounds(s.key[0] + base, n, h->tl.ntiles, s.extra_start, h->stream);
+  HIPCHK(hipGetLastError());
+  s.n = base + n;
+  s.n_extra = n;
+  s.dirty = true;
+  s.counted = false;
+  return CYLPIC_OK;
+}
+
+int cylpic_move_window(cylpic_ctx* h, int ncells) {
+  CHECK_H(h);
+  if (ncells < 0) return fail(CYLPIC_EINVAL, "ncells must be >= 0");
+  if (ncells == 0) return CYLPIC_OK;
+  if (h->g.bc_field[XMIN] == BC_PERIODIC || h->g.bc_field[XMAX] == BC_PERIODIC)
+    return fail(CYLPIC_EINVAL, "a moving window needs non-periodic x boundaries");
+  HIPCHK(hipSetDevice(h->device));
+  Scope sc(h, "window");
+  const long nf = nfield(h);
+  for (int c = 0; c < ncells; ++c) {
+    // window.F90:70-85 without CPML: x_grid_min, xb_min += dx; x_max = xb_global(nx + 1)
+    h->g.x_grid_min = h->g.x_grid_min + h->g.dx;
+    h->g.x_min = h->g.x_min + h->g.dx;
+    h->g.x_max = h->g.x_min + (double)h->g.nx * h->g.dx;
+    if (h->prec == CYLPIC_FP64) {
+      if (!h->spare64) HIPCHK(hipMalloc((void**)&h->spare64, nf * sizeof(cplx<double>)));
+      launch_shift_window_fields(h->g, h->f64, h->spare64, h->stream);
+    } else {
+      if (!h->spare32) HIPCHK(hipMalloc((void**)&h->spare32, nf * sizeof(cplx<float>)));
+      launch_shift_window_fields(h->g, h->f32, h->spare32, h->stream);
+    }
+  }
+  for (auto& s : h->sp) {
+    if (s.n <= 0) continue;
+    if (h->prec == CYLPIC_FP64) launch_window_f64(h->g, view64(s), s.n, h->tl.ntiles, h->stream);
+    else launch_window_mixed(viewm(s), s.n, ncells, h->tl.ntiles, h->stream);
+    s.dirty = true;
+    s.counted = false;
+  }
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
 }
 
 int cylpic_particle_count(cylpic_ctx* h, int species, int64_t* n_out) {
@@ -910,19 +1048,20 @@ int cylpic_push_particles(cylpic_ctx* h) {
     pc.dtco2 = C_LIGHT * (dt / 2.0);
     // the push counts its tiles' new keys by direction for the next
     // migration sort (the tile ranges are those of the last sort)
-    MigCount migc{nullptr, nullptr};
+    MigCount migc{nullptr, nullptr, nullptr};
     if (h->migrate && s.ranged) {
       HIPCHK(hipMemsetAsync(s.mig_flag, 0, 2 * sizeof(int), h->stream));
-      migc = MigCount{s.mig_counts, s.mig_flag};
+      migc = MigCount{s.mig_counts, s.mig_counts_x, s.mig_flag};
     }
     Scope sc(h, "push_deposit");
     s.dirty = true;
     s.counted = migc.counts != nullptr;
     if (h->prec == CYLPIC_FP64) {
-      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, h->f64, geo_of(h), h->gather_lds,
-                              h->d_err, migc, h->stream);
-    } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, h->f32, h->geof, h->ntab,
-                                         h->acc_mode, h->gather_lds, h->d_err, migc, h->stream)) {
+      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, extra_of(s), h->f64, geo_of(h),
+                              h->gather_lds, h->d_err, migc, h->stream);
+    } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, extra_of(s), h->f32,
+                                         h->geof, h->ntab, h->acc_mode, h->gather_lds, h->d_err, migc,
+                                         h->stream)) {
       return fail(CYLPIC_EINVAL, "mixed push: unsupported mode count");
     }
   }

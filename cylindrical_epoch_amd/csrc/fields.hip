@@ -816,6 +816,54 @@ void launch_f2d(const cplx<float>* src, cplx<double>* dst, long n, hipStream_t s
   hipLaunchKernelGGL(k_f2d, dim3(b), dim3(256), 0, s, src, dst, n);
 }
 
+// ------------------------------------------------------------------------
+// moving window: shift_field_modes (window.F90:132-150), out(i) = in(i + 1)
+// for i in [1-ng, nx+ng-1], out(nx+ng) = in(nx+ng); and the x_max
+// incoming-cell fix of shift_fields (window.F90:101-128)
+template <class T>
+__global__ void k_shift_x(Grid g, const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.plane * g.M) return;
+  const int i = (int)(e % g.nx2);  // 0-based column: ix = i + 1 - ng
+  out[e] = i + 1 < g.nx2 ? in[e + 1] : in[e];
+}
+
+template <class T>
+__global__ void k_window_fix(Grid g, cplx<T>* ex, cplx<T>* er, cplx<T>* et, cplx<T>* bx, cplx<T>* br, cplx<T>* bt,
+                             const cplx<T>* bex, const cplx<T>* ber, const cplx<T>* bet, const cplx<T>* bbx,
+                             const cplx<T>* bbr, const cplx<T>* bbt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.ny2 * g.M) return;
+  const int m = t / g.ny2, j = t % g.ny2 + 1 - g.ng;
+  const int nx = g.nx;
+  const long s = g.idx2(j, m);
+  const T h = (T)0.5;
+  ex[g.idx(nx + 1, j, m)] = bex[s];
+  er[g.idx(nx, j, m)] = ber[s];
+  et[g.idx(nx, j, m)] = bet[s];
+  ex[g.idx(nx, j, m)] = rmul(h, cadd(ex[g.idx(nx - 1, j, m)], ex[g.idx(nx + 1, j, m)]));
+  er[g.idx(nx - 1, j, m)] = rmul(h, cadd(er[g.idx(nx - 2, j, m)], er[g.idx(nx, j, m)]));
+  et[g.idx(nx - 1, j, m)] = rmul(h, cadd(et[g.idx(nx - 2, j, m)], et[g.idx(nx, j, m)]));
+  bx[g.idx(nx, j, m)] = bbx[s];
+  br[g.idx(nx + 1, j, m)] = bbr[s];
+  bt[g.idx(nx + 1, j, m)] = bbt[s];
+  bx[g.idx(nx - 1, j, m)] = rmul(h, cadd(bx[g.idx(nx - 2, j, m)], bx[g.idx(nx, j, m)]));
+  br[g.idx(nx, j, m)] = rmul(h, cadd(br[g.idx(nx - 1, j, m)], br[g.idx(nx + 1, j, m)]));
+  bt[g.idx(nx, j, m)] = rmul(h, cadd(bt[g.idx(nx - 1, j, m)], bt[g.idx(nx + 1, j, m)]));
+}
+
+template <class T>
+void launch_shift_window_fields(const Grid& g, FieldSet<T>& F, cplx<T>*& spare, hipStream_t s) {
+  const long n = g.plane * g.M;
+  for (int k = 0; k < 9; ++k) {  // ex .. jt
+    hipLaunchKernelGGL(k_shift_x<T>, dim3(nblk(n)), dim3(256), 0, s, g, F.f[k], spare);
+    std::swap(F.f[k], spare);
+  }
+  hipLaunchKernelGGL(k_window_fix<T>, dim3((g.ny2 * g.M + 255) / 256), dim3(256), 0, s, g, F.ex(), F.er(), F.et(),
+                     F.bx(), F.br(), F.bt(), F.bnd_xmax[0], F.bnd_xmax[1], F.bnd_xmax[2], F.bnd_xmax[3],
+                     F.bnd_xmax[4], F.bnd_xmax[5]);
+}
+
 template <class T>
 void launch_energy(const Grid& g, FieldSet<T>& F, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_energy<T>, dim3(g.ny, g.M), dim3(256), 0, s, g, F.ex(), F.er(), F.et(), F.bx(), F.br(),
@@ -833,7 +881,8 @@ void launch_energy(const Grid& g, FieldSet<T>& F, double* out, hipStream_t s) {
   template void launch_current_axis<T>(const Grid&, FieldSet<T>&, hipStream_t);               \
   template void launch_current_finish<T>(const Grid&, FieldSet<T>&, hipStream_t);             \
   template void launch_fill_zero<T>(cplx<T>*, long, hipStream_t);                             \
-  template void launch_energy<T>(const Grid&, FieldSet<T>&, double*, hipStream_t);
+  template void launch_energy<T>(const Grid&, FieldSet<T>&, double*, hipStream_t);            \
+  template void launch_shift_window_fields<T>(const Grid&, FieldSet<T>&, cplx<T>*&, hipStream_t);
 INST(double)
 INST(float)
 

@@ -73,29 +73,47 @@ template <class T> void launch_current_axis(const Grid&, FieldSet<T>&, hipStream
 template <class T> void launch_current_finish(const Grid&, FieldSet<T>&, hipStream_t);
 template <class T> void launch_fill_zero(cplx<T>*, long n, hipStream_t);
 template <class T> void launch_energy(const Grid&, FieldSet<T>&, double* out, hipStream_t);
+// moving window: shift E, B, J one cell towards x_min (through `spare`, a
+// field-sized buffer that is swapped with each array) and fix the x_max
+// incoming cells from the stored boundary columns
+template <class T> void launch_shift_window_fields(const Grid&, FieldSet<T>&, cplx<T>*& spare, hipStream_t);
 void launch_d2f(const cplx<double>* src, cplx<float>* dst, long n, hipStream_t);
 void launch_f2d(const cplx<float>* src, cplx<double>* dst, long n, hipStream_t);
 
 // push.hip
 // Neighbour-migration direction counts a push leaves for the next sort
-// (counts: [ntiles][MIG_DIRS]; flag: 2 ints zeroed before the push); a null
-// counts pointer skips the counting.
+// (counts: [ntiles][MIG_DIRS] of the tiles' main ranges, counts_x of their
+// extra ranges; flag: 2 ints zeroed before the push); a null counts pointer
+// skips the counting.
 constexpr int MIG_DIRS = 10;
 constexpr int MIG_REMOVED = 9;
 struct MigCount {
   uint32_t* counts;
+  uint32_t* counts_x;
   int* flag;
 };
+
+// Particles appended since the last sort (the moving window's injection):
+// tile t's extra range is [base + start[t], base + start[t + 1]), in key
+// order after the main ranges; start = null: none.
+struct ExtraRange {
+  const int* start;
+  int base;
+};
 void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const PartF64&,
-                             const int* tile_start, FieldSet<double>&, const Geo&, int ldsf, int* err, MigCount,
-                             hipStream_t);
+                             const int* tile_start, ExtraRange, FieldSet<double>&, const Geo&, int ldsf, int* err,
+                             MigCount, hipStream_t);
 // key of every particle: its tile, or with cell_key its tile-major cell (Tiles::key)
 void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, int cell_key, hipStream_t);
 // push_mixed.hip
 int launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, double charge, const PartMixed&,
-                              const int* tile_start, FieldSet<float>&, const float* geo, int ntab, int acc,
-                              int ldsf, int* err, MigCount, hipStream_t);
+                              const int* tile_start, ExtraRange, FieldSet<float>&, const float* geo, int ntab,
+                              int acc, int ldsf, int* err, MigCount, hipStream_t);
 void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, int cell_key, hipStream_t);
+// moving window: remove the particles left of the moved x_min (key = ntiles);
+// the mixed path's cell numbers drop by ncells
+void launch_window_mixed(const PartMixed&, long n, int ncells, int ntiles, hipStream_t);
+void launch_window_f64(const Grid&, const PartF64&, long n, int ntiles, hipStream_t);
 void launch_to_mixed(const Grid&, double mass, const double* const* src, const PartMixed&, long n, hipStream_t);
 void launch_from_mixed(const Grid&, double mass, const PartMixed&, double* const* dst, long n, hipStream_t);
 size_t push_lds_bytes_mixed(const Grid&, const Tiles&, int acc, int ldsf);
@@ -136,8 +154,9 @@ void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_
 // optional) and tile_start_new (ntiles + 2 ints); scratch holds
 // mig_scratch_words(ntiles) words.
 size_t mig_scratch_words(int ntiles);
-void launch_mig_count(const uint32_t* key, const int* tile_start, const Tiles& tl, MigCount mc, hipStream_t);
-void launch_mig_scatter(const uint32_t* key, const int* tile_start, const Tiles& tl, const uint32_t* counts,
+void launch_mig_count(const uint32_t* key, const int* tile_start, ExtraRange xr, const Tiles& tl, MigCount mc,
+                      hipStream_t);
+void launch_mig_scatter(const uint32_t* key, const int* tile_start, ExtraRange xr, const Tiles& tl, MigCount mc,
                         uint32_t* scratch, int* tile_start_new, int word_bytes, const void* const* src,
                         void* const* dst, int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out,
                         hipStream_t);

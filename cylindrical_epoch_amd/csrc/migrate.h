@@ -19,12 +19,12 @@ __device__ __forceinline__ int mig_dir(uint32_t d, int S, int ntx, uint32_t ntil
   return (dty + 1) * 3 + (dtx + 1);
 }
 
-// Counts the keys of tile S's range [p0, p1) by direction into
-// mc.counts[S * MIG_DIRS + dir]; mc.flag[0] |= 1 on a farther move,
-// mc.flag[1] += particles kept.  All work-items of the workgroup call it; c
-// is MIG_DIRS words of LDS.  The keys must be visible to the workgroup.
+// Counts the keys of the range [p0, p1) of tile S by direction into
+// counts[S * MIG_DIRS + dir]; flag[0] |= 1 on a farther move, flag[1] +=
+// particles kept.  All work-items of the workgroup call it; c is MIG_DIRS
+// words of LDS.  The keys must be visible to the workgroup.
 __device__ __forceinline__ void mig_count_tile(const uint32_t* __restrict__ key, int S, int p0, int p1, int ntx,
-                                               int ntiles, const MigCount& mc, uint32_t* c) {
+                                               int ntiles, uint32_t* counts, int* flag, uint32_t* c) {
   if (threadIdx.x < MIG_DIRS) c[threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -40,14 +40,15 @@ __device__ __forceinline__ void mig_count_tile(const uint32_t* __restrict__ key,
       if (lane == 0 && m) atomicAdd(&c[k], (uint32_t)__popcll(m));
     }
   }
-  if (__ballot(far) && lane == 0) atomicOr(mc.flag, 1);
+  if (__ballot(far) && lane == 0) atomicOr(flag, 1);
   __syncthreads();
-  if (threadIdx.x < MIG_DIRS) mc.counts[(long)S * MIG_DIRS + threadIdx.x] = c[threadIdx.x];
+  if (threadIdx.x < MIG_DIRS) counts[(long)S * MIG_DIRS + threadIdx.x] = c[threadIdx.x];
   if (threadIdx.x == 0) {
     uint32_t alive = 0;
     for (int k = 0; k < 9; ++k) alive += c[k];
-    if (alive) atomicAdd(mc.flag + 1, (int)alive);
+    if (alive) atomicAdd(flag + 1, (int)alive);
   }
+  __syncthreads();  // c may be reused by the caller
 }
 
 }  // namespace cylpic

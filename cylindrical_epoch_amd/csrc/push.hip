@@ -153,7 +153,7 @@ __device__ __forceinline__ int bcs_f64(const Grid& g, const Tiles& tl, const Par
 constexpr int FIELD_HALO_F64 = 2;
 template <int LDSF>
 __global__ __launch_bounds__(256) void k_push_deposit_f64(
-    Grid g, Tiles tl, PushConst pc, PartF64 sp, const int* __restrict__ tile_start,
+    Grid g, Tiles tl, PushConst pc, PartF64 sp, const int* __restrict__ tile_start, ExtraRange xr,
     const cplx<double>* __restrict__ ex, const cplx<double>* __restrict__ er,
     const cplx<double>* __restrict__ et, const cplx<double>* __restrict__ bx,
     const cplx<double>* __restrict__ br, const cplx<double>* __restrict__ bt,
@@ -196,13 +196,17 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
   }
   __syncthreads();
 
+  // the tile's particles: its main range, then its extra range
   const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
+  const int q0 = xr.start ? xr.base + xr.start[tile] : 0, q1 = xr.start ? xr.base + xr.start[tile + 1] : 0;
+  const int np_main = p1 - p0, np = np_main + (q1 - q0);
   const int jng = g.ng;
   const double fac = 0.25;
   const double third = 1.0 / 3.0, sixth = 0.5 * third;
   const int M = g.M;
 
-  for (int ip = p0 + (int)threadIdx.x; ip < p1; ip += blockDim.x) {
+  for (int k = threadIdx.x; k < np; k += blockDim.x) {
+    const int ip = k < np_main ? p0 + k : q0 + (k - np_main);
     if (sp.key[ip] >= tl.ntiles) continue;  // removed since the last sort
     const double part_weight = sp.w[ip];
     double part_x = sp.x[ip], part_y = sp.y[ip], part_z = sp.z[ip];
@@ -479,7 +483,23 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
     if (imv != 0.0) unsafeAtomicAdd(p + 1, imv);
   }
   // direction counts of the tile's new keys for the next migration sort
-  if (migc.counts) mig_count_tile((const uint32_t*)sp.key, tile, p0, p1, tl.ntx, tl.ntiles, migc, migs);
+  if (migc.counts) {
+    mig_count_tile((const uint32_t*)sp.key, tile, p0, p1, tl.ntx, tl.ntiles, migc.counts, migc.flag, migs);
+    if (xr.start)
+      mig_count_tile((const uint32_t*)sp.key, tile, q0, q1, tl.ntx, tl.ntiles, migc.counts_x, migc.flag, migs);
+  }
+}
+
+// moving window (remove_particles, window.F90:330-350): particles left of the
+// moved x_min are removed
+__global__ void k_window_f64(Grid g, PartF64 sp, long n, int ntiles) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (sp.x[i] < g.x_min) sp.key[i] = ntiles;
+}
+
+void launch_window_f64(const Grid& g, const PartF64& sp, long n, int ntiles, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_window_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, sp, n, ntiles);
 }
 
 __global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n, int cell_key) {
@@ -498,17 +518,17 @@ size_t push_lds_bytes(const Grid& g, const Tiles& tl, int ldsf) {
 }
 
 void launch_push_deposit_f64(const Grid& g, const Tiles& tl, const PushConst& pc, const PartF64& sp,
-                             const int* tile_start, FieldSet<double>& F, const Geo& geo, int ldsf, int* err,
-                             MigCount migc, hipStream_t s) {
+                             const int* tile_start, ExtraRange xr, FieldSet<double>& F, const Geo& geo, int ldsf,
+                             int* err, MigCount migc, hipStream_t s) {
   const size_t lds = push_lds_bytes(g, tl, ldsf);
   if (ldsf) {
     if (lds > 65536)
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_f64<1>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_push_deposit_f64<1>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
+    hipLaunchKernelGGL(k_push_deposit_f64<1>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start, xr,
                        F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err, migc);
   } else {
-    hipLaunchKernelGGL(k_push_deposit_f64<0>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start,
+    hipLaunchKernelGGL(k_push_deposit_f64<0>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start, xr,
                        F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), geo, err, migc);
   }
 }

@@ -30,12 +30,14 @@
 
 namespace cylpic {
 
-constexpr int FIELD_HALO = 2;
-// Current box border: a particle whose nearest cell c is in the tile deposits
-// into cells c-3 .. c+2 (the half-step cell is c-1 or c, the footprint spans
-// -2..+2 around it, and the always-zero last Jx column and Jr row are not
-// added), so the box reaches 3 cells below the tile and 2 above.
-constexpr int BOX_LO = 3, BOX_HI = 2;
+// LDS regions around a tile.  A particle whose nearest cell c is in the tile
+// gathers from cells c-2 .. c+2 and deposits into c-3 .. c+2 (the half-step
+// cell is c-1 or c, the footprint spans -2..+2 around it, and the always-zero
+// last Jx column and Jr row are not added).  In x the regions reach one cell
+// further down, for the cell a moving-window shift moves every particle by
+// between two sorts.
+constexpr int FIELD_LO_X = 3, FIELD_LO_R = 2, FIELD_HI = 2;
+constexpr int BOX_LO_X = 4, BOX_LO_R = 3, BOX_HI = 2;
 
 
 struct MixConst {
@@ -139,7 +141,7 @@ __device__ __forceinline__ float fixed_scale(float bound) {
 // per-row base register plus an immediate offset; 0: the shape of `tl`.
 template <int M, int ACC, int LDSF, int TX = 0, int TR = 0>
 __global__ __launch_bounds__(256) void k_push_deposit_mixed(
-    Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start,
+    Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start, ExtraRange xr,
     const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
     const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
     cplx<float>* __restrict__ jx, cplx<float>* __restrict__ jr, cplx<float>* __restrict__ jt, GeoF geo,
@@ -151,14 +153,14 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   Boxes<M> b;
   const int tx = TX > 0 ? TX : tl.tx, tr = TR > 0 ? TR : tl.tr;
-  b.fx0 = ttx * tx + 1 - FIELD_HALO;
-  b.fr0 = tty * tr + 1 - FIELD_HALO;
-  b.FX = tx + 2 * FIELD_HALO;
-  b.FR = tr + 2 * FIELD_HALO;
-  b.jx0 = ttx * tx + 1 - BOX_LO;
-  b.jr0 = tty * tr + 1 - BOX_LO;
-  b.LX = tx + BOX_LO + BOX_HI;
-  b.LR = tr + BOX_LO + BOX_HI;
+  b.fx0 = ttx * tx + 1 - FIELD_LO_X;
+  b.fr0 = tty * tr + 1 - FIELD_LO_R;
+  b.FX = tx + FIELD_LO_X + FIELD_HI;
+  b.FR = tr + FIELD_LO_R + FIELD_HI;
+  b.jx0 = ttx * tx + 1 - BOX_LO_X;
+  b.jr0 = tty * tr + 1 - BOX_LO_R;
+  b.LX = tx + BOX_LO_X + BOX_HI;
+  b.LR = tr + BOX_LO_R + BOX_HI;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
   const int nf = LDSF ? 6 * M * b.FR * b.FX : 0;  // complex cells of the field region
   // current box: planes [3][M] of LR x LX cells; each plane holds the real
@@ -174,21 +176,24 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   cplx<float>* fl = (cplx<float>*)(((float*)lds_raw) + ((jwords + 1) & ~1));  // [6][M][FR][FX]
 
   for (int e = threadIdx.x; e < jwords; e += blockDim.x) jf[e] = 0.0f;
+  // the tile's particles: its main range, then its extra range
   const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
+  const int q0 = xr.start ? xr.base + xr.start[tile] : 0, q1 = xr.start ? xr.base + xr.start[tile + 1] : 0;
+  const int np_main = p1 - p0, np = np_main + (q1 - q0);
   // fixed-point scales: |Jx|, |Jr| words of one particle stay below
   // 9 |q fac / dt| w and |Jt| words below 10.5 |q fac c| w (|Hx|, |sum hy| <= 2,
   // |m2| <= 2, |m3| <= 1, |m4| <= 2/3, shape weights <= 1.5, |v_theta| <= c)
   float sxr = 1.0f, st = 1.0f;
   if (AccTraits<ACC>::fixed) {
     float wm = 0.0f;
-    for (int ip = p0 + (int)threadIdx.x; ip < p1; ip += blockDim.x) wm = fmaxf(wm, fabsf(sp.w[ip]));
+    for (int k = threadIdx.x; k < np; k += blockDim.x) wm = fmaxf(wm, fabsf(sp.w[k < np_main ? p0 + k : q0 + (k - np_main)]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o));
     if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = wm;
     __syncthreads();
     wm = fmaxf(fmaxf(wred[0], wred[1]), fmaxf(wred[2], wred[3]));
     // ACC 4 sums in 32 bits: one contribution per particle per word
-    const float cnt = ACC == 4 ? (float)(p1 - p0) : 1.0f;
+    const float cnt = ACC == 4 ? (float)np : 1.0f;
     sxr = fixed_scale(9.0f * fabsf(mc.q_fac_idt) * wm * cnt);
     st = fixed_scale(10.5f * fabsf(mc.q_fac_c) * wm * cnt);
   }
@@ -214,7 +219,8 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   const int jng = g.ng;
   const long fplane = (long)b.FR * b.FX;
 
-  for (int ip = p0 + (int)threadIdx.x; ip < p1; ip += blockDim.x) {
+  for (int k = threadIdx.x; k < np; k += blockDim.x) {
+    const int ip = k < np_main ? p0 + k : q0 + (k - np_main);
     if (sp.key[ip] >= tl.ntiles) continue;
     int cell = sp.cell[ip];
     const float ox = sp.ox[ip], y = sp.y[ip], z = sp.z[ip];
@@ -647,7 +653,11 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     if (imv != 0.0) unsafeAtomicAdd(p + 1, (float)(imv * fac));
   }
   // direction counts of the tile's new keys for the next migration sort
-  if (migc.counts) mig_count_tile((const uint32_t*)sp.key, tile, p0, p1, tl.ntx, tl.ntiles, migc, migs);
+  if (migc.counts) {
+    mig_count_tile((const uint32_t*)sp.key, tile, p0, p1, tl.ntx, tl.ntiles, migc.counts, migc.flag, migs);
+    if (xr.start)
+      mig_count_tile((const uint32_t*)sp.key, tile, q0, q1, tl.ntx, tl.ntiles, migc.counts_x, migc.flag, migs);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -658,6 +668,23 @@ __global__ void k_keys_mixed(Grid g, Tiles tl, MixConst mc, PartMixed sp, long n
   const float y = sp.y[ip], z = sp.z[ip];
   const int cyk = (int)floorf(sqrtf(y * y + z * z) - mc.yoff + 1.5f);
   sp.key[ip] = tl.key(sp.cell[ip], cyk, g.nx, g.ny, cell_key != 0);
+}
+
+// moving window (remove_particles, window.F90:330-350): the grid moved
+// ncells cells towards +x, so the cell numbers drop by ncells; particles now
+// left of x_min (cell < 1: x - x_grid_min = (cell - 1 + ox) dx with ox in
+// [-1/2, 1/2)) are removed
+__global__ void k_window_mixed(PartMixed sp, long n, int ncells, int ntiles) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = sp.cell[i] - ncells;
+  sp.cell[i] = c;
+  if (c < 1) sp.key[i] = ntiles;
+}
+
+void launch_window_mixed(const PartMixed& sp, long n, int ncells, int ntiles, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_window_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sp, n, ncells, ntiles);
 }
 
 // FP64 SI particles -> mixed representation
@@ -696,8 +723,9 @@ __global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, doubl
 
 // ------------------------------------------------------------------------
 size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl, int acc, int ldsf) {
-  const size_t nj = 3 * (size_t)g.M * (tl.tr + BOX_LO + BOX_HI) * (tl.tx + BOX_LO + BOX_HI);
-  const size_t nf = ldsf ? 6 * (size_t)g.M * (tl.tr + 2 * FIELD_HALO) * (tl.tx + 2 * FIELD_HALO) : 0;
+  const size_t nj = 3 * (size_t)g.M * (tl.tr + BOX_LO_R + BOX_HI) * (tl.tx + BOX_LO_X + BOX_HI);
+  const size_t nf =
+      ldsf ? 6 * (size_t)g.M * (tl.tr + FIELD_LO_R + FIELD_HI) * (tl.tx + FIELD_LO_X + FIELD_HI) : 0;
   const size_t jwords = (acc == 0 || acc == 4) ? 2 * nj : 4 * nj;
   return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(cplx<float>);
 }
@@ -718,7 +746,7 @@ MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
 }
 
 int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& pc, double charge,
-                              const PartMixed& sp, const int* tile_start, FieldSet<float>& F,
+                              const PartMixed& sp, const int* tile_start, ExtraRange xr, FieldSet<float>& F,
                               const float* geo, int ntab, int acc, int ldsf, int* err, MigCount migc,
                               hipStream_t s) {
   const size_t lds = push_lds_bytes_mixed(g, tl, acc, ldsf);
@@ -730,7 +758,7 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_mixed<MM, AA, LL, XX, RR>),  \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                  \
     hipLaunchKernelGGL((k_push_deposit_mixed<MM, AA, LL, XX, RR>), dim3(tl.ntiles), dim3(256), lds, s, g, tl, \
-                       mc, sp, tile_start, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), \
+                       mc, sp, tile_start, xr, F.ex(), F.er(), F.et(), F.bx(), F.br(), F.bt(), F.jx(), F.jr(), F.jt(), \
                        gf, err, migc);                                                                  \
   } while (0)
 #define LAUNCH(MM, AA, LL) LAUNCH_T(MM, AA, LL, 0, 0)

@@ -327,19 +327,25 @@ void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_
 constexpr int MIG_BLOCK = 256;
 constexpr int MIG_WAVES = MIG_BLOCK / 64;
 
-// counts[S * MIG_DIRS + dir]; flag[0] |= 1 on a farther move, flag[1] += alive particles
+// counts (and counts_x for the extra ranges) by direction; flag[0] |= 1 on a
+// farther move, flag[1] += particles kept
 __global__ __launch_bounds__(MIG_BLOCK) void k_mig_count(const uint32_t* __restrict__ key,
-                                                         const int* __restrict__ ts, int ntx, int ntiles,
-                                                         MigCount mc) {
+                                                         const int* __restrict__ ts, ExtraRange xr, int ntx,
+                                                         int ntiles, MigCount mc) {
   __shared__ uint32_t c[MIG_DIRS];
   const int S = blockIdx.x;
-  mig_count_tile(key, S, ts[S], ts[S + 1], ntx, ntiles, mc, c);
+  mig_count_tile(key, S, ts[S], ts[S + 1], ntx, ntiles, mc.counts, mc.flag, c);
+  if (xr.start)
+    mig_count_tile(key, S, xr.base + xr.start[S], xr.base + xr.start[S + 1], ntx, ntiles, mc.counts_x, mc.flag, c);
 }
 
-// per destination tile D: tot[D] = arrivals, pre[S * 9 + dir] = arrivals at D
-// from sources before S (ascending tile index); tot[ntiles] = 0
-__global__ void k_mig_offsets(const uint32_t* __restrict__ counts, int ntx, int ntr, int ntiles,
-                              uint32_t* __restrict__ tot, uint32_t* __restrict__ pre) {
+// per destination tile D: tot[D] = arrivals; pre[S * 9 + dir] = arrivals at
+// D from main ranges of sources before S (ascending tile index); the extra
+// ranges follow all main ranges: pre_x[S * 9 + dir] = all main arrivals +
+// extra arrivals from sources before S.  tot[ntiles] = 0.
+__global__ void k_mig_offsets(const uint32_t* __restrict__ counts, const uint32_t* __restrict__ counts_x, int ntx,
+                              int ntr, int ntiles, uint32_t* __restrict__ tot, uint32_t* __restrict__ pre,
+                              uint32_t* __restrict__ pre_x) {
   const int D = blockIdx.x * blockDim.x + threadIdx.x;
   if (D > ntiles) return;
   if (D == ntiles) {
@@ -348,49 +354,54 @@ __global__ void k_mig_offsets(const uint32_t* __restrict__ counts, int ntx, int 
   }
   const int tx = D % ntx, ty = D / ntx;
   int src[9];
-  uint32_t c[9];
+  uint32_t c[9], cx[9];
 #pragma unroll
   for (int dir = 0; dir < 9; ++dir) {
     const int dtx = dir % 3 - 1, dty = dir / 3 - 1;
     const int sty = ty - dty;
     src[dir] = -1;
     c[dir] = 0;
+    cx[dir] = 0;
     if (sty >= 0 && sty < ntr) {
       const int stx = ((tx - dtx) % ntx + ntx) % ntx;
       src[dir] = sty * ntx + stx;
       c[dir] = counts[(long)src[dir] * MIG_DIRS + dir];
+      if (counts_x) cx[dir] = counts_x[(long)src[dir] * MIG_DIRS + dir];
     }
   }
-  uint32_t total = 0;
+  uint32_t total = 0, total_x = 0;
 #pragma unroll
   for (int dir = 0; dir < 9; ++dir) {
     total += c[dir];
+    total_x += cx[dir];
+  }
+#pragma unroll
+  for (int dir = 0; dir < 9; ++dir) {
     if (src[dir] < 0) continue;
-    uint32_t p = 0;
+    uint32_t p = 0, px = 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-      if (src[k] >= 0 && (src[k] < src[dir] || (src[k] == src[dir] && k < dir))) p += c[k];
+      if (src[k] >= 0 && (src[k] < src[dir] || (src[k] == src[dir] && k < dir))) {
+        p += c[k];
+        px += cx[k];
+      }
     pre[(long)src[dir] * 9 + dir] = p;
+    if (counts_x) pre_x[(long)src[dir] * 9 + dir] = total + px;
   }
-  tot[D] = total;
+  tot[D] = total + total_x;
 }
 
 __global__ void k_mig_sentinel(int* ts, int ntiles) { ts[ntiles + 1] = ts[ntiles]; }
 
-// per source tile: stable rank of each particle among the tile's particles
-// with the same direction, then payload copy to its sorted position
+// Stable rank of each particle of the range [p0, p1) among those with the
+// same direction, then payload copy to its sorted position.
 template <class W>
-__global__ __launch_bounds__(MIG_BLOCK) void k_mig_scatter(const uint32_t* __restrict__ key,
-                                                           const int* __restrict__ ts, int ntx, int ntiles,
-                                                           const uint32_t* __restrict__ pre,
-                                                           const int* __restrict__ ts_new, Ptr8 p, int narr,
-                                                           uint32_t* __restrict__ key_out,
-                                                           const uint64_t* __restrict__ id_in,
-                                                           uint64_t* __restrict__ id_out) {
-  __shared__ uint32_t run[9];
-  __shared__ uint32_t wc[MIG_WAVES][9];
-  const int S = blockIdx.x;
-  const int p0 = ts[S], p1 = ts[S + 1];
+__device__ __forceinline__ void mig_scatter_range(const uint32_t* __restrict__ key, int S, int p0, int p1, int ntx,
+                                                  int ntiles, const uint32_t* __restrict__ pre,
+                                                  const int* __restrict__ ts_new, const Ptr8& p, int narr,
+                                                  uint32_t* __restrict__ key_out, const uint64_t* __restrict__ id_in,
+                                                  uint64_t* __restrict__ id_out, uint32_t* run,
+                                                  uint32_t (*wc)[9]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long lt = (1ull << lane) - 1ull;
   if (threadIdx.x < 9) run[threadIdx.x] = 0;
@@ -426,27 +437,49 @@ __global__ __launch_bounds__(MIG_BLOCK) void k_mig_scatter(const uint32_t* __res
   }
 }
 
+// per source tile: its main range, then its extra range
+template <class W>
+__global__ __launch_bounds__(MIG_BLOCK) void k_mig_scatter(const uint32_t* __restrict__ key,
+                                                           const int* __restrict__ ts, ExtraRange xr, int ntx,
+                                                           int ntiles, const uint32_t* __restrict__ pre,
+                                                           const uint32_t* __restrict__ pre_x,
+                                                           const int* __restrict__ ts_new, Ptr8 p, int narr,
+                                                           uint32_t* __restrict__ key_out,
+                                                           const uint64_t* __restrict__ id_in,
+                                                           uint64_t* __restrict__ id_out) {
+  __shared__ uint32_t run[9];
+  __shared__ uint32_t wc[MIG_WAVES][9];
+  const int S = blockIdx.x;
+  mig_scatter_range<W>(key, S, ts[S], ts[S + 1], ntx, ntiles, pre, ts_new, p, narr, key_out, id_in, id_out, run, wc);
+  if (xr.start)
+    mig_scatter_range<W>(key, S, xr.base + xr.start[S], xr.base + xr.start[S + 1], ntx, ntiles, pre_x, ts_new, p,
+                         narr, key_out, id_in, id_out, run, wc);
+}
+
 size_t mig_scratch_words(int ntiles) {
   long scan = 0, m = ntiles + 1;
   while (m > 1) {
     m = (m + SCAN_TILE - 1) / SCAN_TILE;
     scan += m;
   }
-  return (size_t)ntiles * 9 + (size_t)scan + 64;
+  return (size_t)ntiles * 18 + (size_t)scan + 64;
 }
 
-void launch_mig_count(const uint32_t* key, const int* ts, const Tiles& tl, MigCount mc, hipStream_t s) {
-  hipLaunchKernelGGL(k_mig_count, dim3(tl.ntiles), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, tl.ntiles, mc);
+void launch_mig_count(const uint32_t* key, const int* ts, ExtraRange xr, const Tiles& tl, MigCount mc,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_mig_count, dim3(tl.ntiles), dim3(MIG_BLOCK), 0, s, key, ts, xr, tl.ntx, tl.ntiles, mc);
 }
 
-void launch_mig_scatter(const uint32_t* key, const int* ts, const Tiles& tl, const uint32_t* counts,
+void launch_mig_scatter(const uint32_t* key, const int* ts, ExtraRange xr, const Tiles& tl, MigCount mc,
                         uint32_t* scratch, int* ts_new, int word_bytes, const void* const* src, void* const* dst,
                         int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out, hipStream_t s) {
   const int nt = tl.ntiles;
   uint32_t* pre = scratch;
-  uint32_t* tmp = pre + (size_t)nt * 9;
-  hipLaunchKernelGGL(k_mig_offsets, dim3((nt + 1 + 255) / 256), dim3(256), 0, s, counts, tl.ntx, tl.ntr, nt,
-                     (uint32_t*)ts_new, pre);
+  uint32_t* pre_x = pre + (size_t)nt * 9;
+  uint32_t* tmp = pre_x + (size_t)nt * 9;
+  hipLaunchKernelGGL(k_mig_offsets, dim3((nt + 1 + 255) / 256), dim3(256), 0, s, mc.counts,
+                     xr.start ? (const uint32_t*)mc.counts_x : nullptr, tl.ntx, tl.ntr, nt, (uint32_t*)ts_new, pre,
+                     pre_x);
   exclusive_scan((uint32_t*)ts_new, nt + 1, tmp, s);
   hipLaunchKernelGGL(k_mig_sentinel, dim3(1), dim3(1), 0, s, ts_new, nt);
   Ptr8 p;
@@ -455,11 +488,11 @@ void launch_mig_scatter(const uint32_t* key, const int* ts, const Tiles& tl, con
     p.dst[a] = dst[a];
   }
   if (word_bytes == 8)
-    hipLaunchKernelGGL(k_mig_scatter<uint64_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, nt, pre, ts_new,
-                       p, narr, key_out, id_in, id_out);
+    hipLaunchKernelGGL(k_mig_scatter<uint64_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, xr, tl.ntx, nt, pre, pre_x,
+                       ts_new, p, narr, key_out, id_in, id_out);
   else
-    hipLaunchKernelGGL(k_mig_scatter<uint32_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, tl.ntx, nt, pre, ts_new,
-                       p, narr, key_out, id_in, id_out);
+    hipLaunchKernelGGL(k_mig_scatter<uint32_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, xr, tl.ntx, nt, pre, pre_x,
+                       ts_new, p, narr, key_out, id_in, id_out);
 }
 
 }  // namespace cylpic

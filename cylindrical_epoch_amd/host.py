@@ -96,6 +96,20 @@ class Simulation:
         idarr = None if ids is None else np.ascontiguousarray(ids, np.int64)
         check(lib().cylpic_upload_particles(self.h, sid, n, *[_ptr(a) for a in arrs], _ptr(idarr)))
 
+    def inject_particles(self, sid, x, y, z, px, py, pz, w, ids=None):
+        """Append particles (the moving window's insert_particles batch)."""
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, px, py, pz, w)]
+        n = len(arrs[0])
+        idarr = None if ids is None else np.ascontiguousarray(ids, np.int64)
+        check(lib().cylpic_inject_particles(self.h, sid, n, *[_ptr(a) for a in arrs], _ptr(idarr)))
+
+    def move_window(self, ncells):
+        """shift_window (window.F90:62-97) for ncells cells."""
+        check(lib().cylpic_move_window(self.h, int(ncells)))
+        for _ in range(int(ncells)):
+            self.x_min = self.x_min + self.dx
+            self.x_max = self.x_min + self.nx * self.dx
+
     def particle_count(self, sid):
         n = C.c_int64()
         check(lib().cylpic_particle_count(self.h, sid, C.byref(n)))

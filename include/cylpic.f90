@@ -184,6 +184,25 @@ MODULE cylpic
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_set_sort_interval
 
+    FUNCTION cylpic_inject_particles(h, species, n, x, y, z, px, py, pz, w, ids) &
+        BIND(C, NAME='cylpic_inject_particles') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_INT64_T, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: species
+      INTEGER(C_INT64_T), VALUE :: n
+      REAL(C_DOUBLE), INTENT(IN) :: x(*), y(*), z(*), px(*), py(*), pz(*), w(*)
+      TYPE(C_PTR), VALUE :: ids
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_inject_particles
+
+    FUNCTION cylpic_move_window(h, ncells) BIND(C, NAME='cylpic_move_window') &
+        RESULT(ierr)
+      IMPORT :: C_PTR, C_INT
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: ncells
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_move_window
+
     FUNCTION cylpic_set_sort_method(h, method) &
         BIND(C, NAME='cylpic_set_sort_method') RESULT(ierr)
       IMPORT :: C_PTR, C_INT

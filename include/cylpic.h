@@ -125,6 +125,27 @@ int cylpic_update_eb_fields_final(cylpic_ctx *h, const double *src_xmin,
  * (src/epoch2d.F90:213-265 without optional physics, output or window). */
 int cylpic_step(cylpic_ctx *h, const double *src_xmin, const double *src_xmax);
 
+/* Moving window (src/housekeeping/window.F90).  The host keeps the
+ * reference's moving_window bookkeeping (window.F90:353-376: the shift
+ * fraction, start and stop times, the after-move boundaries) and its particle
+ * injection (insert_particles, window.F90:157-325: parser densities and the
+ * RNG), and calls:
+ *   cylpic_inject_particles  - before the shift, with the new particles of the
+ *                              ncells columns beyond x_max (SI, as in upload);
+ *                              they join the species without a full re-sort;
+ *   cylpic_move_window       - shift_window for ncells cells: per cell the
+ *                              grid moves by dx, E, B and J move one cell
+ *                              towards x_min with the x_max incoming-cell fix
+ *                              (shift_fields, window.F90:101-150), and the
+ *                              particles left of x_min are removed
+ *                              (remove_particles, window.F90:330-350).
+ * x must not be periodic. */
+int cylpic_inject_particles(cylpic_ctx *h, int species, int64_t n,
+                            const double *x, const double *y, const double *z,
+                            const double *px, const double *py, const double *pz,
+                            const double *w, const int64_t *ids);
+int cylpic_move_window(cylpic_ctx *h, int ncells);
+
 /* Re-sort every species' particles by tile key (stable), dropping removed
  * particles.  The key of a particle is its cell (src/housekeeping/
  * split_particle.F90:61-62) mapped to its tile.  Sorting happens on its own

@@ -1165,3 +1165,72 @@ void orc_mode_energy(const orc_grid *g, const orc_fields *f, double *w_e, double
     w_b[m] = kappa * 0.5 / mu0 * sb * g->dx * g->dy;
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* moving window, window.F90:62-153, 330-350                           */
+/* ------------------------------------------------------------------ */
+/* shift_field_modes, window.F90:132-150: f(i) = f(i+1) for i in
+ * [1-ng, nx+ng-1]; f(nx+ng) keeps its value.  field_mode_bc is an MPI halo
+ * exchange, a no-op on one rank with non-periodic x. */
+static void shift_field_modes(const orc_grid *g, ocplx *f) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng;
+  for (int m = 0; m < g->n_mode; ++m)
+    for (int j = 1 - ng; j <= ny + ng; ++j)
+      for (int i = 1 - ng; i <= nx + ng - 1; ++i) f[I3(g, i, j, m)] = f[I3(g, i + 1, j, m)];
+}
+
+/* shift_fields, window.F90:101-128 (x_max is a domain boundary on one rank) */
+static void shift_fields(const orc_grid *g, orc_fields *f) {
+  ocplx *arr[9] = {f->ex, f->er, f->et, f->bx, f->br, f->bt, f->jx, f->jr, f->jt};
+  for (int k = 0; k < 9; ++k) shift_field_modes(g, arr[k]);
+  const int nx = g->nx, ny = g->ny, ng = g->ng;
+  for (int m = 0; m < g->n_mode; ++m)
+    for (int j = 1 - ng; j <= ny + ng; ++j) {
+      const size_t s = I2(g, j, m);
+      f->ex[I3(g, nx + 1, j, m)] = f->bnd_xmax[0][s];
+      f->er[I3(g, nx, j, m)] = f->bnd_xmax[1][s];
+      f->et[I3(g, nx, j, m)] = f->bnd_xmax[2][s];
+      f->ex[I3(g, nx, j, m)] = rmul(0.5, cadd(f->ex[I3(g, nx - 1, j, m)], f->ex[I3(g, nx + 1, j, m)]));
+      f->er[I3(g, nx - 1, j, m)] = rmul(0.5, cadd(f->er[I3(g, nx - 2, j, m)], f->er[I3(g, nx, j, m)]));
+      f->et[I3(g, nx - 1, j, m)] = rmul(0.5, cadd(f->et[I3(g, nx - 2, j, m)], f->et[I3(g, nx, j, m)]));
+      f->bx[I3(g, nx, j, m)] = f->bnd_xmax[3][s];
+      f->br[I3(g, nx + 1, j, m)] = f->bnd_xmax[4][s];
+      f->bt[I3(g, nx + 1, j, m)] = f->bnd_xmax[5][s];
+      f->bx[I3(g, nx - 1, j, m)] = rmul(0.5, cadd(f->bx[I3(g, nx - 2, j, m)], f->bx[I3(g, nx, j, m)]));
+      f->br[I3(g, nx, j, m)] = rmul(0.5, cadd(f->br[I3(g, nx - 1, j, m)], f->br[I3(g, nx + 1, j, m)]));
+      f->bt[I3(g, nx, j, m)] = rmul(0.5, cadd(f->bt[I3(g, nx - 1, j, m)], f->bt[I3(g, nx + 1, j, m)]));
+    }
+}
+
+/* remove_particles, window.F90:330-350: drop x < x_min, keeping the order */
+static void remove_left(const orc_grid *g, orc_species *sp, int n_species) {
+  for (int s = 0; s < n_species; ++s) {
+    orc_species *p = &sp[s];
+    int64_t k = 0;
+    for (int64_t i = 0; i < p->n; ++i) {
+      if (p->x[i] < g->x_min) continue;
+      p->x[k] = p->x[i];
+      p->y[k] = p->y[i];
+      p->z[k] = p->z[i];
+      p->px[k] = p->px[i];
+      p->py[k] = p->py[i];
+      p->pz[k] = p->pz[i];
+      p->w[k] = p->w[i];
+      if (p->id) p->id[k] = p->id[i];
+      ++k;
+    }
+    p->n = k;
+  }
+}
+
+/* shift_window, window.F90:62-97 (no CPML: x_min = xb_min) */
+void orc_shift_window(orc_grid *g, orc_fields *f, orc_species *sp, int n_species, int ncells) {
+  for (int c = 0; c < ncells; ++c) {
+    /* x_global(1) + dx, xb_global(1) + dx, xb_global(nx + 1) */
+    g->x_grid_min = g->x_grid_min + g->dx;
+    g->x_min = g->x_min + g->dx;
+    g->x_max = g->x_min + (double)g->nx * g->dx;
+    remove_left(g, sp, n_species);
+    shift_fields(g, f);
+  }
+}

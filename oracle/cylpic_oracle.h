@@ -93,6 +93,12 @@ void orc_current_bcs(const orc_grid *g, orc_fields *f);
 void orc_current_bcs_r_min_final(const orc_grid *g, orc_fields *f);
 void orc_current_finish(const orc_grid *g, orc_fields *f);
 void orc_particle_bcs(const orc_grid *g, orc_species *sp, int n_species);
+/* ---- moving window (src/housekeeping/window.F90) -------------------- */
+/* shift_window for ncells cells: per cell the grid moves by dx, particles
+ * with x < x_min are removed and E, B, J shift one cell towards x_min with
+ * the x_max incoming-cell fix.  The injection of new particles is the
+ * caller's (append them to the species before the call). */
+void orc_shift_window(orc_grid *g, orc_fields *f, orc_species *sp, int n_species, int ncells);
 void orc_geometry(const orc_grid *g, double *inv_area_rt, double *inv_area_xt,
                   double *inv_volume, double *ratio_area_xt);
 

@@ -80,6 +80,7 @@ def _declare(L):
         "orc_push_particles": [gp, fp, sp, C.c_int],
         "orc_current_bcs": [gp, fp], "orc_current_bcs_r_min_final": [gp, fp],
         "orc_current_finish": [gp, fp], "orc_particle_bcs": [gp, sp, C.c_int],
+        "orc_shift_window": [gp, fp, sp, C.c_int, C.c_int],
         "orc_geometry": [gp, dp, dp, dp, dp],
         "orc_deposit_from_weights": [gp, fp, dp, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_double, C.c_double, C.c_double,
@@ -190,6 +191,23 @@ class State:
 
     def setup_field_boundaries(self):
         lib().orc_setup_field_boundaries(C.byref(self.g), C.byref(self.f))
+
+    def append_particles(self, species, x, y, z, px, py, pz, w, ids=None):
+        """Append particles to a species (the window's injection)."""
+        sp = self.species[species]
+        d = dict(x=x, y=y, z=z, px=px, py=py, pz=pz, w=w)
+        n = len(x)
+        for k, v in d.items():
+            sp[k] = np.concatenate([sp[k][:sp["n"]], np.asarray(v, np.float64)])
+        new_ids = (np.arange(n, dtype=np.int64) if ids is None else np.asarray(ids, np.int64))
+        sp["id"] = np.concatenate([sp["id"][:sp["n"]], new_ids])
+        sp["n"] = sp["n"] + n
+
+    def shift_window(self, ncells):
+        """shift_window (window.F90:62-97) for ncells cells."""
+        arr = self._species_array()
+        lib().orc_shift_window(C.byref(self.g), C.byref(self.f), arr, len(self.species), int(ncells))
+        self._species_back(arr)
 
     def step(self, src_xmin=None, src_xmax=None):
         """One timestep of the reference main loop, epoch2d.F90:213-265

@@ -49,7 +49,7 @@ def load_fields(st, sim, fields):
 
 
 def add_plasma(st, sim, nx, ny, dx, dy, ppc, density=1e25, temp_ev=1000.0, drift=(0, 0, 0),
-               seed=1, q=-Q0, m=M0, shuffle=False):
+               seed=1, q=-Q0, m=M0, shuffle=False, capacity=None):
     p = deck.uniform_plasma(nx, ny, dx, dy, ppc, density, temp_ev, mass=m, seed=seed)
     for k, d in zip(("px", "py", "pz"), drift):
         p[k] = p[k] + d * m * C_LIGHT
@@ -58,7 +58,7 @@ def add_plasma(st, sim, nx, ny, dx, dy, ppc, density=1e25, temp_ev=1000.0, drift
         p = {k: v[perm] for k, v in p.items()}
     n = len(p["x"])
     st.add_species(q, m, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"], ids=p["id"])
-    sid = sim.set_species(q, m, False, capacity=n)
+    sid = sim.set_species(q, m, False, capacity=n if capacity is None else capacity)
     sim.upload_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"], ids=p["id"])
     return p
 

@@ -1,0 +1,68 @@
+"""The oracle's moving window (src/housekeeping/window.F90:62-153, 330-350)
+checked by what a window shift must do: translate the fields one cell
+towards x_min, keep a field that is uniform in x uniform, bring in the stored
+x_max boundary columns, move the grid by dx per cell and drop exactly the
+particles left of the new x_min, keeping the order of the others."""
+import numpy as np
+import pytest
+
+FIELDS9 = ("ex", "er", "et", "bx", "br", "bt", "jx", "jr", "jt")
+
+
+def _state(oracle, nx=24, ny=10, M=2):
+    dx, dy = 1e-7, 1e-7
+    bf = ("simple_outflow", "simple_outflow", "other", "open")
+    bp = ("open", "open", "other", "open")
+    return oracle.State(nx, ny, M, dx, dy, oracle.set_dt(dx, dy, 0.95), bc_field=bf, bc_particle=bp)
+
+
+def test_shift_translates_interior(oracle):
+    st = _state(oracle)
+    rng = np.random.default_rng(1)
+    for n in FIELDS9:
+        st.arr[n][:] = rng.normal(size=st.shape) + 1j * rng.normal(size=st.shape)
+    st.setup_field_boundaries()
+    before = {n: st.arr[n].copy() for n in FIELDS9}
+    k = 3
+    st.shift_window(k)
+    ng, nx = 5, st.g.nx
+    # array column c holds Fortran ix = c + 1 - ng; after k shifts f(ix) = f_old(ix + k)
+    # for every column untouched by the x_max fix (ix <= nx - 2 - ... for the last shift)
+    for n in FIELDS9:
+        a, b = st.arr[n], before[n]
+        last = nx - 3 + ng - 1  # array column of ix = nx - 3
+        assert np.array_equal(a[:, :, : last - k + 1], b[:, :, k: last + 1]), n
+
+
+def test_uniform_field_stays_uniform(oracle):
+    st = _state(oracle)
+    rng = np.random.default_rng(2)
+    prof = {n: rng.normal(size=(st.shape[0], st.shape[1], 1)) for n in FIELDS9}
+    for n in FIELDS9:
+        st.arr[n][:] = np.broadcast_to(prof[n], st.shape)
+    st.setup_field_boundaries()
+    st.shift_window(2)
+    for n in FIELDS9:
+        # every column equals the profile: the boundary columns stored from a
+        # uniform field and their averages reproduce it
+        assert np.allclose(st.arr[n], np.broadcast_to(prof[n], st.shape), rtol=0, atol=1e-15), n
+
+
+def test_grid_moves_and_particles_left_of_x_min_go(oracle):
+    st = _state(oracle)
+    dx = st.g.dx
+    x0_min, x0_grid = st.g.x_min, st.g.x_grid_min
+    n = 400
+    rng = np.random.default_rng(3)
+    x = x0_min + rng.uniform(-0.5, 5.0, n) * dx
+    y = rng.uniform(0.1, 5.0, n) * st.g.dy
+    z = np.zeros(n)
+    st.add_species(-1.6e-19, 9.1e-31, x, y, z, z, z, z, np.ones(n), ids=np.arange(n))
+    st.shift_window(3)
+    assert st.g.x_grid_min == x0_grid + dx + dx + dx
+    assert st.g.x_min == x0_min + dx + dx + dx
+    assert st.g.x_max == st.g.x_min + st.g.nx * dx
+    keep = x >= st.g.x_min
+    sp = st.species[0]
+    assert sp["n"] == keep.sum()
+    assert np.array_equal(sp["id"][: sp["n"]], np.arange(n)[keep])  # order kept
