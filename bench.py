@@ -3,13 +3,14 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed|both]
 
 A step is one full timestep of the reference's main loop (update_eb_fields_half,
-push_particles, current_finish, update_eb_fields_final; src/epoch2d.F90:213-265)
-over the whole particle load, with every input resident in HBM.  Workload: the
-8192 x 512 cell, 3-mode, 32 particles-per-cell single run (134,217,728
-electrons, n_e = 5e23 m^-3 plateau, dx = 0.02 um, dy = 0.1 um) with a Gaussian
-laser (lambda 0.8 um, a0 = 3, waist 15 um, 25 fs) entering through x_min, open
-particle boundaries, no moving window.  Synthetic data: particles from a seeded
-generator with a 100 eV thermal spread.
+push_particles, current_finish, update_eb_fields_final, moving_window;
+src/epoch2d.F90:213-265) over the whole particle load, with every input
+resident in HBM.  Workload: the 8192 x 512 cell, 3-mode, 32 particles-per-cell
+single run (134,217,728 electrons, n_e = 5e23 m^-3 plateau, dx = 0.02 um,
+dy = 0.1 um) with a Gaussian laser (lambda 0.8 um, a0 = 3, waist 15 um, 25 fs)
+entering through x_min, open particle boundaries and a window moving at c
+(a column of 16,384 new particles injected per cell moved).  Synthetic data:
+particles from a seeded generator with a 100 eV thermal spread.
 
 Prints one JSON line.  The CPU baseline is the FP64 oracle (one host core) timed
 on a sample of the same workload and extrapolated per particle.
@@ -58,7 +59,44 @@ class Laser:
         return np.concatenate([base * math.cos(self.pol), base * math.sin(self.pol)])
 
 
-def make_sim(w, precision, particles, tune=None):
+class Window:
+    """The reference's moving_window (window.F90:353-376) at window_v_x = c
+    from t = 0: the shift fraction accumulates dt v / dx, and each whole cell
+    injects a column of new particles beyond x_max (insert_particles,
+    window.F90:157-325) and shifts the window.  The columns are drawn ahead
+    of time from the same generator as the initial load (synthetic data)."""
+
+    def __init__(self, w, dt, max_cells, seed=11):
+        self.w, self.dt, self.fraction = w, dt, 0.0
+        self.v = deck.C_LIGHT
+        self.batches = []
+        for k in range(max_cells):
+            p = deck.uniform_plasma(1, w["ny"], w["dx"], w["dy"], w["ppc"], w["density"], 100.0,
+                                    seed=seed + k, cells_x=[0])
+            self.batches.append({key: p[key] for key in ("x", "y", "z", "px", "py", "pz", "w")})
+        self.used = 0
+        self.injected = 0
+
+    @staticmethod
+    def cells_for(w, dt, steps):
+        return int(math.ceil(steps * dt * deck.C_LIGHT / w["dx"])) + 2
+
+    def advance(self, sim, sid):
+        self.fraction += self.dt * self.v / self.w["dx"]
+        k = int(math.floor(self.fraction))
+        if k <= 0:
+            return
+        for c in range(k):
+            p = self.batches[self.used]
+            self.used += 1
+            sim.inject_particles(sid, p["x"] + (sim.x_max + c * self.w["dx"]), p["y"], p["z"], p["px"],
+                                 p["py"], p["pz"], p["w"])
+            self.injected += len(p["x"])
+        sim.move_window(k)
+        self.fraction -= k
+
+
+def make_sim(w, precision, particles, tune=None, extra_capacity=0):
     dx, dy = w["dx"], w["dy"]
     laser = Laser(w["ny"], dy)
     dt = deck.set_dt(dx, dy, 0.95, plasma=[(w["density"], 100.0 * deck.EV / deck.KB, deck.M0)],
@@ -74,29 +112,33 @@ def make_sim(w, precision, particles, tune=None):
     if tune.get("sort_method"):
         sim.set_sort_method(tune["sort_method"])
     n = len(particles["x"])
-    sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=n)
+    sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=n + extra_capacity)
     sim.upload_particles(sid, particles["x"], particles["y"], particles["z"], particles["px"],
                          particles["py"], particles["pz"], particles["w"])
     sim.setup_field_boundaries()
     return sim, laser, dt
 
 
-def run(sim, laser, dt, steps, warmup):
+def run(sim, laser, dt, steps, warmup, window=None):
     t = 0.0
     for _ in range(warmup):
         t += dt
         sim.step(laser.sources(t))
+        if window:
+            window.advance(sim, 0)
     sim.synchronize()
     sim.profile(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         t += dt
         sim.step(laser.sources(t))
+        if window:
+            window.advance(sim, 0)
     sim.synchronize()
     t1 = time.perf_counter()
     prof = {k: sim.kernel_time(k) for k in ("push_deposit", "update_e", "update_b", "field_bcs",
                                             "copy_old", "zero_j", "current_axis", "particle_bcs",
-                                            "current_finish", "sort")}
+                                            "current_finish", "sort", "window", "inject")}
     sim.profile(False)
     return (t1 - t0) / steps, prof
 
@@ -200,6 +242,7 @@ def main():
     ap.add_argument("--acc-mode", type=int, default=None, help="mixed current accumulator 0..4")
     ap.add_argument("--gather-lds", type=int, default=None, help="mixed gather from LDS (1) or L1/L2 (0)")
     ap.add_argument("--sort-method", default=None, choices=["radix", "migrate"])
+    ap.add_argument("--no-window", action="store_true", help="keep the window fixed (no injection)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -224,9 +267,19 @@ def main():
         tune["tile_" + k] = tuple(int(v) for v in t.split("x")) if t else None
     results = {}
     for prec in precs:
-        sim, laser, dt = make_sim(w, prec, parts, tune)
+        win = None
+        extra = 0
+        if not a.no_window:
+            dt0 = deck.set_dt(w["dx"], w["dy"], 0.95)
+            cells = Window.cells_for(w, dt0, a.steps + a.warmup)
+            extra = cells * w["ny"] * w["ppc"]
+        sim, laser, dt = make_sim(w, prec, parts, tune, extra_capacity=extra)
+        if not a.no_window:
+            win = Window(w, dt, Window.cells_for(w, dt, a.steps + a.warmup))
         log(f"{prec}: uploaded, dt = {dt:.4e}")
-        ms, prof = run(sim, laser, dt, a.steps, a.warmup)
+        ms, prof = run(sim, laser, dt, a.steps, a.warmup, win)
+        if win:
+            log(f"{prec}: window moved {win.used} cells, {win.injected} particles injected")
         sorts = sim.sort_counts()
         sim.close()
         results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof, sorts=sorts)
@@ -246,7 +299,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": a.workload, "nx": w["nx"], "ny": w["ny"], "n_mode": w["n_mode"],
                    "ppc": w["ppc"], "particles": n, "precision": main_prec, "parallelism": "single-gpu",
-                   "laser": "x_min, a0=3, lambda 0.8 um, waist 15 um, 25 fs", "moving_window": False},
+                   "laser": "x_min, a0=3, lambda 0.8 um, waist 15 um, 25 fs",
+                   "moving_window": not a.no_window, "window_v_x": "c" if not a.no_window else None},
         "field_cell_updates_per_s": round(cells / (r["ms"] / 1e3), 1),
         "per_precision": {k: {"ms_per_step": round(v["ms"], 4), "particle_steps_per_s": round(v["value"], 1),
                               "kernel_ms_per_step": {kk: round(kms / a.steps, 4) for kk, (kms, cnt) in

@@ -256,7 +256,11 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   Tiles t;
   t.tx = tx;
   t.tr = tr;
-  t.ntx = (h->g.nx + tx - 1) / tx;
+  t.shift = 0;
+  // a spare column for the moving window's tile offset (non-periodic x only)
+  const bool periodic_x = h->g.bc_field[XMIN] == BC_PERIODIC || h->g.bc_field[XMAX] == BC_PERIODIC ||
+                          h->g.bc_particle[XMIN] == BC_PERIODIC || h->g.bc_particle[XMAX] == BC_PERIODIC;
+  t.ntx = (h->g.nx + tx - 1) / tx + (periodic_x ? 0 : 1);
   t.ntr = (h->g.ny + tr - 1) / tr;
   t.ntiles = t.ntx * t.ntr;
   size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t, h->acc_mode, h->gather_lds)
@@ -808,6 +812,7 @@ int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double*
   if (!s.ranged) return fail(CYLPIC_ESTATE, "upload the species' particles before injecting");
   if ((ids != nullptr) != s.ids) return fail(CYLPIC_EINVAL, "ids must be given exactly when the upload gave them");
   HIPCHK(hipSetDevice(h->device));
+  Scope scope(h, "inject");
   if (s.n_extra > 0) {  // merge the previous batch first
     int rc = sort_species(h, s);
     if (rc) return rc;
@@ -871,7 +876,8 @@ int cylpic_move_window(cylpic_ctx* h, int ncells) {
   CHECK_H(h);
   if (ncells < 0) return fail(CYLPIC_EINVAL, "ncells must be >= 0");
   if (ncells == 0) return CYLPIC_OK;
-  if (h->g.bc_field[XMIN] == BC_PERIODIC || h->g.bc_field[XMAX] == BC_PERIODIC)
+  if (h->g.bc_field[XMIN] == BC_PERIODIC || h->g.bc_field[XMAX] == BC_PERIODIC ||
+      h->g.bc_particle[XMIN] == BC_PERIODIC || h->g.bc_particle[XMAX] == BC_PERIODIC)
     return fail(CYLPIC_EINVAL, "a moving window needs non-periodic x boundaries");
   HIPCHK(hipSetDevice(h->device));
   Scope sc(h, "window");
@@ -895,6 +901,28 @@ int cylpic_move_window(cylpic_ctx* h, int ncells) {
     else launch_window_mixed(viewm(s), s.n, ncells, h->tl.ntiles, h->stream);
     s.dirty = true;
     s.counted = false;
+  }
+  // the tiles move with the plasma: raise their offset; each time it reaches
+  // a whole tile, relabel the columns one down (order and ranges unchanged;
+  // the vacated first column held only particles that just left the grid)
+  for (int c = 0; c < ncells; ++c) {
+    if (++h->tl.shift < h->tl.tx) continue;
+    h->tl.shift = 0;
+    for (auto& s : h->sp) {
+      if (!s.tile_start) continue;
+      launch_relabel_tiles(s.key[0], s.n, h->tl.ntiles, s.tile_start, s.tile_start_alt, h->stream);
+      std::swap(s.tile_start, s.tile_start_alt);
+      s.d_count = s.tile_start + h->tl.ntiles;
+      if (s.n_extra > 0) {
+        if (!h->mig) {
+          HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
+          HIPCHK(hipHostMalloc((void**)&h->mig_host, 2 * sizeof(int), 0));
+        }
+        launch_relabel_tiles(nullptr, 0, h->tl.ntiles, s.extra_start, (int*)h->mig, h->stream);
+        HIPCHK(hipMemcpyAsync(s.extra_start, h->mig, (h->tl.ntiles + 2) * sizeof(int), hipMemcpyDeviceToDevice,
+                              h->stream));
+      }
+    }
   }
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;

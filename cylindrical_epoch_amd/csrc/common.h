@@ -51,16 +51,24 @@ struct Grid {
 
 // Tile decomposition of the push/deposit: tiles of tile_x x tile_r cells,
 // numbered row-major in x; H halo cells around each tile in the LDS current box.
+// Tile column t covers cells [t tx - shift + 1, (t + 1) tx - shift]: a moving
+// window raises `shift` by one per cell it moves, so the tiles move with the
+// plasma and no particle changes tile because of a window shift; at shift =
+// tx the columns are relabelled one down.  With non-periodic x there is one
+// spare tile column for that offset.
 struct Tiles {
   int tx, tr, ntx, ntr, ntiles;
+  int shift;
   // Key of a (clamped) cell: its tile (cell_key = false), or the tile-major
   // cell number tile * tx * tr + row-in-tile * tx + column-in-tile.
   __host__ __device__ __forceinline__ int key(int cx, int cy, int nx, int ny, bool cell_key) const {
     cx = cx < 1 ? 1 : (cx > nx ? nx : cx);
     cy = cy < 1 ? 1 : (cy > ny ? ny : cy);
-    const int lx = cx - 1, ly = cy - 1, t = (ly / tr) * ntx + lx / tx;
+    const int lx = cx - 1 + shift, ly = cy - 1, t = (ly / tr) * ntx + lx / tx;
     return cell_key ? (t * tr + ly % tr) * tx + lx % tx : t;
   }
+  // first cell (x) of tile column ttx
+  __host__ __device__ __forceinline__ int x0(int ttx) const { return ttx * tx + 1 - shift; }
 };
 constexpr int DEPOSIT_HALO = 3;
 

@@ -134,6 +134,9 @@ size_t sort_scratch_words(long n);
 void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, SortScratch& s,
                 uint32_t* tiles_out, uint32_t** perm_out, hipStream_t);
 void launch_tile_bounds(const uint32_t* sorted_tiles, long n, int ntiles, int* tile_start, hipStream_t);
+// tile columns relabelled one down (keys of n particles, null: ranges only);
+// ts_out = the ranges under the new labels
+void launch_relabel_tiles(uint32_t* key, long n, int ntiles, const int* ts_in, int* ts_out, hipStream_t);
 // key[i] /= div (tile-major cell keys -> tile keys)
 void launch_div_keys(uint32_t* key, long n, uint32_t div, hipStream_t);
 // Payload permutation dst[d(i)] = src[perm[i]]; d(i) = i, or with `inter`

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
   const int tile = blockIdx.x;
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   LdsBox box;
-  box.x0 = ttx * tl.tx + 1 - DEPOSIT_HALO;
+  box.x0 = tl.x0(ttx) - DEPOSIT_HALO;
   box.r0 = tty * tl.tr + 1 - DEPOSIT_HALO;
   box.LX = tl.tx + 2 * DEPOSIT_HALO;
   box.LR = tl.tr + 2 * DEPOSIT_HALO;
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
   // field region [6][M][FR][FX] after the current box: the tile plus the
   // 2-cell reach of the gather stencils
   cplx<double>* fl = (cplx<double>*)(lds + 2 * nl);
-  const int fx0 = ttx * tl.tx + 1 - FIELD_HALO_F64, fr0 = tty * tl.tr + 1 - FIELD_HALO_F64;
+  const int fx0 = tl.x0(ttx) - FIELD_HALO_F64, fr0 = tty * tl.tr + 1 - FIELD_HALO_F64;
   const int FX = tl.tx + 2 * FIELD_HALO_F64, FR = tl.tr + 2 * FIELD_HALO_F64;
   const long fplane = (long)FX * FR;
   if (LDSF) {

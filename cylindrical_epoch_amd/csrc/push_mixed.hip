@@ -153,11 +153,11 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   Boxes<M> b;
   const int tx = TX > 0 ? TX : tl.tx, tr = TR > 0 ? TR : tl.tr;
-  b.fx0 = ttx * tx + 1 - FIELD_LO_X;
+  b.fx0 = ttx * tx + 1 - tl.shift - FIELD_LO_X;
   b.fr0 = tty * tr + 1 - FIELD_LO_R;
   b.FX = tx + FIELD_LO_X + FIELD_HI;
   b.FR = tr + FIELD_LO_R + FIELD_HI;
-  b.jx0 = ttx * tx + 1 - BOX_LO_X;
+  b.jx0 = ttx * tx + 1 - tl.shift - BOX_LO_X;
   b.jr0 = tty * tr + 1 - BOX_LO_R;
   b.LX = tx + BOX_LO_X + BOX_HI;
   b.LR = tr + BOX_LO_R + BOX_HI;
@@ -600,11 +600,8 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     int key = tl.ntiles;
     if (!out) {
       const float rk = sqrtf(yn * yn + zn * zn);
-      int cyk = (int)floorf(rk - mc.yoff + 1.5f);
-      int cxk = cell;
-      cxk = cxk < 1 ? 1 : (cxk > g.nx ? g.nx : cxk);
-      cyk = cyk < 1 ? 1 : (cyk > g.ny ? g.ny : cyk);
-      key = ((cyk - 1) / tl.tr) * tl.ntx + (cxk - 1) / tl.tx;
+      const int cyk = (int)floorf(rk - mc.yoff + 1.5f);
+      key = tl.key(cell, cyk, g.nx, g.ny, false);
     }
     sp.key[ip] = key;
   }

@@ -238,6 +238,24 @@ void launch_tile_bounds(const uint32_t* sorted_keys, long n, int ntiles, int* ti
                      ntiles, tile_start);
 }
 
+// tile columns relabelled one down: keys k < ntiles -> k - 1, and range
+// starts ts_out[k] = ts_in[k + 1] (k < ntiles), the two last entries kept
+__global__ void k_relabel_keys(uint32_t* __restrict__ key, long n, uint32_t ntiles) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && key[i] < ntiles) key[i] -= 1;
+}
+__global__ void k_relabel_ranges(const int* __restrict__ ts_in, int* __restrict__ ts_out, int ntiles) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ntiles) ts_out[k] = ts_in[k + 1];
+  else if (k <= ntiles + 1) ts_out[k] = ts_in[k];
+}
+
+void launch_relabel_tiles(uint32_t* key, long n, int ntiles, const int* ts_in, int* ts_out, hipStream_t s) {
+  if (key && n > 0)
+    hipLaunchKernelGGL(k_relabel_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, key, n, (uint32_t)ntiles);
+  hipLaunchKernelGGL(k_relabel_ranges, dim3((ntiles + 2 + 255) / 256), dim3(256), 0, s, ts_in, ts_out, ntiles);
+}
+
 __global__ void k_div_keys(uint32_t* __restrict__ key, long n, uint32_t div) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) key[i] /= div;

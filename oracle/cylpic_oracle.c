@@ -1101,9 +1101,9 @@ void orc_particle_bcs(const orc_grid *g, orc_species *sp, int n_species) {
 /* cell key (split_particle.F90:61-62) and stable counting sort         */
 /* ------------------------------------------------------------------ */
 void orc_cell_keys(const orc_grid *g, int64_t n, const double *x, const double *y,
-                   const double *z, int tile_x, int tile_r, int32_t *keys) {
+                   const double *z, int tile_x, int tile_r, int shift, int32_t *keys) {
   const int nx = g->nx, ny = g->ny;
-  const int ntx = (nx + tile_x - 1) / tile_x;
+  const int ntx = (nx + shift + tile_x - 1) / tile_x;
   for (int64_t i = 0; i < n; ++i) {
     double r = sqrt(y[i] * y[i] + z[i] * z[i]);
     int cx = (int)floor((x[i] - g->x_grid_min) / g->dx + 1.5);
@@ -1111,7 +1111,7 @@ void orc_cell_keys(const orc_grid *g, int64_t n, const double *x, const double *
     /* a particle on the domain edge within rounding maps to the edge cell */
     cx = cx < 1 ? 1 : (cx > nx ? nx : cx);
     cy = cy < 1 ? 1 : (cy > ny ? ny : cy);
-    int lx = cx - 1, ly = cy - 1;
+    int lx = cx - 1 + shift, ly = cy - 1;
     int tx = lx / tile_x, ty = ly / tile_r;
     keys[i] = (int32_t)((((int64_t)ty * ntx + tx) * tile_r + (ly - ty * tile_r)) * tile_x +
                         (lx - tx * tile_x));

@@ -120,11 +120,12 @@ void orc_deposit_from_weights(const orc_grid *g, orc_fields *f,
 
 /* ---- cell key sort (key: src/housekeeping/split_particle.F90:61-62) --- */
 /* Tile-major cell key: tiles of tile_x * tile_r cells, tiles row-major in x,
- * cells inside a tile row-major in x.  Cells are clamped into 1..nx, 1..ny
- * (a particle on the outer edge within rounding). */
+ * cells inside a tile row-major in x; the tile columns start `shift` cells
+ * before cell 1.  Cells are clamped into 1..nx, 1..ny (a particle on the
+ * outer edge within rounding). */
 void orc_cell_keys(const orc_grid *g, int64_t n, const double *x,
                    const double *y, const double *z, int tile_x, int tile_r,
-                   int32_t *keys);
+                   int shift, int32_t *keys);
 /* Stable counting sort of keys in [0, nkeys): perm[i] = source index. */
 void orc_stable_sort_keys(int64_t n, const int32_t *keys, int32_t nkeys,
                           int64_t *perm);

@@ -86,7 +86,7 @@ def _declare(L):
                                      C.c_double, C.c_double, C.c_double,
                                      OCplx, OCplx, C.c_double,
                                      dp, dp, dp, dp],
-        "orc_cell_keys": [gp, C.c_int64, dp, dp, dp, C.c_int, C.c_int, dp],
+        "orc_cell_keys": [gp, C.c_int64, dp, dp, dp, C.c_int, C.c_int, C.c_int, dp],
         "orc_stable_sort_keys": [C.c_int64, dp, C.c_int32, dp],
         "orc_mode_energy": [gp, fp, dp, dp],
     }
@@ -261,11 +261,11 @@ def set_dt(dx, dy, dt_multiplier=1.0, dt_plasma=0.0, dt_laser=0.0):
     return lib().orc_set_dt(dx, dy, dt_multiplier, dt_plasma, dt_laser)
 
 
-def cell_keys(state, x, y, z, tile_x, tile_r):
+def cell_keys(state, x, y, z, tile_x, tile_r, shift=0):
     n = len(x)
     keys = np.zeros(n, np.int32)
     x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
-    lib().orc_cell_keys(C.byref(state.g), n, _ptr(x), _ptr(y), _ptr(z), tile_x, tile_r, _ptr(keys))
+    lib().orc_cell_keys(C.byref(state.g), n, _ptr(x), _ptr(y), _ptr(z), tile_x, tile_r, shift, _ptr(keys))
     return keys
 
 

@@ -112,9 +112,11 @@ def test_injected_batch_sorts_as_stable_cpu_sort(oracle):
     st, got, order, ntiles, cells, tile, c0, c1 = _inject_push(oracle, "fp64", "migrate")
     assert c1["migrate"] == c0["migrate"] + 1
     pos = {i: k for k, i in enumerate(got["id"])}
-    keys_alive = oracle.cell_keys(st, got["x"], got["y"], got["z"], *tile) // cells
-    keys = np.array([keys_alive[pos[i]] if i in pos else ntiles for i in order], np.int32)
-    want = order[oracle.stable_sort(keys, ntiles + 1)][: len(got["id"])]
+    # the tiles moved with the plasma: one window cell = tile offset 1
+    keys_alive = oracle.cell_keys(st, got["x"], got["y"], got["z"], *tile, shift=1) // cells
+    removed = 1 << 16  # beyond every tile key of the offset tiling
+    keys = np.array([keys_alive[pos[i]] if i in pos else removed for i in order], np.int32)
+    want = order[oracle.stable_sort(keys, removed + 1)][: len(got["id"])]
     log(f"  {len(order)} -> {len(want)} particles")
     assert np.array_equal(got["id"], want)
 
