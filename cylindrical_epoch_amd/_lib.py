@@ -26,6 +26,7 @@ SIGNATURES = {
     "cylpic_particle_count": [_p, _i, _p],
     "cylpic_inject_particles": [_p, _i, _l, _p, _p, _p, _p, _p, _p, _p, _p],
     "cylpic_move_window": [_p, _i],
+    "cylpic_set_smoothing": [_p, _i, _i, _p, _i],
     "cylpic_upload_field": [_p, _i, _p],
     "cylpic_download_field": [_p, _i, _p],
     "cylpic_setup_field_boundaries": [_p],

@@ -92,8 +92,10 @@ struct cylpic_ctx {
   bool migrate = true;      // use the neighbour migration sort when it applies
   double* stage = nullptr;  // FP64 staging of host particles for the mixed path
   long stage_cap = 0;
-  cplx<double>* spare64 = nullptr;  // field-sized buffer of the moving window's shifts
+  cplx<double>* spare64 = nullptr;  // field-sized buffer of the moving window's shifts and smoothing
   cplx<float>* spare32 = nullptr;
+  int smooth_its = 0, smooth_comp_its = 0;  // current smoothing (current_smooth.F90)
+  std::vector<int> smooth_strides{1};
   long sort_radix = 0, sort_migrate = 0;
   int acc_mode = 3;    // mixed path: current accumulator (0 FP32, 1 FP32 + 2Sum, 2 FP64, 3 int64, 4 2x32 packed)
   int gather_lds = 1;  // gather from the LDS-staged field region (1) or L1/L2 (0); init sets it per precision
@@ -1124,7 +1126,33 @@ int cylpic_current_finish(cylpic_ctx* h) {
   HIPCHK(hipSetDevice(h->device));
   Scope s(h, "current_finish");
   with_fields(h, [&](auto& F) { launch_current_finish(h->g, F, h->stream); });
+  if (h->smooth_its + h->smooth_comp_its > 0) {
+    const long nf = nfield(h);
+    if (h->prec == CYLPIC_FP64) {
+      if (!h->spare64) HIPCHK(hipMalloc((void**)&h->spare64, nf * sizeof(cplx<double>)));
+      launch_smooth_current(h->g, h->f64, h->spare64, h->smooth_its, h->smooth_comp_its, h->smooth_strides.data(),
+                            (int)h->smooth_strides.size(), h->stream);
+    } else {
+      if (!h->spare32) HIPCHK(hipMalloc((void**)&h->spare32, nf * sizeof(cplx<float>)));
+      launch_smooth_current(h->g, h->f32, h->spare32, h->smooth_its, h->smooth_comp_its, h->smooth_strides.data(),
+                            (int)h->smooth_strides.size(), h->stream);
+    }
+  }
   HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+int cylpic_set_smoothing(cylpic_ctx* h, int its, int comp_its, const int* strides, int nstrides) {
+  CHECK_H(h);
+  if (its < 0 || comp_its < 0) return fail(CYLPIC_EINVAL, "iteration counts must be >= 0");
+  std::vector<int> st;
+  if (strides && nstrides > 0) st.assign(strides, strides + nstrides);
+  else st.assign(1, 1);
+  for (int v : st)
+    if (v < 1 || v > h->g.ng) return fail(CYLPIC_EINVAL, "smoothing strides must be in 1..ng");
+  h->smooth_its = its;
+  h->smooth_comp_its = comp_its;
+  h->smooth_strides = st;
   return CYLPIC_OK;
 }
 

@@ -864,6 +864,55 @@ void launch_shift_window_fields(const Grid& g, FieldSet<T>& F, cplx<T>*& spare, 
                      F.bnd_xmax[4], F.bnd_xmax[5]);
 }
 
+// ------------------------------------------------------------------------
+// smooth_current (current_smooth.F90:50-58, smooth_mode_array :145-196): a
+// pass writes the interior of `out` from the work array; the work array's
+// interior is then refreshed from `out`
+template <class T>
+__global__ void k_smooth_pass(Grid g, const cplx<T>* __restrict__ wk, cplx<T>* __restrict__ out, T alpha, T beta,
+                              int stride) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)g.nx * g.ny * g.M) return;
+  const int ix = (int)(e % g.nx) + 1, iy = (int)((e / g.nx) % g.ny) + 1, m = (int)(e / ((long)g.nx * g.ny));
+  const long k = g.idx(ix, iy, m);
+  const long sx = stride, sy = (long)stride * g.nx2;
+  const cplx<T> sum = cadd(cadd(cadd(wk[k - sx], wk[k + sx]), wk[k - sy]), wk[k + sy]);
+  out[k] = cadd(rmul(alpha, wk[k]), rmul(beta, sum));
+}
+
+template <class T>
+__global__ void k_copy_interior(Grid g, const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)g.nx * g.ny * g.M) return;
+  const int ix = (int)(e % g.nx) + 1, iy = (int)((e / g.nx) % g.ny) + 1, m = (int)(e / ((long)g.nx * g.ny));
+  const long k = g.idx(ix, iy, m);
+  dst[k] = src[k];
+}
+
+template <class T>
+void launch_smooth_current(const Grid& g, FieldSet<T>& F, cplx<T>* wk, int its, int comp_its, const int* strides,
+                           int nstrides, hipStream_t s) {
+  const long n = g.plane * g.M, ni = (long)g.nx * g.ny * g.M;
+  const bool per_max = g.bc_field[XMAX] == BC_PERIODIC, per_min = g.bc_field[XMIN] == BC_PERIODIC;
+  cplx<T>* J[3] = {F.jx(), F.jr(), F.jt()};
+  for (int c = 0; c < 3; ++c) {
+    (void)hipMemcpyAsync(wk, J[c], n * sizeof(cplx<T>), hipMemcpyDeviceToDevice, s);
+    double alpha = 0.5;
+    const double beta = (1.0 - alpha) * 0.25;
+    for (int it = 1; it <= its + comp_its; ++it) {
+      for (int is = 0; is < nstrides; ++is) {
+        if (per_max || per_min)  // field_mode_bc: the periodic exchange of the work array
+          hipLaunchKernelGGL(k_halo_x<T>, dim3(nblk((long)g.ny2 * g.M)), dim3(256), 0, s, g, wk, 1 - g.ng,
+                             g.ny + g.ng, (int)per_max, (int)per_min);
+        hipLaunchKernelGGL(k_smooth_pass<T>, dim3(nblk(ni)), dim3(256), 0, s, g, wk, J[c], (T)alpha, (T)beta,
+                           strides[is]);
+        hipLaunchKernelGGL(k_copy_interior<T>, dim3(nblk(ni)), dim3(256), 0, s, g, J[c], wk);
+      }
+      if (it > its) alpha = (double)its * 0.5 + 1.0;
+    }
+  }
+}
+
 template <class T>
 void launch_energy(const Grid& g, FieldSet<T>& F, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_energy<T>, dim3(g.ny, g.M), dim3(256), 0, s, g, F.ex(), F.er(), F.et(), F.bx(), F.br(),
@@ -882,7 +931,9 @@ void launch_energy(const Grid& g, FieldSet<T>& F, double* out, hipStream_t s) {
   template void launch_current_finish<T>(const Grid&, FieldSet<T>&, hipStream_t);             \
   template void launch_fill_zero<T>(cplx<T>*, long, hipStream_t);                             \
   template void launch_energy<T>(const Grid&, FieldSet<T>&, double*, hipStream_t);            \
-  template void launch_shift_window_fields<T>(const Grid&, FieldSet<T>&, cplx<T>*&, hipStream_t);
+  template void launch_shift_window_fields<T>(const Grid&, FieldSet<T>&, cplx<T>*&, hipStream_t); \
+  template void launch_smooth_current<T>(const Grid&, FieldSet<T>&, cplx<T>*, int, int, const int*, int,  \
+                                         hipStream_t);
 INST(double)
 INST(float)
 

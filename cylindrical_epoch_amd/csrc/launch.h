@@ -77,6 +77,12 @@ template <class T> void launch_energy(const Grid&, FieldSet<T>&, double* out, hi
 // field-sized buffer that is swapped with each array) and fix the x_max
 // incoming cells from the stored boundary columns
 template <class T> void launch_shift_window_fields(const Grid&, FieldSet<T>&, cplx<T>*& spare, hipStream_t);
+// smooth_current: `its` binomial iterations plus `comp_its` compensating
+// ones with the given strides (host array, each in 1..ng); wk: a field-sized
+// work buffer
+template <class T>
+void launch_smooth_current(const Grid&, FieldSet<T>&, cplx<T>* wk, int its, int comp_its, const int* strides,
+                           int nstrides, hipStream_t);
 void launch_d2f(const cplx<double>* src, cplx<float>* dst, long n, hipStream_t);
 void launch_f2d(const cplx<float>* src, cplx<double>* dst, long n, hipStream_t);
 

@@ -153,6 +153,12 @@ class Simulation:
     def current_finish(self):
         check(lib().cylpic_current_finish(self.h))
 
+    def set_smoothing(self, its, comp_its=0, strides=(1,)):
+        """smooth_currents of the deck: smooth_its, smooth_comp_its,
+        smooth_strides (current_smooth.F90:50-58, 145-196)."""
+        st = np.ascontiguousarray(strides, np.int32)
+        check(lib().cylpic_set_smoothing(self.h, int(its), int(comp_its), _ptr(st), len(st)))
+
     def update_eb_fields_final(self, src_xmin=None, src_xmax=None):
         a = None if src_xmin is None else np.ascontiguousarray(src_xmin, np.float64)
         b = None if src_xmax is None else np.ascontiguousarray(src_xmax, np.float64)

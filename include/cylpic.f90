@@ -203,6 +203,15 @@ MODULE cylpic
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_move_window
 
+    FUNCTION cylpic_set_smoothing(h, its, comp_its, strides, nstrides) &
+        BIND(C, NAME='cylpic_set_smoothing') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: its, comp_its, nstrides
+      INTEGER(C_INT), INTENT(IN) :: strides(*)
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_set_smoothing
+
     FUNCTION cylpic_set_sort_method(h, method) &
         BIND(C, NAME='cylpic_set_sort_method') RESULT(ierr)
       IMPORT :: C_PTR, C_INT

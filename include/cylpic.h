@@ -121,6 +121,13 @@ int cylpic_current_finish(cylpic_ctx *h);
 int cylpic_update_eb_fields_final(cylpic_ctx *h, const double *src_xmin,
                                   const double *src_xmax);
 
+/* Current smoothing inside current_finish (smooth_current,
+ * src/housekeeping/current_smooth.F90:50-58 with smooth_mode_array
+ * :145-196): the deck's smooth_its, smooth_comp_its and smooth_strides (NULL
+ * or nstrides = 0: stride 1; each stride in 1..ng).  its + comp_its = 0 turns
+ * smoothing off (the default). */
+int cylpic_set_smoothing(cylpic_ctx *h, int its, int comp_its, const int *strides, int nstrides);
+
 /* The four calls above in main-loop order plus time += dt
  * (src/epoch2d.F90:213-265 without optional physics, output or window). */
 int cylpic_step(cylpic_ctx *h, const double *src_xmin, const double *src_xmax);

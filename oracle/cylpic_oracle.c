@@ -696,6 +696,52 @@ void orc_current_finish(const orc_grid *g, orc_fields *f) {
   halo_x(g, f->jt, 1 - g->ng, g->ny + g->ng);
 }
 
+/* smooth_mode_array, current_smooth.F90:145-196: strided binomial passes
+ * over the interior; the work array's ghost cells keep their values except
+ * for the periodic exchange (field_mode_bc), strides <= ng.  alpha becomes
+ * its/2 + 1 after each pass beyond `its` (beta stays 1/8), as written. */
+static void smooth_mode_array(const orc_grid *g, ocplx *array, int its, int comp_its, const int *strides,
+                              int nstrides) {
+  const int nx = g->nx, ny = g->ny, M = g->n_mode;
+  const size_t n = (size_t)g->n_mode * (nx + 2 * g->ng) * (ny + 2 * g->ng);
+  ocplx *wk = malloc(n * sizeof(ocplx));
+  memcpy(wk, array, n * sizeof(ocplx));
+  double alpha = 0.5;
+  const double beta = (1.0 - alpha) * 0.25;
+  const int one = 1;
+  if (nstrides <= 0) {
+    strides = &one;
+    nstrides = 1;
+  }
+  for (int it = 1; it <= its + comp_its; ++it) {
+    for (int is = 0; is < nstrides; ++is) {
+      halo_x(g, wk, 1 - g->ng, ny + g->ng);
+      const int s = strides[is];
+      for (int m = 0; m < M; ++m)
+        for (int iy = 1; iy <= ny; ++iy)
+          for (int ix = 1; ix <= nx; ++ix) {
+            const ocplx sum = cadd(cadd(cadd(wk[I3(g, ix - s, iy, m)], wk[I3(g, ix + s, iy, m)]),
+                                        wk[I3(g, ix, iy - s, m)]),
+                                   wk[I3(g, ix, iy + s, m)]);
+            array[I3(g, ix, iy, m)] = cadd(rmul(alpha, wk[I3(g, ix, iy, m)]), rmul(beta, sum));
+          }
+      for (int m = 0; m < M; ++m)
+        for (int iy = 1; iy <= ny; ++iy)
+          for (int ix = 1; ix <= nx; ++ix) wk[I3(g, ix, iy, m)] = array[I3(g, ix, iy, m)];
+    }
+    if (it > its) alpha = (double)its * 0.5 + 1.0;
+  }
+  free(wk);
+}
+
+/* smooth_current, current_smooth.F90:50-58 */
+void orc_smooth_current(const orc_grid *g, orc_fields *f, int its, int comp_its, const int *strides,
+                        int nstrides) {
+  smooth_mode_array(g, f->jx, its, comp_its, strides, nstrides);
+  smooth_mode_array(g, f->jr, its, comp_its, strides, nstrides);
+  smooth_mode_array(g, f->jt, its, comp_its, strides, nstrides);
+}
+
 /* ------------------------------------------------------------------ */
 /* particle push and deposit: src/particles.F90                       */
 /* ------------------------------------------------------------------ */

@@ -92,6 +92,9 @@ void orc_push_particles(const orc_grid *g, orc_fields *f,
 void orc_current_bcs(const orc_grid *g, orc_fields *f);
 void orc_current_bcs_r_min_final(const orc_grid *g, orc_fields *f);
 void orc_current_finish(const orc_grid *g, orc_fields *f);
+/* smooth_current (current_smooth.F90:50-58, smooth_mode_array :145-196) */
+void orc_smooth_current(const orc_grid *g, orc_fields *f, int its, int comp_its,
+                        const int *strides, int nstrides);
 void orc_particle_bcs(const orc_grid *g, orc_species *sp, int n_species);
 /* ---- moving window (src/housekeeping/window.F90) -------------------- */
 /* shift_window for ncells cells: per cell the grid moves by dx, particles

@@ -81,6 +81,7 @@ def _declare(L):
         "orc_current_bcs": [gp, fp], "orc_current_bcs_r_min_final": [gp, fp],
         "orc_current_finish": [gp, fp], "orc_particle_bcs": [gp, sp, C.c_int],
         "orc_shift_window": [gp, fp, sp, C.c_int, C.c_int],
+        "orc_smooth_current": [gp, fp, C.c_int, C.c_int, C.c_void_p, C.c_int],
         "orc_geometry": [gp, dp, dp, dp, dp],
         "orc_deposit_from_weights": [gp, fp, dp, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_double, C.c_double, C.c_double,
@@ -186,8 +187,18 @@ class State:
         lib().orc_push_particles(C.byref(self.g), C.byref(self.f), arr, len(self.species))
         self._species_back(arr)
 
-    def current_finish(self):
+    def current_finish(self, smooth=None):
+        """current_finish (current_smooth.F90:29-46); smooth = (its,
+        comp_its, strides) applies smooth_current after the current bcs."""
         lib().orc_current_finish(C.byref(self.g), C.byref(self.f))
+        if smooth:
+            self.smooth_current(*smooth)
+
+    def smooth_current(self, its, comp_its=0, strides=(1,)):
+        """smooth_current (current_smooth.F90:50-58, 145-196) on jx, jr, jt."""
+        st = np.ascontiguousarray(strides, np.int32)
+        lib().orc_smooth_current(C.byref(self.g), C.byref(self.f), int(its), int(comp_its),
+                                 st.ctypes.data_as(C.c_void_p), len(st))
 
     def setup_field_boundaries(self):
         lib().orc_setup_field_boundaries(C.byref(self.g), C.byref(self.f))

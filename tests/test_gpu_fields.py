@@ -73,6 +73,59 @@ def test_current_finish(oracle):
         sim.close()
 
 
+SMOOTH = [(1, 0, (1,)), (2, 1, (1, 2)), (1, 3, (3,)), (3, 0, (1, 2, 5))]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("sm", range(len(SMOOTH)))
+def test_current_finish_with_smoothing(oracle, case, sm):
+    """smooth_current after the current bcs: FP64 bit-identical."""
+    log("current_finish smoothing case", case, "smooth", SMOOTH[sm])
+    bf, bp = CASES[case]
+    st, sim = make_pair(oracle, 27, 14, 3, bc_field=bf, bc_particle=bp)
+    rng = np.random.default_rng(50 + case + 10 * sm)
+    load_fields(st, sim, random_fields(st.shape, rng))
+    its, comp, strides = SMOOTH[sm]
+    sim.set_smoothing(its, comp, strides)
+    for _ in range(2):  # the second call reuses the work buffer
+        st.current_finish(smooth=SMOOTH[sm])
+        sim.current_finish()
+    for n in ("jx", "jr", "jt"):
+        got = sim.download_field(n)
+        assert np.array_equal(got, st.arr[n]), (bf, n, np.max(np.abs(got - st.arr[n])))
+    sim.close()
+
+
+def test_current_finish_with_smoothing_mixed(oracle):
+    """The FP32 field path: the same passes in FP32, relative error of a
+    few FP32 ulps of the largest current."""
+    log("current_finish smoothing mixed")
+    bf, bp = CASES[2]
+    st, sim = make_pair(oracle, 27, 14, 3, bc_field=bf, bc_particle=bp, precision="mixed")
+    rng = np.random.default_rng(77)
+    f = random_fields(st.shape, rng)
+    load_fields(st, sim, f)
+    for n in f:  # the oracle starts from the FP32-rounded arrays the device holds
+        st.arr[n][:] = sim.download_field(n)
+    sim.set_smoothing(2, 1, (1, 2))
+    st.current_finish(smooth=(2, 1, (1, 2)))
+    sim.current_finish()
+    for n in ("jx", "jr", "jt"):
+        got = sim.download_field(n)
+        scale = np.max(np.abs(st.arr[n]))
+        assert np.max(np.abs(got - st.arr[n])) <= 1e-6 * scale, n
+    sim.close()
+
+
+def test_smoothing_rejects_bad_strides(oracle):
+    st, sim = make_pair(oracle, 16, 8, 1)
+    with pytest.raises(Exception):
+        sim.set_smoothing(1, 0, (6,))  # beyond ng = 5
+    with pytest.raises(Exception):
+        sim.set_smoothing(-1, 0, (1,))
+    sim.close()
+
+
 def test_mode_energy(oracle):
     log("mode_energy")
     st, sim = make_pair(oracle, 40, 20, 3)
