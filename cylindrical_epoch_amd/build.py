@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcylpic.so")
-SOURCES = ["api.cpp", "fields.hip", "push.hip", "push_mixed.hip", "sort.hip"]
+SOURCES = ["api.cpp", "fields.hip", "push.hip", "push_mixed.hip", "sort.hip", "diag.hip"]
 # the mixed-precision push contracts multiply-adds; everything else keeps the
 # reference's operation order
 CONTRACT = {"push_mixed.hip": "-ffp-contract=fast"}

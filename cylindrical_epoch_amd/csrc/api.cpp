@@ -94,6 +94,10 @@ struct cylpic_ctx {
   long stage_cap = 0;
   cplx<double>* spare64 = nullptr;  // field-sized buffer of the moving window's shifts and smoothing
   cplx<float>* spare32 = nullptr;
+  cplx<double>* diag64 = nullptr;  // field-sized FP64 buffer of the number-density diagnostic
+  double* diag_part = nullptr;      // per-block partial ranges of the distribution functions
+  double* diag_hist = nullptr;      // distribution function bins
+  long diag_hist_cap = 0;
   int smooth_its = 0, smooth_comp_its = 0;  // current smoothing (current_smooth.F90)
   std::vector<int> smooth_strides{1};
   long sort_radix = 0, sort_migrate = 0;
@@ -521,6 +525,9 @@ void free_all(cylpic_ctx* h) {
   if (h->stage) (void)hipFree(h->stage);
   if (h->spare64) (void)hipFree(h->spare64);
   if (h->spare32) (void)hipFree(h->spare32);
+  if (h->diag64) (void)hipFree(h->diag64);
+  if (h->diag_part) (void)hipFree(h->diag_part);
+  if (h->diag_hist) (void)hipFree(h->diag_hist);
   if (h->mig_host) (void)hipHostFree(h->mig_host);
   if (h->d_energy) (void)hipFree(h->d_energy);
   if (h->sc_mem) (void)hipFree(h->sc_mem);
@@ -1219,6 +1226,154 @@ int cylpic_mode_energy(cylpic_ctx* h, double* w_e, double* w_b) {
     w_b[m] = kappa * 0.5 / mu0 * out[2 * m + 1] * h->g.dx * h->g.dy;
   }
   return CYLPIC_OK;
+}
+
+// A species' alive particles at the front of its arrays (sorting first when
+// a push may have left removed ones, as a download does).
+static int settle_species(cylpic_ctx* h, Species& s) {
+  int rc = s.dirty ? sort_species(h, s) : CYLPIC_OK;
+  if (rc) return rc;
+  return refresh_count(h, s);
+}
+
+int cylpic_number_density_modes(cylpic_ctx* h, int species, double* out) {
+  CHECK_H(h);
+  if (!out) return fail(CYLPIC_EINVAL, "null output");
+  if (species >= (int)h->sp.size()) return fail(CYLPIC_EINVAL, "unknown species");
+  HIPCHK(hipSetDevice(h->device));
+  const long nf = nfield(h);
+  if (!h->diag64) HIPCHK(hipMalloc((void**)&h->diag64, nf * sizeof(cplx<double>)));
+  launch_fill_zero(h->diag64, nf, h->stream);
+  const int s0 = species < 0 ? 0 : species, s1 = species < 0 ? (int)h->sp.size() - 1 : species;
+  for (int is = s0; is <= s1; ++is) {
+    Species& s = h->sp[is];
+    int rc = settle_species(h, s);
+    if (rc) return rc;
+    if (h->prec == CYLPIC_FP64) launch_number_density_f64(h->g, view64(s), s.n, h->diag64, h->stream);
+    else launch_number_density_mixed(h->g, viewm(s), s.n, h->diag64, h->stream);
+  }
+  launch_density_bcs(h->g, h->diag64, h->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, h->diag64, nf * sizeof(cplx<double>), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return check_device_errors(h);
+}
+
+int cylpic_dist_fn(cylpic_ctx* h, int species, int ndims, const int* direction, double* ranges, int* resolution,
+                   const double* restrictions, const int* use_restrictions, double* out, int64_t capacity,
+                   int64_t* nbins) {
+  CHECK_H(h);
+  if (species < 0 || species >= (int)h->sp.size()) return fail(CYLPIC_EINVAL, "unknown species");
+  if (ndims < 1 || ndims > DF_MAXDIMS || !direction || !ranges || !resolution || !nbins)
+    return fail(CYLPIC_EINVAL, "ndims must be 1..3 with direction, ranges, resolution and nbins given");
+  HIPCHK(hipSetDevice(h->device));
+  *nbins = 0;
+  const Grid& g = h->g;
+  DistFn d{};
+  d.ndims = ndims;
+  bool parallel[3] = {false, false, false}, any_calc = false;
+  for (int k = 0; k < ndims; ++k) {
+    const int dir = direction[k];
+    d.direction[k] = dir;
+    double r0 = ranges[2 * k], r1 = ranges[2 * k + 1];
+    if (dir == DF_X || dir == DF_Y) {  // dist_fn.F90:197-286
+      const double hh = dir == DF_X ? g.dx : g.dy;
+      const double lo = dir == DF_X ? g.x_min : g.y_min, hi = dir == DF_X ? g.x_max : g.y_max;
+      const double gmin = dir == DF_X ? g.x_grid_min : g.y_grid_min;
+      const double gmax = gmin + (double)((dir == DF_X ? g.nx : g.ny) - 1) * hh;
+      if (std::fabs(r0 - r1) <= 2.2250738585072014e-308) {
+        r0 = gmin - 0.5 * hh;
+        r1 = gmax + 0.5 * hh;
+      } else {
+        r0 = std::fmax(r0, lo);
+        r1 = std::fmin(r1, hi);
+        r0 = lo + std::floor((r0 - lo) / hh) * hh;
+        r1 = lo + std::ceil((r1 - lo) / hh) * hh;
+        r0 = std::fmax(r0, gmin - 0.5 * hh);
+        r1 = std::fmin(r1, gmax + 0.5 * hh);
+      }
+      d.resolution[k] = (int)std::max(0L, std::lround((r1 - r0) / hh));
+      d.dgrid[k] = hh;
+      parallel[k] = true;
+    } else {
+      if (dir < DF_PX || dir > DF_MOD_P) return fail(CYLPIC_EINVAL, "unknown distribution function direction");
+      if (resolution[k] < 1) return fail(CYLPIC_EINVAL, "momentum-space resolution must be >= 1");
+      d.resolution[k] = resolution[k];
+      if (std::fabs(r0 - r1) <= 2.2250738585072014e-308) {
+        d.calc_range[k] = 1;
+        any_calc = true;
+      }
+      if (dir >= DF_XY_ANGLE && dir <= DF_ZX_ANGLE) d.need_angle[dir - DF_XY_ANGLE] = 1;
+    }
+    d.range[k][0] = r0;
+    d.range[k][1] = r1;
+  }
+  for (int k = ndims; k < 3; ++k) d.resolution[k] = 1;
+  for (int k = 0; k < DF_MAXDIRS; ++k) {
+    d.use_restriction[k] = use_restrictions ? use_restrictions[k] : 0;
+    if (d.use_restriction[k]) {
+      d.restriction[k][0] = restrictions[2 * k];
+      d.restriction[k][1] = restrictions[2 * k + 1];
+    }
+  }
+  for (int k = 0; k < 3; ++k)  // an angle that is only restricted is computed too
+    if (d.use_restriction[DF_XY_ANGLE - 1 + k]) d.need_angle[k] = 1;
+  Species& s = h->sp[species];
+  int rc = settle_species(h, s);
+  if (rc) return rc;
+  if (s.n < 1) return CYLPIC_OK;  // the reference writes nothing for an empty species
+  const double mc = s.m * C_LIGHT;
+  const int nblocks = (int)std::min<long>(1024, (s.n + 255) / 256);
+  if (any_calc) {  // dist_fn.F90:341-453
+    if (!h->diag_part) HIPCHK(hipMalloc((void**)&h->diag_part, 1024 * 6 * sizeof(double)));
+    if (h->prec == CYLPIC_FP64) launch_df_range_f64(g, view64(s), s.n, mc, d, h->diag_part, nblocks, h->stream);
+    else launch_df_range_mixed(g, viewm(s), s.n, mc, d, h->diag_part, nblocks, h->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part((size_t)nblocks * 6);
+    HIPCHK(hipMemcpyAsync(part.data(), h->diag_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < ndims; ++k)
+      if (d.calc_range[k]) {
+        double lo = 1.7976931348623157e308 * 0.4, hi = -lo;
+        for (int b = 0; b < nblocks; ++b) {
+          lo = std::fmin(lo, part[(size_t)b * 6 + 2 * k]);
+          hi = std::fmax(hi, part[(size_t)b * 6 + 2 * k + 1]);
+        }
+        d.range[k][0] = lo;
+        d.range[k][1] = hi;
+      }
+  }
+  for (int k = 0; k < ndims; ++k) {  // dist_fn.F90:455-472
+    if (std::fabs(d.range[k][0] - d.range[k][1]) <= 2.2250738585072014e-308) {
+      d.range[k][0] = -1.0;
+      d.range[k][1] = 1.0;
+    }
+    if (d.direction[k] >= DF_XY_ANGLE && d.direction[k] <= DF_ZX_ANGLE)
+      d.angle_max[d.direction[k] - DF_XY_ANGLE] = d.range[k][1];
+    if (!parallel[k]) d.dgrid[k] = (d.range[k][1] - d.range[k][0]) / (double)d.resolution[k];
+    ranges[2 * k] = d.range[k][0];
+    ranges[2 * k + 1] = d.range[k][1];
+    resolution[k] = d.resolution[k];
+  }
+  const long nb = (long)d.resolution[0] * d.resolution[1] * d.resolution[2];
+  *nbins = nb;
+  if (!out || nb == 0) return CYLPIC_OK;
+  if (nb > capacity) return fail(CYLPIC_EINVAL, "output capacity smaller than the number of bins");
+  if (nb > h->diag_hist_cap) {
+    if (h->diag_hist) HIPCHK(hipFree(h->diag_hist));
+    h->diag_hist = nullptr;
+    h->diag_hist_cap = 0;
+    HIPCHK(hipMalloc((void**)&h->diag_hist, nb * sizeof(double)));
+    h->diag_hist_cap = nb;
+  }
+  HIPCHK(hipMemsetAsync(h->diag_hist, 0, nb * sizeof(double), h->stream));
+  if (h->prec == CYLPIC_FP64) launch_df_bin_f64(g, view64(s), s.n, mc, d, h->diag_hist, nblocks, h->stream);
+  else launch_df_bin_mixed(g, viewm(s), s.n, mc, d, h->diag_hist, nblocks, h->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, h->diag_hist, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return check_device_errors(h);
 }
 
 int cylpic_synchronize(cylpic_ctx* h) {

@@ -1,0 +1,337 @@
+// Output diagnostics computed from the device-resident particles: the mode
+// decomposition of a species' number density (calc_number_density_modes,
+// src/io/calc_df.F90:588-661) and the binned distribution function
+// (general_dist_fn, src/io/dist_fn.F90:118-715).  Both read the particle
+// arrays once (HBM-bound) and accumulate with FP64 atomics; the sums are
+// exact up to the order of the additions.  Particles are reconstructed in
+// FP64 exactly as cylpic_download_particles returns them, so the cell
+// indices, shape weights and bins equal those of a CPU pass over the
+// downloaded particles.
+#include "cplx_ops.h"
+#include "launch.h"
+
+namespace cylpic {
+
+namespace {
+
+constexpr double PI_D = 3.141592653589793238462643383279503;
+constexpr double C_TINY_D = 2.2250738585072014e-308;  // TINY(1.0_num)
+
+__device__ __forceinline__ void load_particle(const Grid&, const PartF64& p, long i, double& x, double& y,
+                                              double& z, double* pm, double& w, double) {
+  x = p.x[i];
+  y = p.y[i];
+  z = p.z[i];
+  if (pm) {
+    pm[0] = p.px[i];
+    pm[1] = p.py[i];
+    pm[2] = p.pz[i];
+  }
+  w = p.w[i];
+}
+
+// the FP64 values cylpic_download_particles reconstructs
+__device__ __forceinline__ void load_particle(const Grid& g, const PartMixed& p, long i, double& x, double& y,
+                                              double& z, double* pm, double& w, double mc) {
+  x = g.x_grid_min + ((double)(p.cell[i] - 1) + (double)p.ox[i]) * g.dx;
+  y = (double)p.y[i] * g.dy;
+  z = (double)p.z[i] * g.dy;
+  if (pm) {
+    pm[0] = (double)p.ux[i] * mc;
+    pm[1] = (double)p.uy[i] * mc;
+    pm[2] = (double)p.uz[i] * mc;
+  }
+  w = (double)p.w[i];
+}
+
+// One particle's contribution: particle_to_grid.inc with triangle/gxfac.inc
+// (quadratic weights summing to one; r < dy folds the inner radial weight
+// into the nearest cell), the macro-particle volume 2 pi dx dy r
+// (partlist.F90:999-1013) and the mode factor 1 or 2 e^{i m theta}.
+template <class P>
+__global__ __launch_bounds__(256) void k_number_density(Grid g, P p, long n, double mc, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x, y, z, w;
+  load_particle(g, p, i, x, y, z, nullptr, w, mc);
+  const double part_r = sqrt(y * y + z * z);
+  const double cxr = (x - g.x_grid_min) / g.dx;
+  const double cyr = (part_r - g.y_grid_min) / g.dy;
+  int cx = (int)floor(cxr + 0.5), cy = (int)floor(cyr + 0.5);
+  const double fx = (double)cx - cxr, fy = (double)cy - cyr;
+  cx += 1;
+  cy += 1;
+  if (cx - 1 < 1 - g.ng || cx + 1 > g.nx + g.ng || cy - 1 < 1 - g.ng || cy + 1 > g.ny + g.ng) return;
+  const double cx2 = fx * fx, cy2 = fy * fy;
+  const double gx[3] = {0.5 * ((0.25 + cx2) + fx), 0.75 - cx2, 0.5 * ((0.25 + cx2) - fx)};
+  double gy[3] = {0.5 * ((0.25 + cy2) + fy), 0.75 - cy2, 0.5 * ((0.25 + cy2) - fy)};
+  if (part_r < g.dy) {
+    gy[1] = gy[1] + gy[0];
+    gy[0] = 0.0;
+  }
+  const double vol = (((2.0 * PI_D) * g.dx) * g.dy) * part_r;
+  const double pnd = w / vol;
+  const cplx<double> eith = mk(y / part_r, z / part_r);
+  cplx<double> eim = mk(1.0, 0.0), mf = mk(1.0, 0.0);
+  for (int m = 0; m < g.M; ++m) {
+    if (m > 0) {
+      eim = cmul(eim, eith);
+      mf = mk(2.0 * eim.re, 2.0 * eim.im);
+    }
+#pragma unroll
+    for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+      for (int ix = 0; ix < 3; ++ix) {
+        const double v = (gx[ix] * gy[iy]) * pnd;
+        double* o = out + 2 * g.idx(cx + ix - 1, cy + iy - 1, m);
+        unsafeAtomicAdd(o, v * mf.re);
+        if (m > 0) unsafeAtomicAdd(o + 1, v * mf.im);  // m = 0 adds v * 0
+      }
+  }
+}
+
+// calc_boundary_modes (calc_df.F90:35-55) = the real processor_summation_bcs
+// (boundary.F90:1305-1326) on both parts: particle_reflection_bcs without a
+// flip (:833-914), x part, one thread per (row, mode)
+__global__ void k_dens_reflect_x(Grid g, cplx<double>* __restrict__ a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.ny2 * g.M) return;
+  const int j = 1 - g.ng + t % g.ny2, m = t / g.ny2, nx = g.nx, ng = g.ng;
+  if (g.bc_particle[XMIN] == BC_REFLECT)
+    for (int i = 1; i <= ng - 1; ++i) {
+      a[g.idx(i, j, m)] = cadd(a[g.idx(i, j, m)], a[g.idx(1 - i, j, m)]);
+      a[g.idx(1 - i, j, m)] = czero<double>();
+    }
+  if (g.bc_particle[XMAX] == BC_REFLECT)
+    for (int i = 1; i <= ng; ++i) {
+      a[g.idx(nx + 1 - i, j, m)] = cadd(a[g.idx(nx + 1 - i, j, m)], a[g.idx(nx + i, j, m)]);
+      a[g.idx(nx + i, j, m)] = czero<double>();
+    }
+}
+
+// ... its r_max part, one thread per (column, mode)
+__global__ void k_dens_reflect_rmax(Grid g, cplx<double>* __restrict__ a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nx2 * g.M) return;
+  const int ix = 1 - g.ng + t % g.nx2, m = t / g.nx2, ny = g.ny;
+  for (int i = 1; i <= g.ng; ++i) {
+    a[g.idx(ix, ny + 1 - i, m)] = cadd(a[g.idx(ix, ny + 1 - i, m)], a[g.idx(ix, ny + i, m)]);
+    a[g.idx(ix, ny + i, m)] = czero<double>();
+  }
+}
+
+// particle_periodic_bcs on one rank (boundary.F90:1019-1129): the ghost
+// columns are added into the interior and keep their values
+__global__ void k_dens_periodic(Grid g, cplx<double>* __restrict__ a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.ny2 * g.M) return;
+  const int j = 1 - g.ng + t % g.ny2, m = t / g.ny2, nx = g.nx, ng = g.ng;
+  for (int i = 1; i <= ng; ++i) a[g.idx(i, j, m)] = cadd(a[g.idx(i, j, m)], a[g.idx(nx + i, j, m)]);
+  for (int i = 1; i <= ng; ++i)
+    a[g.idx(nx - ng + i, j, m)] = cadd(a[g.idx(nx - ng + i, j, m)], a[g.idx(i - ng, j, m)]);
+}
+
+// field_mode_zero_gradient of a cell-centred array (boundary.F90:654-707):
+// x_min and x_max per row, then y_min and y_max per column
+__global__ void k_dens_zero_gradient_x(Grid g, cplx<double>* __restrict__ a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.ny2 * g.M) return;
+  const int j = 1 - g.ng + t % g.ny2, m = t / g.ny2, nx = g.nx, ng = g.ng;
+  if (g.bc_field[XMIN] != BC_PERIODIC)
+    for (int i = 1; i <= ng; ++i) a[g.idx(i - ng, j, m)] = a[g.idx(ng + 1 - i, j, m)];
+  if (g.bc_field[XMAX] != BC_PERIODIC)
+    for (int i = 1; i <= ng; ++i) a[g.idx(nx + i, j, m)] = a[g.idx(nx + 1 - i, j, m)];
+}
+
+__global__ void k_dens_zero_gradient_y(Grid g, cplx<double>* __restrict__ a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nx2 * g.M) return;
+  const int ix = 1 - g.ng + t % g.nx2, m = t / g.nx2, ny = g.ny, ng = g.ng;
+  if (g.bc_field[YMIN] != BC_PERIODIC)
+    for (int i = 1; i <= ng; ++i) a[g.idx(ix, i - ng, m)] = a[g.idx(ix, ng + 1 - i, m)];
+  if (g.bc_field[YMAX] != BC_PERIODIC)
+    for (int i = 1; i <= ng; ++i) a[g.idx(ix, ny + i, m)] = a[g.idx(ix, ny + 1 - i, m)];
+}
+
+// ------------------------------------------------------------------------
+// distribution functions
+
+// particle_data(:) of a non-photon particle (dist_fn.F90:486-553); angles
+// only where `need` asks; false when a needed angle has |p| < tiny (CYCLE)
+template <class P>
+__device__ __forceinline__ bool df_data(const Grid& g, const P& p, long i, double mc, double mc2, const DistFn& d,
+                                        bool wrap, double* pd, double& w) {
+  double x, y, z, pm[3];
+  load_particle(g, p, i, x, y, z, pm, w, mc);
+  const double ux = pm[0] / mc, uy = pm[1] / mc, uz = pm[2] / mc;
+  const double u2 = (ux * ux + uy * uy) + uz * uz;
+  const double gam = sqrt(u2 + 1.0);
+  const double gm1 = u2 / (gam + 1.0);
+  pd[DF_X] = x;
+  pd[DF_Y] = y;
+  pd[DF_PX] = pm[0];
+  pd[DF_PY] = pm[1];
+  pd[DF_PZ] = pm[2];
+  pd[DF_EN] = gm1 * mc2;
+  pd[DF_GAMMA_M1] = gm1;
+  pd[DF_MOD_P] = sqrt((pm[0] * pm[0] + pm[1] * pm[1]) + pm[2] * pm[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    pd[DF_XY_ANGLE + k] = 0.0;
+    if (!d.need_angle[k]) continue;
+    const double a1 = pm[k], a2 = pm[(k + 1) % 3];
+    const double pp = sqrt(a1 * a1 + a2 * a2);
+    if (pp < C_TINY_D) return false;
+    double th = asin(a2 / pp);
+    if (a1 < 0.0) th = copysign(1.0, a2) * PI_D - th;
+    if (wrap && th + 2.0 * PI_D < d.angle_max[k]) th = th + 2.0 * PI_D;
+    pd[DF_XY_ANGLE + k] = th;
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool df_restricted(const DistFn& d, const double* pd) {
+  for (int k = 1; k <= DF_MAXDIRS; ++k)
+    if (d.use_restriction[k - 1] && (pd[k] < d.restriction[k - 1][0] || pd[k] > d.restriction[k - 1][1]))
+      return true;
+  return false;
+}
+
+// per-block minima and maxima of the auto-ranged directions
+template <class P>
+__global__ __launch_bounds__(256) void k_df_range(Grid g, P p, long n, double mc, DistFn d, double* __restrict__ part) {
+  __shared__ double smin[3][256], smax[3][256];
+  double lo[3] = {1.7976931348623157e308 * 0.4, 1.7976931348623157e308 * 0.4, 1.7976931348623157e308 * 0.4};
+  double hi[3] = {-lo[0], -lo[1], -lo[2]};
+  const double mc2 = mc * C_LIGHT;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double pd[DF_MAXDIRS + 1], w;
+    if (!df_data(g, p, i, mc, mc2, d, false, pd, w)) continue;
+    if (df_restricted(d, pd)) continue;
+    for (int k = 0; k < d.ndims; ++k)
+      if (d.calc_range[k]) {
+        const double v = pd[d.direction[k]];
+        lo[k] = v < lo[k] ? v : lo[k];
+        hi[k] = v > hi[k] ? v : hi[k];
+      }
+  }
+  for (int k = 0; k < 3; ++k) {
+    smin[k][threadIdx.x] = lo[k];
+    smax[k][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) {
+        smin[k][threadIdx.x] = fmin(smin[k][threadIdx.x], smin[k][threadIdx.x + s]);
+        smax[k][threadIdx.x] = fmax(smax[k][threadIdx.x], smax[k][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) {
+      part[(long)blockIdx.x * 6 + 2 * k] = smin[k][0];
+      part[(long)blockIdx.x * 6 + 2 * k + 1] = smax[k][0];
+    }
+}
+
+// the binning pass (dist_fn.F90:482-578): a histogram of up to DF_LDS_BINS
+// bins is accumulated per workgroup in LDS and added to `out` at the end;
+// larger ones go straight to `out`
+template <class P, bool LDS>
+__global__ __launch_bounds__(256) void k_df_bin(Grid g, P p, long n, double mc, DistFn d, double* __restrict__ out) {
+  __shared__ double hist[LDS ? DF_LDS_BINS : 1];
+  const long nbins = (long)d.resolution[0] * d.resolution[1] * d.resolution[2];
+  if (LDS) {
+    for (int k = threadIdx.x; k < nbins; k += blockDim.x) hist[k] = 0.0;
+    __syncthreads();
+  }
+  const double mc2 = mc * C_LIGHT;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double pd[DF_MAXDIRS + 1], w;
+    if (!df_data(g, p, i, mc, mc2, d, true, pd, w)) continue;
+    if (df_restricted(d, pd)) continue;
+    long cell[3] = {0, 0, 0};
+    bool ok = true;
+    for (int k = 0; k < d.ndims; ++k) {
+      const double c = floor((pd[d.direction[k]] - d.range[k][0]) / d.dgrid[k]);
+      if (!(c >= 0.0 && c < (double)d.resolution[k])) {
+        ok = false;
+        break;
+      }
+      cell[k] = (long)c;
+    }
+    if (!ok) continue;
+    const long b = (cell[2] * d.resolution[1] + cell[1]) * d.resolution[0] + cell[0];
+    if (LDS) atomicAdd(&hist[b], w);
+    else unsafeAtomicAdd(&out[b], w);
+  }
+  if (LDS) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+      if (hist[k] != 0.0) unsafeAtomicAdd(&out[k], hist[k]);
+  }
+}
+
+inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+template <class P>
+void number_density(const Grid& g, const P& p, long n, double mc, cplx<double>* out, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_number_density<P>, dim3(nblk(n)), dim3(256), 0, s, g, p, n, mc, (double*)out);
+}
+
+template <class P>
+void df_range(const Grid& g, const P& p, long n, double mc, const DistFn& d, double* part, int nblocks,
+              hipStream_t s) {
+  hipLaunchKernelGGL(k_df_range<P>, dim3(nblocks), dim3(256), 0, s, g, p, n, mc, d, part);
+}
+
+template <class P>
+void df_bin(const Grid& g, const P& p, long n, double mc, const DistFn& d, double* out, int nblocks,
+            hipStream_t s) {
+  const long nbins = (long)d.resolution[0] * d.resolution[1] * d.resolution[2];
+  if (nbins <= DF_LDS_BINS)
+    hipLaunchKernelGGL((k_df_bin<P, true>), dim3(nblocks), dim3(256), 0, s, g, p, n, mc, d, out);
+  else
+    hipLaunchKernelGGL((k_df_bin<P, false>), dim3(nblocks), dim3(256), 0, s, g, p, n, mc, d, out);
+}
+
+}  // namespace
+
+void launch_number_density_f64(const Grid& g, const PartF64& p, long n, cplx<double>* out, hipStream_t s) {
+  number_density(g, p, n, 0.0, out, s);
+}
+void launch_number_density_mixed(const Grid& g, const PartMixed& p, long n, cplx<double>* out, hipStream_t s) {
+  number_density(g, p, n, 0.0, out, s);
+}
+
+void launch_density_bcs(const Grid& g, cplx<double>* a, hipStream_t s) {
+  const bool xref = g.bc_particle[XMIN] == BC_REFLECT || g.bc_particle[XMAX] == BC_REFLECT;
+  const bool per = g.bc_particle[XMIN] == BC_PERIODIC && g.bc_particle[XMAX] == BC_PERIODIC;
+  const unsigned brow = nblk((long)g.ny2 * g.M), bcol = nblk((long)g.nx2 * g.M);
+  if (xref) hipLaunchKernelGGL(k_dens_reflect_x, dim3(brow), dim3(256), 0, s, g, a);
+  if (g.bc_particle[YMAX] == BC_REFLECT) hipLaunchKernelGGL(k_dens_reflect_rmax, dim3(bcol), dim3(256), 0, s, g, a);
+  if (per) hipLaunchKernelGGL(k_dens_periodic, dim3(brow), dim3(256), 0, s, g, a);
+  hipLaunchKernelGGL(k_dens_zero_gradient_x, dim3(brow), dim3(256), 0, s, g, a);
+  hipLaunchKernelGGL(k_dens_zero_gradient_y, dim3(bcol), dim3(256), 0, s, g, a);
+}
+
+void launch_df_range_f64(const Grid& g, const PartF64& p, long n, double mc, const DistFn& d, double* part,
+                         int nblocks, hipStream_t s) {
+  df_range(g, p, n, mc, d, part, nblocks, s);
+}
+void launch_df_range_mixed(const Grid& g, const PartMixed& p, long n, double mc, const DistFn& d, double* part,
+                           int nblocks, hipStream_t s) {
+  df_range(g, p, n, mc, d, part, nblocks, s);
+}
+void launch_df_bin_f64(const Grid& g, const PartF64& p, long n, double mc, const DistFn& d, double* out,
+                       int nblocks, hipStream_t s) {
+  df_bin(g, p, n, mc, d, out, nblocks, s);
+}
+void launch_df_bin_mixed(const Grid& g, const PartMixed& p, long n, double mc, const DistFn& d, double* out,
+                         int nblocks, hipStream_t s) {
+  df_bin(g, p, n, mc, d, out, nblocks, s);
+}
+
+}  // namespace cylpic

@@ -170,4 +170,41 @@ void launch_mig_scatter(const uint32_t* key, const int* tile_start, ExtraRange x
                         void* const* dst, int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out,
                         hipStream_t);
 
+// diag.hip: output diagnostics
+// number density modes (complex FP64, field-shaped `out`, zeroed by the
+// caller); then launch_density_bcs applies the ghost sums and zero-gradient
+// ghosts of calc_boundary_modes / calc_number_density_modes
+void launch_number_density_f64(const Grid&, const PartF64&, long n, cplx<double>* out, hipStream_t);
+void launch_number_density_mixed(const Grid&, const PartMixed&, long n, cplx<double>* out, hipStream_t);
+void launch_density_bcs(const Grid&, cplx<double>* a, hipStream_t);
+
+// Distribution functions: c_dir_* numbering (constants.F90:231-247)
+enum {
+  DF_X = 1, DF_Y = 2, DF_PX = 3, DF_PY = 4, DF_PZ = 5, DF_EN = 6, DF_GAMMA_M1 = 7, DF_XY_ANGLE = 8,
+  DF_YZ_ANGLE = 9, DF_ZX_ANGLE = 10, DF_MOD_P = 11, DF_MAXDIRS = 11, DF_MAXDIMS = 3
+};
+constexpr int DF_LDS_BINS = 4096;  // histograms up to this size are summed per workgroup in LDS
+struct DistFn {
+  int ndims;
+  int direction[3];
+  double range[3][2];
+  double dgrid[3];
+  int resolution[3];
+  int calc_range[3];
+  int need_angle[3];  // xy, yz, zx
+  double angle_max[3];
+  double restriction[DF_MAXDIRS][2];
+  int use_restriction[DF_MAXDIRS];
+};
+// per-block (min, max) of each auto-ranged direction: part[nblocks][6]
+void launch_df_range_f64(const Grid&, const PartF64&, long n, double mc, const DistFn&, double* part, int nblocks,
+                         hipStream_t);
+void launch_df_range_mixed(const Grid&, const PartMixed&, long n, double mc, const DistFn&, double* part,
+                           int nblocks, hipStream_t);
+// adds each particle's weight into its bin of `out` (zeroed by the caller)
+void launch_df_bin_f64(const Grid&, const PartF64&, long n, double mc, const DistFn&, double* out, int nblocks,
+                       hipStream_t);
+void launch_df_bin_mixed(const Grid&, const PartMixed&, long n, double mc, const DistFn&, double* out,
+                         int nblocks, hipStream_t);
+
 }  // namespace cylpic

@@ -24,6 +24,8 @@ BC = dict(periodic=1, other=2, simple_laser=3, simple_outflow=4, open=5,
 FIELDS = ("ex", "er", "et", "bx", "br", "bt", "jx", "jr", "jt",
           "bx_old", "br_old", "bt_old", "jx_old", "jr_old", "jt_old")
 PRECISION = {"fp64": 0, "mixed": 1}
+# distribution function directions, c_dir_* of constants.F90:231-247
+DIRS = dict(x=1, y=2, px=3, py=4, pz=5, en=6, gamma_m1=7, xy_angle=8, yz_angle=9, zx_angle=10, mod_p=11)
 
 
 def _code(b):
@@ -152,6 +154,52 @@ class Simulation:
 
     def current_finish(self):
         check(lib().cylpic_current_finish(self.h))
+
+    # ------------------------------------------------------------ diagnostics
+    def number_density_modes(self, species=-1):
+        """number_density_mode output (calc_number_density_modes,
+        calc_df.F90:588-661); species < 0 sums every species."""
+        a = np.empty(self.shape, np.complex128)
+        check(lib().cylpic_number_density_modes(self.h, int(species), _ptr(a)))
+        return a
+
+    def dist_fn(self, species, directions, ranges=None, resolution=None, restrictions=None):
+        """A distribution function (general_dist_fn, dist_fn.F90:118-715).
+        directions: 1-3 of DIRS (names or c_dir_* numbers); ranges: (lo, hi)
+        per direction, equal values for the automatic range; resolution:
+        bins per momentum direction; restrictions: {direction: (lo, hi)}.
+        Returns (bins with the first direction fastest, used ranges, used
+        resolution), or None for an empty species."""
+        nd = len(directions)
+        d = np.zeros(3, np.int32)
+        d[:nd] = [DIRS[k] if isinstance(k, str) else k for k in directions]
+        rg = np.zeros(6)
+        if ranges is not None:
+            rg[: 2 * nd] = np.ravel(ranges)
+        res = np.ones(3, np.int32)
+        if resolution is not None:
+            res[:nd] = resolution
+        restr = np.zeros(22)
+        use = np.zeros(11, np.int32)
+        for k, (lo, hi) in (restrictions or {}).items():
+            i = (DIRS[k] if isinstance(k, str) else k) - 1
+            restr[2 * i: 2 * i + 2] = lo, hi
+            use[i] = 1
+        nb = C.c_int64(0)
+        check(lib().cylpic_dist_fn(self.h, int(species), nd, _ptr(d), _ptr(rg), _ptr(res), _ptr(restr), _ptr(use),
+                                   None, 0, C.byref(nb)))
+        if nb.value == 0:
+            return None
+        out = np.zeros(nb.value)
+        rg2 = np.zeros(6)
+        if ranges is not None:
+            rg2[: 2 * nd] = np.ravel(ranges)
+        res2 = np.ones(3, np.int32)
+        if resolution is not None:
+            res2[:nd] = resolution
+        check(lib().cylpic_dist_fn(self.h, int(species), nd, _ptr(d), _ptr(rg2), _ptr(res2), _ptr(restr),
+                                   _ptr(use), _ptr(out), nb.value, C.byref(nb)))
+        return out.reshape(res2[2], res2[1], res2[0]), rg2[: 2 * nd].reshape(nd, 2), res2[:nd].copy()
 
     def set_smoothing(self, its, comp_its=0, strides=(1,)):
         """smooth_currents of the deck: smooth_its, smooth_comp_its,

@@ -12,6 +12,11 @@ MODULE cylpic
 
   INTEGER(C_INT), PARAMETER :: CYLPIC_FP64 = 0, CYLPIC_MIXED = 1
   INTEGER(C_INT), PARAMETER :: CYLPIC_SORT_RADIX = 0, CYLPIC_SORT_MIGRATE = 1
+  ! distribution function directions (c_dir_* of constants.F90)
+  INTEGER(C_INT), PARAMETER :: CYLPIC_DIR_X = 1, CYLPIC_DIR_Y = 2, CYLPIC_DIR_PX = 3, &
+      CYLPIC_DIR_PY = 4, CYLPIC_DIR_PZ = 5, CYLPIC_DIR_EN = 6, CYLPIC_DIR_GAMMA_M1 = 7, &
+      CYLPIC_DIR_XY_ANGLE = 8, CYLPIC_DIR_YZ_ANGLE = 9, CYLPIC_DIR_ZX_ANGLE = 10, &
+      CYLPIC_DIR_MOD_P = 11
 
   ! field identifiers, in the order of src/shared_data.F90:474-478
   INTEGER(C_INT), PARAMETER :: CYLPIC_EXM = 0, CYLPIC_ERM = 1, CYLPIC_ETM = 2
@@ -202,6 +207,32 @@ MODULE cylpic
       INTEGER(C_INT), VALUE :: ncells
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_move_window
+
+    FUNCTION cylpic_number_density_modes(h, species, out) &
+        BIND(C, NAME='cylpic_number_density_modes') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_DOUBLE_COMPLEX
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: species
+      COMPLEX(C_DOUBLE_COMPLEX), INTENT(OUT) :: out(*)
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_number_density_modes
+
+    FUNCTION cylpic_dist_fn(h, species, ndims, direction, ranges, resolution, &
+        restrictions, use_restrictions, out, capacity, nbins) &
+        BIND(C, NAME='cylpic_dist_fn') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_INT64_T, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: species, ndims
+      INTEGER(C_INT), INTENT(IN) :: direction(*)
+      REAL(C_DOUBLE), INTENT(INOUT) :: ranges(*)
+      INTEGER(C_INT), INTENT(INOUT) :: resolution(*)
+      REAL(C_DOUBLE), INTENT(IN) :: restrictions(*)
+      INTEGER(C_INT), INTENT(IN) :: use_restrictions(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: out(*)
+      INTEGER(C_INT64_T), VALUE :: capacity
+      INTEGER(C_INT64_T), INTENT(OUT) :: nbins
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_dist_fn
 
     FUNCTION cylpic_set_smoothing(h, its, comp_its, strides, nstrides) &
         BIND(C, NAME='cylpic_set_smoothing') RESULT(ierr)

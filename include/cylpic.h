@@ -186,6 +186,34 @@ int cylpic_tune(cylpic_ctx *h, int acc_mode, int gather_lds);
  * setup.F90:526-571), w_e[n_mode], w_b[n_mode] in joules. */
 int cylpic_mode_energy(cylpic_ctx *h, double *w_e, double *w_b);
 
+/* Output diagnostics from the device-resident particles.  A species that a
+ * push has left with removed particles is sorted first (as for a download).
+ *
+ * number_density_mode (calc_number_density_modes, src/io/calc_df.F90:588-661,
+ * with the ghost sums of calc_boundary_modes :35-55): species >= 0 one
+ * species, < 0 the sum over all species; out: one complex mode array of the
+ * field shape (as cylpic_download_field). */
+int cylpic_number_density_modes(cylpic_ctx *h, int species, double *out);
+
+/* Distribution functions (general_dist_fn, src/io/dist_fn.F90:118-715, one
+ * rank).  direction[ndims] (ndims 1..3): CYLPIC_DIR_*; ranges[2*ndims]:
+ * (lo, hi) per direction, lo == hi asks for the automatic range (momentum
+ * directions) or the whole domain (x, y); resolution[ndims]: bins of each
+ * momentum direction (x and y bin per cell).  On return ranges and
+ * resolution hold the values used and *nbins their product;
+ * restrictions[2*11] / use_restrictions[11] (or NULL) restrict each
+ * direction (index direction - 1) to [lo, hi].  out (capacity doubles, first
+ * direction fastest) receives the summed weights unless NULL.  An empty
+ * species writes nothing and returns *nbins = 0. */
+enum {
+  CYLPIC_DIR_X = 1, CYLPIC_DIR_Y = 2, CYLPIC_DIR_PX = 3, CYLPIC_DIR_PY = 4, CYLPIC_DIR_PZ = 5,
+  CYLPIC_DIR_EN = 6, CYLPIC_DIR_GAMMA_M1 = 7, CYLPIC_DIR_XY_ANGLE = 8, CYLPIC_DIR_YZ_ANGLE = 9,
+  CYLPIC_DIR_ZX_ANGLE = 10, CYLPIC_DIR_MOD_P = 11
+};
+int cylpic_dist_fn(cylpic_ctx *h, int species, int ndims, const int *direction, double *ranges,
+                   int *resolution, const double *restrictions, const int *use_restrictions,
+                   double *out, int64_t capacity, int64_t *nbins);
+
 /* Wait for all queued device work of this handle. */
 int cylpic_synchronize(cylpic_ctx *h);
 

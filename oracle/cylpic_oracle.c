@@ -1280,3 +1280,243 @@ void orc_shift_window(orc_grid *g, orc_fields *f, orc_species *sp, int n_species
     shift_fields(g, f);
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* output diagnostics: src/io/calc_df.F90, src/io/dist_fn.F90         */
+/* ------------------------------------------------------------------ */
+
+/* calc_boundary_modes without mixed particle bcs (calc_df.F90:35-55): the
+ * real processor_summation_bcs (boundary.F90:1305-1326) on the real and
+ * imaginary parts, i.e. particle_reflection_bcs without a flip direction
+ * (:833-914, cell-centred folds) then the one-rank particle_periodic_bcs
+ * (:1019-1129). */
+static void density_summation_bcs(const orc_grid *g, ocplx *a) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng;
+  for (int m = 0; m < g->n_mode; ++m) {
+    if (g->bc_particle[ORC_XMIN] == ORC_BC_REFLECT)
+      for (int j = 1 - ng; j <= ny + ng; ++j)
+        for (int i = 1; i <= ng - 1; ++i) {
+          a[I3(g, i, j, m)] = cadd(a[I3(g, i, j, m)], a[I3(g, 1 - i, j, m)]);
+          a[I3(g, 1 - i, j, m)] = CZERO;
+        }
+    if (g->bc_particle[ORC_XMAX] == ORC_BC_REFLECT)
+      for (int j = 1 - ng; j <= ny + ng; ++j)
+        for (int i = 1; i <= ng; ++i) {
+          a[I3(g, nx + 1 - i, j, m)] = cadd(a[I3(g, nx + 1 - i, j, m)], a[I3(g, nx + i, j, m)]);
+          a[I3(g, nx + i, j, m)] = CZERO;
+        }
+    if (g->bc_particle[ORC_YMAX] == ORC_BC_REFLECT)
+      for (int i = 1; i <= ng; ++i)
+        for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
+          a[I3(g, ix, ny + 1 - i, m)] = cadd(a[I3(g, ix, ny + 1 - i, m)], a[I3(g, ix, ny + i, m)]);
+          a[I3(g, ix, ny + i, m)] = CZERO;
+        }
+    if (g->bc_particle[ORC_XMIN] == ORC_BC_PERIODIC && g->bc_particle[ORC_XMAX] == ORC_BC_PERIODIC) {
+      for (int j = 1 - ng; j <= ny + ng; ++j)
+        for (int i = 1; i <= ng; ++i) a[I3(g, i, j, m)] = cadd(a[I3(g, i, j, m)], a[I3(g, nx + i, j, m)]);
+      for (int j = 1 - ng; j <= ny + ng; ++j)
+        for (int i = 1; i <= ng; ++i)
+          a[I3(g, nx - ng + i, j, m)] = cadd(a[I3(g, nx - ng + i, j, m)], a[I3(g, i - ng, j, m)]);
+    }
+  }
+}
+
+/* field_mode_zero_gradient for a cell-centred array (boundary.F90:654-707)
+ * on x_min, x_max, y_min, y_max in that order, skipping periodic ones. */
+static void centre_zero_gradient(const orc_grid *g, ocplx *a) {
+  const int nx = g->nx, ny = g->ny, ng = g->ng;
+  for (int m = 0; m < g->n_mode; ++m) {
+    if (g->bc_field[ORC_XMIN] != ORC_BC_PERIODIC)
+      for (int j = 1 - ng; j <= ny + ng; ++j)
+        for (int i = 1; i <= ng; ++i) a[I3(g, i - ng, j, m)] = a[I3(g, ng + 1 - i, j, m)];
+    if (g->bc_field[ORC_XMAX] != ORC_BC_PERIODIC)
+      for (int j = 1 - ng; j <= ny + ng; ++j)
+        for (int i = 1; i <= ng; ++i) a[I3(g, nx + i, j, m)] = a[I3(g, nx + 1 - i, j, m)];
+    if (g->bc_field[ORC_YMIN] != ORC_BC_PERIODIC)
+      for (int i = 1; i <= ng; ++i)
+        for (int ix = 1 - ng; ix <= nx + ng; ++ix) a[I3(g, ix, i - ng, m)] = a[I3(g, ix, ng + 1 - i, m)];
+    if (g->bc_field[ORC_YMAX] != ORC_BC_PERIODIC)
+      for (int i = 1; i <= ng; ++i)
+        for (int ix = 1 - ng; ix <= nx + ng; ++ix) a[I3(g, ix, ny + i, m)] = a[I3(g, ix, ny + 1 - i, m)];
+  }
+}
+
+/* calc_number_density_modes, calc_df.F90:588-661, with particle_to_grid.inc
+ * and triangle/gxfac.inc (weights summing to one, the r < dy fold) and
+ * get_macro_part_volume (partlist.F90:999-1013): each particle adds
+ * gx gy w / (2 pi dx dy r) times 1 (m = 0) or 2 e^{i m theta} (m > 0). */
+void orc_number_density_modes(const orc_grid *g, const orc_species *sp, int n_species,
+                              int species, ocplx *out) {
+  const size_t n = (size_t)g->n_mode * NX2(g) * NY2(g);
+  memset(out, 0, n * sizeof(ocplx));
+  const int s0 = species < 0 ? 0 : species, s1 = species < 0 ? n_species - 1 : species;
+  for (int is = s0; is <= s1; ++is) {
+    const orc_species *s = &sp[is];
+    for (int64_t ip = 0; ip < s->n; ++ip) {
+      const double y = s->y[ip], z = s->z[ip];
+      const double part_r = sqrt(y * y + z * z);
+      const double cxr = (s->x[ip] - g->x_grid_min) / g->dx;
+      const double cyr = (part_r - g->y_grid_min) / g->dy;
+      int cx = (int)floor(cxr + 0.5), cy = (int)floor(cyr + 0.5);
+      const double fx = (double)cx - cxr, fy = (double)cy - cyr;
+      cx += 1;
+      cy += 1;
+      double gx[3], gy[3];
+      const double cx2 = fx * fx, cy2 = fy * fy;
+      gx[0] = 0.5 * ((0.25 + cx2) + fx);
+      gx[1] = 0.75 - cx2;
+      gx[2] = 0.5 * ((0.25 + cx2) - fx);
+      gy[0] = 0.5 * ((0.25 + cy2) + fy);
+      gy[1] = 0.75 - cy2;
+      gy[2] = 0.5 * ((0.25 + cy2) - fy);
+      if (part_r < g->dy) {
+        gy[1] = gy[1] + gy[0];
+        gy[0] = 0.0;
+      }
+      if (cx - 1 < 1 - g->ng || cx + 1 > g->nx + g->ng || cy - 1 < 1 - g->ng || cy + 1 > g->ny + g->ng)
+        continue; /* outside the array: no valid particle gets here */
+      const double vol = (((2.0 * PI) * g->dx) * g->dy) * part_r;
+      const double pnd = s->w[ip] / vol;
+      const ocplx eith = cc(y / part_r, z / part_r);
+      ocplx eim = cc(1.0, 0.0), mf = cc(1.0, 0.0);
+      for (int m = 0; m < g->n_mode; ++m) {
+        if (m == 0) {
+          mf = cc(1.0, 0.0);
+        } else {
+          eim = cmul(eim, eith);
+          mf = rmul(2.0, eim);
+        }
+        for (int iy = -1; iy <= 1; ++iy)
+          for (int ix = -1; ix <= 1; ++ix) {
+            const size_t k = I3(g, cx + ix, cy + iy, m);
+            out[k] = cadd(out[k], rmul((gx[ix + 1] * gy[iy + 1]) * pnd, mf));
+          }
+      }
+    }
+  }
+  density_summation_bcs(g, out);
+  centre_zero_gradient(g, out);
+}
+
+/* the particle_data(:) of general_dist_fn for a non-photon particle
+ * (dist_fn.F90:486-553); angles only where `need` asks; returns 0 when an
+ * angle is needed and its momentum pair has |p| < tiny (CYCLE). */
+static int df_particle_data(const orc_species *s, int64_t ip, const int *need, const double *amax,
+                            double *pd) {
+  const double mc = s->m * C_LIGHT, mc2 = mc * C_LIGHT;
+  const double px = s->px[ip], py = s->py[ip], pz = s->pz[ip];
+  const double ux = px / mc, uy = py / mc, uz = pz / mc;
+  const double u2 = (ux * ux + uy * uy) + uz * uz;
+  const double gam = sqrt(u2 + 1.0);
+  const double gm1 = u2 / (gam + 1.0);
+  pd[ORC_DIR_X] = s->x[ip];
+  pd[ORC_DIR_Y] = s->y[ip];
+  pd[ORC_DIR_PX] = px;
+  pd[ORC_DIR_PY] = py;
+  pd[ORC_DIR_PZ] = pz;
+  pd[ORC_DIR_EN] = gm1 * mc2;
+  pd[ORC_DIR_GAMMA_M1] = gm1;
+  pd[ORC_DIR_MOD_P] = sqrt((px * px + py * py) + pz * pz);
+  const double a1[3] = {px, py, pz}, a2[3] = {py, pz, px};
+  for (int k = 0; k < 3; ++k) {
+    pd[ORC_DIR_XY_ANGLE + k] = 0.0;
+    if (!need[k]) continue;
+    const double p = sqrt(a1[k] * a1[k] + a2[k] * a2[k]);
+    if (p < C_TINY) return 0;
+    double th = asin(a2[k] / p);
+    if (a1[k] < 0.0) th = copysign(1.0, a2[k]) * PI - th;
+    if (amax && th + 2.0 * PI < amax[k]) th = th + 2.0 * PI;
+    pd[ORC_DIR_XY_ANGLE + k] = th;
+  }
+  return 1;
+}
+
+static int df_restricted(const double *pd, const double *restr, const int *use) {
+  for (int d = 1; d <= ORC_DF_MAXDIRS; ++d)
+    if (use[d - 1] && (pd[d] < restr[2 * (d - 1)] || pd[d] > restr[2 * (d - 1) + 1])) return 1;
+  return 0;
+}
+
+int orc_dist_fn(const orc_grid *g, const orc_species *s, int ndims, const int *direction,
+                double *ranges, int *resolution, const double *restrictions,
+                const int *use_restrictions, double *out) {
+  if (ndims < 1 || ndims > ORC_DF_MAXDIMS) return 1;
+  if (s->n < 1) return 2;
+  int calc[3] = {0, 0, 0}, any_calc = 0, parallel[3] = {0, 0, 0};
+  int need[3] = {0, 0, 0};
+  double dgrid[3] = {1.0, 1.0, 1.0};
+  for (int d = 0; d < ndims; ++d) {
+    const int dir = direction[d];
+    double *r = ranges + 2 * d;
+    if (dir == ORC_DIR_X || dir == ORC_DIR_Y) {
+      const double h = dir == ORC_DIR_X ? g->dx : g->dy;
+      const double lo = dir == ORC_DIR_X ? g->x_min : g->y_min;
+      const double hi = dir == ORC_DIR_X ? g->x_max : g->y_max;
+      const double gmin = dir == ORC_DIR_X ? g->x_grid_min : g->y_grid_min;
+      const double gmax = gmin + (double)((dir == ORC_DIR_X ? g->nx : g->ny) - 1) * h;
+      if (fabs(r[0] - r[1]) <= C_TINY) { /* dist_fn.F90:200-205, 248-253 */
+        r[0] = gmin - 0.5 * h;
+        r[1] = gmax + 0.5 * h;
+      } else { /* :206-229: snap to cell boundaries inside the domain */
+        r[0] = fmax(r[0], lo);
+        r[1] = fmin(r[1], hi);
+        r[0] = lo + floor((r[0] - lo) / h) * h;
+        r[1] = lo + ceil((r[1] - lo) / h) * h;
+        r[0] = fmax(r[0], gmin - 0.5 * h);
+        r[1] = fmin(r[1], gmax + 0.5 * h);
+      }
+      resolution[d] = (int)lround((r[1] - r[0]) / h);
+      if (resolution[d] <= 0) resolution[d] = 0;
+      dgrid[d] = h;
+      parallel[d] = 1;
+      continue;
+    }
+    if (dir < ORC_DIR_PX || dir > ORC_DIR_MOD_P) return 1;
+    if (fabs(r[0] - r[1]) <= C_TINY) { calc[d] = 1; any_calc = 1; }
+    if (dir >= ORC_DIR_XY_ANGLE && dir <= ORC_DIR_ZX_ANGLE) need[dir - ORC_DIR_XY_ANGLE] = 1;
+  }
+  for (int k = 0; k < 3; ++k)
+    if (use_restrictions[ORC_DIR_XY_ANGLE - 1 + k]) need[k] = 1;
+  double pd[ORC_DF_MAXDIRS + 1];
+  if (any_calc) { /* :341-453 */
+    for (int d = 0; d < ndims; ++d)
+      if (calc[d]) { ranges[2 * d] = 1.7976931348623157e308 * 0.4; ranges[2 * d + 1] = -1.7976931348623157e308 * 0.4; }
+    for (int64_t ip = 0; ip < s->n; ++ip) {
+      if (!df_particle_data(s, ip, need, NULL, pd)) continue;
+      if (df_restricted(pd, restrictions, use_restrictions)) continue;
+      for (int d = 0; d < ndims; ++d)
+        if (calc[d]) {
+          const double v = pd[direction[d]];
+          if (v < ranges[2 * d]) ranges[2 * d] = v;
+          if (v > ranges[2 * d + 1]) ranges[2 * d + 1] = v;
+        }
+    }
+  }
+  double amax[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < ndims; ++d) { /* :455-472 */
+    double *r = ranges + 2 * d;
+    if (fabs(r[0] - r[1]) <= C_TINY) { r[0] = -1.0; r[1] = 1.0; }
+    const int dir = direction[d];
+    if (dir >= ORC_DIR_XY_ANGLE && dir <= ORC_DIR_ZX_ANGLE) amax[dir - ORC_DIR_XY_ANGLE] = r[1];
+    if (!parallel[d]) dgrid[d] = (r[1] - r[0]) / (double)resolution[d];
+  }
+  for (int d = ndims; d < 3; ++d) resolution[d] = 1;
+  if (!out) return 0;
+  const int64_t ncell = (int64_t)resolution[0] * resolution[1] * resolution[2];
+  for (int64_t k = 0; k < ncell; ++k) out[k] = 0.0;
+  if (ncell == 0) return 0;
+  for (int64_t ip = 0; ip < s->n; ++ip) { /* :482-578 */
+    if (!df_particle_data(s, ip, need, amax, pd)) continue;
+    if (df_restricted(pd, restrictions, use_restrictions)) continue;
+    int64_t cell[3] = {0, 0, 0};
+    int ok = 1;
+    for (int d = 0; d < ndims; ++d) {
+      const int64_t c = (int64_t)floor((pd[direction[d]] - ranges[2 * d]) / dgrid[d]);
+      if (c < 0 || c >= resolution[d]) { ok = 0; break; }
+      cell[d] = c;
+    }
+    if (!ok) continue;
+    out[(cell[2] * resolution[1] + cell[1]) * resolution[0] + cell[0]] += s->w[ip];
+  }
+  return 0;
+}

@@ -142,6 +142,27 @@ double orc_set_dt(double dx, double dy, double dt_multiplier,
 void orc_mode_energy(const orc_grid *g, const orc_fields *f, double *w_e,
                      double *w_b);
 
+/* ---- output diagnostics (src/io/calc_df.F90, src/io/dist_fn.F90) ----- */
+/* calc_number_density_modes (calc_df.F90:588-661): species >= 0 one
+ * species, < 0 the sum over all; out: one mode array. */
+void orc_number_density_modes(const orc_grid *g, const orc_species *sp, int n_species,
+                              int species, ocplx *out);
+/* Directions of a distribution function: c_dir_* (constants.F90:231-247). */
+enum {
+  ORC_DIR_X = 1, ORC_DIR_Y = 2, ORC_DIR_PX = 3, ORC_DIR_PY = 4, ORC_DIR_PZ = 5,
+  ORC_DIR_EN = 6, ORC_DIR_GAMMA_M1 = 7, ORC_DIR_XY_ANGLE = 8, ORC_DIR_YZ_ANGLE = 9,
+  ORC_DIR_ZX_ANGLE = 10, ORC_DIR_MOD_P = 11, ORC_DF_MAXDIRS = 11, ORC_DF_MAXDIMS = 3
+};
+/* general_dist_fn on one rank (dist_fn.F90:118-715).  ranges[2*3] and
+ * resolution[3] are the requested values on entry and the ones used on
+ * return; restrictions[2*11] / use_restrictions[11] per direction (index
+ * direction - 1).  out (resolution[0]*resolution[1]*resolution[2] doubles,
+ * first index fastest) is written unless NULL.  Returns 0, 1 for an unknown
+ * direction, 2 for an empty species (nothing written). */
+int orc_dist_fn(const orc_grid *g, const orc_species *s, int ndims, const int *direction,
+                double *ranges, int *resolution, const double *restrictions,
+                const int *use_restrictions, double *out);
+
 #ifdef __cplusplus
 }
 #endif

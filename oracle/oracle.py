@@ -90,11 +90,14 @@ def _declare(L):
         "orc_cell_keys": [gp, C.c_int64, dp, dp, dp, C.c_int, C.c_int, C.c_int, dp],
         "orc_stable_sort_keys": [C.c_int64, dp, C.c_int32, dp],
         "orc_mode_energy": [gp, fp, dp, dp],
+        "orc_number_density_modes": [gp, sp, C.c_int, C.c_int, dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = None
+    L.orc_dist_fn.argtypes = [gp, sp, C.c_int, dp, dp, dp, dp, dp, dp]
+    L.orc_dist_fn.restype = C.c_int
     L.orc_set_dt.argtypes = [C.c_double] * 5
     L.orc_set_dt.restype = C.c_double
 
@@ -239,11 +242,60 @@ class State:
         lib().orc_mode_energy(C.byref(self.g), C.byref(self.f), _ptr(we), _ptr(wb))
         return we, wb
 
+    def number_density_modes(self, species=-1):
+        """calc_number_density_modes (calc_df.F90:588-661); species < 0 sums
+        every species."""
+        out = np.zeros(self.shape, np.complex128)
+        arr = self._species_array()
+        lib().orc_number_density_modes(C.byref(self.g), arr, len(self.species), int(species), _ptr(out))
+        return out
+
+    def dist_fn(self, species, directions, ranges=None, resolution=None, restrictions=None):
+        """general_dist_fn (dist_fn.F90:118-715) on one rank.  directions:
+        1-3 of DIRS; ranges: per direction (lo, hi), equal values meaning
+        the automatic range; resolution: bins per momentum direction;
+        restrictions: {direction: (lo, hi)}.  Returns (histogram with the
+        first direction fastest, used ranges, used resolution) or None for
+        an empty species."""
+        nd = len(directions)
+        d = np.zeros(3, np.int32)
+        d[:nd] = [DIRS[k] if isinstance(k, str) else k for k in directions]
+        rg = np.zeros(6)
+        if ranges is not None:
+            rg[: 2 * nd] = np.ravel(ranges)
+        res = np.ones(3, np.int32)
+        if resolution is not None:
+            res[:nd] = resolution
+        restr = np.zeros(22)
+        use = np.zeros(11, np.int32)
+        for k, (lo, hi) in (restrictions or {}).items():
+            i = (DIRS[k] if isinstance(k, str) else k) - 1
+            restr[2 * i: 2 * i + 2] = lo, hi
+            use[i] = 1
+        bound = 1
+        for k in range(nd):
+            bound *= {1: self.g.nx + 1, 2: self.g.ny + 1}.get(int(d[k]), int(res[k]))
+        out = np.zeros(max(1, bound))
+        arr = self._species_array()
+        sp = C.cast(C.byref(arr[species]), C.POINTER(Species))
+        rc = lib().orc_dist_fn(C.byref(self.g), sp, nd, _ptr(d), _ptr(rg), _ptr(res), _ptr(restr), _ptr(use),
+                               _ptr(out))
+        if rc == 2:
+            return None
+        if rc:
+            raise ValueError("bad distribution function direction")
+        n = int(res[0]) * int(res[1]) * int(res[2])
+        return out[:n].reshape(res[2], res[1], res[0]), rg[: 2 * nd].reshape(nd, 2), res[:nd].copy()
+
     def geometry(self):
         n = self.g.ny + 2 * self.g.ng + 1
         a = [np.zeros(n) for _ in range(4)]
         lib().orc_geometry(C.byref(self.g), *[_ptr(x) for x in a])
         return a
+
+
+# c_dir_* of constants.F90:231-247
+DIRS = dict(x=1, y=2, px=3, py=4, pz=5, en=6, gamma_m1=7, xy_angle=8, yz_angle=9, zx_angle=10, mod_p=11)
 
 
 def map_field_bc(name):

@@ -1,0 +1,192 @@
+"""The oracle's output diagnostics checked against what fixes them
+independently of the reference's text:
+
+* number density modes (calc_df.F90:588-661): each particle carries the
+  quadratic B-spline weights 0.5 (1/2 + f)^2, 3/4 - f^2, 0.5 (1/2 - f)^2, so
+  a particle's 3x3 stencil sums to w / (2 pi dx dy r) times 1 (m = 0) or
+  2 e^{+i m theta} (m > 0); the periodic and reflecting ghost sums keep that
+  total inside the domain; a uniform plasma gives its density in m = 0;
+* the distribution function (dist_fn.F90:118-715): numpy's histogram over
+  the same edges, gamma - 1 computed directly, arctan2 for the angles."""
+import numpy as np
+import pytest
+
+ME, QE, C = 9.1093837015e-31, 1.602176634e-19, 2.99792458e8
+NX, NY, M, NG = 20, 12, 3, 5
+DX = DY = 1e-7
+
+
+def _state(oracle, bf=("periodic", "periodic", "other", "conduct"), bp=("periodic", "periodic", "other", "reflect")):
+    return oracle.State(NX, NY, M, DX, DY, 1e-17, bc_field=bf, bc_particle=bp)
+
+
+def _one(st, x, y, z, w=3.0):
+    st.species.clear()
+    one = np.array([1.0])
+    st.add_species(-QE, ME, one * x, one * y, one * z, one * 0, one * 0, one * 0, one * w)
+
+
+def _interior(a):
+    return a[:, NG:NG + NY, NG:NG + NX]
+
+
+@pytest.mark.parametrize("px,pr,th", [(7.3, 5.2, 0.4), (0.2, 3.7, 2.5), (19.9, 8.1, -1.2), (10.5, 0.3, 1.0),
+                                      (4.0, 11.8, 0.0)])
+def test_density_stencil_total_and_mode_phase(oracle, px, pr, th):
+    st = _state(oracle)
+    x, r = px * DX, pr * DY
+    _one(st, x, r * np.cos(th), r * np.sin(th))
+    d = st.number_density_modes()
+    pnd = 3.0 / (2 * np.pi * DX * DY * r)
+    tot = _interior(d).sum(axis=(1, 2))
+    assert np.isclose(tot[0], pnd, rtol=1e-12)
+    for m in range(1, M):
+        assert np.isclose(tot[m], 2 * pnd * np.exp(1j * m * th), rtol=1e-12), m
+
+
+@pytest.mark.parametrize("f", [-0.49, -0.2, 0.0, 0.31, 0.5])
+def test_density_weights_are_quadratic_bspline(oracle, f):
+    st = _state(oracle)
+    ix, iy = 9, 6  # Fortran cell indices; cell centre at x_grid_min + (ix - 1) dx
+    fy = 0.13
+    x = st.g.x_grid_min + (ix - 1 - f) * DX  # cell_frac = cell - cell_r = f
+    r = st.g.y_grid_min + (iy - 1 - fy) * DY
+    _one(st, x, r, 0.0, w=1.0)
+    d = st.number_density_modes()[0].real
+    pnd = 1.0 / (2 * np.pi * DX * DY * r)
+    bx = [0.5 * (0.5 + f) ** 2, 0.75 - f * f, 0.5 * (0.5 - f) ** 2]
+    by = [0.5 * (0.5 + fy) ** 2, 0.75 - fy * fy, 0.5 * (0.5 - fy) ** 2]
+    cx = int(np.floor((x - st.g.x_grid_min) / DX + 0.5)) + 1
+    for a in range(3):
+        for b in range(3):
+            got = d[iy + a - 2 + NG, cx + b - 2 + NG]
+            assert np.isclose(got, bx[b] * by[a] * pnd, rtol=1e-12, atol=0), (a, b)
+
+
+@pytest.mark.parametrize("rr", [0.0001, 0.3, 0.5, 0.99])
+def test_density_axis_fold(oracle, rr):
+    """r < dy is exactly when the stencil reaches the iy = 0 row; its weight
+    moves into iy = 1, so the whole weight stays inside: row 1 carries
+    gy(0) + gy(-1) of the quadratic B-spline."""
+    st = _state(oracle)
+    r = rr * DY
+    _one(st, 10.0 * DX, r, 0.0, w=1.0)
+    d = st.number_density_modes()[0].real
+    pnd = 1.0 / (2 * np.pi * DX * DY * r)
+    rows = d.sum(axis=1)  # sum over x: the radial weights times pnd
+    assert np.isclose(rows[NG:NG + NY].sum(), pnd, rtol=1e-12)
+    fy = 0.0 - (r - st.g.y_grid_min) / DY  # cell_y = 0 before the +1
+    assert np.isclose(rows[NG], (0.75 - fy * fy + 0.5 * (0.5 + fy) ** 2) * pnd, rtol=1e-12)
+
+
+def test_density_zero_gradient_ghosts(oracle):
+    st = _state(oracle, bf=("conduct", "conduct", "other", "conduct"), bp=("open", "open", "other", "open"))
+    rng = np.random.default_rng(3)
+    n = 400
+    x = rng.uniform(0.6, NX - 0.6, n) * DX
+    r = rng.uniform(1.2, NY - 0.6, n) * DY
+    th = rng.uniform(0, 2 * np.pi, n)
+    z = np.zeros(n)
+    st.add_species(-QE, ME, x, r * np.cos(th), r * np.sin(th), z, z, z, np.ones(n))
+    d = st.number_density_modes()
+    for i in range(1, NG + 1):  # f(i - ng) = f(ng + 1 - i) and the mirror at the far sides
+        assert np.array_equal(d[:, :, i - 1], d[:, :, 2 * NG - i])
+        assert np.array_equal(d[:, :, NG + NX + i - 1], d[:, :, NG + NX - i])
+        assert np.array_equal(d[:, i - 1, :], d[:, 2 * NG - i, :])
+        assert np.array_equal(d[:, NG + NY + i - 1, :], d[:, NG + NY - i, :])
+
+
+@pytest.mark.parametrize("bp", [("periodic", "periodic", "other", "reflect"), ("reflect", "reflect", "other", "reflect")])
+def test_density_ghost_sums_keep_the_total(oracle, bp):
+    st = _state(oracle, bp=bp)
+    rng = np.random.default_rng(11)
+    n = 3000
+    x = rng.uniform(0.0, NX, n) * DX
+    r = rng.uniform(0.05, NY, n) * DY
+    th = rng.uniform(0, 2 * np.pi, n)
+    w = rng.uniform(0.5, 2.0, n)
+    z = np.zeros(n)
+    st.add_species(-QE, ME, x, r * np.cos(th), r * np.sin(th), z, z, z, w)
+    d = st.number_density_modes()
+    pnd = w / (2 * np.pi * DX * DY * r)
+    want = [pnd.sum()] + [(2 * pnd * np.exp(1j * m * th)).sum() for m in range(1, M)]
+    tot = _interior(d).sum(axis=(1, 2))
+    for m in range(M):
+        assert np.isclose(tot[m], want[m], rtol=1e-11, atol=1e-12 * abs(want[0])), m
+
+
+def test_uniform_plasma_density(oracle):
+    from cylindrical_epoch_amd import deck
+    st = _state(oracle)
+    n0 = 1e25
+    p = deck.uniform_plasma(NX, NY, DX, DY, 64, n0, 10.0, mass=ME, seed=5)
+    st.add_species(-QE, ME, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+    d = st.number_density_modes()[0].real
+    bulk = d[NG + 2:NG + NY - 2, NG:NG + NX]
+    assert abs(bulk.mean() / n0 - 1) < 0.02
+
+
+def _thermal(oracle, n=4000, seed=2):
+    st = _state(oracle)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, NX, n) * DX
+    r = rng.uniform(0.1, NY, n) * DY
+    th = rng.uniform(0, 2 * np.pi, n)
+    p = rng.normal(size=(3, n)) * ME * C * 0.3
+    w = rng.uniform(0.5, 1.5, n)
+    st.add_species(-QE, ME, x, r * np.cos(th), r * np.sin(th), p[0], p[1], p[2], w)
+    return st, dict(x=x, y=r * np.cos(th), px=p[0], py=p[1], pz=p[2], w=w)
+
+
+def test_energy_spectrum_matches_numpy_histogram(oracle):
+    st, p = _thermal(oracle)
+    u2 = (p["px"] ** 2 + p["py"] ** 2 + p["pz"] ** 2) / (ME * C) ** 2
+    en = (np.sqrt(1 + u2) - 1) * ME * C * C
+    lo, hi, nb = 0.0, 0.9 * en.max(), 40
+    h, rg, res = st.dist_fn(0, ["en"], ranges=[(lo, hi)], resolution=[nb])
+    ref, _ = np.histogram(en, bins=np.linspace(lo, hi, nb + 1), weights=p["w"])
+    # particles within 1e-9 of a bin edge may fall either side
+    edges = np.linspace(lo, hi, nb + 1)
+    near = np.min(np.abs(en[:, None] - edges[None, :]), axis=1) < 1e-9 * hi
+    assert not near.any()
+    assert np.allclose(h.ravel(), ref, rtol=1e-12, atol=0)
+    assert res[0] == nb and tuple(rg[0]) == (lo, hi)
+
+
+def test_automatic_range_excludes_the_maximum(oracle):
+    st, p = _thermal(oracle)
+    h, rg, res = st.dist_fn(0, ["px"], resolution=[25])
+    assert rg[0, 0] == p["px"].min() and rg[0, 1] == p["px"].max()
+    # FLOOR((max - min) / dgrid) + 1 = resolution + 1: the largest particle drops out
+    assert np.isclose(h.sum(), p["w"].sum() - p["w"][np.argmax(p["px"])], rtol=1e-12)
+
+
+def test_spatial_axis_and_restriction(oracle):
+    st, p = _thermal(oracle)
+    h, rg, res = st.dist_fn(0, ["x", "py"], ranges=[(3.3 * DX, 11.2 * DX), (-1e-22, 1e-22)], resolution=[0, 10],
+                            restrictions={"pz": (0.0, 1e30)})
+    # x snaps to whole cells inside the domain: [3 dx, 12 dx), one bin per cell
+    assert np.allclose(rg[0], [3 * DX, 12 * DX], rtol=1e-12) and res[0] == 9
+    keep = p["pz"] >= 0.0
+    ref, _, _ = np.histogram2d(p["py"][keep], p["x"][keep], bins=[np.linspace(-1e-22, 1e-22, 11),
+                                                                    np.linspace(3 * DX, 12 * DX, 10)],
+                               weights=p["w"][keep])
+    assert np.allclose(h[0], ref, rtol=1e-12, atol=1e-12 * ref.max())
+
+
+def test_xy_angle_is_arctan2(oracle):
+    st, p = _thermal(oracle)
+    h, rg, res = st.dist_fn(0, ["xy_angle"], ranges=[(-np.pi, np.pi)], resolution=[36])
+    ang = np.arctan2(p["py"], p["px"])
+    ref, _ = np.histogram(ang, bins=np.linspace(-np.pi, np.pi, 37), weights=p["w"])
+    assert np.allclose(h.ravel(), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_empty_species_and_bad_direction(oracle):
+    st = _state(oracle)
+    e = np.zeros(0)
+    st.add_species(-QE, ME, e, e, e, e, e, e, e)
+    assert st.dist_fn(0, ["en"], resolution=[10]) is None
+    st2, _ = _thermal(oracle, n=10)
+    with pytest.raises(ValueError):
+        st2.dist_fn(0, [12], resolution=[10])
