@@ -66,3 +66,31 @@ def test_grid_moves_and_particles_left_of_x_min_go(oracle):
     sp = st.species[0]
     assert sp["n"] == keep.sum()
     assert np.array_equal(sp["id"][: sp["n"]], np.arange(n)[keep])  # order kept
+
+
+def test_x_max_fix_keeps_a_line_straight(oracle):
+    """A field linear in x, whose stored x_max columns continue the line at
+    the columns they refill, stays exactly that line through one shift: the
+    averaged staggered neighbour of the incoming cell is the midpoint of the
+    line's two values around it."""
+    st = _state(oracle)
+    rng = np.random.default_rng(7)
+    ng, nx = 5, st.g.nx
+    ix = np.arange(st.shape[2]) + 1 - ng  # Fortran column index per array column
+    lines = {}
+    for n in FIELDS9:
+        a = rng.normal(size=(st.shape[0], st.shape[1], 1)) + 1j * rng.normal(size=(st.shape[0], st.shape[1], 1))
+        b = rng.normal(size=(st.shape[0], st.shape[1], 1)) + 1j * rng.normal(size=(st.shape[0], st.shape[1], 1))
+        lines[n] = (a, b)
+        st.arr[n][:] = a + b * ix[None, None, :]
+    # the column each stored boundary value refills, window.F90:101-128
+    refill = dict(ex=nx + 1, er=nx, et=nx, bx=nx, br=nx + 1, bt=nx + 1)
+    for k, n in enumerate(("ex", "er", "et", "bx", "br", "bt")):
+        a, b = lines[n]
+        st.bnd["xmax"][k][:] = (a + b * (refill[n] + 1))[:, :, 0]
+    st.shift_window(1)
+    for n in ("ex", "er", "et", "bx", "br", "bt"):
+        a, b = lines[n]
+        want = a + b * (ix[None, None, :] + 1)
+        cols = slice(0, nx + 1 + ng)  # every column up to ix = nx + 1
+        assert np.allclose(st.arr[n][:, :, cols], want[:, :, cols], rtol=0, atol=1e-12), n
