@@ -167,6 +167,74 @@ def test_continuity_mode0_finite_volume(oracle, seed):
         assert abs(res) < 1e-12 * scale, (cx, cy, res)
 
 
+def _mode_charge(g, pos, q, w, M):
+    """Charge modes on the staggered grid at a position: quadratic-spline
+    fractions times 1 (m = 0) or 2 e^{+i m theta} (m > 0)."""
+    xr = (pos[0] - g.x_grid_min) / g.dx
+    rr = (math.hypot(pos[1], pos[2]) - g.y_grid_min) / g.dy
+    th = math.atan2(pos[2], pos[1])
+    cx, wx = _stagger_weights(xr)
+    cy, wy = _stagger_weights(rr)
+    Q = {}
+    for m in range(M):
+        f = 1.0 if m == 0 else 2 * np.exp(1j * m * th)
+        for i in range(3):
+            for j in range(3):
+                Q[(cx - 1 + i, cy - 1 + j, m)] = q * w * wx[i] * wy[j] * f
+    return Q
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_mode_continuity_structure(oracle, seed):
+    """All modes of one push (particles.F90:508-665).  With Q_m the charge
+    modes at x(t+dt/2) and x(t+3dt/2), R = Q_m(1) - Q_m(0) + dt (x and r
+    face fluxes of Jx, Jr) must be exactly what the azimuthal current has
+    to carry: R = i m dt K cy Jt in every cell and mode, with one K per
+    particle.  m = 0 gives R = 0.  For m > 0 the same K in every cell and
+    mode pins the m-factors of Jx and Jr against the exact phases
+    e^{i m theta} of the charge; K differs between particles because the
+    reference deposits Jt with v_theta(t+dt) rather than r dtheta/dt, so
+    per-mode continuity for m > 0 is not a discrete identity of the
+    reference's scheme."""
+    rng = np.random.default_rng(100 + seed)
+    M = 3
+    st = _state(oracle, nx=24, ny=20, M=M)
+    g = st.g
+    x = (8 + rng.random()) * g.dx
+    r = (9 + rng.random()) * g.dy
+    th = rng.uniform(-3, 3)
+    u = rng.normal(size=3) * 2.0
+    q, w = -Q0, 1.7e3
+    sp = _one_particle(st, x, r, th, p=u * M0 * C_LIGHT, w=w)
+    gam = math.sqrt(1 + u @ u)
+    pos0 = np.array([sp["x"][0], sp["y"][0], sp["z"][0]])
+    half = pos0 + u * C_LIGHT * g.dt / 2 / gam
+    st.push_particles()
+    u1 = np.array([sp["px"][0], sp["py"][0], sp["pz"][0]]) / (M0 * C_LIGHT)
+    pos1 = np.array([sp["x"][0], sp["y"][0], sp["z"][0]])
+    end15 = pos1 + u1 * C_LIGHT * g.dt / 2 / math.sqrt(1 + u1 @ u1)
+    inv_rt, inv_xt, inv_vol, _ = st.geometry()
+    ng = g.ng
+    Qa, Qb = _mode_charge(g, half, q, w, M), _mode_charge(g, end15, q, w, M)
+    scale = abs(q * w)
+    ks = []
+    for (cx, cy, m) in sorted(set(Qa) | set(Qb)):
+        flux = ((st.at("jx", cx + 1, cy, m) - st.at("jx", cx, cy, m)) / inv_rt[cy + ng]
+                + st.at("jr", cx, cy + 1, m) / inv_xt[cy + ng] - st.at("jr", cx, cy, m) / inv_xt[cy - 1 + ng])
+        R = Qb.get((cx, cy, m), 0.0) - Qa.get((cx, cy, m), 0.0) + g.dt * flux
+        jt = st.at("jt", cx, cy, m)
+        if m == 0:
+            assert abs(R) < 1e-12 * scale, (cx, cy, R)
+        elif abs(jt) * g.dt * g.dx * g.dy * cy > 1e-9 * scale:
+            ks.append(R / (1j * m * g.dt * jt * g.dx * g.dy * cy))
+        else:
+            assert abs(R) < 1e-11 * scale
+    ks = np.array(ks)
+    assert len(ks) >= 12
+    assert np.allclose(ks, ks[0], rtol=1e-8, atol=0), ks
+    assert abs(ks[0].imag) < 1e-8 * abs(ks[0]) and 0.3 < ks[0].real < 1.5
+
+
 def test_small_angle_switch_is_continuous(oracle):
     """The m>0 factors switch from the exact form to the small-angle Taylor
     form at |m dtheta| = 1e-4 (particles.F90:589-626); the deposited current
