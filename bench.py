@@ -143,6 +143,26 @@ def run(sim, laser, dt, steps, warmup, window=None):
     return (t1 - t0) / steps, prof
 
 
+def accuracy_step(sim, laser, t, window=None):
+    """One more step, outside the timed region, in the main loop's order
+    with the m = 0 continuity check around the push; returns the residual
+    relative to the largest per-cell charge change and to the largest
+    per-cell charge, and the on-axis Ex (Re exm(1:nx, 0, 0)) after it."""
+    sim.update_eb_fields_half()
+    sim.continuity_begin()
+    sim.push_particles()
+    res, dq, q = sim.continuity_end()
+    sim.current_finish()
+    sim.update_eb_fields_final(laser.sources(t))
+    if window:
+        window.advance(sim, 0)
+    ng = 5
+    ex = sim.download_field("ex")
+    axis = ex[0, ng - 1, ng:ng + sim.nx].real.copy()  # iy = 0 is r = 0
+    return {"residual_over_max_dq": res / dq if dq > 0 else None,
+            "residual_over_max_q": res / q if q > 0 else None}, axis
+
+
 def measured_traffic(precision):
     """HBM bytes per push_deposit launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
@@ -278,11 +298,13 @@ def main():
             win = Window(w, dt, Window.cells_for(w, dt, a.steps + a.warmup))
         log(f"{prec}: uploaded, dt = {dt:.4e}")
         ms, prof = run(sim, laser, dt, a.steps, a.warmup, win)
+        sorts = sim.sort_counts()
+        cont, axis = accuracy_step(sim, laser, (a.warmup + a.steps + 1) * dt, win)
         if win:
             log(f"{prec}: window moved {win.used} cells, {win.injected} particles injected")
-        sorts = sim.sort_counts()
+        log(f"{prec}: m = 0 continuity residual {cont}")
         sim.close()
-        results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof, sorts=sorts)
+        results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof, sorts=sorts, cont=cont, axis=axis)
         log(f"{prec}: {ms * 1e3:.3f} ms/step, {n / ms:.4e} particle-steps/s")
         for k, (kms, cnt) in prof.items():
             if cnt:
@@ -309,8 +331,14 @@ def main():
                           for k, v in results.items()},
         "roofline": roofline(w, n, r["prof"]["push_deposit"], main_prec),
     }
+    acc = {"after_steps": a.warmup + a.steps + 1,
+           "continuity_m0": {k: v["cont"] for k, v in results.items()}}
     if "fp64" in results and "mixed" in results:
         out["mixed_over_fp64"] = round(results["mixed"]["value"] / results["fp64"]["value"], 3)
+        ref = results["fp64"]["axis"]
+        acc["on_axis_ex_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(results["mixed"]["axis"] - ref)) /
+                                                        max(np.max(np.abs(ref)), 1e-300))
+    out["accuracy"] = acc
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_sample_cols)

@@ -98,6 +98,9 @@ struct cylpic_ctx {
   double* diag_part = nullptr;      // per-block partial ranges of the distribution functions
   double* diag_hist = nullptr;      // distribution function bins
   long diag_hist_cap = 0;
+  double* cont_q = nullptr;               // two m = 0 charge planes of the continuity check
+  unsigned long long* cont_out = nullptr;  // its two maxima
+  bool cont_armed = false;
   int smooth_its = 0, smooth_comp_its = 0;  // current smoothing (current_smooth.F90)
   std::vector<int> smooth_strides{1};
   long sort_radix = 0, sort_migrate = 0;
@@ -528,6 +531,8 @@ void free_all(cylpic_ctx* h) {
   if (h->diag64) (void)hipFree(h->diag64);
   if (h->diag_part) (void)hipFree(h->diag_part);
   if (h->diag_hist) (void)hipFree(h->diag_hist);
+  if (h->cont_q) (void)hipFree(h->cont_q);
+  if (h->cont_out) (void)hipFree(h->cont_out);
   if (h->mig_host) (void)hipHostFree(h->mig_host);
   if (h->d_energy) (void)hipFree(h->d_energy);
   if (h->sc_mem) (void)hipFree(h->sc_mem);
@@ -1373,6 +1378,54 @@ int cylpic_dist_fn(cylpic_ctx* h, int species, int ndims, const int* direction, 
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, h->diag_hist, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  return check_device_errors(h);
+}
+
+// the m = 0 charge of every mobile species at x + v dt/2 into q
+static int charge_half(cylpic_ctx* h, double* q) {
+  HIPCHK(hipMemsetAsync(q, 0, h->g.plane * sizeof(double), h->stream));
+  for (auto& s : h->sp) {
+    if (s.immobile || s.n <= 0) continue;
+    if (h->prec == CYLPIC_FP64) launch_charge_half_f64(h->g, view64(s), s.n, h->tl.ntiles, s.m, s.q, q, h->stream);
+    else launch_charge_half_mixed(h->g, viewm(s), s.n, h->tl.ntiles, s.q, q, h->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+int cylpic_continuity_begin(cylpic_ctx* h) {
+  CHECK_H(h);
+  HIPCHK(hipSetDevice(h->device));
+  if (!h->cont_q) {
+    HIPCHK(hipMalloc((void**)&h->cont_q, 2 * h->g.plane * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&h->cont_out, 3 * sizeof(unsigned long long)));
+  }
+  int rc = charge_half(h, h->cont_q);
+  if (rc) return rc;
+  h->cont_armed = true;
+  return CYLPIC_OK;
+}
+
+int cylpic_continuity_end(cylpic_ctx* h, double* residual, double* dq, double* q) {
+  CHECK_H(h);
+  if (!residual || !dq || !q) return fail(CYLPIC_EINVAL, "null output");
+  if (!h->cont_armed) return fail(CYLPIC_ESTATE, "cylpic_continuity_begin was not called before this push");
+  HIPCHK(hipSetDevice(h->device));
+  h->cont_armed = false;
+  double* q1 = h->cont_q + h->g.plane;
+  int rc = charge_half(h, q1);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(h->cont_out, 0, 3 * sizeof(unsigned long long), h->stream));
+  with_fields(h, [&](auto& F) {
+    launch_continuity_m0(h->g, h->cont_q, q1, F.jx(), F.jr(), geo_of(h), h->cont_out, h->stream);
+  });
+  HIPCHK(hipGetLastError());
+  unsigned long long bits[3];
+  HIPCHK(hipMemcpyAsync(bits, h->cont_out, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::memcpy(residual, &bits[0], sizeof(double));
+  std::memcpy(dq, &bits[1], sizeof(double));
+  std::memcpy(q, &bits[2], sizeof(double));
   return check_device_errors(h);
 }
 

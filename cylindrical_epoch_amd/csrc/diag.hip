@@ -7,6 +7,8 @@
 // FP64 exactly as cylpic_download_particles returns them, so the cell
 // indices, shape weights and bins equal those of a CPU pass over the
 // downloaded particles.
+#include <algorithm>
+
 #include "cplx_ops.h"
 #include "launch.h"
 
@@ -273,6 +275,75 @@ __global__ __launch_bounds__(256) void k_df_bin(Grid g, P p, long n, double mc, 
   }
 }
 
+// ------------------------------------------------------------------------
+// continuity check of the FP64 deposit: the m = 0 charge of a particle at
+// x + v dt/2 (the half push of particles.F90:296-312) on the staggered grid
+// with quadratic-spline fractions summing to one, the grid on which the
+// charge-conserving deposit (particles.F90:508-665) keeps
+// Q(t+3dt/2) - Q(t+dt/2) + dt (net current out of the cell) = 0.  The mixed
+// path's charge (its own FP32 half step) is in push_mixed.hip.
+template <class P>
+__global__ __launch_bounds__(256) void k_charge_half(Grid g, P p, long n, int ntiles, double mc, double q,
+                                                     double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (unsigned)p.key[i] >= (unsigned)ntiles) return;
+  double x, y, z, pm[3], w;
+  load_particle(g, p, i, x, y, z, pm, w, mc);
+  const double ux = pm[0] / mc, uy = pm[1] / mc, uz = pm[2] / mc;
+  const double root = (C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+  const double xh = x + ux * root, yh = y + uy * root, zh = z + uz * root;
+  const double xr = (xh - g.x_grid_min) / g.dx;
+  const double rr = (sqrt(yh * yh + zh * zh) - g.y_grid_min) / g.dy;
+  const double fcx = floor(xr), fcy = floor(rr);
+  const int cx = (int)fcx, cy = (int)fcy;  // first of the three cells (Fortran index)
+  if (cx < 1 - g.ng || cx + 2 > g.nx + g.ng || cy < 1 - g.ng || cy + 2 > g.ny + g.ng) return;
+  const double fx = fcx - xr + 0.5, fy = fcy - rr + 0.5;
+  const double wx[3] = {0.5 * ((0.25 + fx * fx) + fx), 0.5 * (1.5 - 2.0 * fx * fx), 0.5 * ((0.25 + fx * fx) - fx)};
+  const double wy[3] = {0.5 * ((0.25 + fy * fy) + fy), 0.5 * (1.5 - 2.0 * fy * fy), 0.5 * ((0.25 + fy * fy) - fy)};
+  const double qw = q * w;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) unsafeAtomicAdd(out + g.idx(cx + k, cy + j, 0), qw * (wx[k] * wy[j]));
+}
+
+// max |Q1 - Q0 + dt (fluxes of Jx, Jr)|, max |Q1 - Q0| and max |Q1| of the
+// m = 0 cells at least ng from every edge; out[0..2] hold the bit patterns
+// of the non-negative maxima (ordered as unsigned integers)
+template <class T>
+__global__ __launch_bounds__(256) void k_continuity_m0(Grid g, const double* __restrict__ q0,
+                                                       const double* __restrict__ q1, const cplx<T>* __restrict__ jx,
+                                                       const cplx<T>* __restrict__ jr, Geo geo,
+                                                       unsigned long long* __restrict__ out) {
+  const int nxi = g.nx - 2 * g.ng, nyi = g.ny - 2 * g.ng;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double res = 0.0, dq = 0.0, qq = 0.0;
+  if (nxi > 0 && nyi > 0 && e < (long)nxi * nyi) {
+    const int cx = g.ng + 1 + (int)(e % nxi), cy = g.ng + 1 + (int)(e / nxi);
+    const int jng = g.ng;
+    const double flux = ((double)jx[g.idx(cx + 1, cy, 0)].re - (double)jx[g.idx(cx, cy, 0)].re) /
+                            geo.inv_area_rt[cy + jng] +
+                        (double)jr[g.idx(cx, cy + 1, 0)].re / geo.inv_area_xt[cy + jng] -
+                        (double)jr[g.idx(cx, cy, 0)].re / geo.inv_area_xt[cy - 1 + jng];
+    const long k = g.idx(cx, cy, 0);
+    const double d = q1[k] - q0[k];
+    res = fabs(d + g.dt * flux);
+    dq = fabs(d);
+    qq = fabs(q1[k]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    res = fmax(res, __shfl_xor(res, o));
+    dq = fmax(dq, __shfl_xor(dq, o));
+    qq = fmax(qq, __shfl_xor(qq, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out, (unsigned long long)__double_as_longlong(res));
+    atomicMax(out + 1, (unsigned long long)__double_as_longlong(dq));
+    atomicMax(out + 2, (unsigned long long)__double_as_longlong(qq));
+  }
+}
+
 inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 template <class P>
@@ -305,6 +376,23 @@ void launch_number_density_f64(const Grid& g, const PartF64& p, long n, cplx<dou
 void launch_number_density_mixed(const Grid& g, const PartMixed& p, long n, cplx<double>* out, hipStream_t s) {
   number_density(g, p, n, 0.0, out, s);
 }
+
+void launch_charge_half_f64(const Grid& g, const PartF64& p, long n, int ntiles, double mass, double q,
+                            double* out, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_charge_half<PartF64>, dim3(nblk(n)), dim3(256), 0, s, g, p, n, ntiles, mass * C_LIGHT, q,
+                       out);
+}
+template <class T>
+void launch_continuity_m0(const Grid& g, const double* q0, const double* q1, const cplx<T>* jx, const cplx<T>* jr,
+                          const Geo& geo, unsigned long long* out, hipStream_t s) {
+  const long n = (long)std::max(0, g.nx - 2 * g.ng) * std::max(0, g.ny - 2 * g.ng);
+  if (n > 0) hipLaunchKernelGGL(k_continuity_m0<T>, dim3(nblk(n)), dim3(256), 0, s, g, q0, q1, jx, jr, geo, out);
+}
+template void launch_continuity_m0<float>(const Grid&, const double*, const double*, const cplx<float>*,
+                                          const cplx<float>*, const Geo&, unsigned long long*, hipStream_t);
+template void launch_continuity_m0<double>(const Grid&, const double*, const double*, const cplx<double>*,
+                                           const cplx<double>*, const Geo&, unsigned long long*, hipStream_t);
 
 void launch_density_bcs(const Grid& g, cplx<double>* a, hipStream_t s) {
   const bool xref = g.bc_particle[XMIN] == BC_REFLECT || g.bc_particle[XMAX] == BC_REFLECT;

@@ -70,6 +70,29 @@ __device__ __forceinline__ void w3f(float frac, float& wm, float& w0, float& wp)
 
 // f2, f3, f4 of the m-factors: m_fac_k = 2 exp(i m theta0) f_k(m dtheta).
 // f2 = (e^{ix}-1)/(ix), f3 = (e^{ix}(1-ix)-1)/x^2, f4 = i(e^{ix}(2-x^2-2ix)-2)/x^3
+// The half push to t + dt/2 (particles.F90:296-312) in cell units, shared by
+// the push and the continuity check so that both work with the same FP32
+// values: the position, 1/r, r, the radial cell coordinate and the floors
+// that place the staggered weights (hx_dcell.inc).
+struct HalfStep {
+  float oxh, yh, zh, irh, rh, cyr, fl2x, fl2y;
+};
+__device__ __forceinline__ HalfStep half_step(const MixConst& mc, float ox, float y, float z, float ux, float uy,
+                                              float uz) {
+  HalfStep h;
+  const float ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
+  h.oxh = ox + ux * (mc.kx * ig);
+  h.yh = y + uy * (mc.ky * ig);
+  h.zh = z + uz * (mc.ky * ig);
+  const float rh2 = h.yh * h.yh + h.zh * h.zh;
+  h.irh = __builtin_amdgcn_rsqf(rh2);
+  h.rh = rh2 * h.irh;
+  h.cyr = h.rh - mc.yoff;
+  h.fl2x = floorf(h.oxh);
+  h.fl2y = floorf(h.cyr);
+  return h;
+}
+
 __device__ __forceinline__ void mfactors(float x, cplx<float>& f2, cplx<float>& f3, cplx<float>& f4) {
   const float ax = fabsf(x);
   if (ax < 0.7f) {
@@ -228,20 +251,15 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     const float wgt = sp.w[ip];
 
     // half push to t + dt/2 (particles.F90:296-312), in cell units
-    float ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
-    const float oxh = ox + ux * (mc.kx * ig);
-    const float yh = y + uy * (mc.ky * ig);
-    const float zh = z + uz * (mc.ky * ig);
-    const float rh2 = yh * yh + zh * zh;
-    const float irh = __builtin_amdgcn_rsqf(rh2);
-    const float rh = rh2 * irh;
+    const HalfStep hs = half_step(mc, ox, y, z, ux, uy, uz);
+    const float oxh = hs.oxh, yh = hs.yh, zh = hs.zh, irh = hs.irh, rh = hs.rh;
     const cplx<float> emi = mk(yh * irh, -(zh * irh));  // exp(-i theta_05)
 
     // weights at t + dt/2 (gx.inc, hx_dcell.inc) relative to the cell
-    const float fl1x = floorf(oxh + 0.5f), fl2x = floorf(oxh);
+    const float fl1x = floorf(oxh + 0.5f), fl2x = hs.fl2x;
     const int cx1 = cell + (int)fl1x, cx2 = cell + (int)fl2x;
-    const float cyr = rh - mc.yoff;
-    const float fl1y = floorf(cyr + 0.5f), fl2y = floorf(cyr);
+    const float cyr = hs.cyr;
+    const float fl1y = floorf(cyr + 0.5f), fl2y = hs.fl2y;
     const int cy1 = (int)fl1y + 1, cy2 = (int)fl2y + 1;
     float gx[5], gy[5], hx[5], hy[5];
     gx[0] = gx[4] = gy[0] = gy[4] = hx[0] = hx[4] = hy[0] = hy[4] = 0.0f;
@@ -335,7 +353,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     ux = fmaf(mc.cmratio, ex_p, uxp);
     uy = fmaf(mc.cmratio, ey_p, uyp);
     uz = fmaf(mc.cmratio, ez_p, uzp);
-    ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
+    const float ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
     const float dxn = ux * (mc.kx * ig), dyn = uy * (mc.ky * ig), dzn = uz * (mc.ky * ig);
     float oxn = oxh + dxn;
     float yn = yh + dyn, zn = zh + dzn;
@@ -658,6 +676,29 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
 }
 
 // ------------------------------------------------------------------------
+// continuity check: the m = 0 charge q w hx hy / 4 of each particle at its
+// half-step position, with the push's own FP32 half step and staggered
+// weights (the charge whose motion the deposit's currents carry), summed
+// in FP64 into one plane
+__global__ __launch_bounds__(256) void k_charge_half_mixed(Grid g, MixConst mc, PartMixed sp, long n, int ntiles,
+                                                           double qfac, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (unsigned)sp.key[i] >= (unsigned)ntiles) return;
+  const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], sp.ux[i], sp.uy[i], sp.uz[i]);
+  const int cx2 = sp.cell[i] + (int)hs.fl2x, cy2 = (int)hs.fl2y + 1;
+  if (cx2 - 1 < 1 - g.ng || cx2 + 1 > g.nx + g.ng || cy2 - 1 < 1 - g.ng || cy2 + 1 > g.ny + g.ng) return;
+  float hx[3], hy[3];
+  w3f((hs.fl2x - hs.oxh) + 0.5f, hx[0], hx[1], hx[2]);
+  w3f((hs.fl2y - hs.cyr) + 0.5f, hy[0], hy[1], hy[2]);
+  const double qw = qfac * (double)sp.w[i];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      unsafeAtomicAdd(out + g.idx(cx2 + k - 1, cy2 + j - 1, 0), qw * ((double)hx[k] * (double)hy[j]));
+}
+
+// ------------------------------------------------------------------------
 // key of an uploaded particle (no push)
 __global__ void k_keys_mixed(Grid g, Tiles tl, MixConst mc, PartMixed sp, long n, int cell_key) {
   const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -740,6 +781,14 @@ MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
   mc.nyf = (float)(g.y_max / g.dy);
   mc.png_shift = (1 + 3) / 2;
   return mc;
+}
+
+void launch_charge_half_mixed(const Grid& g, const PartMixed& sp, long n, int ntiles, double charge, double* out,
+                              hipStream_t s) {
+  // only the half-step constants of mix_const are used
+  if (n > 0)
+    hipLaunchKernelGGL(k_charge_half_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g,
+                       mix_const(g, PushConst{}, charge), sp, n, ntiles, charge * 0.25, out);
 }
 
 int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& pc, double charge,

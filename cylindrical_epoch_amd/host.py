@@ -201,6 +201,18 @@ class Simulation:
                                    _ptr(use), _ptr(out), nb.value, C.byref(nb)))
         return out.reshape(res2[2], res2[1], res2[0]), rg2[: 2 * nd].reshape(nd, 2), res2[:nd].copy()
 
+    def continuity_begin(self):
+        """Store the m = 0 charge at x + v dt/2 before a push."""
+        check(lib().cylpic_continuity_begin(self.h))
+
+    def continuity_end(self):
+        """After the push (before current_finish): (max |residual|, max
+        |charge change|, max |charge|) of the m = 0 continuity equation over
+        the cells at least ng from every edge, in coulombs."""
+        r, dq, q = C.c_double(), C.c_double(), C.c_double()
+        check(lib().cylpic_continuity_end(self.h, C.byref(r), C.byref(dq), C.byref(q)))
+        return r.value, dq.value, q.value
+
     def set_smoothing(self, its, comp_its=0, strides=(1,)):
         """smooth_currents of the deck: smooth_its, smooth_comp_its,
         smooth_strides (current_smooth.F90:50-58, 145-196)."""

@@ -208,6 +208,20 @@ MODULE cylpic
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_move_window
 
+    FUNCTION cylpic_continuity_begin(h) BIND(C, NAME='cylpic_continuity_begin') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_continuity_begin
+
+    FUNCTION cylpic_continuity_end(h, residual, dq, q) &
+        BIND(C, NAME='cylpic_continuity_end') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(OUT) :: residual, dq, q
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_continuity_end
+
     FUNCTION cylpic_number_density_modes(h, species, out) &
         BIND(C, NAME='cylpic_number_density_modes') RESULT(ierr)
       IMPORT :: C_PTR, C_INT, C_DOUBLE_COMPLEX

@@ -210,6 +210,20 @@ enum {
   CYLPIC_DIR_EN = 6, CYLPIC_DIR_GAMMA_M1 = 7, CYLPIC_DIR_XY_ANGLE = 8, CYLPIC_DIR_YZ_ANGLE = 9,
   CYLPIC_DIR_ZX_ANGLE = 10, CYLPIC_DIR_MOD_P = 11
 };
+/* Continuity check of one push (no counterpart in the reference; it checks
+ * the identity its charge-conserving deposit keeps for m = 0,
+ * src/particles.F90:508-665).  cylpic_continuity_begin, called right before
+ * cylpic_push_particles, stores the m = 0 charge of every mobile species at
+ * x + v dt/2 on the staggered grid (quadratic-spline fractions; the mixed
+ * path uses its push's own FP32 half step and weights); then
+ * cylpic_continuity_end, right after the push and before
+ * cylpic_current_finish, returns max |Q(t+3dt/2) - Q(t+dt/2) + dt (net Jx, Jr
+ * current out of the cell)|, max |Q(t+3dt/2) - Q(t+dt/2)| and max
+ * |Q(t+3dt/2)| in coulombs over the cells at least ng from every edge.  For m > 0 the reference's scheme
+ * has no such identity (J_theta is deposited with v_theta(t+dt)). */
+int cylpic_continuity_begin(cylpic_ctx *h);
+int cylpic_continuity_end(cylpic_ctx *h, double *residual, double *dq, double *q);
+
 int cylpic_dist_fn(cylpic_ctx *h, int species, int ndims, const int *direction, double *ranges,
                    int *resolution, const double *restrictions, const int *use_restrictions,
                    double *out, int64_t capacity, int64_t *nbins);
