@@ -111,6 +111,8 @@ def make_sim(w, precision, particles, tune=None, extra_capacity=0):
                      gather_lds=tune.get("gather_lds") if mixed else None)
     if tune.get("sort_method"):
         sim.set_sort_method(tune["sort_method"])
+    if tune.get("sort_interval"):
+        sim.set_sort_interval(tune["sort_interval"])
     n = len(particles["x"])
     sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=n + extra_capacity)
     sim.upload_particles(sid, particles["x"], particles["y"], particles["z"], particles["px"],
@@ -263,6 +265,9 @@ def main():
     ap.add_argument("--gather-lds", type=int, default=None, help="mixed gather from LDS (1) or L1/L2 (0)")
     ap.add_argument("--sort-method", default=None, choices=["radix", "migrate"])
     ap.add_argument("--no-window", action="store_true", help="keep the window fixed (no injection)")
+    ap.add_argument("--sort-interval", type=int, default=4,
+                    help="sort every n pushes (library default 1; 4 measured fastest on this workload, with no "
+                         "slower push: the cold plasma moves well under a tile in 4 steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -281,7 +286,8 @@ def main():
     n = len(parts["x"])
     log(f"{n} particles in {time.perf_counter() - t0:.1f} s")
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
-    tune = {"acc_mode": a.acc_mode, "gather_lds": a.gather_lds, "sort_method": a.sort_method}
+    tune = {"acc_mode": a.acc_mode, "gather_lds": a.gather_lds, "sort_method": a.sort_method,
+            "sort_interval": a.sort_interval}
     for k in ("mixed", "fp64"):
         t = getattr(a, "tile_" + k)
         tune["tile_" + k] = tuple(int(v) for v in t.split("x")) if t else None
@@ -322,7 +328,8 @@ def main():
         "config": {"workload": a.workload, "nx": w["nx"], "ny": w["ny"], "n_mode": w["n_mode"],
                    "ppc": w["ppc"], "particles": n, "precision": main_prec, "parallelism": "single-gpu",
                    "laser": "x_min, a0=3, lambda 0.8 um, waist 15 um, 25 fs",
-                   "moving_window": not a.no_window, "window_v_x": "c" if not a.no_window else None},
+                   "moving_window": not a.no_window, "window_v_x": "c" if not a.no_window else None,
+                   "sort_interval": a.sort_interval},
         "field_cell_updates_per_s": round(cells / (r["ms"] / 1e3), 1),
         "per_precision": {k: {"ms_per_step": round(v["ms"], 4), "particle_steps_per_s": round(v["value"], 1),
                               "kernel_ms_per_step": {kk: round(kms / a.steps, 4) for kk, (kms, cnt) in

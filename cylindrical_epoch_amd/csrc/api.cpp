@@ -827,25 +827,26 @@ int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double*
   if ((ids != nullptr) != s.ids) return fail(CYLPIC_EINVAL, "ids must be given exactly when the upload gave them");
   HIPCHK(hipSetDevice(h->device));
   Scope scope(h, "inject");
-  if (s.n_extra > 0) {  // merge the previous batch first
-    int rc = sort_species(h, s);
-    if (rc) return rc;
-  }
+  // the extra region holds every batch injected since the last sort; this
+  // batch joins it and the region is stable-sorted again by current key
+  // (earlier batches in their current order first, removed particles last)
+  const long xbase = s.n - s.n_extra;
   const long base = s.n;
   if (base + n > s.cap) return fail(CYLPIC_EINVAL, "injection exceeds the species capacity");
   const double* src[7] = {x, y, z, px, py, pz, w};
   int rc = stage_in(h, s, base, n, src);
   if (rc) return rc;
   if (ids) HIPCHK(hipMemcpyAsync(s.id[0] + base, ids, n * sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
-  // tile keys of the batch, a stable sort of the batch by key in place, and
-  // its tile ranges: the push then treats it as each tile's extra range
+  // tile keys of the batch, a stable sort of the extra region by key in
+  // place, and its tile ranges: the push treats it as each tile's extra range
   if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s, base), n, 0, h->stream);
   else launch_keys_mixed(h->g, h->tl, viewm(s, base), n, 0, h->stream);
-  rc = ensure_sort_scratch(h, n);
+  const long nr = base + n - xbase;
+  rc = ensure_sort_scratch(h, nr);
   if (rc) return rc;
   uint32_t* perm = nullptr;
-  radix_sort(s.key[0] + base, n, h->tl.ntiles, 0, h->sc, s.key[1] + base, &perm, h->stream);
-  HIPCHK(hipMemcpyAsync(s.key[0] + base, s.key[1] + base, n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+  radix_sort(s.key[0] + xbase, nr, h->tl.ntiles, 0, h->sc, s.key[1] + xbase, &perm, h->stream);
+  HIPCHK(hipMemcpyAsync(s.key[0] + xbase, s.key[1] + xbase, nr * sizeof(uint32_t), hipMemcpyDeviceToDevice,
                         h->stream));
   const bool f64 = h->prec == CYLPIC_FP64;
   const int narr = f64 ? 7 : 8;
@@ -854,33 +855,34 @@ int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double*
     double* sa[7];
     double* sb[7];
     for (int k = 0; k < 7; ++k) {
-      sa[k] = s.a[k] + base;
-      sb[k] = s.b[k] + base;
+      sa[k] = s.a[k] + xbase;
+      sb[k] = s.b[k] + xbase;
     }
-    launch_gather_f64(perm, n, sa, sb, 7, nullptr, nullptr, 0, h->stream);
+    launch_gather_f64(perm, nr, sa, sb, 7, nullptr, nullptr, 0, h->stream);
   } else {
     uint32_t* sa[8];
     uint32_t* sb[8];
     for (int k = 0; k < 8; ++k) {
-      sa[k] = s.ma[k] + base;
-      sb[k] = s.mb[k] + base;
+      sa[k] = s.ma[k] + xbase;
+      sb[k] = s.mb[k] + xbase;
     }
-    launch_gather_32(perm, n, sa, sb, 8, nullptr, nullptr, 0, h->stream);
+    launch_gather_32(perm, nr, sa, sb, 8, nullptr, nullptr, 0, h->stream);
   }
   for (int k = 0; k < narr; ++k) {
-    void* dst = f64 ? (void*)(s.a[k] + base) : (void*)(s.ma[k] + base);
-    const void* from = f64 ? (const void*)(s.b[k] + base) : (const void*)(s.mb[k] + base);
-    HIPCHK(hipMemcpyAsync(dst, from, n * wb, hipMemcpyDeviceToDevice, h->stream));
+    void* dst = f64 ? (void*)(s.a[k] + xbase) : (void*)(s.ma[k] + xbase);
+    const void* from = f64 ? (const void*)(s.b[k] + xbase) : (const void*)(s.mb[k] + xbase);
+    HIPCHK(hipMemcpyAsync(dst, from, nr * wb, hipMemcpyDeviceToDevice, h->stream));
   }
-  if (ids) {
-    launch_gather_64(perm, n, (const uint64_t*)s.id[0] + base, (uint64_t*)s.id[1] + base, nullptr, nullptr, 0,
+  if (s.ids) {
+    launch_gather_64(perm, nr, (const uint64_t*)s.id[0] + xbase, (uint64_t*)s.id[1] + xbase, nullptr, nullptr, 0,
                      h->stream);
-    HIPCHK(hipMemcpyAsync(s.id[0] + base, s.id[1] + base, n * sizeof(int64_t), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(s.id[0] + xbase, s.id[1] + xbase, nr * sizeof(int64_t), hipMemcpyDeviceToDevice,
+                          h->stream));
   }
-  launch_tile_bounds(s.key[0] + base, n, h->tl.ntiles, s.extra_start, h->stream);
+  launch_tile_bounds(s.key[0] + xbase, nr, h->tl.ntiles, s.extra_start, h->stream);
   HIPCHK(hipGetLastError());
   s.n = base + n;
-  s.n_extra = n;
+  s.n_extra = nr;
   s.dirty = true;
   s.counted = false;
   return CYLPIC_OK;

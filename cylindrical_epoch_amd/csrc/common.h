@@ -49,6 +49,18 @@ struct Grid {
   }
 };
 
+// Workgroup -> tile: the dispatcher hands workgroup b to XCD b % 8, and each
+// XCD has its own L2.  Giving XCD x the contiguous tile range
+// [x q + min(x, r), ...) (q = n / 8, r = n % 8) in dispatch order keeps
+// neighbouring tiles, whose field and current halos overlap, on one L2.
+// A bijection of [0, n).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n / NUM_XCD, r = n % NUM_XCD;
+  const int x = b % NUM_XCD, k = b / NUM_XCD;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // Tile decomposition of the push/deposit: tiles of tile_x x tile_r cells,
 // numbered row-major in x; H halo cells around each tile in the LDS current box.
 // Tile column t covers cells [t tx - shift + 1, (t + 1) tx - shift]: a moving

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64(
     int* __restrict__ err, MigCount migc) {
   extern __shared__ double lds[];
   __shared__ uint32_t migs[MIG_DIRS];
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   LdsBox box;
   box.x0 = tl.x0(ttx) - DEPOSIT_HALO;

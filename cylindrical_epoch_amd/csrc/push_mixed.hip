@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   extern __shared__ double lds_raw[];
   __shared__ float wred[8];
   __shared__ uint32_t migs[MIG_DIRS];
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   Boxes<M> b;
   const int tx = TX > 0 ? TX : tl.tx, tr = TR > 0 ? TR : tl.tr;

@@ -213,6 +213,10 @@ class Simulation:
         check(lib().cylpic_continuity_end(self.h, C.byref(r), C.byref(dq), C.byref(q)))
         return r.value, dq.value, q.value
 
+    def set_sort_interval(self, n):
+        """Sort the particles every n pushes."""
+        check(lib().cylpic_set_sort_interval(self.h, int(n)))
+
     def set_smoothing(self, its, comp_its=0, strides=(1,)):
         """smooth_currents of the deck: smooth_its, smooth_comp_its,
         smooth_strides (current_smooth.F90:50-58, 145-196)."""

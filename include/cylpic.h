@@ -139,7 +139,10 @@ int cylpic_step(cylpic_ctx *h, const double *src_xmin, const double *src_xmax);
  * RNG), and calls:
  *   cylpic_inject_particles  - before the shift, with the new particles of the
  *                              ncells columns beyond x_max (SI, as in upload);
- *                              they join the species without a full re-sort;
+ *                              they join the species without a full re-sort:
+ *                              the batches since the last sort form one
+ *                              region after the sorted particles, stable-
+ *                              sorted by current tile key at each injection;
  *   cylpic_move_window       - shift_window for ncells cells: per cell the
  *                              grid moves by dx, E, B and J move one cell
  *                              towards x_min with the x_max incoming-cell fix

@@ -79,7 +79,7 @@ def test_window_mixed_energies(oracle):
     sim.close()
 
 
-def _inject_push(oracle, precision, method):
+def _inject_push(oracle, precision, method, nbatch=1):
     nx, ny, dx, dy = 32, 12, 1e-7, 1e-7
     tile = (8, 4)
     st, sim = make_pair(oracle, nx, ny, 1, dx, dy, bc_field=BF, bc_particle=BP, precision=precision, tile=tile,
@@ -87,14 +87,19 @@ def _inject_push(oracle, precision, method):
     sim.set_sort_method(method)
     add_plasma(st, sim, nx, ny, dx, dy, ppc=4, temp_ev=2.0e4, seed=9, capacity=4 * nx * ny * 4)
     before = sim.download_particles(0)["id"]
-    p = _column(st, nx, ny, dx, dy, 4, 11, 10 ** 6)
-    sim.inject_particles(0, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"], ids=p["id"])
     ntiles = (nx // tile[0]) * (ny // tile[1])
     cells = tile[0] * tile[1]
-    inj_keys = oracle.cell_keys(st, p["x"], p["y"], p["z"], *tile) // cells
-    batch = p["id"][oracle.stable_sort(inj_keys, ntiles)]
-    sim.move_window(1)
-    st.shift_window(1)  # the oracle's grid follows (its own particles are not used)
+    ids, keys = [], []
+    for b in range(nbatch):  # batches since the last sort: one region, stable-sorted by key at each injection
+        p = _column(st, nx, ny, dx, dy, 4, 11 + b, 10 ** 6 * (b + 1))
+        p["x"] = p["x"] + b * dx
+        sim.inject_particles(0, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"], ids=p["id"])
+        ids.append(p["id"])
+        keys.append(oracle.cell_keys(st, p["x"], p["y"], p["z"], *tile) // cells)
+    ids, keys = np.concatenate(ids), np.concatenate(keys)
+    batch = ids[oracle.stable_sort(keys.astype(np.int32), ntiles)]
+    sim.move_window(nbatch)
+    st.shift_window(nbatch)  # the oracle's grid follows (its own particles are not used)
     sim.push_particles()
     c0 = sim.sort_counts()
     got = sim.download_particles(0)
@@ -103,17 +108,18 @@ def _inject_push(oracle, precision, method):
     return st, got, np.concatenate([before, batch]), ntiles, cells, tile, c0, c1
 
 
-def test_injected_batch_sorts_as_stable_cpu_sort(oracle):
+@pytest.mark.parametrize("nbatch", [1, 2, 3])
+def test_injected_batch_sorts_as_stable_cpu_sort(oracle, nbatch):
     """FP64: injected particles join the tile ranges without a full re-sort;
     after the next push the order equals the stable CPU sort by tile key of
-    the previous order followed by the batch (itself in stable tile-key
+    the previous order followed by the batches (together in stable tile-key
     order), removed particles dropped (bit-exact)."""
-    log("window sort fp64")
-    st, got, order, ntiles, cells, tile, c0, c1 = _inject_push(oracle, "fp64", "migrate")
+    log("window sort fp64", nbatch)
+    st, got, order, ntiles, cells, tile, c0, c1 = _inject_push(oracle, "fp64", "migrate", nbatch)
     assert c1["migrate"] == c0["migrate"] + 1
     pos = {i: k for k, i in enumerate(got["id"])}
     # the tiles moved with the plasma: one window cell = tile offset 1
-    keys_alive = oracle.cell_keys(st, got["x"], got["y"], got["z"], *tile, shift=1) // cells
+    keys_alive = oracle.cell_keys(st, got["x"], got["y"], got["z"], *tile, shift=nbatch) // cells
     removed = 1 << 16  # beyond every tile key of the offset tiling
     keys = np.array([keys_alive[pos[i]] if i in pos else removed for i in order], np.int32)
     want = order[oracle.stable_sort(keys, removed + 1)][: len(got["id"])]
@@ -121,12 +127,13 @@ def test_injected_batch_sorts_as_stable_cpu_sort(oracle):
     assert np.array_equal(got["id"], want)
 
 
-def test_injected_batch_mixed_migrate_equals_radix(oracle):
-    """Mixed precision: the migration merge of an injected batch gives the same
-    order and state as the full radix sort of the same run."""
-    log("window sort mixed")
-    _, got, *_ , c0, c1 = _inject_push(oracle, "mixed", "migrate")
+@pytest.mark.parametrize("nbatch", [1, 2])
+def test_injected_batch_mixed_migrate_equals_radix(oracle, nbatch):
+    """Mixed precision: the migration merge of injected batches gives the
+    same order and state as the full radix sort of the same run."""
+    log("window sort mixed", nbatch)
+    _, got, *_ , c0, c1 = _inject_push(oracle, "mixed", "migrate", nbatch)
     assert c1["migrate"] == c0["migrate"] + 1
-    _, ref, *_ = _inject_push(oracle, "mixed", "radix")
+    _, ref, *_ = _inject_push(oracle, "mixed", "radix", nbatch)
     for k in ("id", "x", "y", "z", "px", "py", "pz", "w"):
         assert np.array_equal(got[k], ref[k]), k
