@@ -141,6 +141,7 @@ class State:
             self.f.bnd_xmin[k] = self.bnd["xmin"][k].ctypes.data
             self.f.bnd_xmax[k] = self.bnd["xmax"][k].ctypes.data
         self.species = []
+        self.smooth = None  # (its, comp_its, strides) of smooth_current, or None
 
     # Fortran-index view helpers
     def at(self, name, i, j, m):
@@ -228,7 +229,7 @@ class State:
         (no moving window, no optional physics)."""
         self.update_eb_fields_half()
         self.push_particles()
-        self.current_finish()
+        self.current_finish(smooth=self.smooth)
         self.g.time += self.g.dt
         self.update_eb_fields_final(src_xmin, src_xmax)
 
