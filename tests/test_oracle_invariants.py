@@ -338,6 +338,62 @@ def test_update_e_gauss_law(oracle, m):
         assert abs(after[k] - want) < floor, k
 
 
+def _div_e_axis(st, comp=("ex", "er")):
+    """m = 0 divergence over the axis disk r < dy/2 around each Ex(x, r=0)
+    point: d/dx of Ex on the axis plus the flux of Er through r = dy/2,
+    2 pi (dy/2) Er / (pi (dy/2)^2) = 4 Er / dy."""
+    g, ng = st.g, st.g.ng
+    ex, er = st.arr[comp[0]][0], st.arr[comp[1]][0]
+    J0 = ng - 1  # row iy = 0, r = 0
+    return np.array([(ex[J0, i + ng] - ex[J0, i + ng - 1]) / g.dx + 4.0 * er[J0 + 1, i + ng - 1] / g.dy
+                     for i in range(0, g.nx)])
+
+
+def test_axis_gauss_law_m0(oracle):
+    """The r = 0 row of update_e_field for m = 0 (fields.f90:116-134, Ex on
+    the axis from 4 c^2 / dr Btheta(1)) keeps Gauss's law over the axis
+    disk: its divergence changes by -dt/2 div J / eps0 (the curl-B terms
+    cancel to round-off), as in the interior."""
+    rng = np.random.default_rng(31)
+    st = _state(oracle, nx=12, ny=10, M=3)
+    for c in ("ex", "er", "et", "bx", "br", "bt"):
+        st.arr[c][:] = rng.normal(size=st.shape) + 1j * rng.normal(size=st.shape)
+    for c in ("jx", "jr", "jt"):
+        st.arr[c][:] = 1e6 * (rng.normal(size=st.shape) + 1j * rng.normal(size=st.shape))
+    before, div_j = _div_e_axis(st), _div_e_axis(st, ("jx", "jr"))
+    st.call("orc_update_e_field")
+    after = _div_e_axis(st)
+    dt = st.g.dt
+    bmax = max(np.max(np.abs(st.arr[c])) for c in ("bx", "br", "bt"))
+    floor = 64 * 2.2e-16 * C_LIGHT ** 2 * dt * bmax / (st.g.dx * st.g.dy) * 10
+    assert floor < 1e-5 * np.max(np.abs(0.5 * dt * div_j / EPS0))
+    assert np.max(np.abs(after - (before - 0.5 * dt * div_j / EPS0))) < floor
+
+
+@pytest.mark.parametrize("m", [0, 1, 2])
+def test_first_cell_div_b(oracle, m):
+    """div B over the first radial cell (0 < r < dy, the disk around the
+    axis, whose inner face has zero area) is preserved by update_b_field's
+    row-1 update (fields.f90:186-248), which reads the E values of the r = 0
+    row."""
+    rng = np.random.default_rng(40 + m)
+    st = _state(oracle, nx=12, ny=10, M=3)
+    for c in ("ex", "er", "et", "bx", "br", "bt"):
+        st.arr[c][:] = rng.normal(size=st.shape) + 1j * rng.normal(size=st.shape)
+    g, ng = st.g, st.g.ng
+
+    def div1():
+        bx, br, bt = st.arr["bx"][m], st.arr["br"][m], st.arr["bt"][m]
+        J = ng  # row iy = 1
+        return np.array([(bx[J, i + ng - 1] - bx[J, i + ng - 2]) / g.dx + 2.0 * br[J, i + ng - 1] / g.dy
+                         - 1j * m * bt[J, i + ng - 1] / (0.5 * g.dy) for i in range(1, g.nx + 1)])
+
+    before = div1()
+    st.call("orc_update_b_field")
+    after = div1()
+    assert np.max(np.abs(after - before)) < 1e-9 * np.max(np.abs(before))
+
+
 def test_vacuum_energy_bounded(oracle):
     """A vacuum m=1 pulse in a conducting box: the mode energy stays bounded
     (the leapfrog's discrete energy is conserved up to O(dt^2) oscillation)."""
