@@ -265,6 +265,8 @@ def main():
     ap.add_argument("--gather-lds", type=int, default=None, help="mixed gather from LDS (1) or L1/L2 (0)")
     ap.add_argument("--sort-method", default=None, choices=["radix", "migrate"])
     ap.add_argument("--no-window", action="store_true", help="keep the window fixed (no injection)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the library's autotuner pick the tile shape and gather source before timing")
     ap.add_argument("--sort-interval", type=int, default=4,
                     help="sort every n pushes (library default 1; 4 measured fastest on this workload, with no "
                          "slower push: the cold plasma moves well under a tile in 4 steps)")
@@ -303,6 +305,10 @@ def main():
         if not a.no_window:
             win = Window(w, dt, Window.cells_for(w, dt, a.steps + a.warmup))
         log(f"{prec}: uploaded, dt = {dt:.4e}")
+        tuned = None
+        if a.autotune:
+            tuned = sim.autotune(pushes=2)
+            log(f"{prec}: autotune chose tile {tuned[0]}x{tuned[1]}, gather_lds={tuned[2]} ({tuned[3]:.3f} ms/push)")
         ms, prof = run(sim, laser, dt, a.steps, a.warmup, win)
         sorts = sim.sort_counts()
         cont, axis = accuracy_step(sim, laser, (a.warmup + a.steps + 1) * dt, win)
@@ -310,7 +316,7 @@ def main():
             log(f"{prec}: window moved {win.used} cells, {win.injected} particles injected")
         log(f"{prec}: m = 0 continuity residual {cont}")
         sim.close()
-        results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof, sorts=sorts, cont=cont, axis=axis)
+        results[prec] = dict(ms=ms * 1e3, value=n / ms, prof=prof, sorts=sorts, cont=cont, axis=axis, tuned=tuned)
         log(f"{prec}: {ms * 1e3:.3f} ms/step, {n / ms:.4e} particle-steps/s")
         for k, (kms, cnt) in prof.items():
             if cnt:
@@ -334,7 +340,10 @@ def main():
         "per_precision": {k: {"ms_per_step": round(v["ms"], 4), "particle_steps_per_s": round(v["value"], 1),
                               "kernel_ms_per_step": {kk: round(kms / a.steps, 4) for kk, (kms, cnt) in
                                                      v["prof"].items() if cnt},
-                              "sorts": v["sorts"]}
+                              "sorts": v["sorts"],
+                              "autotuned": None if v["tuned"] is None else
+                              {"tile": f"{v['tuned'][0]}x{v['tuned'][1]}", "gather_lds": v["tuned"][2],
+                               "push_ms": round(v["tuned"][3], 4)}}
                           for k, v in results.items()},
         "roofline": roofline(w, n, r["prof"]["push_deposit"], main_prec),
     }

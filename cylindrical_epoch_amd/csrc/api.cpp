@@ -574,6 +574,60 @@ int alloc_species_arrays(cylpic_ctx* h, Species& s) {
   return CYLPIC_OK;
 }
 
+// Change the tile shape with species present: each species is first
+// settled under the old tiles (removed particles dropped, injected batches
+// merged), then its per-tile tables are reallocated and it is re-sorted by
+// tile-major cell key as on upload.
+int retile(cylpic_ctx* h, int tx, int tr) {
+  for (auto& s : h->sp)
+    if (s.cap > 0 && s.n > 0 && (s.dirty || s.n_extra > 0)) {
+      int rc = sort_species(h, s);
+      if (rc) return rc;
+      rc = refresh_count(h, s);
+      if (rc) return rc;
+    }
+  int rc = set_tiles(h, tx, tr);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->mig) {
+    HIPCHK(hipFree(h->mig));
+    h->mig = nullptr;
+  }
+  for (auto& s : h->sp) {
+    if (s.cap <= 0) continue;
+    int* bufs[3] = {s.tile_start, s.tile_start_alt, s.extra_start};
+    for (int* b : bufs)
+      if (b) HIPCHK(hipFree(b));
+    if (s.mig_counts) HIPCHK(hipFree(s.mig_counts));
+    if (s.mig_counts_x) HIPCHK(hipFree(s.mig_counts_x));
+    s.tile_start = s.tile_start_alt = s.extra_start = nullptr;
+    s.mig_counts = s.mig_counts_x = nullptr;
+    const int nt = h->tl.ntiles;
+    HIPCHK(hipMalloc((void**)&s.tile_start, (nt + 2) * sizeof(int)));
+    HIPCHK(hipMemsetAsync(s.tile_start, 0, (nt + 2) * sizeof(int), h->stream));
+    HIPCHK(hipMalloc((void**)&s.tile_start_alt, (nt + 2) * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&s.mig_counts, (size_t)nt * MIG_DIRS * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void**)&s.mig_counts_x, (size_t)nt * MIG_DIRS * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void**)&s.extra_start, (nt + 2) * sizeof(int)));
+    s.d_count = s.tile_start + nt;
+    s.n_extra = 0;
+    s.ranged = false;
+    s.counted = false;
+    s.dirty = false;
+    if (s.n <= 0) {
+      launch_tile_bounds(nullptr, 0, nt, s.tile_start, h->stream);
+      s.ranged = true;
+      continue;
+    }
+    if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), s.n, 1, h->stream);
+    else launch_keys_mixed(h->g, h->tl, viewm(s), s.n, 1, h->stream);
+    rc = sort_species(h, s, 1);
+    if (rc) return rc;
+  }
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1428,6 +1482,131 @@ int cylpic_continuity_end(cylpic_ctx* h, double* residual, double* dq, double* q
   std::memcpy(residual, &bits[0], sizeof(double));
   std::memcpy(dq, &bits[1], sizeof(double));
   std::memcpy(q, &bits[2], sizeof(double));
+  return check_device_errors(h);
+}
+
+int cylpic_autotune(cylpic_ctx* h, int pushes, int* tile_x, int* tile_r, int* gather_lds, double* ms_best) {
+  CHECK_H(h);
+  if (pushes < 1) return fail(CYLPIC_EINVAL, "pushes must be >= 1");
+  HIPCHK(hipSetDevice(h->device));
+  // settle every species so that the snapshot is the sorted particle set
+  for (auto& s : h->sp) {
+    if (s.cap <= 0 || !s.tile_start) continue;
+    int rc = settle_species(h, s);
+    if (rc) return rc;
+  }
+  // snapshot: each species' particles (and ids), the currents and their
+  // *_old copies, and the step bookkeeping the pushes advance
+  const bool f64 = h->prec == CYLPIC_FP64;
+  const int narr = f64 ? 7 : 8;
+  const size_t wb = f64 ? sizeof(double) : sizeof(uint32_t);
+  size_t bytes = 6 * nfield(h) * cplx_bytes(h);
+  for (auto& s : h->sp) bytes += (size_t)s.n * (narr * wb + (s.ids ? sizeof(int64_t) : 0));
+  char* snap = nullptr;
+  HIPCHK(hipMalloc((void**)&snap, bytes));
+  std::vector<long> n0;
+  auto save = [&](bool restore) -> int {
+    char* p = snap;
+    auto mv = [&](void* live, size_t nb) -> int {
+      if (nb == 0) return CYLPIC_OK;
+      HIPCHK(hipMemcpyAsync(restore ? live : (void*)p, restore ? (const void*)p : (const void*)live, nb,
+                            hipMemcpyDeviceToDevice, h->stream));
+      p += nb;
+      return CYLPIC_OK;
+    };
+    for (int k = 0; k < 6; ++k) {
+      int rc = mv(field_ptr(h, k < 3 ? 6 + k : 12 + (k - 3)), nfield(h) * cplx_bytes(h));
+      if (rc) return rc;
+    }
+    for (size_t i = 0; i < h->sp.size(); ++i) {
+      Species& s = h->sp[i];
+      const long n = restore ? n0[i] : s.n;
+      for (int k = 0; k < narr; ++k) {
+        int rc = mv(f64 ? (void*)s.a[k] : (void*)s.ma[k], n * wb);
+        if (rc) return rc;
+      }
+      if (s.ids) {
+        int rc = mv(s.id[0], n * sizeof(int64_t));
+        if (rc) return rc;
+      }
+      if (restore) {
+        s.n = n;
+        s.n_extra = 0;
+        s.dirty = false;
+      }
+    }
+    return CYLPIC_OK;
+  };
+  for (auto& s : h->sp) n0.push_back(s.n);
+  const long push_count0 = h->push_count, radix0 = h->sort_radix, migrate0 = h->sort_migrate;
+  const int interval0 = h->sort_interval;
+  h->sort_interval = 1 << 30;  // time the pushes alone
+  const int tx0 = h->tl.tx, tr0 = h->tl.tr, g0 = h->gather_lds, shift0 = h->tl.shift;
+  int rc = save(false);
+  if (rc) {
+    (void)hipFree(snap);
+    return rc;
+  }
+  struct Cand {
+    int tx, tr, g;
+  };
+  std::vector<Cand> cands;
+  if (f64) cands = {{16, 8, 0}, {16, 4, 0}, {8, 8, 0}, {8, 4, 0}, {32, 4, 0}, {16, 8, 1}};
+  else cands = {{16, 4, 1}, {8, 8, 1}, {8, 4, 1}, {16, 8, 1}, {32, 4, 1}, {16, 4, 0}};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = -1.0;
+  Cand pick{tx0, tr0, g0};
+  for (const Cand& c : cands) {
+    if (c.tx > h->g.nx || c.tr > h->g.ny) continue;
+    rc = save(true);
+    if (rc) break;
+    h->gather_lds = c.g;
+    if (retile(h, c.tx, c.tr) != CYLPIC_OK) continue;  // too much LDS for this shape
+    rc = cylpic_push_particles(h);  // warm-up
+    if (rc) break;
+    HIPCHK(hipEventRecord(e0, h->stream));
+    for (int i = 0; i < pushes && !rc; ++i) rc = cylpic_push_particles(h);
+    if (rc) break;
+    HIPCHK(hipEventRecord(e1, h->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (best < 0 || ms < best) {
+      best = ms;
+      pick = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // back to the snapshot, under the chosen shape
+  int rc2 = save(true);
+  if (!rc2) {
+    h->gather_lds = pick.g;
+    rc2 = retile(h, pick.tx, pick.tr);
+    if (rc2) {  // cannot happen for a shape that ran; fall back to the old one
+      h->gather_lds = g0;
+      rc2 = retile(h, tx0, tr0);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  (void)hipFree(snap);
+  h->push_count = push_count0;
+  h->sort_interval = interval0;
+  h->sort_radix = radix0;
+  h->sort_migrate = migrate0;
+  (void)shift0;  // retile starts the tile offset at 0: keys are recomputed from the positions
+  if (rc) return rc;
+  if (rc2) return rc2;
+  for (auto& s : h->sp) {
+    rc = refresh_count(h, s);
+    if (rc) return rc;
+  }
+  if (tile_x) *tile_x = pick.tx;
+  if (tile_r) *tile_r = pick.tr;
+  if (gather_lds) *gather_lds = pick.g;
+  if (ms_best) *ms_best = best / pushes;
   return check_device_errors(h);
 }
 

@@ -213,6 +213,14 @@ class Simulation:
         check(lib().cylpic_continuity_end(self.h, C.byref(r), C.byref(dq), C.byref(q)))
         return r.value, dq.value, q.value
 
+    def autotune(self, pushes=2):
+        """Pick the fastest tile shape and gather source for the current
+        particles (timed pushes, state restored); returns (tile_x, tile_r,
+        gather_lds, ms per push)."""
+        tx, tr, gl, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().cylpic_autotune(self.h, int(pushes), C.byref(tx), C.byref(tr), C.byref(gl), C.byref(ms)))
+        return tx.value, tr.value, gl.value, ms.value
+
     def set_sort_interval(self, n):
         """Sort the particles every n pushes."""
         check(lib().cylpic_set_sort_interval(self.h, int(n)))

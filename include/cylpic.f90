@@ -208,6 +208,16 @@ MODULE cylpic
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_move_window
 
+    FUNCTION cylpic_autotune(h, pushes, tile_x, tile_r, gather_lds, ms_per_push) &
+        BIND(C, NAME='cylpic_autotune') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT), VALUE :: pushes
+      INTEGER(C_INT), INTENT(OUT) :: tile_x, tile_r, gather_lds
+      REAL(C_DOUBLE), INTENT(OUT) :: ms_per_push
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_autotune
+
     FUNCTION cylpic_continuity_begin(h) BIND(C, NAME='cylpic_continuity_begin') RESULT(ierr)
       IMPORT :: C_PTR, C_INT
       TYPE(C_PTR), VALUE :: h

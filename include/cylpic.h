@@ -184,6 +184,17 @@ int cylpic_set_tile(cylpic_ctx *h, int tile_x, int tile_r);
  * no counterpart in the reference (its deposit is serial FP64). */
 int cylpic_tune(cylpic_ctx *h, int acc_mode, int gather_lds);
 
+/* Autotuner (no counterpart in the reference): times `pushes` pushes (plus
+ * one warm-up) of the current particles for each candidate tile shape and
+ * gather source of this precision that fits in LDS, restores the particles,
+ * the currents and their *_old copies after each, and keeps the fastest
+ * candidate (the particles re-sorted under it).  Sorting is suspended
+ * while it times, so every candidate is timed on the same pushes.  Returns
+ * the choice and its time per push.  Call it with profiling off; the step
+ * counters are restored. */
+int cylpic_autotune(cylpic_ctx *h, int pushes, int *tile_x, int *tile_r, int *gather_lds,
+                    double *ms_per_push);
+
 /* Per-mode field energies (FP64 reduction; the reference's Cartesian
  * energy sums are never updated in cylindrical mode, src/housekeeping/
  * setup.F90:526-571), w_e[n_mode], w_b[n_mode] in joules. */

@@ -212,3 +212,32 @@ def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, te
     log(f"  {len(before)} -> {len(want)} particles, sort counts {c1}")
     assert np.array_equal(got["id"], want)
     sim.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_restores_state(oracle, precision):
+    """The autotuner times pushes under each candidate tile shape and puts
+    the particles and currents back: the steps after it equal the oracle's
+    as without it (to the multi-step tolerances), and the choice is one of
+    the candidates."""
+    log("autotune", precision)
+    nx, ny, dx, dy = 48, 24, 1e-7, 1e-7
+    st, sim = make_pair(oracle, nx, ny, 2, dx, dy, precision=precision)
+    add_plasma(st, sim, nx, ny, dx, dy, ppc=6, temp_ev=5000.0, seed=12, drift=(0.2, 0.0, 0.1))
+    st.step()
+    sim.step()
+    tx, tr, gl, ms = sim.autotune(pushes=2)
+    log(f"  chose {tx}x{tr} gather_lds={gl}, {ms:.4f} ms/push")
+    assert (tx, tr) in {(16, 8), (16, 4), (8, 8), (8, 4), (32, 4)} and gl in (0, 1) and ms > 0
+    for _ in range(2):
+        st.step()
+        sim.step()
+    got = by_id(sim.download_particles(0))
+    want = by_id({k: st.species[0][k][: st.species[0]["n"]] for k in ("x", "y", "z", "px", "py", "pz", "w", "id")})
+    assert np.array_equal(got["id"], want["id"])
+    tol = 1e-9 if precision == "fp64" else 1e-4  # the multi-step tolerances
+    for k in ("x", "y", "z", "px", "py", "pz"):
+        assert max_rel(got[k], want[k]) < tol, k
+    jm = sim.download_field("jx")
+    assert max_rel(jm[0], st.arr["jx"][0]) < (1e-10 if precision == "fp64" else 1e-3)
+    sim.close()
