@@ -144,6 +144,19 @@ struct Boxes {
 // inv_area_xt(iy) / inv_area_xt(iy-1), particles.F90:209-216, 636-647), so it
 // is applied once per box cell when the box is flushed.
 // LDSF: gather the fields from the LDS-staged tile region (1) or from L1/L2 (0).
+// Row pitch of the current box: the box's width, except that the int64 box
+// (ACC 3) of 16-wide tiles uses 24 words.  A wavefront's lanes sit on the
+// tile's 4 x 16 cells, so its 64-bit adds at one stencil offset touch four
+// rows of 16 consecutive words; at a pitch of 24 words (48 banks) those rows
+// start at banks 0, 48, 32 and 16, and every bank serves exactly two lanes.
+__host__ __device__ inline int box_pitch(int acc, int tx) {
+  const int w = tx + BOX_LO_X + BOX_HI;
+  return (acc == 3 && tx == 16) ? 24 : w;
+}
+// int64 box planes per component (ACC 3): the m = 0 current has no
+// imaginary part, so mode 0 keeps only its real plane
+__host__ __device__ inline int acc3_planes(int M) { return 2 * M - 1; }
+
 template <int ACC>
 struct AccTraits {
   static constexpr bool fixed = ACC >= 3;
@@ -182,20 +195,26 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   b.FR = tr + FIELD_LO_R + FIELD_HI;
   b.jx0 = ttx * tx + 1 - tl.shift - BOX_LO_X;
   b.jr0 = tty * tr + 1 - BOX_LO_R;
-  b.LX = tx + BOX_LO_X + BOX_HI;
+  b.LX = box_pitch(ACC, tx);
   b.LR = tr + BOX_LO_R + BOX_HI;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
   const int nf = LDSF ? 6 * M * b.FR * b.FX : 0;  // complex cells of the field region
   // current box: planes [3][M] of LR x LX cells; each plane holds the real
   // words then the imaginary words (FP64 for ACC 2, int64 for ACC 3, FP32 for
   // ACC 0, 1, where ACC 1 adds a second FP32 box for the low words), or one
-  // packed int64 per cell (ACC 4).  Consecutive cells are consecutive words,
-  // so lanes at consecutive cells hit distinct LDS banks.
+  // packed int64 per cell (ACC 4).  ACC 3 stores mode 0's real words only.
+  // Consecutive cells are consecutive words, so lanes at consecutive cells
+  // hit distinct LDS banks.
+  const int ncell = b.LR * b.LX;
+  const int jmode = ACC == 4 ? ncell : 2 * ncell;                       // words per mode plane (m > 0)
+  const int jcomp = ACC == 3 ? acc3_planes(M) * ncell : M * jmode;      // words per component
+  // first word of mode m's real plane within a component
+  auto mode_off = [&](int m) { return ACC == 3 ? (m == 0 ? 0 : (2 * m - 1) * ncell) : m * jmode; };
   double* jl = lds_raw;
   float* jf = (float*)lds_raw;
   float* jlo = jf + 2 * nj;
   unsigned long long* jq = (unsigned long long*)lds_raw;
-  const int jwords = AccTraits<ACC>::words * nj;  // 32-bit words
+  const int jwords = ACC == 3 ? 2 * 3 * jcomp : AccTraits<ACC>::words * nj;  // 32-bit words
   cplx<float>* fl = (cplx<float>*)(((float*)lds_raw) + ((jwords + 1) & ~1));  // [6][M][FR][FX]
 
   for (int e = threadIdx.x; e < jwords; e += blockDim.x) jf[e] = 0.0f;
@@ -450,10 +469,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     }
     // is the whole footprint inside the LDS current box?
     const bool jin = (cx2 - 2) >= b.jx0 && (cx2 + 2) < b.jx0 + b.LX && (cy2 - 2) >= b.jr0 && (cy2 + 2) < b.jr0 + b.LR;
-    const int ncell = b.LR * b.LX;
     const int jb = (cy2 - b.jr0) * b.LX + (cx2 - b.jx0);      // box cell of (cx2, cy2), comp 0, mode 0
-    const int jmode = ACC == 4 ? ncell : 2 * ncell;            // words per mode plane
-    const int jcomp = M * jmode;                               // words per component
     // real (part 0) or imaginary (part 1) word of the cell at word index c (ACC 0..3)
     auto lds_add = [&](int c, int part, float v) {
       const int off = c + part * ncell;
@@ -509,7 +525,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
 #pragma unroll
           for (int m = 0; m < M; ++m) {
             const cplx<float> w_xt = mk(m2[m].re * gxv + m3[m].re * hxv, m2[m].im * gxv + m3[m].im * hxv);
-            const int cm = c + m * jmode;
+            const int cm = c + mode_off(m);
             if (ix != xmax) lds_add_c(cm + 1, PX * wrt[m].re, PX * wrt[m].im, m != 0);      // (cx+1, cy)
             if (iy != ymax) lds_add_c(cm + jcomp + b.LX, -R * w_xt.re, -R * w_xt.im, m != 0);  // (cx, cy+1)
             lds_add_c(cm + 2 * jcomp, m2[m].re * A + m3[m].re * B + m4[m].re * C,            // (cx, cy)
@@ -624,7 +640,6 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
     sp.key[ip] = key;
   }
   __syncthreads();
-  const int ncell = b.LR * b.LX;
   for (int e = threadIdx.x; e < nj; e += blockDim.x) {
     // complex cell e = plane * ncell + cell; real and imaginary words
     const int wr = ACC == 4 ? e : e + (e / ncell) * ncell, wi = wr + ncell;
@@ -635,8 +650,11 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
       re = (double)((long long)(q - (unsigned long long)(long long)lo) >> 32);
       imv = (double)lo;
     } else if (ACC == 3) {
-      re = (double)(long long)jq[wr];
-      imv = (double)(long long)jq[wi];
+      const int plane = e / ncell, cell = e - plane * ncell;
+      const int comp3 = plane / M, mm = plane - comp3 * M;
+      const int w3 = comp3 * jcomp + mode_off(mm) + cell;
+      re = (double)(long long)jq[w3];
+      imv = mm == 0 ? 0.0 : (double)(long long)jq[w3 + ncell];
     } else if (ACC == 2) {
       re = jl[wr];
       imv = jl[wi];
@@ -761,10 +779,11 @@ __global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, doubl
 
 // ------------------------------------------------------------------------
 size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl, int acc, int ldsf) {
-  const size_t nj = 3 * (size_t)g.M * (tl.tr + BOX_LO_R + BOX_HI) * (tl.tx + BOX_LO_X + BOX_HI);
+  const size_t ncell = (size_t)(tl.tr + BOX_LO_R + BOX_HI) * box_pitch(acc, tl.tx);
+  const size_t nj = 3 * (size_t)g.M * ncell;
   const size_t nf =
       ldsf ? 6 * (size_t)g.M * (tl.tr + FIELD_LO_R + FIELD_HI) * (tl.tx + FIELD_LO_X + FIELD_HI) : 0;
-  const size_t jwords = (acc == 0 || acc == 4) ? 2 * nj : 4 * nj;
+  const size_t jwords = acc == 3 ? 2 * 3 * acc3_planes(g.M) * ncell : ((acc == 0 || acc == 4) ? 2 * nj : 4 * nj);
   return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(cplx<float>);
 }
 
