@@ -786,7 +786,9 @@ int cylpic_tune(cylpic_ctx* h, int acc_mode, int gather_lds) {
   if (acc_mode < 0 || acc_mode > 4) return fail(CYLPIC_EINVAL, "acc_mode must be 0..4");
   const int old_acc = h->acc_mode, old_g = h->gather_lds;
   h->acc_mode = acc_mode;
-  h->gather_lds = gather_lds ? 1 : 0;
+  if (gather_lds < 0 || gather_lds > 2 || (gather_lds == 2 && (h->prec != CYLPIC_MIXED || acc_mode != 3)))
+    return fail(CYLPIC_EINVAL, "gather_lds must be 0 or 1 (2: E from LDS, mixed path with acc_mode 3 only)");
+  h->gather_lds = gather_lds;
   int rc = set_tiles(h, h->tl.tx, h->tl.tr);
   if (rc) {
     h->acc_mode = old_acc;
@@ -1552,7 +1554,7 @@ int cylpic_autotune(cylpic_ctx* h, int pushes, int* tile_x, int* tile_r, int* ga
   };
   std::vector<Cand> cands;
   if (f64) cands = {{16, 8, 0}, {16, 4, 0}, {8, 8, 0}, {8, 4, 0}, {32, 4, 0}, {16, 8, 1}};
-  else cands = {{16, 4, 1}, {8, 8, 1}, {8, 4, 1}, {16, 8, 1}, {32, 4, 1}, {16, 4, 0}};
+  else cands = {{16, 4, 1}, {8, 8, 1}, {8, 4, 1}, {16, 8, 1}, {32, 4, 1}, {16, 4, 0}, {16, 4, 2}, {32, 4, 2}};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1560,6 +1562,7 @@ int cylpic_autotune(cylpic_ctx* h, int pushes, int* tile_x, int* tile_r, int* ga
   Cand pick{tx0, tr0, g0};
   for (const Cand& c : cands) {
     if (c.tx > h->g.nx || c.tr > h->g.ny) continue;
+    if (c.g == 2 && h->acc_mode != 3) continue;  // the split gather exists for the int64 box only
     rc = save(true);
     if (rc) break;
     h->gather_lds = c.g;

@@ -143,7 +143,8 @@ struct Boxes {
 // inv_area_xt(iy) times a running sum because ratio_area_xt(iy) =
 // inv_area_xt(iy) / inv_area_xt(iy-1), particles.F90:209-216, 636-647), so it
 // is applied once per box cell when the box is flushed.
-// LDSF: gather the fields from the LDS-staged tile region (1) or from L1/L2 (0).
+// LDSF: gather the fields from the LDS-staged tile region (1), from L1/L2
+// (0), or E from LDS and B from L1/L2 (2: half the LDS reads and region).
 // Row pitch of the current box: the box's width, except that the int64 box
 // (ACC 3) of 16-wide tiles uses 24 words.  A wavefront's lanes sit on the
 // tile's 4 x 16 cells, so its 64-bit adds at one stencil offset touch four
@@ -198,7 +199,8 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   b.LX = box_pitch(ACC, tx);
   b.LR = tr + BOX_LO_R + BOX_HI;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
-  const int nf = LDSF ? 6 * M * b.FR * b.FX : 0;  // complex cells of the field region
+  constexpr int NFC = LDSF == 2 ? 3 : 6;              // components staged in LDS
+  const int nf = LDSF ? NFC * M * b.FR * b.FX : 0;  // complex cells of the field region
   // current box: planes [3][M] of LR x LX cells; each plane holds the real
   // words then the imaginary words (FP64 for ACC 2, int64 for ACC 3, FP32 for
   // ACC 0, 1, where ACC 1 adds a second FP32 box for the low words), or one
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           cplx<float> S = mk(0.0f, 0.0f);
-          if (in_lds) {
+          if (in_lds && c < NFC) {
             const cplx<float>* base =
                 fl + ((long)(c * M + m) * fplane + (long)(ccy[c] - 1 - b.fr0) * b.FX + (ccx[c] - 1 - b.fx0));
 #pragma unroll
@@ -781,8 +783,9 @@ __global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, doubl
 size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl, int acc, int ldsf) {
   const size_t ncell = (size_t)(tl.tr + BOX_LO_R + BOX_HI) * box_pitch(acc, tl.tx);
   const size_t nj = 3 * (size_t)g.M * ncell;
-  const size_t nf =
-      ldsf ? 6 * (size_t)g.M * (tl.tr + FIELD_LO_R + FIELD_HI) * (tl.tx + FIELD_LO_X + FIELD_HI) : 0;
+  const size_t nf = ldsf ? (ldsf == 2 ? 3 : 6) * (size_t)g.M * (tl.tr + FIELD_LO_R + FIELD_HI) *
+                               (tl.tx + FIELD_LO_X + FIELD_HI)
+                         : 0;
   const size_t jwords = acc == 3 ? 2 * 3 * acc3_planes(g.M) * ncell : ((acc == 0 || acc == 4) ? 2 * nj : 4 * nj);
   return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(cplx<float>);
 }
@@ -814,7 +817,7 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
                               const PartMixed& sp, const int* tile_start, ExtraRange xr, FieldSet<float>& F,
                               const float* geo, int ntab, int acc, int ldsf, int* err, MigCount migc,
                               hipStream_t s) {
-  const size_t lds = push_lds_bytes_mixed(g, tl, acc, ldsf);
+  size_t lds = push_lds_bytes_mixed(g, tl, acc, ldsf);
   MixConst mc = mix_const(g, pc, charge);
   GeoF gf{geo, geo + ntab, geo + 2 * ntab, geo + 3 * ntab};
 #define LAUNCH_T(MM, AA, LL, XX, RR)                                                                    \
@@ -850,10 +853,14 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
   if (tl.tx == 8 && tl.tr == 8) { BY_MODE_T(AA, LL, 8, 8); return 0; }       \
   if (tl.tx == 8 && tl.tr == 4) { BY_MODE_T(AA, LL, 8, 4); return 0; }
   // compile-time tile shapes for the fixed-point accumulators
-  if (acc == 3 && !ldsf) { BY_SHAPE_T(3, 0) }
-  if (acc == 3 && ldsf) { BY_SHAPE_T(3, 1) }
+  if (acc == 3 && ldsf == 0) { BY_SHAPE_T(3, 0) }
+  if (acc == 3 && ldsf == 1) { BY_SHAPE_T(3, 1) }
+  if (acc == 3 && ldsf == 2) { BY_SHAPE_T(3, 2) }
   if (acc == 4 && !ldsf) { BY_SHAPE_T(4, 0) }
   if (acc == 4 && ldsf) { BY_SHAPE_T(4, 1) }
+  // the split gather (2) exists for the compile-time shapes only: stage
+  // everything elsewhere
+  if (ldsf == 2) lds = push_lds_bytes_mixed(g, tl, acc, 1);
   const int sel = acc * 2 + (ldsf ? 1 : 0);
   switch (sel) {
     case 0: BY_MODE(0, 0); break;

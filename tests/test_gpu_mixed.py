@@ -30,11 +30,12 @@ def test_mixed_fields_half(oracle, M):
 
 @pytest.mark.parametrize("M,acc,gather_lds", [(1, 3, 1), (2, 3, 1), (3, 3, 1), (3, 3, 0), (4, 3, 0),
                                                (3, 0, 1), (3, 1, 1), (3, 2, 0), (1, 4, 0), (3, 4, 1),
-                                               (2, 0, 0)])
+                                               (2, 0, 0), (3, 3, 2), (1, 3, 2)])
 def test_mixed_push(oracle, M, acc, gather_lds):
     """One push of a warm plasma in given fields, for each current
     accumulator (FP32, FP32 + 2Sum, FP64, int64 fixed point, packed 2 x 32-bit
-    fixed point) and gather source (LDS, L1/L2).
+    fixed point) and gather source (LDS, L1/L2, or E from LDS and B from
+    L1/L2).
     Positions and momenta carry FP32 rounding (~1e-7 relative per
     operation); the current of a particle is a difference of O(1) shape
     weights, so FP32 positions put ~1e-5 relative error on slow particles'
