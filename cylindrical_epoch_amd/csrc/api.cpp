@@ -78,7 +78,9 @@ struct cylpic_ctx {
   float* geof = nullptr;  // FP32 copy for the mixed path
   int ntab = 0;
   double* src_dev[2] = {};
-  double* src_host[2] = {};
+  double* src_host[2] = {};       // two pinned slots per boundary, used in turn
+  hipEvent_t src_ev[2][2] = {};   // [boundary][slot]: that slot's copy to the device is done
+  int src_slot = 0;
   int* d_err = nullptr;
   double* d_energy = nullptr;
   std::vector<Species> sp;
@@ -522,6 +524,8 @@ void free_all(cylpic_ctx* h) {
   for (int k = 0; k < 2; ++k) {
     if (h->src_dev[k]) (void)hipFree(h->src_dev[k]);
     if (h->src_host[k]) (void)hipHostFree(h->src_host[k]);
+    for (int slot = 0; slot < 2; ++slot)
+      if (h->src_ev[k][slot]) (void)hipEventDestroy(h->src_ev[k][slot]);
   }
   if (h->d_err) (void)hipFree(h->d_err);
   if (h->mig) (void)hipFree(h->mig);
@@ -740,8 +744,11 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
   for (int k = 0; k < 2; ++k) {
     if (hipMalloc((void**)&h->src_dev[k], 2 * (ny + 1) * sizeof(double)) != hipSuccess)
       return bail(fail(CYLPIC_ENOMEM, "source allocation failed"));
-    if (hipHostMalloc((void**)&h->src_host[k], 2 * (ny + 1) * sizeof(double), 0) != hipSuccess)
+    if (hipHostMalloc((void**)&h->src_host[k], 2 * 2 * (ny + 1) * sizeof(double), 0) != hipSuccess)
       return bail(fail(CYLPIC_ENOMEM, "pinned source allocation failed"));
+    for (int slot = 0; slot < 2; ++slot)
+      if (hipEventCreateWithFlags(&h->src_ev[k][slot], hipEventDisableTiming) != hipSuccess)
+        return bail(fail(CYLPIC_EHIP, "event creation failed"));
   }
   if (hipMalloc((void**)&h->d_err, sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&h->d_energy, 2 * n_mode * sizeof(double)) != hipSuccess)
@@ -1232,13 +1239,18 @@ int cylpic_update_eb_fields_final(cylpic_ctx* h, const double* src_xmin, const d
   const size_t sb = 2 * (size_t)(h->g.ny + 1) * sizeof(double);
   const double* dsrc[2] = {nullptr, nullptr};
   const double* hsrc[2] = {src_xmin, src_xmax};
+  // stage through pinned memory in two slots used in turn: the slot written
+  // now was last read by the copy two calls ago, which its event covers, so
+  // the host does not wait for the stream
+  const int slot = h->src_slot;
+  h->src_slot ^= 1;
   for (int k = 0; k < 2; ++k) {
     if (!hsrc[k]) continue;
-    // stage through pinned memory; the previous step's copy has completed
-    // once the stream reaches this point
-    HIPCHK(hipStreamSynchronize(h->stream));
-    std::memcpy(h->src_host[k], hsrc[k], sb);
-    HIPCHK(hipMemcpyAsync(h->src_dev[k], h->src_host[k], sb, hipMemcpyHostToDevice, h->stream));
+    double* pin = h->src_host[k] + slot * 2 * (size_t)(h->g.ny + 1);
+    HIPCHK(hipEventSynchronize(h->src_ev[k][slot]));
+    std::memcpy(pin, hsrc[k], sb);
+    HIPCHK(hipMemcpyAsync(h->src_dev[k], pin, sb, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipEventRecord(h->src_ev[k][slot], h->stream));
     dsrc[k] = h->src_dev[k];
   }
   {
