@@ -1324,8 +1324,10 @@ int cylpic_number_density_modes(cylpic_ctx* h, int species, double* out) {
     Species& s = h->sp[is];
     int rc = settle_species(h, s);
     if (rc) return rc;
-    if (h->prec == CYLPIC_FP64) launch_number_density_f64(h->g, view64(s), s.n, h->diag64, h->stream);
-    else launch_number_density_mixed(h->g, viewm(s), s.n, h->diag64, h->stream);
+    const int* ts = s.ranged ? s.tile_start : nullptr;
+    if (h->prec == CYLPIC_FP64)
+      launch_number_density_f64(h->g, h->tl, view64(s), ts, extra_of(s), s.n, h->diag64, h->stream);
+    else launch_number_density_mixed(h->g, h->tl, viewm(s), ts, extra_of(s), s.n, h->diag64, h->stream);
   }
   launch_density_bcs(h->g, h->diag64, h->stream);
   HIPCHK(hipGetLastError());
@@ -1456,8 +1458,10 @@ static int charge_half(cylpic_ctx* h, double* q) {
   HIPCHK(hipMemsetAsync(q, 0, h->g.plane * sizeof(double), h->stream));
   for (auto& s : h->sp) {
     if (s.immobile || s.n <= 0) continue;
-    if (h->prec == CYLPIC_FP64) launch_charge_half_f64(h->g, view64(s), s.n, h->tl.ntiles, s.m, s.q, q, h->stream);
-    else launch_charge_half_mixed(h->g, viewm(s), s.n, h->tl.ntiles, s.q, q, h->stream);
+    const int* ts = s.ranged ? s.tile_start : nullptr;
+    if (h->prec == CYLPIC_FP64)
+      launch_charge_half_f64(h->g, h->tl, view64(s), ts, extra_of(s), s.n, s.m, s.q, q, h->stream);
+    else launch_charge_half_mixed(h->g, h->tl, viewm(s), ts, extra_of(s), s.n, s.q, q, h->stream);
   }
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;

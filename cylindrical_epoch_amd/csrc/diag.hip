@@ -11,6 +11,7 @@
 
 #include "cplx_ops.h"
 #include "launch.h"
+#include "tile_scatter.h"
 
 namespace cylpic {
 
@@ -91,6 +92,53 @@ __global__ __launch_bounds__(256) void k_number_density(Grid g, P p, long n, dou
       }
   }
 }
+
+// The same contribution as a tile-scatter operation: NC = 2 M words per cell
+// (the real and imaginary part of each mode)
+template <class P>
+struct DensityOp {
+  Grid g;
+  P p;
+  __device__ bool stencil(long i, int& cx0, int& cy0, double w[3][3], double* f) const {
+    double x, y, z, wt;
+    load_particle(g, p, i, x, y, z, nullptr, wt, 0.0);
+    const double part_r = sqrt(y * y + z * z);
+    const double cxr = (x - g.x_grid_min) / g.dx;
+    const double cyr = (part_r - g.y_grid_min) / g.dy;
+    int cx = (int)floor(cxr + 0.5), cy = (int)floor(cyr + 0.5);
+    const double fx = (double)cx - cxr, fy = (double)cy - cyr;
+    cx += 1;
+    cy += 1;
+    if (cx - 1 < 1 - g.ng || cx + 1 > g.nx + g.ng || cy - 1 < 1 - g.ng || cy + 1 > g.ny + g.ng) return false;
+    const double cx2 = fx * fx, cy2 = fy * fy;
+    const double gx[3] = {0.5 * ((0.25 + cx2) + fx), 0.75 - cx2, 0.5 * ((0.25 + cx2) - fx)};
+    double gy[3] = {0.5 * ((0.25 + cy2) + fy), 0.75 - cy2, 0.5 * ((0.25 + cy2) - fy)};
+    if (part_r < g.dy) {
+      gy[1] = gy[1] + gy[0];
+      gy[0] = 0.0;
+    }
+    const double pnd = wt / ((((2.0 * PI_D) * g.dx) * g.dy) * part_r);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) w[b][a] = (gx[a] * gy[b]) * pnd;
+    const cplx<double> eith = mk(y / part_r, z / part_r);
+    cplx<double> eim = mk(1.0, 0.0);
+    f[0] = 1.0;
+    f[1] = 0.0;
+    for (int m = 1; m < g.M; ++m) {
+      eim = cmul(eim, eith);
+      f[2 * m] = 2.0 * eim.re;
+      f[2 * m + 1] = 2.0 * eim.im;
+    }
+    cx0 = cx - 1;
+    cy0 = cy - 1;
+    return true;
+  }
+  __device__ long out_index(const Grid& gg, int cx, int cy, int k) const {
+    return 2 * gg.idx(cx, cy, k >> 1) + (k & 1);
+  }
+};
 
 // calc_boundary_modes (calc_df.F90:35-55) = the real processor_summation_bcs
 // (boundary.F90:1305-1326) on both parts: particle_reflection_bcs without a
@@ -307,6 +355,37 @@ __global__ __launch_bounds__(256) void k_charge_half(Grid g, P p, long n, int nt
     for (int k = 0; k < 3; ++k) unsafeAtomicAdd(out + g.idx(cx + k, cy + j, 0), qw * (wx[k] * wy[j]));
 }
 
+// the same charge as a tile-scatter operation (one real word per cell)
+struct ChargeOp64 {
+  Grid g;
+  PartF64 p;
+  double mc, q;
+  __device__ bool stencil(long i, int& cx0, int& cy0, double w[3][3], double* f) const {
+    double x, y, z, pm[3], wt;
+    load_particle(g, p, i, x, y, z, pm, wt, mc);
+    const double ux = pm[0] / mc, uy = pm[1] / mc, uz = pm[2] / mc;
+    const double root = (C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+    const double xh = x + ux * root, yh = y + uy * root, zh = z + uz * root;
+    const double xr = (xh - g.x_grid_min) / g.dx;
+    const double rr = (sqrt(yh * yh + zh * zh) - g.y_grid_min) / g.dy;
+    const double fcx = floor(xr), fcy = floor(rr);
+    cx0 = (int)fcx;
+    cy0 = (int)fcy;
+    if (cx0 < 1 - g.ng || cx0 + 2 > g.nx + g.ng || cy0 < 1 - g.ng || cy0 + 2 > g.ny + g.ng) return false;
+    const double fx = fcx - xr + 0.5, fy = fcy - rr + 0.5;
+    const double wx[3] = {0.5 * ((0.25 + fx * fx) + fx), 0.5 * (1.5 - 2.0 * fx * fx), 0.5 * ((0.25 + fx * fx) - fx)};
+    const double wy[3] = {0.5 * ((0.25 + fy * fy) + fy), 0.5 * (1.5 - 2.0 * fy * fy), 0.5 * ((0.25 + fy * fy) - fy)};
+    const double qw = q * wt;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) w[b][a] = qw * (wx[a] * wy[b]);
+    f[0] = 1.0;
+    return true;
+  }
+  __device__ long out_index(const Grid& gg, int cx, int cy, int) const { return gg.idx(cx, cy, 0); }
+};
+
 // max |Q1 - Q0 + dt (fluxes of Jx, Jr)|, max |Q1 - Q0| and max |Q1| of the
 // m = 0 cells at least ng from every edge; out[0..2] hold the bit patterns
 // of the non-negative maxima (ordered as unsigned integers)
@@ -315,10 +394,11 @@ __global__ __launch_bounds__(256) void k_continuity_m0(Grid g, const double* __r
                                                        const double* __restrict__ q1, const cplx<T>* __restrict__ jx,
                                                        const cplx<T>* __restrict__ jr, Geo geo,
                                                        unsigned long long* __restrict__ out) {
+  __shared__ double red[3][4];
   const int nxi = g.nx - 2 * g.ng, nyi = g.ny - 2 * g.ng;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)nxi * nyi;
   double res = 0.0, dq = 0.0, qq = 0.0;
-  if (nxi > 0 && nyi > 0 && e < (long)nxi * nyi) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     const int cx = g.ng + 1 + (int)(e % nxi), cy = g.ng + 1 + (int)(e / nxi);
     const int jng = g.ng;
     const double flux = ((double)jx[g.idx(cx + 1, cy, 0)].re - (double)jx[g.idx(cx, cy, 0)].re) /
@@ -327,9 +407,9 @@ __global__ __launch_bounds__(256) void k_continuity_m0(Grid g, const double* __r
                         (double)jr[g.idx(cx, cy, 0)].re / geo.inv_area_xt[cy - 1 + jng];
     const long k = g.idx(cx, cy, 0);
     const double d = q1[k] - q0[k];
-    res = fabs(d + g.dt * flux);
-    dq = fabs(d);
-    qq = fabs(q1[k]);
+    res = fmax(res, fabs(d + g.dt * flux));
+    dq = fmax(dq, fabs(d));
+    qq = fmax(qq, fabs(q1[k]));
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -337,10 +417,17 @@ __global__ __launch_bounds__(256) void k_continuity_m0(Grid g, const double* __r
     dq = fmax(dq, __shfl_xor(dq, o));
     qq = fmax(qq, __shfl_xor(qq, o));
   }
+  const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(out, (unsigned long long)__double_as_longlong(res));
-    atomicMax(out + 1, (unsigned long long)__double_as_longlong(dq));
-    atomicMax(out + 2, (unsigned long long)__double_as_longlong(qq));
+    red[0][wv] = res;
+    red[1][wv] = dq;
+    red[2][wv] = qq;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[threadIdx.x][w]);
+    atomicMax(out + threadIdx.x, (unsigned long long)__double_as_longlong(v));
   }
 }
 
@@ -370,24 +457,49 @@ void df_bin(const Grid& g, const P& p, long n, double mc, const DistFn& d, doubl
 
 }  // namespace
 
-void launch_number_density_f64(const Grid& g, const PartF64& p, long n, cplx<double>* out, hipStream_t s) {
-  number_density(g, p, n, 0.0, out, s);
+template <class P>
+static void number_density_tiled(const Grid& g, const Tiles& tl, const P& p, const int* tile_start, ExtraRange xr,
+                                 long n, cplx<double>* out, hipStream_t s) {
+  const DensityOp<P> op{g, p};
+  const size_t lds = (size_t)2 * g.M * (tl.tx + 4) * (tl.tr + 4) * sizeof(double);
+  if (n <= 0) return;
+  if (!tile_start || lds > 65536) {  // no tile ranges, or a box beyond the default LDS: one atomic per word
+    number_density(g, p, n, 0.0, out, s);
+    return;
+  }
+  switch (g.M) {
+    case 1: launch_tile_scatter<2>(g, tl, op, p.key, tile_start, xr, (double*)out, s); break;
+    case 2: launch_tile_scatter<4>(g, tl, op, p.key, tile_start, xr, (double*)out, s); break;
+    case 3: launch_tile_scatter<6>(g, tl, op, p.key, tile_start, xr, (double*)out, s); break;
+    default: number_density(g, p, n, 0.0, out, s); break;
+  }
 }
-void launch_number_density_mixed(const Grid& g, const PartMixed& p, long n, cplx<double>* out, hipStream_t s) {
-  number_density(g, p, n, 0.0, out, s);
+void launch_number_density_f64(const Grid& g, const Tiles& tl, const PartF64& p, const int* tile_start,
+                               ExtraRange xr, long n, cplx<double>* out, hipStream_t s) {
+  number_density_tiled(g, tl, p, tile_start, xr, n, out, s);
+}
+void launch_number_density_mixed(const Grid& g, const Tiles& tl, const PartMixed& p, const int* tile_start,
+                                 ExtraRange xr, long n, cplx<double>* out, hipStream_t s) {
+  number_density_tiled(g, tl, p, tile_start, xr, n, out, s);
 }
 
-void launch_charge_half_f64(const Grid& g, const PartF64& p, long n, int ntiles, double mass, double q,
-                            double* out, hipStream_t s) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_charge_half<PartF64>, dim3(nblk(n)), dim3(256), 0, s, g, p, n, ntiles, mass * C_LIGHT, q,
+void launch_charge_half_f64(const Grid& g, const Tiles& tl, const PartF64& p, const int* tile_start, ExtraRange xr,
+                            long n, double mass, double q, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  const size_t lds = (size_t)(tl.tx + 4) * (tl.tr + 4) * sizeof(double);
+  if (tile_start && lds <= 65536)
+    launch_tile_scatter<1>(g, tl, ChargeOp64{g, p, mass * C_LIGHT, q}, p.key, tile_start, xr, out, s);
+  else
+    hipLaunchKernelGGL(k_charge_half<PartF64>, dim3(nblk(n)), dim3(256), 0, s, g, p, n, tl.ntiles, mass * C_LIGHT, q,
                        out);
 }
 template <class T>
 void launch_continuity_m0(const Grid& g, const double* q0, const double* q1, const cplx<T>* jx, const cplx<T>* jr,
                           const Geo& geo, unsigned long long* out, hipStream_t s) {
   const long n = (long)std::max(0, g.nx - 2 * g.ng) * std::max(0, g.ny - 2 * g.ng);
-  if (n > 0) hipLaunchKernelGGL(k_continuity_m0<T>, dim3(nblk(n)), dim3(256), 0, s, g, q0, q1, jx, jr, geo, out);
+  if (n > 0)
+    hipLaunchKernelGGL(k_continuity_m0<T>, dim3(std::min(nblk(n), 1024u)), dim3(256), 0, s, g, q0, q1, jx, jr, geo,
+                       out);
 }
 template void launch_continuity_m0<float>(const Grid&, const double*, const double*, const cplx<float>*,
                                           const cplx<float>*, const Geo&, unsigned long long*, hipStream_t);

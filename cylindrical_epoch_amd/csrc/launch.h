@@ -174,17 +174,20 @@ void launch_mig_scatter(const uint32_t* key, const int* tile_start, ExtraRange x
 // number density modes (complex FP64, field-shaped `out`, zeroed by the
 // caller); then launch_density_bcs applies the ghost sums and zero-gradient
 // ghosts of calc_boundary_modes / calc_number_density_modes
-void launch_number_density_f64(const Grid&, const PartF64&, long n, cplx<double>* out, hipStream_t);
-void launch_number_density_mixed(const Grid&, const PartMixed&, long n, cplx<double>* out, hipStream_t);
+// (one workgroup per tile over the tile ranges when tile_start is given)
+void launch_number_density_f64(const Grid&, const Tiles&, const PartF64&, const int* tile_start, ExtraRange,
+                               long n, cplx<double>* out, hipStream_t);
+void launch_number_density_mixed(const Grid&, const Tiles&, const PartMixed&, const int* tile_start, ExtraRange,
+                                 long n, cplx<double>* out, hipStream_t);
 void launch_density_bcs(const Grid&, cplx<double>* a, hipStream_t);
 // continuity check: the m = 0 charge at x + v dt/2 on the staggered grid
 // (one plane, zeroed by the caller; particles with key >= ntiles skipped),
 // then the residual maxima (out[3], bit patterns, zeroed by the caller)
-void launch_charge_half_f64(const Grid&, const PartF64&, long n, int ntiles, double mass, double q, double* out,
-                            hipStream_t);
+void launch_charge_half_f64(const Grid&, const Tiles&, const PartF64&, const int* tile_start, ExtraRange, long n,
+                            double mass, double q, double* out, hipStream_t);
 // the mixed path's charge with the push's own FP32 half step (push_mixed.hip)
-void launch_charge_half_mixed(const Grid&, const PartMixed&, long n, int ntiles, double charge, double* out,
-                              hipStream_t);
+void launch_charge_half_mixed(const Grid&, const Tiles&, const PartMixed&, const int* tile_start, ExtraRange,
+                              long n, double charge, double* out, hipStream_t);
 template <class T>
 void launch_continuity_m0(const Grid&, const double* q0, const double* q1, const cplx<T>* jx, const cplx<T>* jr,
                           const Geo& geo, unsigned long long* out, hipStream_t);

@@ -27,6 +27,7 @@
 #include "cplx_ops.h"
 #include "launch.h"
 #include "migrate.h"
+#include "tile_scatter.h"
 
 namespace cylpic {
 
@@ -805,12 +806,43 @@ MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
   return mc;
 }
 
-void launch_charge_half_mixed(const Grid& g, const PartMixed& sp, long n, int ntiles, double charge, double* out,
-                              hipStream_t s) {
+// the same charge as a tile-scatter operation (one real word per cell)
+struct ChargeOpMixed {
+  Grid g;
+  MixConst mc;
+  PartMixed sp;
+  double qfac;
+  __device__ bool stencil(long i, int& cx0, int& cy0, double w[3][3], double* f) const {
+    const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], sp.ux[i], sp.uy[i], sp.uz[i]);
+    const int cx2 = sp.cell[i] + (int)hs.fl2x, cy2 = (int)hs.fl2y + 1;
+    if (cx2 - 1 < 1 - g.ng || cx2 + 1 > g.nx + g.ng || cy2 - 1 < 1 - g.ng || cy2 + 1 > g.ny + g.ng) return false;
+    float hx[3], hy[3];
+    w3f((hs.fl2x - hs.oxh) + 0.5f, hx[0], hx[1], hx[2]);
+    w3f((hs.fl2y - hs.cyr) + 0.5f, hy[0], hy[1], hy[2]);
+    const double qw = qfac * (double)sp.w[i];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) w[b][a] = qw * ((double)hx[a] * (double)hy[b]);
+    f[0] = 1.0;
+    cx0 = cx2 - 1;
+    cy0 = cy2 - 1;
+    return true;
+  }
+  __device__ long out_index(const Grid& gg, int cx, int cy, int) const { return gg.idx(cx, cy, 0); }
+};
+
+void launch_charge_half_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, const int* tile_start,
+                              ExtraRange xr, long n, double charge, double* out, hipStream_t s) {
+  if (n <= 0) return;
   // only the half-step constants of mix_const are used
-  if (n > 0)
-    hipLaunchKernelGGL(k_charge_half_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g,
-                       mix_const(g, PushConst{}, charge), sp, n, ntiles, charge * 0.25, out);
+  const MixConst mc = mix_const(g, PushConst{}, charge);
+  const size_t lds = (size_t)(tl.tx + 4) * (tl.tr + 4) * sizeof(double);
+  if (tile_start && lds <= 65536)
+    launch_tile_scatter<1>(g, tl, ChargeOpMixed{g, mc, sp, charge * 0.25}, sp.key, tile_start, xr, out, s);
+  else
+    hipLaunchKernelGGL(k_charge_half_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, mc, sp, n,
+                       tl.ntiles, charge * 0.25, out);
 }
 
 int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& pc, double charge,
