@@ -235,6 +235,56 @@ def cpu_baseline(w, sample_cols):
             "push_particle_steps_per_s": round(ns / (t2 - t1), 1)}
 
 
+def scan_runs(n=32, seed=20231):
+    """A parameter scan of uneven runs: nx in [512, 2048], ny in [64, 256],
+    1-3 modes and 8-32 particles per cell, drawn from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    return [dict(nx=int(rng.integers(512, 2049)), ny=int(rng.integers(64, 257)), n_mode=int(rng.integers(1, 4)),
+                 ppc=int(rng.integers(8, 33)), dx=0.02e-6, dy=0.1e-6, density=1e24) for _ in range(n)]
+
+
+def scan(precision, steps, warmup):
+    """Steps every run of the scan, each in its own library handle (and HIP
+    stream): first one run after another, then all runs interleaved step by
+    step so that their kernels overlap on the GPU.  Thermal plasma (100 eV),
+    periodic in x, conducting at r_max."""
+    runs = scan_runs()
+    sims, total = [], 0
+    for k, w in enumerate(runs):
+        dt = deck.set_dt(w["dx"], w["dy"], 0.95)
+        sim = Simulation(w["nx"], w["ny"], w["n_mode"], w["dx"], w["dy"], dt,
+                         bc_field=("periodic", "periodic", "other", "conduct"),
+                         bc_particle=("periodic", "periodic", "other", "reflect"), precision=precision)
+        sim.set_sort_interval(4)
+        p = deck.uniform_plasma(w["nx"], w["ny"], w["dx"], w["dy"], w["ppc"], w["density"], 100.0, seed=100 + k)
+        sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=len(p["x"]))
+        sim.upload_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+        sim.setup_field_boundaries()
+        total += len(p["x"])
+        sims.append(sim)
+    for sim in sims:
+        for _ in range(warmup):
+            sim.step()
+        sim.synchronize()
+    t0 = time.perf_counter()
+    for sim in sims:
+        for _ in range(steps):
+            sim.step()
+        sim.synchronize()
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim in sims:
+            sim.step()
+    for sim in sims:
+        sim.synchronize()
+    t_int = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
+    return dict(runs=len(runs), particles=total, cells=sum(w["nx"] * w["ny"] * w["n_mode"] for w in runs),
+                seq_s=t_seq, int_s=t_int, value_seq=total * steps / t_seq, value_int=total * steps / t_int)
+
+
 def copy_calibration():
     """A plain device copy of 2^28 float32 (1 GiB read + 1 GiB written per
     launch), three launches."""
@@ -270,12 +320,31 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=4,
                     help="sort every n pushes (library default 1; 4 measured fastest on this workload, with no "
                          "slower push: the cold plasma moves well under a tile in 4 steps)")
+    ap.add_argument("--scan", action="store_true",
+                    help="instead of the single run: the 32-run parameter scan (nx 512-2048, ny 64-256, 1-3 modes, "
+                         "8-32 ppc), stepped one run after another and interleaved")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.scan:
+        prec = "mixed" if a.precision == "both" else a.precision
+        r = scan(prec, a.steps, a.warmup)
+        log(f"scan {prec}: {r['runs']} runs, {r['particles']} particles; one after another {r['seq_s']:.3f} s, "
+            f"interleaved {r['int_s']:.3f} s")
+        print(json.dumps({
+            "metric": "particle-steps/s, 32-run parameter scan (aggregate), 1 MI355X",
+            "value": round(r["value_int"], 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(r["int_s"] / a.steps * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == "fp64" else "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": "scan_32_runs_nx512-2048_ny64-256_m1-3_ppc8-32", "runs": r["runs"],
+                       "particles": r["particles"], "cell_modes": r["cells"], "precision": prec,
+                       "parallelism": "single-gpu, one stream per run"},
+            "one_after_another_particle_steps_per_s": round(r["value_seq"], 1)}), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no "
