@@ -347,7 +347,7 @@ def main():
             "one_after_another_particle_steps_per_s": round(r["value_seq"], 1)}), flush=True)
         return
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no "
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the library has no "
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
     w = deck.WORKLOADS[a.workload]

@@ -137,7 +137,7 @@ void orc_stable_sort_keys(int64_t n, const int32_t *keys, int32_t nkeys,
 double orc_set_dt(double dx, double dy, double dt_multiplier,
                   double dt_plasma_frequency, double dt_laser);
 
-/* ---- diagnostics defined by this build (the reference's Cartesian
+/* ---- diagnostics with no reference routine (the reference's Cartesian
  * energy sums are never updated in the cylindrical code) -------------- */
 void orc_mode_energy(const orc_grid *g, const orc_fields *f, double *w_e,
                      double *w_b);
