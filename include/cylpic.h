@@ -180,8 +180,9 @@ int cylpic_set_tile(cylpic_ctx *h, int tile_x, int tile_r);
  * exact, order-independent tile sums) or two 32-bit fixed-point halves packed
  * in one int64 per complex cell (4: half the atomics, ~1e-5 relative
  * resolution); gather_lds (both paths): stage the tile's fields in LDS (1,
- * the mixed path's default) or gather from L1/L2 (0, the FP64 default).  Has
- * no counterpart in the reference (its deposit is serial FP64). */
+ * the mixed path's default) or gather from L1/L2 (0, the FP64 default); 2
+ * (mixed path, acc_mode 3) stages E only and gathers B from L1/L2.  Has no
+ * counterpart in the reference (its deposit is serial FP64). */
 int cylpic_tune(cylpic_ctx *h, int acc_mode, int gather_lds);
 
 /* Autotuner (no counterpart in the reference): times `pushes` pushes (plus
