@@ -649,6 +649,10 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
   if (ng != 5) return fail(CYLPIC_EINVAL, "ng must be 5 (triangle shape: png = 3, ng = png + 2)");
   if (nx < 2 * ng) return fail(CYLPIC_EINVAL, "nx must be >= 2*ng");
   if (ny < 3) return fail(CYLPIC_EINVAL, "ny must be >= 3");
+  // the r_max current reflection (boundary.F90:986-1013) scales ghost row i
+  // by radius ratios whose denominator r_max - i dy is zero at i = ny
+  if (bc_particle && bc_particle[YMAX] == BC_REFLECT && ny <= ng)
+    return fail(CYLPIC_EINVAL, "a reflecting r_max needs ny > ng");
   if (n_mode < 1 || n_mode > 8) return fail(CYLPIC_EINVAL, "n_mode must be in 1..8");
   if (!(dx > 0) || !(dy > 0) || !(dt > 0)) return fail(CYLPIC_EINVAL, "dx, dy, dt must be positive");
   if (precision != CYLPIC_FP64 && precision != CYLPIC_MIXED)

@@ -71,6 +71,14 @@ static inline ocplx crecip(ocplx z) {
 }
 static const ocplx CZERO = {0.0, 0.0};
 
+/* nonzero once a push met a particle it cannot deposit (orc_take_error) */
+static int orc_error = 0;
+int orc_take_error(void) {
+  const int e = orc_error;
+  orc_error = 0;
+  return e;
+}
+
 /* ------------------------------------------------------------------ */
 /* field solver: src/fields.f90                                       */
 /* ------------------------------------------------------------------ */
@@ -953,6 +961,14 @@ void orc_push_particles(const orc_grid *g, orc_fields *f, orc_species *sp, int n
       cell_y2 += 1;
       weights3(cell_frac_x, hx);
       weights3(cell_frac_y, hy);
+      if (!(fabs(cell_x_r) < 1e9 && fabs(cell_y_r) < 1e9) || cell_x1 < 2 - g->ng || cell_x1 > g->nx + g->ng - 1 ||
+          cell_y1 < 2 - g->ng || cell_y1 > g->ny + g->ng - 1 || cell_x2 < 2 - g->ng || cell_x2 > g->nx + g->ng - 1 ||
+          cell_y2 < 2 - g->ng || cell_y2 > g->ny + g->ng - 1) {
+        /* a stencil outside the arrays (a position the reference's grid
+         * cannot hold, or not finite): flag and skip */
+        orc_error |= 2;
+        continue;
+      }
 
       /* e_part.inc / b_part.inc */
       ocplx e_m = cc(1.0, 0.0);
@@ -1034,6 +1050,12 @@ void orc_push_particles(const orc_grid *g, orc_fields *f, orc_species *sp, int n
       cell_y3 += 1;
       for (int k = -2; k <= 2; ++k) { hx[k] = 0.0; hy[k] = 0.0; }
       int dcellx = cell_x3 - cell_x2, dcelly = cell_y3 - cell_y2;
+      if (dcellx < -1 || dcellx > 1 || dcelly < -1 || dcelly > 1) {
+        /* a move of more than one cell per step (or a non-finite position):
+         * the reference's weight arrays would be overrun; flag and skip */
+        orc_error |= 1;
+        continue;
+      }
       weights3(cell_frac_x, hx + dcellx);
       weights3(cell_frac_y, hy + dcelly);
       for (int k = -2; k <= 2; ++k) { hx[k] = hx[k] - gx[k]; hy[k] = hy[k] - gy[k]; }

@@ -67,6 +67,12 @@ typedef struct {
   int immobile;
 } orc_species;
 
+/* Returns and clears the error flag of the last pushes: nonzero when a
+ * particle moved more than one cell in a step (or had a non-finite
+ * position), which the reference's deposit cannot represent; that
+ * particle's update was skipped. */
+int orc_take_error(void);
+
 /* ---- field solver (src/fields.f90) ---------------------------------- */
 void orc_update_e_field(const orc_grid *g, orc_fields *f);
 void orc_update_b_field(const orc_grid *g, orc_fields *f);

@@ -98,6 +98,8 @@ def _declare(L):
         fn.restype = None
     L.orc_dist_fn.argtypes = [gp, sp, C.c_int, dp, dp, dp, dp, dp, dp]
     L.orc_dist_fn.restype = C.c_int
+    L.orc_take_error.argtypes = []
+    L.orc_take_error.restype = C.c_int
     L.orc_set_dt.argtypes = [C.c_double] * 5
     L.orc_set_dt.restype = C.c_double
 
@@ -190,6 +192,9 @@ class State:
         arr = self._species_array()
         lib().orc_push_particles(C.byref(self.g), C.byref(self.f), arr, len(self.species))
         self._species_back(arr)
+        if lib().orc_take_error():
+            raise FloatingPointError("oracle push: a particle moved more than one cell in a step "
+                                     "(or its position is not finite)")
 
     def current_finish(self, smooth=None):
         """current_finish (current_smooth.F90:29-46); smooth = (its,

@@ -495,3 +495,14 @@ def test_mode_factor_sensitivity(oracle):
             out.append(np.concatenate([st.arr[c][1].ravel() for c in ("jx", "jr", "jt")]))
         worst = max(worst, np.max(np.abs(out[0] - out[1])) / np.max(np.abs(out[0])))
     assert 1e-13 < worst < 1e-8
+
+
+def test_oracle_refuses_multi_cell_moves(oracle):
+    """The oracle flags a particle whose stencil it cannot place (here a
+    non-finite momentum) instead of overrunning the reference's 5-point
+    weight arrays or reading outside the field arrays."""
+    st = _state(oracle, nx=16, ny=8, M=1)
+    one = np.ones(1)
+    st.add_species(-Q0, M0, one * 5e-7, one * 3e-7, one * 0, one * np.nan, one * 0, one * 0, one)
+    with pytest.raises(FloatingPointError):
+        st.push_particles()
