@@ -317,9 +317,10 @@ def main():
     ap.add_argument("--no-window", action="store_true", help="keep the window fixed (no injection)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the library's autotuner pick the tile shape and gather source before timing")
-    ap.add_argument("--sort-interval", type=int, default=4,
-                    help="sort every n pushes (library default 1; 4 measured fastest on this workload, with no "
-                         "slower push: the cold plasma moves well under a tile in 4 steps)")
+    ap.add_argument("--sort-interval", type=int, default=8,
+                    help="sort every n pushes (library default 1; 8 measured fastest on this workload, with no "
+                         "slower push: in 8 steps a particle moves at most 4.8 cells in x and about one in r, "
+                         "within a 16 x 4 tile's neighbours)")
     ap.add_argument("--scan", action="store_true",
                     help="instead of the single run: the 32-run parameter scan (nx 512-2048, ny 64-256, 1-3 modes, "
                          "8-32 ppc), stepped one run after another and interleaved")
