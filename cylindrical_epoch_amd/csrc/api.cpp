@@ -346,6 +346,13 @@ int stage_in(cylpic_ctx* h, Species& s, long base, long n, const double* const s
       HIPCHK(hipMemcpyAsync(s.a[k] + base, src[k], n * sizeof(double), hipMemcpyHostToDevice, h->stream));
     return CYLPIC_OK;
   }
+  // the mixed push forms 1 + |u|^2 in FP32 (u = p / (m c)): each component
+  // must stay below 1e18 so that the sum cannot overflow
+  const double umax = 1.0e18 * s.m * C_LIGHT;
+  for (int k = 3; k < 6; ++k)
+    for (long i = 0; i < n; ++i)
+      if (!(std::fabs(src[k][i]) < umax))
+        return fail(CYLPIC_EINVAL, "momentum beyond the mixed path's range (|p| / (m c) must be below 1e18 and finite)");
   if (7 * n > h->stage_cap) {
     HIPCHK(hipStreamSynchronize(h->stream));
     if (h->stage) HIPCHK(hipFree(h->stage));

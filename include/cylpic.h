@@ -80,7 +80,10 @@ int cylpic_set_species(cylpic_ctx *h, int id, double charge, double mass,
 
 /* Copy a species' particles to / from the device, SoA staging arrays filled
  * from the linked list (src/housekeeping/partlist.F90:414-488 wire order:
- * pos3, p3, weight).  ids may be NULL (no identity tracking). */
+ * pos3, p3, weight).  ids may be NULL (no identity tracking).  On the mixed
+ * path every momentum component must be finite with |p| / (m c) < 1e18
+ * (CYLPIC_EINVAL otherwise, nothing copied); the same holds for
+ * cylpic_inject_particles. */
 int cylpic_upload_particles(cylpic_ctx *h, int species, int64_t n,
                             const double *x, const double *y, const double *z,
                             const double *px, const double *py, const double *pz,
