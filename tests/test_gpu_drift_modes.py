@@ -10,8 +10,8 @@ import pytest
 from cylindrical_epoch_amd import deck
 from cylindrical_epoch_amd.host import CylpicError, Simulation
 from gpu_helpers import log
-from test_oracle_drift_modes import (DRIFT, DX, DY, M, NX, NY, check_mode_structure, drift_plasma,
-                                     expected_current, interior_means)
+from test_oracle_drift_modes import (DRIFT, DX, DY, M, NX, NY, check_axis_rows, check_mode_structure,
+                                     drift_plasma, expected_current, interior_means)
 
 pytestmark = pytest.mark.gpu
 
@@ -42,6 +42,7 @@ def test_drift_current_modes_gpu(precision):
     j = expected_current(drift)
     log("  J_x,0", means["jx"][0] / j[0], "J_r,1", means["jr"][1] / j[1], "J_t,1", means["jt"][1] / (-1j * j[1]))
     check_mode_structure(means, j)
+    check_axis_rows(lambda c: fields[c], 5, j)
     sim.close()
 
 

@@ -57,6 +57,29 @@ def check_mode_structure(means, j, rel=0.012, leak=0.015):
         assert abs(means[c][m]) < leak * abs(jy), (c, m, means[c][m])
 
 
+def check_axis_rows(get, ng, j, nx=NX, rel=0.04, leak=0.03):
+    """Rows 0..3, where the axis fold and the r = 0 rules of the current act
+    (boundary.F90:1909-1959), at the same closed form: J_x,0 on the axis,
+    J_theta,1 on the axis, and J_r,1 on the row reflected to r = -dr/2 (the
+    uniform transverse current continues across the axis).  Rows near the
+    axis hold few particles and see the largest angle changes per step, so
+    the tolerance is looser than in the interior.  J_x,1 and J_theta,0 are
+    zero on the axis."""
+    jx, jy = j[0], j[1]
+    cols = slice(ng, ng + nx)
+
+    def row(c, m, r):
+        return get(c)[m][ng - 1 + r, cols].mean()
+
+    for r in range(4):
+        assert abs(row("jx", 0, r) - jx) < rel * abs(jx), ("jx", 0, r)
+        assert abs(row("jr", 1, r) - jy) < rel * abs(jy), ("jr", 1, r)
+        assert abs(row("jt", 1, r) - (-1j * jy)) < rel * abs(jy), ("jt", 1, r)
+        for c, m in (("jx", 1), ("jr", 0), ("jt", 0)):
+            assert abs(row(c, m, r)) < leak * abs(jy), (c, m, r)
+    assert not get("jx")[1][ng - 1, :].any() and not get("jt")[0][ng - 1, :].any()
+
+
 def test_drift_current_magnitude_closed_form():
     """The manual's figure value: |J_x| = |J_y| = 3.40e-6 A m^-2."""
     j = expected_current()
@@ -75,6 +98,7 @@ def test_drift_current_modes(oracle):
     st.current_finish()
     means = interior_means(lambda c: st.arr[c], st.g.ng)
     check_mode_structure(means, expected_current())
+    check_axis_rows(lambda c: st.arr[c], st.g.ng, expected_current())
 
 
 def test_drift_check_catches_a_dropped_mode_factor(oracle):

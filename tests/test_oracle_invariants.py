@@ -167,6 +167,59 @@ def test_continuity_mode0_finite_volume(oracle, seed):
         assert abs(res) < 1e-12 * scale, (cx, cy, res)
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_continuity_mode0_after_axis_fold(oracle, seed):
+    """A particle within 1.4 cells of the axis: after current_finish (the
+    fold of the r < 0 ghost rows, boundary.F90:1917-1931), every cell
+    with r >= dr conserves charge to round-off, its charge being the
+    stencil plus the stencil's mirror (the node at -r folds onto +r).  The
+    axis cell itself is left out: the reference replaces jxm(:,0,0) by
+    (4 jxm(:,1,0) - jxm(:,2,0)) / 3."""
+    rng = np.random.default_rng(100 + seed)
+    st = _state(oracle, nx=24, ny=20, M=1)
+    g = st.g
+    x = (8 + rng.random()) * g.dx
+    r = (0.05 + 1.35 * rng.random()) * g.dy
+    th = rng.uniform(-3, 3)
+    u = rng.normal(size=3) * 1.0
+    p = u * M0 * C_LIGHT
+    q, w = -Q0, 1.7e3
+    sp = _one_particle(st, x, r, th, p=p, w=w)
+    gam = math.sqrt(1 + u @ u)
+    dtco2 = C_LIGHT * g.dt / 2
+    pos0 = np.array([sp["x"][0], sp["y"][0], sp["z"][0]])
+    st.push_particles()
+    st.current_finish()
+    inv_rt, inv_xt, _, _ = st.geometry()
+    ng = g.ng
+
+    def charge(pos):
+        xr = (pos[0] - g.x_grid_min) / g.dx
+        rr = (math.hypot(pos[1], pos[2]) - g.y_grid_min) / g.dy
+        cx, wx = _stagger_weights(xr)
+        cy, wy = _stagger_weights(rr)
+        Q = {}
+        for i in range(3):
+            for j in range(3):
+                key = (cx - 1 + i, abs(cy - 1 + j))
+                Q[key] = Q.get(key, 0.0) + q * w * wx[i] * wy[j]
+        return Q
+
+    Q0_ = charge(pos0 + u * dtco2 / gam)
+    Q1_ = charge(pos0 + 3 * u * dtco2 / gam)
+    cells = {c for c in set(Q0_) | set(Q1_) if c[1] >= 1}
+    assert cells
+    scale = abs(q * w)
+    for (cx, cy) in cells:
+        jx1 = st.at("jx", cx + 1, cy, 0).real
+        jx0 = st.at("jx", cx, cy, 0).real
+        jr1 = st.at("jr", cx, cy + 1, 0).real
+        jr0 = st.at("jr", cx, cy, 0).real
+        flux = (jx1 - jx0) / inv_rt[cy + ng] + jr1 / inv_xt[cy + ng] - jr0 / inv_xt[cy - 1 + ng]
+        res = Q1_.get((cx, cy), 0.0) - Q0_.get((cx, cy), 0.0) + g.dt * flux
+        assert abs(res) < 1e-12 * scale, (cx, cy, res)
+
+
 def _mode_charge(g, pos, q, w, M):
     """Charge modes on the staggered grid at a position: quadratic-spline
     fractions times 1 (m = 0) or 2 e^{+i m theta} (m > 0)."""
