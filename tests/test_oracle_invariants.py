@@ -526,6 +526,31 @@ def test_particle_bcs_periodic_and_reflect(oracle):
     assert sp["pz"][0] == pytest.approx(p[2], rel=1e-14)
 
 
+@pytest.mark.parametrize("theta", [0.7, 2.4, -1.9])
+def test_rmax_reflection_at_any_angle(oracle, theta):
+    """r_max reflection (boundary.F90:1776-1790): the radius is mirrored
+    about y_max at the same angle; the radial momentum reverses, the
+    azimuthal and axial momenta and |p| are kept."""
+    st = _state(oracle, nx=10, ny=8, M=1, bc_particle=("periodic", "periodic", "other", "reflect"))
+    g = st.g
+    r = g.y_max + 0.3 * g.dy
+    p = np.array([1.3e-22, -0.7e-22, 2.1e-22])
+    c, s = math.cos(theta), math.sin(theta)
+    pr, pt = p[1] * c + p[2] * s, -p[1] * s + p[2] * c
+    sp = st.add_species(-Q0, M0, [0.5 * (g.x_min + g.x_max)], [r * c], [r * s], [p[0]], [p[1]], [p[2]], [1.0])
+    arr = st._species_array()
+    oracle.lib().orc_particle_bcs(oracle.C.byref(st.g), arr, 1)
+    st._species_back(arr)
+    y, z = sp["y"][0], sp["z"][0]
+    assert math.hypot(y, z) == pytest.approx(g.y_max - 0.3 * g.dy, rel=1e-13)
+    assert math.atan2(z, y) == pytest.approx(theta, abs=1e-13)
+    py, pz = sp["py"][0], sp["pz"][0]
+    assert py * c + pz * s == pytest.approx(-pr, rel=1e-12)
+    assert -py * s + pz * c == pytest.approx(pt, rel=1e-12)
+    assert sp["px"][0] == p[0]
+    assert math.hypot(py, pz) == pytest.approx(math.hypot(p[1], p[2]), rel=1e-14)
+
+
 def test_mode_factor_sensitivity(oracle):
     """The m>0 deposit's exact-branch factors are ill-conditioned near the
     small-angle switch: one ulp of dtheta (the difference two atan2
