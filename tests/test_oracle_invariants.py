@@ -93,6 +93,43 @@ def test_boris_rotation_uniform_bx(oracle):
     assert got == pytest.approx(ang, rel=1e-10)
 
 
+@pytest.mark.parametrize("vd", [0.05, 0.3])
+def test_e_cross_b_drift(oracle, vd):
+    """Uniform E along y and B along z (m = 1 fields: E_r,1 = E, E_theta,1 =
+    -i E, B_r,1 = i B, B_theta,1 = B).  A particle starting at the drift
+    velocity E x B / B^2 = (E / B) x feels no force in the continuum.  The
+    relativistic Boris step (half kick, rotation with the gamma of the
+    kicked momentum, half kick; particles.F90:404-451) holds it there up to
+    a gyration whose amplitude, relative to the drift momentum, is eps^2,
+    eps = q E dt / (2 m c) the half kick in u = p / (m c): the kick changes
+    gamma by O(eps^2) before the rotation.  Over 200 pushes the largest
+    relative momentum excursion is eps^2 to 5 %, and x advances by
+    (E / B) t to eps^2."""
+    st = _state(oracle, nx=24, ny=32, M=2, bc_particle=("periodic", "periodic", "other", "reflect"))
+    B0 = 1.0e4
+    E0 = vd * C_LIGHT * B0
+    st.arr["er"][1] = E0
+    st.arr["et"][1] = -1j * E0
+    st.arr["br"][1] = 1j * B0
+    st.arr["bt"][1] = B0
+    gam = 1.0 / math.sqrt(1.0 - vd * vd)
+    u0 = np.array([gam * vd, 0.0, 0.0])
+    x0, y0 = 3.3 * st.g.dx, 10.4 * st.g.dy
+    sp = _one_particle(st, x0, y0, 0.0, p=u0 * M0 * C_LIGHT)
+    eps = Q0 * E0 * st.g.dt / (2 * M0 * C_LIGHT)
+    n, dev = 200, 0.0
+    for _ in range(n):
+        st.push_particles()
+        u = np.array([sp["px"][0], sp["py"][0], sp["pz"][0]]) / (M0 * C_LIGHT)
+        dev = max(dev, np.abs(u - u0).max() / u0[0])
+    assert dev == pytest.approx(eps * eps, rel=0.05)
+    L = st.g.x_max - st.g.x_min
+    travelled = (sp["x"][0] - x0) % L
+    dist = vd * C_LIGHT * n * st.g.dt
+    assert travelled == pytest.approx(dist % L, abs=eps * eps * dist)
+    assert sp["y"][0] == pytest.approx(y0, abs=eps * eps * dist)
+
+
 def test_free_streaming(oracle):
     """No fields: x(t+dt) = x + v dt."""
     st = _state(oracle)
