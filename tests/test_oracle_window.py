@@ -94,3 +94,39 @@ def test_x_max_fix_keeps_a_line_straight(oracle):
         want = a + b * (ix[None, None, :] + 1)
         cols = slice(0, nx + 1 + ng)  # every column up to ix = nx + 1
         assert np.allclose(st.arr[n][:, :, cols], want[:, :, cols], rtol=0, atol=1e-12), n
+
+
+def test_injection_and_removal_commute(oracle):
+    """The reference inserts the new column (insert_particles) before the
+    shift removes the particles left of x_min.  The inserted particles lie
+    beyond x_max and the removed ones below the new x_min, so the two sets
+    are disjoint and the order of the two operations cannot change the
+    result: inserting after the shift gives the same particles in the same
+    order."""
+    rng = np.random.default_rng(4)
+    runs = []
+    for insert_first in (True, False):
+        st = _state(oracle)
+        dx, dy = st.g.dx, st.g.dy
+        n = 300
+        x = st.g.x_min + rng.uniform(-0.5, st.g.nx, n) * dx if insert_first else runs[0][1]
+        y = rng.uniform(0.1, 5.0, n) * dy if insert_first else runs[0][2]
+        z = np.zeros(n)
+        st.add_species(-1.6e-19, 9.1e-31, x, y, z, z, z, z, np.ones(n), ids=np.arange(n))
+        k = 2
+        xi = st.g.x_max + (np.arange(40) % 20 + 0.5) / 20 * k * dx
+        yi = (np.arange(40) // 20 + 0.5) * dy
+        zi = np.zeros(40)
+        args = (0, xi, yi, zi, zi, zi, zi, np.ones(40))
+        if insert_first:
+            st.append_particles(*args, ids=np.arange(n, n + 40))
+            st.shift_window(k)
+        else:
+            st.shift_window(k)
+            st.append_particles(*args, ids=np.arange(n, n + 40))
+        sp = st.species[0]
+        runs.append((sp["id"][: sp["n"]].copy(), x, y, sp["x"][: sp["n"]].copy()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][3], runs[1][3])
+    ids = runs[0][0]
+    assert len(ids) < 340 and np.count_nonzero(ids >= 300) == 40    # some removed, all inserted kept
