@@ -101,7 +101,8 @@ struct cylpic_ctx {
   double* diag_hist = nullptr;      // distribution function bins
   long diag_hist_cap = 0;
   double* cont_q = nullptr;               // two m = 0 charge planes of the continuity check
-  unsigned long long* cont_out = nullptr;  // its two maxima
+  double* gauss_q = nullptr;              // two m = 0 charge planes of the Gauss check
+  unsigned long long* cont_out = nullptr;  // the three maxima of either check
   bool cont_armed = false;
   int smooth_its = 0, smooth_comp_its = 0;  // current smoothing (current_smooth.F90)
   std::vector<int> smooth_strides{1};
@@ -543,6 +544,7 @@ void free_all(cylpic_ctx* h) {
   if (h->diag_part) (void)hipFree(h->diag_part);
   if (h->diag_hist) (void)hipFree(h->diag_hist);
   if (h->cont_q) (void)hipFree(h->cont_q);
+  if (h->gauss_q) (void)hipFree(h->gauss_q);
   if (h->cont_out) (void)hipFree(h->cont_out);
   if (h->mig_host) (void)hipHostFree(h->mig_host);
   if (h->d_energy) (void)hipFree(h->d_energy);
@@ -1465,14 +1467,17 @@ int cylpic_dist_fn(cylpic_ctx* h, int species, int ndims, const int* direction, 
 }
 
 // the m = 0 charge of every mobile species at x + v dt/2 into q
-static int charge_half(cylpic_ctx* h, double* q) {
+// The m = 0 charge of the mobile species at x + vs v dt/2; with `immobile`,
+// immobile species too, at x.
+static int charge_half(cylpic_ctx* h, double* q, bool immobile = false, double vs = 1.0) {
   HIPCHK(hipMemsetAsync(q, 0, h->g.plane * sizeof(double), h->stream));
   for (auto& s : h->sp) {
-    if (s.immobile || s.n <= 0) continue;
+    if ((s.immobile && !immobile) || s.n <= 0) continue;
     const int* ts = s.ranged ? s.tile_start : nullptr;
+    const double v = s.immobile ? 0.0 : vs;
     if (h->prec == CYLPIC_FP64)
-      launch_charge_half_f64(h->g, h->tl, view64(s), ts, extra_of(s), s.n, s.m, s.q, q, h->stream);
-    else launch_charge_half_mixed(h->g, h->tl, viewm(s), ts, extra_of(s), s.n, s.q, q, h->stream);
+      launch_charge_half_f64(h->g, h->tl, view64(s), ts, extra_of(s), s.n, s.m, s.q, v, q, h->stream);
+    else launch_charge_half_mixed(h->g, h->tl, viewm(s), ts, extra_of(s), s.n, s.q, v, q, h->stream);
   }
   HIPCHK(hipGetLastError());
   return CYLPIC_OK;
@@ -1481,10 +1486,8 @@ static int charge_half(cylpic_ctx* h, double* q) {
 int cylpic_continuity_begin(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
-  if (!h->cont_q) {
-    HIPCHK(hipMalloc((void**)&h->cont_q, 2 * h->g.plane * sizeof(double)));
-    HIPCHK(hipMalloc((void**)&h->cont_out, 3 * sizeof(unsigned long long)));
-  }
+  if (!h->cont_q) HIPCHK(hipMalloc((void**)&h->cont_q, 2 * h->g.plane * sizeof(double)));
+  if (!h->cont_out) HIPCHK(hipMalloc((void**)&h->cont_out, 3 * sizeof(unsigned long long)));
   int rc = charge_half(h, h->cont_q);
   if (rc) return rc;
   h->cont_armed = true;
@@ -1511,6 +1514,33 @@ int cylpic_continuity_end(cylpic_ctx* h, double* residual, double* dq, double* q
   std::memcpy(residual, &bits[0], sizeof(double));
   std::memcpy(dq, &bits[1], sizeof(double));
   std::memcpy(q, &bits[2], sizeof(double));
+  return check_device_errors(h);
+}
+
+int cylpic_gauss_residual_m0(cylpic_ctx* h, double* residual, double* q, double* flux) {
+  CHECK_H(h);
+  if (!residual || !q || !flux) return fail(CYLPIC_EINVAL, "null output");
+  HIPCHK(hipSetDevice(h->device));
+  if (!h->gauss_q) HIPCHK(hipMalloc((void**)&h->gauss_q, 2 * h->g.plane * sizeof(double)));
+  if (!h->cont_out) HIPCHK(hipMalloc((void**)&h->cont_out, 3 * sizeof(unsigned long long)));
+  // the charges half a step behind and ahead (their mean is the charge E
+  // is consistent with between steps); immobile species at x in both
+  double* qa = h->gauss_q;
+  double* qb = h->gauss_q + h->g.plane;
+  int rc = charge_half(h, qa, true, -1.0);
+  if (!rc) rc = charge_half(h, qb, true, 1.0);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(h->cont_out, 0, 3 * sizeof(unsigned long long), h->stream));
+  with_fields(h, [&](auto& F) {
+    launch_gauss_m0(h->g, qa, qb, F.ex(), F.er(), geo_of(h), h->cont_out, h->stream);
+  });
+  HIPCHK(hipGetLastError());
+  unsigned long long bits[3];
+  HIPCHK(hipMemcpyAsync(bits, h->cont_out, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::memcpy(residual, &bits[0], sizeof(double));
+  std::memcpy(q, &bits[1], sizeof(double));
+  std::memcpy(flux, &bits[2], sizeof(double));
   return check_device_errors(h);
 }
 

@@ -330,15 +330,17 @@ __global__ __launch_bounds__(256) void k_df_bin(Grid g, P p, long n, double mc, 
 // charge-conserving deposit (particles.F90:508-665) keeps
 // Q(t+3dt/2) - Q(t+dt/2) + dt (net current out of the cell) = 0.  The mixed
 // path's charge (its own FP32 half step) is in push_mixed.hip.
+// The charge at x + vs v dt/2: vs = 1 the half step ahead, -1 the half step
+// behind, 0 at x itself (a species that does not move).
 template <class P>
-__global__ __launch_bounds__(256) void k_charge_half(Grid g, P p, long n, int ntiles, double mc, double q,
+__global__ __launch_bounds__(256) void k_charge_half(Grid g, P p, long n, int ntiles, double mc, double q, double vs,
                                                      double* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || (unsigned)p.key[i] >= (unsigned)ntiles) return;
   double x, y, z, pm[3], w;
   load_particle(g, p, i, x, y, z, pm, w, mc);
   const double ux = pm[0] / mc, uy = pm[1] / mc, uz = pm[2] / mc;
-  const double root = (C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+  const double root = vs * ((C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0));
   const double xh = x + ux * root, yh = y + uy * root, zh = z + uz * root;
   const double xr = (xh - g.x_grid_min) / g.dx;
   const double rr = (sqrt(yh * yh + zh * zh) - g.y_grid_min) / g.dy;
@@ -359,12 +361,12 @@ __global__ __launch_bounds__(256) void k_charge_half(Grid g, P p, long n, int nt
 struct ChargeOp64 {
   Grid g;
   PartF64 p;
-  double mc, q;
+  double mc, q, vs;
   __device__ bool stencil(long i, int& cx0, int& cy0, double w[3][3], double* f) const {
     double x, y, z, pm[3], wt;
     load_particle(g, p, i, x, y, z, pm, wt, mc);
     const double ux = pm[0] / mc, uy = pm[1] / mc, uz = pm[2] / mc;
-    const double root = (C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+    const double root = vs * ((C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0));
     const double xh = x + ux * root, yh = y + uy * root, zh = z + uz * root;
     const double xr = (xh - g.x_grid_min) / g.dx;
     const double rr = (sqrt(yh * yh + zh * zh) - g.y_grid_min) / g.dy;
@@ -431,6 +433,55 @@ __global__ __launch_bounds__(256) void k_continuity_m0(Grid g, const double* __r
   }
 }
 
+// Gauss's law for m = 0 on the cells of the continuity check.  A step
+// changes E by -dt (J_prev + J) / (2 eps0) plus a curl whose net flux
+// vanishes, J_prev and J carrying the charge between the half-step positions
+// of two successive pushes, so eps0 (net flux of Ex, Er out of the cell) - Q
+// keeps its value from step to step, Q the mean of the charges at
+// x - v dt/2 (qa) and x + v dt/2 (qb) between steps.  out[0..2]: max |that
+// difference|, max |Q|, max |eps0 net flux| (bit patterns, as above).
+template <class T>
+__global__ __launch_bounds__(256) void k_gauss_m0(Grid g, const double* __restrict__ qa, const double* __restrict__ qb,
+                                                  const cplx<T>* __restrict__ ex,
+                                                  const cplx<T>* __restrict__ er, Geo geo,
+                                                  unsigned long long* __restrict__ out) {
+  __shared__ double red[3][4];
+  const int nxi = g.nx - 2 * g.ng, nyi = g.ny - 2 * g.ng;
+  const long n = (long)nxi * nyi;
+  double res = 0.0, qq = 0.0, ff = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int cx = g.ng + 1 + (int)(e % nxi), cy = g.ng + 1 + (int)(e / nxi);
+    const int jng = g.ng;
+    const double flux = EPSILON0 * (((double)ex[g.idx(cx + 1, cy, 0)].re - (double)ex[g.idx(cx, cy, 0)].re) /
+                                        geo.inv_area_rt[cy + jng] +
+                                    (double)er[g.idx(cx, cy + 1, 0)].re / geo.inv_area_xt[cy + jng] -
+                                    (double)er[g.idx(cx, cy, 0)].re / geo.inv_area_xt[cy - 1 + jng]);
+    const long k = g.idx(cx, cy, 0);
+    const double qc = 0.5 * (qa[k] + qb[k]);
+    res = fmax(res, fabs(flux - qc));
+    qq = fmax(qq, fabs(qc));
+    ff = fmax(ff, fabs(flux));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    res = fmax(res, __shfl_xor(res, o));
+    qq = fmax(qq, __shfl_xor(qq, o));
+    ff = fmax(ff, __shfl_xor(ff, o));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wv] = res;
+    red[1][wv] = qq;
+    red[2][wv] = ff;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[threadIdx.x][w]);
+    atomicMax(out + threadIdx.x, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
 inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 template <class P>
@@ -484,14 +535,14 @@ void launch_number_density_mixed(const Grid& g, const Tiles& tl, const PartMixed
 }
 
 void launch_charge_half_f64(const Grid& g, const Tiles& tl, const PartF64& p, const int* tile_start, ExtraRange xr,
-                            long n, double mass, double q, double* out, hipStream_t s) {
+                            long n, double mass, double q, double vs, double* out, hipStream_t s) {
   if (n <= 0) return;
   const size_t lds = (size_t)(tl.tx + 4) * (tl.tr + 4) * sizeof(double);
   if (tile_start && lds <= 65536)
-    launch_tile_scatter<1>(g, tl, ChargeOp64{g, p, mass * C_LIGHT, q}, p.key, tile_start, xr, out, s);
+    launch_tile_scatter<1>(g, tl, ChargeOp64{g, p, mass * C_LIGHT, q, vs}, p.key, tile_start, xr, out, s);
   else
     hipLaunchKernelGGL(k_charge_half<PartF64>, dim3(nblk(n)), dim3(256), 0, s, g, p, n, tl.ntiles, mass * C_LIGHT, q,
-                       out);
+                       vs, out);
 }
 template <class T>
 void launch_continuity_m0(const Grid& g, const double* q0, const double* q1, const cplx<T>* jx, const cplx<T>* jr,
@@ -501,6 +552,17 @@ void launch_continuity_m0(const Grid& g, const double* q0, const double* q1, con
     hipLaunchKernelGGL(k_continuity_m0<T>, dim3(std::min(nblk(n), 1024u)), dim3(256), 0, s, g, q0, q1, jx, jr, geo,
                        out);
 }
+template <class T>
+void launch_gauss_m0(const Grid& g, const double* qa, const double* qb, const cplx<T>* ex, const cplx<T>* er,
+                     const Geo& geo, unsigned long long* out, hipStream_t s) {
+  const long n = (long)std::max(0, g.nx - 2 * g.ng) * std::max(0, g.ny - 2 * g.ng);
+  if (n > 0)
+    hipLaunchKernelGGL(k_gauss_m0<T>, dim3(std::min(nblk(n), 1024u)), dim3(256), 0, s, g, qa, qb, ex, er, geo, out);
+}
+template void launch_gauss_m0<float>(const Grid&, const double*, const double*, const cplx<float>*,
+                                     const cplx<float>*, const Geo&, unsigned long long*, hipStream_t);
+template void launch_gauss_m0<double>(const Grid&, const double*, const double*, const cplx<double>*,
+                                      const cplx<double>*, const Geo&, unsigned long long*, hipStream_t);
 template void launch_continuity_m0<float>(const Grid&, const double*, const double*, const cplx<float>*,
                                           const cplx<float>*, const Geo&, unsigned long long*, hipStream_t);
 template void launch_continuity_m0<double>(const Grid&, const double*, const double*, const cplx<double>*,

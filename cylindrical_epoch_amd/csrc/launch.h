@@ -180,17 +180,25 @@ void launch_number_density_f64(const Grid&, const Tiles&, const PartF64&, const 
 void launch_number_density_mixed(const Grid&, const Tiles&, const PartMixed&, const int* tile_start, ExtraRange,
                                  long n, cplx<double>* out, hipStream_t);
 void launch_density_bcs(const Grid&, cplx<double>* a, hipStream_t);
-// continuity check: the m = 0 charge at x + v dt/2 on the staggered grid
-// (one plane, zeroed by the caller; particles with key >= ntiles skipped),
-// then the residual maxima (out[3], bit patterns, zeroed by the caller)
+// continuity check: the m = 0 charge at x + vs v dt/2 on the staggered grid
+// (one plane, zeroed by the caller; particles with key >= ntiles skipped;
+// vs = 1 for the continuity check, -1 for the half step behind, 0 for a
+// species that does not move), then the residual maxima (out[3], bit
+// patterns, zeroed by the caller)
 void launch_charge_half_f64(const Grid&, const Tiles&, const PartF64&, const int* tile_start, ExtraRange, long n,
-                            double mass, double q, double* out, hipStream_t);
+                            double mass, double q, double vs, double* out, hipStream_t);
 // the mixed path's charge with the push's own FP32 half step (push_mixed.hip)
 void launch_charge_half_mixed(const Grid&, const Tiles&, const PartMixed&, const int* tile_start, ExtraRange,
-                              long n, double charge, double* out, hipStream_t);
+                              long n, double charge, double vs, double* out, hipStream_t);
 template <class T>
 void launch_continuity_m0(const Grid&, const double* q0, const double* q1, const cplx<T>* jx, const cplx<T>* jr,
                           const Geo& geo, unsigned long long* out, hipStream_t);
+// Gauss's law for m = 0 on the same cells: max |eps0 (net flux of Ex, Er out
+// of the cell) - (qa + qb) / 2|, max |(qa + qb) / 2|, max |eps0 net flux|
+// (out[3] as above)
+template <class T>
+void launch_gauss_m0(const Grid&, const double* qa, const double* qb, const cplx<T>* ex, const cplx<T>* er,
+                     const Geo& geo, unsigned long long* out, hipStream_t);
 
 // Distribution functions: c_dir_* numbering (constants.F90:231-247)
 enum {

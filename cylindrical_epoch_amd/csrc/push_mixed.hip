@@ -702,10 +702,10 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
 // weights (the charge whose motion the deposit's currents carry), summed
 // in FP64 into one plane
 __global__ __launch_bounds__(256) void k_charge_half_mixed(Grid g, MixConst mc, PartMixed sp, long n, int ntiles,
-                                                           double qfac, double* __restrict__ out) {
+                                                           double qfac, float vs, double* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || (unsigned)sp.key[i] >= (unsigned)ntiles) return;
-  const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], sp.ux[i], sp.uy[i], sp.uz[i]);
+  const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], vs * sp.ux[i], vs * sp.uy[i], vs * sp.uz[i]);
   const int cx2 = sp.cell[i] + (int)hs.fl2x, cy2 = (int)hs.fl2y + 1;
   if (cx2 - 1 < 1 - g.ng || cx2 + 1 > g.nx + g.ng || cy2 - 1 < 1 - g.ng || cy2 + 1 > g.ny + g.ng) return;
   float hx[3], hy[3];
@@ -812,8 +812,9 @@ struct ChargeOpMixed {
   MixConst mc;
   PartMixed sp;
   double qfac;
+  float vs;  // the charge at x + vs v dt/2 (0: at x, a species that does not move)
   __device__ bool stencil(long i, int& cx0, int& cy0, double w[3][3], double* f) const {
-    const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], sp.ux[i], sp.uy[i], sp.uz[i]);
+    const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], vs * sp.ux[i], vs * sp.uy[i], vs * sp.uz[i]);
     const int cx2 = sp.cell[i] + (int)hs.fl2x, cy2 = (int)hs.fl2y + 1;
     if (cx2 - 1 < 1 - g.ng || cx2 + 1 > g.nx + g.ng || cy2 - 1 < 1 - g.ng || cy2 + 1 > g.ny + g.ng) return false;
     float hx[3], hy[3];
@@ -833,16 +834,17 @@ struct ChargeOpMixed {
 };
 
 void launch_charge_half_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, const int* tile_start,
-                              ExtraRange xr, long n, double charge, double* out, hipStream_t s) {
+                              ExtraRange xr, long n, double charge, double vsd, double* out, hipStream_t s) {
   if (n <= 0) return;
   // only the half-step constants of mix_const are used
   const MixConst mc = mix_const(g, PushConst{}, charge);
+  const float vs = (float)vsd;
   const size_t lds = (size_t)(tl.tx + 4) * (tl.tr + 4) * sizeof(double);
   if (tile_start && lds <= 65536)
-    launch_tile_scatter<1>(g, tl, ChargeOpMixed{g, mc, sp, charge * 0.25}, sp.key, tile_start, xr, out, s);
+    launch_tile_scatter<1>(g, tl, ChargeOpMixed{g, mc, sp, charge * 0.25, vs}, sp.key, tile_start, xr, out, s);
   else
     hipLaunchKernelGGL(k_charge_half_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, mc, sp, n,
-                       tl.ntiles, charge * 0.25, out);
+                       tl.ntiles, charge * 0.25, vs, out);
 }
 
 int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& pc, double charge,

@@ -213,6 +213,15 @@ class Simulation:
         check(lib().cylpic_continuity_end(self.h, C.byref(r), C.byref(dq), C.byref(q)))
         return r.value, dq.value, q.value
 
+    def gauss_residual_m0(self):
+        """Between steps: (max |eps0 net flux of Ex, Er - Q|, max |Q|, max
+        |eps0 net flux|) of the m = 0 Gauss law over the cells at least ng
+        from every edge, in coulombs (Q: the mean of the mobile species'
+        charges at x - v dt/2 and x + v dt/2, immobile ones at x)."""
+        r, q, f = C.c_double(), C.c_double(), C.c_double()
+        check(lib().cylpic_gauss_residual_m0(self.h, C.byref(r), C.byref(q), C.byref(f)))
+        return r.value, q.value, f.value
+
     def autotune(self, pushes=2):
         """Pick the fastest tile shape and gather source for the current
         particles (timed pushes, state restored); returns (tile_x, tile_r,

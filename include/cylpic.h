@@ -242,6 +242,19 @@ enum {
 int cylpic_continuity_begin(cylpic_ctx *h);
 int cylpic_continuity_end(cylpic_ctx *h, double *residual, double *dq, double *q);
 
+/* Gauss's law for m = 0 (no counterpart in the reference; the quantity its
+ * charge-conserving deposit keeps, src/particles.F90:508-665, with the E
+ * updates of src/fields.f90:53-113, 316-353, which apply the previous and
+ * the new current for half a step each).  Called between steps: Q is the
+ * mean of the m = 0 charges of every mobile species at x - v dt/2 and at
+ * x + v dt/2 (staggered quadratic-spline fractions, as in the continuity
+ * check) plus every immobile species at x; returns max |eps0 (net Ex, Er
+ * flux out of the cell) - Q|, max |Q| and max |eps0 net flux| in coulombs
+ * over the cells at least ng from every edge.  The first keeps its initial
+ * value (zero for a neutral start at rest with E = 0 and J = 0) while the
+ * charge stays away from the edges. */
+int cylpic_gauss_residual_m0(cylpic_ctx *h, double *residual, double *q, double *flux);
+
 int cylpic_dist_fn(cylpic_ctx *h, int species, int ndims, const int *direction, double *ranges,
                    int *resolution, const double *restrictions, const int *use_restrictions,
                    double *out, int64_t capacity, int64_t *nbins);

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "cylpic.h")).read()
-    return sorted(set(re.findall(r"\b(cylpic_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(cylpic_[a-z0-9_]+)\s*\(", txt)))
 
 
 @pytest.fixture(scope="module")
