@@ -3,7 +3,6 @@ and mixed: the pulse from the x_min laser enters at the source's amplitude
 and polarisation, delayed by x / c, and leaves through the x_max outflow
 with under 1 % reflected; the r_max outflow absorbs a cylindrical pulse
 that a conducting r_max keeps."""
-import numpy as np
 import pytest
 
 from cylindrical_epoch_amd import deck
