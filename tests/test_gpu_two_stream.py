@@ -22,6 +22,7 @@ import pytest
 from cylindrical_epoch_amd import deck
 from cylindrical_epoch_amd.host import Simulation
 from gpu_helpers import log
+from plasma_waves import em_wave, langmuir
 
 pytestmark = pytest.mark.gpu
 
@@ -67,99 +68,50 @@ def test_two_stream_growth_rate(precision):
     sim.close()
 
 
+def _device(precision):
+    def make(nx, ny, M, dx, dy, dt, bc_field, bc_particle):
+        sim = Simulation(nx, ny, M, dx, dy, dt, bc_field=bc_field, bc_particle=bc_particle, precision=precision)
+        _made.append(sim)
+        return sim
+    return make
+
+
+_made = []
+
+
+@pytest.fixture(autouse=True)
+def _close_sims():
+    yield
+    while _made:
+        _made.pop().close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_langmuir_frequency(precision):
-    """A cold electron plasma on immobile ions (neutral, E = 0), the
-    electrons given u_x = 1e-3 sin(k x) with k = 2 pi / (80 dx), uniform in
-    r, oscillates at the plasma frequency w_p = sqrt(n e^2 / (eps0 m)): the
-    E_x amplitude at that wavenumber changes sign every half period.
-    Shape-function and time-step corrections are below 0.2 % here; the
-    bound is 1 %.  A wrong factor in the current, the charge or eps0 moves
-    the frequency by its square root."""
-    C, Q0, M0 = deck.C_LIGHT, deck.Q0, deck.M0
-    kx = 2 * math.pi / (80 * DX)
-    n = 2.3e21
-    w_p = math.sqrt(n * Q0 ** 2 / (EPS0 * M0))
-    dt = deck.set_dt(DX, DX, 0.95)
-    log("Langmuir", precision, f"w_p dt = {w_p * dt:.2e}")
-    ny = 32
-    sim = Simulation(NX, ny, 1, DX, DX, dt, bc_field=("periodic", "periodic", "other", "conduct"),
-                     bc_particle=("periodic", "periodic", "other", "reflect"), precision=precision)
-    p = deck.uniform_plasma(NX, ny, DX, DX, 16, n, 0.0, seed=21)
-    ions = dict(p)
-    e = dict(p)
-    e["px"] = 1e-3 * M0 * C * np.sin(kx * p["x"])
-    for sp, q, m, still in ((e, -Q0, M0, False), (ions, Q0, 1836.15 * M0, True)):
-        sid = sim.set_species(q, m, still, capacity=len(sp["x"]))
-        sim.upload_particles(sid, sp["x"], sp["y"], sp["z"], sp["px"], sp["py"], sp["pz"], sp["w"])
-    ng, ts, sig, c0 = sim.ng, [], [], None
-    for step in range(int(3.2 * 2 * math.pi / (w_p * dt))):
-        sim.step()
-        if step % 10 == 9:
-            ex = sim.download_field("ex")[0][ng - 1 + 10, ng:ng + NX].real
-            c = np.fft.rfft(ex)[5]
-            c0 = c if c0 is None else c0
-            sig.append((c * np.conj(c0)).real)
-            ts.append((step + 1) * dt)
-    sig, ts = np.array(sig), np.array(ts)
-    cross = [ts[i] - sig[i] * (ts[i + 1] - ts[i]) / (sig[i + 1] - sig[i])
-             for i in range(len(sig) - 1) if sig[i] * sig[i + 1] < 0]
-    assert len(cross) >= 5
-    half = np.diff(cross).mean()
-    got = math.pi / half
-    log(f"  measured {got / w_p:.4f} w_p over {len(cross) - 1} half periods")
+    """Cold electrons on immobile ions (neutral, E = 0), u_x = 1e-3 sin(k x)
+    with k = 2 pi / (80 dx) on 400 x 32 cells, oscillate at the plasma
+    frequency w_p = sqrt(n e^2 / (eps0 m)) to 1 % (measured 0.998; the r_max
+    boundary and the axis pull it down by about 1 / ny^2).  A wrong factor in
+    the current, the charge or eps0 moves the frequency by its square root."""
+    got, w_p, halves = langmuir(_device(precision), nx=NX, ny=32, ppc=16, nwaves=5)
+    log(f"Langmuir {precision}: measured {got / w_p:.4f} w_p over {halves} half periods")
+    assert halves >= 5
     assert got == pytest.approx(w_p, rel=0.01)
-    sim.close()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_em_wave_in_plasma(precision):
     """An electromagnetic standing wave in a cold plasma (electrons on
-    immobile ions, n chosen so that w_p = c k): the m = 1 field uniform
-    across the axis, E along y, E_r,1 = E0 cos(k x), E_theta,1 = -i E0 cos(k x)
-    with k = 2 pi / (16 dx), drives the m = 1 currents of the deposit and
-    their axis rows.  It oscillates at
+    immobile ions, w_p = c k): the m = 1 field uniform across the axis, E
+    along y, with k = 2 pi / (16 dx), drives the m = 1 currents of the
+    deposit and their axis rows.  It oscillates at
     (2 / dt)^2 sin^2(w dt / 2) = w_p^2 cos^2(w dt / 2) + c^2 k_d^2,
     k_d = (2 / dx) sin(k dx / 2) the Yee wavenumber; the cos^2 comes from the
     step applying the previous and the new current for half a step each
     (without it the frequency would be 0.7 % higher here).  Bound 0.2 %, on a
     row that r_max has not reached.  Without the plasma's current the wave
     would oscillate at c k_d, 30 % slower."""
-    C, Q0, M0 = deck.C_LIGHT, deck.Q0, deck.M0
-    nx, ny, dx = 16, 96, 1e-7
-    k = 2 * math.pi / (nx * dx)
-    w_p = C * k
-    n = w_p ** 2 * EPS0 * M0 / Q0 ** 2
-    dt = deck.set_dt(dx, dx, 0.95)
-    kd = 2.0 / dx * math.sin(k * dx / 2)
-    want = 2.0 / dt * math.asin(math.sqrt((w_p ** 2 + (C * kd) ** 2) / ((2.0 / dt) ** 2 + w_p ** 2)))
-    log("EM wave in plasma", precision, f"w dt = {want * dt:.3f}")
-    sim = Simulation(nx, ny, 2, dx, dx, dt, bc_field=("periodic", "periodic", "other", "conduct"),
-                     bc_particle=("periodic", "periodic", "other", "reflect"), precision=precision)
-    p = deck.uniform_plasma(nx, ny, dx, dx, 8, n, 0.0, seed=23)
-    for q, m, still in ((-Q0, M0, False), (Q0, 1836.15 * M0, True)):
-        sid = sim.set_species(q, m, still, capacity=len(p["x"]))
-        sim.upload_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
-    ng = sim.ng
-    x = sim.x_min + dx / 2 + (np.arange(1 - ng, nx + ng + 1) - 1) * dx
-    E0 = 1.0e8
-    for name, ph in (("er", 1.0), ("et", -1j)):
-        f = np.zeros(sim.shape, complex)
-        f[1][:, :] = ph * E0 * np.cos(k * x)[None, :]
-        sim.upload_field(name, f)
-    sim.setup_field_boundaries()
-    ts, sig, c0 = [], [], None
-    for step in range(95):
-        sim.step()
-        c = np.fft.fft(sim.download_field("er")[1][ng - 1 + 3, ng:ng + nx].real)[1]
-        c0 = c if c0 is None else c0
-        sig.append((c * np.conj(c0)).real)
-        ts.append((step + 1) * dt)
-    sig, ts = np.array(sig), np.array(ts)
-    cross = [ts[i] - sig[i] * (ts[i + 1] - ts[i]) / (sig[i + 1] - sig[i])
-             for i in range(len(sig) - 1) if sig[i] * sig[i + 1] < 0]
-    assert len(cross) >= 6
-    got = math.pi / np.diff(cross).mean()
-    log(f"  measured {got / want:.4f} of the expected frequency ({len(cross) - 1} half periods)")
+    got, want, halves = em_wave(_device(precision))
+    log(f"EM wave {precision}: measured {got / want:.4f} of the expected frequency ({halves} half periods)")
+    assert halves >= 6
     assert got == pytest.approx(want, rel=0.002)
-    sim.close()
