@@ -28,6 +28,17 @@
 #define PI 3.141592653589793238462643383279503                /* constants.F90:172 */
 #define C_TINY 2.2250738585072014e-308                        /* TINY(1.0_num) */
 
+/* Planted mistakes, for checking the checks: built with -DORACLE_PLANT=k
+ * (oracle/Makefile target `plant`), mistake k replaces the restated term by
+ * a plausible slip (a flipped sign, a dropped or swapped term, a wrong
+ * factor).  tests/test_oracle_planted.py holds the table of mistakes and
+ * requires each one to fail a committed test.  The default build (0) is the
+ * restatement itself. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+#define PLANT(k, good, bad) ((ORACLE_PLANT == (k)) ? (bad) : (good))
+
 /* ------------------------------------------------------------------ */
 /* indexing                                                           */
 /* ------------------------------------------------------------------ */
@@ -96,9 +107,9 @@ void orc_update_e_field(const orc_grid *g, orc_fields *f) {
     for (int ir = 1; ir <= ny; ++ir) {
       double r_d = fabs((double)(ir - 1) * dy + g->y_grid_min);
       double r_p = r_d + 0.5 * dy;
-      double fac_x = c2 / r_p;
+      double fac_x = c2 / PLANT(5, r_p, r_d);
       double a_x = (double)m * fac_x;       /* im_fac_x = (0, a_x) */
-      double a_r = ((double)m * c2) / r_d;  /* im_fac_r = (0, a_r) */
+      double a_r = ((double)m * c2) / PLANT(6, r_d, r_p);  /* im_fac_r = (0, a_r) */
       for (int ix = 0; ix <= nx; ++ix) {
         size_t k = I3(g, ix, ir, m);
         size_t kr1 = I3(g, ix, ir + 1, m);
@@ -132,10 +143,10 @@ void orc_update_e_field(const orc_grid *g, orc_fields *f) {
     {
       const int m = 0;
       size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m);
-      ocplx s = csub(rmul((4.0 * c2) / dy, bt[k1]), cdivr(jx[k0], EPSILON0));
+      ocplx s = csub(rmul((PLANT(1, 4.0, 2.0) * c2) / dy, bt[k1]), cdivr(jx[k0], EPSILON0));
       ex[k0] = cadd(ex[k0], cmulr(cmulr(s, 0.5), dt));
       et[k0] = CZERO;
-      er[k0] = cneg(er[k1]);
+      er[k0] = PLANT(4, cneg(er[k1]), er[k1]);
       for (int ir = 1 - ng; ir <= -1; ++ir) {
         et[I3(g, ix, ir, m)] = cneg(et[I3(g, ix, -ir, m)]);
         er[I3(g, ix, ir, m)] = cneg(er[I3(g, ix, -ir + 1, m)]);
@@ -147,9 +158,14 @@ void orc_update_e_field(const orc_grid *g, orc_fields *f) {
       const int m = 1;
       size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m), k2 = I3(g, ix, 2, m);
       ex[k0] = CZERO;
+#if ORACLE_PLANT != 3
       er[k0] = csub(imul(2.0, et[k0]), er[k1]);
+#endif
       ocplx w = csub(rmul(9.0, er[k1]), er[k2]);
-      et[k0] = cc(0.125 * w.im, -(0.125 * w.re));
+      et[k0] = PLANT(2, cc(0.125 * w.im, -(0.125 * w.re)), cc(-(0.125 * w.im), 0.125 * w.re));
+#if ORACLE_PLANT == 3
+      er[k0] = csub(imul(2.0, et[k0]), er[k1]);
+#endif
       for (int ir = 1 - ng; ir <= -1; ++ir) {
         et[I3(g, ix, ir, m)] = et[I3(g, ix, -ir, m)];
         er[I3(g, ix, ir, m)] = er[I3(g, ix, -ir + 1, m)];
@@ -163,7 +179,7 @@ void orc_update_e_field(const orc_grid *g, orc_fields *f) {
       ex[k0] = CZERO;
       et[k0] = CZERO;
       er[k0] = cneg(er[k1]);
-      er[k1] = cdivr(er[k2], 9.0);
+      er[k1] = cdivr(er[k2], PLANT(7, 9.0, 4.0));
       for (int ir = 1 - ng; ir <= -1; ++ir) {
         et[I3(g, ix, ir, m)] = cneg(rmul(mode_sign, et[I3(g, ix, -ir, m)]));
         er[I3(g, ix, ir, m)] = cneg(rmul(mode_sign, er[I3(g, ix, -ir + 1, m)]));
@@ -219,7 +235,7 @@ void orc_update_b_field(const orc_grid *g, orc_fields *f) {
       const int m = 0;
       size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m);
       br[k0] = CZERO;
-      bx[k0] = bx[k1];
+      bx[k0] = PLANT(8, bx[k1], cneg(bx[k1]));
       bt[k0] = cneg(bt[k1]);
       for (int ir = 1 - ng; ir <= -1; ++ir) {
         bt[I3(g, ix, ir, m)] = cneg(bt[I3(g, ix, -ir + 1, m)]);
@@ -234,10 +250,10 @@ void orc_update_b_field(const orc_grid *g, orc_fields *f) {
       if (ix >= 2 - ng) {  /* (22) */
         ocplx t1 = imul(idy, ex[k0]);
         ocplx t2 = cdivr(csub(et[k0], et[I3(g, ix - 1, 0, m)]), dx);
-        ocplx s = cadd(t1, t2);
+        ocplx s = PLANT(10, cadd(t1, t2), csub(t1, t2));
         br[k0] = cadd(br[k0], cmulr(cmulr(s, 0.5), dt));
       }
-      bt[k0] = csub(imul(-2.0, br[k0]), bt[k1]);
+      bt[k0] = PLANT(9, csub(imul(-2.0, br[k0]), bt[k1]), cadd(imul(-2.0, br[k0]), bt[k1]));
       for (int ir = 1 - ng; ir <= -1; ++ir) {
         bt[I3(g, ix, ir, m)] = bt[I3(g, ix, -ir + 1, m)];
         br[I3(g, ix, ir, m)] = br[I3(g, ix, -ir, m)];
@@ -411,7 +427,7 @@ void orc_setup_field_boundaries(const orc_grid *g, orc_fields *f) {
 /* The m=1 plane-wave sources of laser.f90:476-489: source_t = s1 + i s2,
  * source_r = -i s1 + s2. */
 static inline ocplx src_t(const double *src, int ny, int ir) {
-  return cc(src[ir], src[ny + 1 + ir]);
+  return cc(src[ir], PLANT(21, src[ny + 1 + ir], -src[ny + 1 + ir]));
 }
 static inline ocplx src_r(const double *src, int ny, int ir) {
   return cc(src[ny + 1 + ir], -src[ir]);
@@ -432,13 +448,13 @@ void orc_outflow_bcs_x_min(const orc_grid *g, orc_fields *f, const double *src) 
   for (int m = 0; m < M; ++m) {
     for (int ir = 1; ir <= ny; ++ir) {
       ocplx st = (m == 1 && src) ? src_t(src, ny, ir) : CZERO;
-      double r_d = fabs((double)(ir - 1 - 1) * g->dy + g->y_grid_min);
-      ocplx a = rmul(4.0, st);
+      double r_d = fabs((double)(ir - 1 - PLANT(29, 1, 0)) * g->dy + g->y_grid_min);
+      ocplx a = rmul(PLANT(20, 4.0, 2.0), st);
       ocplx b = rmul(2.0, cadd(f->bnd_xmin[1][I2(g, ir, m)], rmul(c, f->bnd_xmin[5][I2(g, ir, m)])));
       ocplx d = rmul(2.0, f->er[I3(g, 1, ir, m)]);
       ocplx e = cdivr(imul(((double)m * (c * c)) * dt, f->bx[I3(g, 1, ir, m)]), r_d);
       ocplx h = rmul(dt_eps, f->jr[I3(g, 1, ir, m)]);
-      ocplx k = rmul(diff, f->bt[I3(g, 2, ir, m)]);
+      ocplx k = rmul(PLANT(22, diff, -diff), f->bt[I3(g, 2, ir, m)]);
       f->bt[I3(g, 1, ir, m)] = rmul(sum, cadd(cadd(cadd(csub(cadd(a, b), d), e), h), k));
     }
     const int ir_l = 1, ir_h = ny - 1;
@@ -446,7 +462,7 @@ void orc_outflow_bcs_x_min(const orc_grid *g, orc_fields *f, const double *src) 
       ocplx sr = (m == 1 && src) ? src_r(src, ny, ir) : CZERO;
       ocplx a = rmul(-4.0, sr);
       ocplx b = rmul(2.0, cadd(f->bnd_xmin[2][I2(g, ir, m)], rmul(c, f->bnd_xmin[4][I2(g, ir, m)])));
-      ocplx d = rmul(2.0, f->et[I3(g, 1, ir, m)]);
+      ocplx d = rmul(2.0, f->et[I3(g, PLANT(23, 1, 2), ir, m)]);
       ocplx e = rmul(lr, csub(f->bx[I3(g, 1, ir + 1, m)], f->bx[I3(g, 1, ir, m)]));
       ocplx h = rmul(dt_eps, f->jt[I3(g, 1, ir, m)]);
       ocplx k = rmul(diff, f->br[I3(g, 2, ir, m)]);
@@ -461,7 +477,7 @@ void orc_outflow_bcs_x_max(const orc_grid *g, orc_fields *f, const double *src) 
   const int nx = g->nx, ny = g->ny, M = g->n_mode;
   const double c = C_LIGHT, dt = g->dt;
   const double dtc2 = dt * (c * c), lx = dtc2 / g->dx, lr = dtc2 / g->dy;
-  const double sum = 1.0 / (lx + c), diff = lx - c, dt_eps = dt / EPSILON0;
+  const double sum = 1.0 / PLANT(24, lx + c, lx), diff = lx - c, dt_eps = dt / EPSILON0;
   for (int m = 0; m < M; ++m)
     for (int ir = 0; ir <= ny; ++ir) f->bx[I3(g, nx, ir, m)] = f->bnd_xmax[3][I2(g, ir, m)];
   for (int m = 0; m < M; ++m) {
@@ -505,8 +521,8 @@ void orc_outflow_bcs_r_max(const orc_grid *g, orc_fields *f) {
   for (int m = 0; m < M; ++m) {
     for (int ix = ix_l; ix <= ix_h; ++ix) {
       ocplx a = cneg(cmulr(f->bx[I3(g, ix, ny - 1, m)], c - ly));
-      ocplx b = cmulr(f->bx_old[I3(g, ix, ny, m)], -c + ly);
-      ocplx d = cmulr(f->bx_old[I3(g, ix, ny - 1, m)], -c - ly);
+      ocplx b = cmulr(PLANT(27, f->bx_old, f->bx)[I3(g, ix, ny, m)], -c + ly);
+      ocplx d = cmulr(f->bx_old[I3(g, ix, ny - 1, m)], PLANT(28, -c - ly, c + ly));
       ocplx e = rmul((c * dt) * inv_r, f->et[I3(g, ix, ny - 1, m)]);
       ocplx h = rmul(0.5 * lx,
                      csub(cadd(csub(f->br[I3(g, ix + 1, ny - 1, m)], f->br[I3(g, ix, ny - 1, m)]),
@@ -515,13 +531,13 @@ void orc_outflow_bcs_r_max(const orc_grid *g, orc_fields *f) {
       ocplx k = imul(icdt * (double)m, cadd(f->er[I3(g, ix, ny, m)], f->er[I3(g, ix, ny - 1, m)]));
       ocplx l = rmul(dt_2eps, cadd(f->jt[I3(g, ix, ny - 1, m)], f->jt_old[I3(g, ix, ny - 1, m)]));
       f->bx[I3(g, ix, ny, m)] =
-          rmul(sum_x, csub(csub(cadd(csub(csub(csub(a, b), d), e), h), k), l));
+          rmul(PLANT(25, sum_x, sum_t), csub(csub(cadd(csub(csub(csub(a, b), d), e), h), k), l));
     }
     for (int ix = 1; ix <= nx; ++ix) {
       ocplx a = cneg(cmulr(f->bt[I3(g, ix, ny - 1, m)], (c - ly) + dtc2_4r));
-      ocplx b = cmulr(f->bt_old[I3(g, ix, ny, m)], (-c + ly) + dtc2_4r);
+      ocplx b = cmulr(PLANT(30, f->bt_old, f->bt)[I3(g, ix, ny, m)], (-c + ly) + dtc2_4r);
       ocplx d = cmulr(f->bt_old[I3(g, ix, ny - 1, m)], (-c - ly) + dtc2_4r);
-      ocplx e = rmul((0.5 * lx) / c,
+      ocplx e = rmul(PLANT(26, 1.0, -1.0) * ((0.5 * lx) / c),
                      csub(csub(cadd(f->er[I3(g, ix, ny, m)], f->er[I3(g, ix, ny - 1, m)]),
                                f->er[I3(g, ix - 1, ny, m)]),
                           f->er[I3(g, ix - 1, ny - 1, m)]));
@@ -621,6 +637,7 @@ static void reflect_current(const orc_grid *g, ocplx *J, int m, int flip_dir) {
         if (flip_dir == 2) {
           double num = r_max + ((double)i - 0.5) * g->dy;
           double den = r_max - ((double)i - 0.5) * g->dy;
+          if (ORACLE_PLANT == 19) { double t = num; num = den; den = t; }
           J[I3(g, ix, ny + 1 - i, m)] =
               csub(J[I3(g, ix, ny + 1 - i, m)], cdivr(cmulr(J[I3(g, ix, ny + i, m)], num), den));
         } else if (flip_dir == 1) {
@@ -645,6 +662,7 @@ static void periodic_current(const orc_grid *g, ocplx *J, int m) {
   for (int j = 1 - ng; j <= ny + ng; ++j)
     for (int i = 1; i <= ng; ++i)
       J[I3(g, i, j, m)] = cadd(J[I3(g, i, j, m)], J[I3(g, nx + i, j, m)]);
+  if (ORACLE_PLANT == 18) return;
   for (int j = 1 - ng; j <= ny + ng; ++j)
     for (int i = 1; i <= ng; ++i)
       J[I3(g, nx - ng + i, j, m)] = cadd(J[I3(g, nx - ng + i, j, m)], J[I3(g, i - ng, j, m)]);
@@ -671,24 +689,27 @@ void orc_current_bcs_r_min_final(const orc_grid *g, orc_fields *f) {
     for (int j = 2; j <= ng; ++j)
       for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
         size_t a = I3(g, ix, 1 - j, m);
-        jx[I3(g, ix, j - 1, m)] = cadd(jx[I3(g, ix, j - 1, m)], rmul(mode_sign, jx[a]));
+        jx[I3(g, ix, j - 1, m)] = PLANT(12, cadd(jx[I3(g, ix, j - 1, m)], rmul(mode_sign, jx[a])),
+                                        csub(jx[I3(g, ix, j - 1, m)], rmul(mode_sign, jx[a])));
         jr[I3(g, ix, j, m)] = csub(jr[I3(g, ix, j, m)], rmul(mode_sign, jr[a]));
-        jt[I3(g, ix, j - 1, m)] = csub(jt[I3(g, ix, j - 1, m)], rmul(mode_sign, jt[a]));
+        jt[I3(g, ix, j - 1, m)] = PLANT(11, csub(jt[I3(g, ix, j - 1, m)], rmul(mode_sign, jt[a])),
+                                        cadd(jt[I3(g, ix, j - 1, m)], rmul(mode_sign, jt[a])));
         jx[a] = CZERO; jr[a] = CZERO; jt[a] = CZERO;
       }
     for (int ix = 1 - ng; ix <= nx + ng; ++ix)
-      jr[I3(g, ix, 1, m)] = csub(jr[I3(g, ix, 1, m)], rmul(mode_sign, jr[I3(g, ix, 0, m)]));
-    mode_sign = -mode_sign;
+      if (ORACLE_PLANT != 13)
+        jr[I3(g, ix, 1, m)] = csub(jr[I3(g, ix, 1, m)], rmul(mode_sign, jr[I3(g, ix, 0, m)]));
+    mode_sign = PLANT(17, -mode_sign, mode_sign);
   }
   for (int m = 0; m < M; ++m)
     for (int ix = 1 - ng; ix <= nx + ng; ++ix) {
       size_t k0 = I3(g, ix, 0, m), k1 = I3(g, ix, 1, m), k2 = I3(g, ix, 2, m);
       if (m > 0) jx[k0] = CZERO;
-      else jx[k0] = cdivr(csub(rmul(4.0, jx[k1]), jx[k2]), 3.0);
+      else jx[k0] = cdivr(csub(rmul(4.0, jx[k1]), jx[k2]), PLANT(14, 3.0, 4.0));
       if (m == 1) {
         ocplx w = csub(rmul(9.0, jr[k1]), jr[k2]);
-        jt[k0] = cc(w.im / 8.0, -(w.re / 8.0));
-        jr[k0] = csub(imul(2.0, jt[k0]), jr[k1]);
+        jt[k0] = PLANT(15, cc(w.im / 8.0, -(w.re / 8.0)), cc(-(w.im / 8.0), w.re / 8.0));
+        jr[k0] = PLANT(16, csub(imul(2.0, jt[k0]), jr[k1]), cadd(imul(2.0, jt[k0]), jr[k1]));
       } else {
         jt[k0] = CZERO;
         jr[k0] = cneg(jr[k1]);
@@ -737,7 +758,7 @@ static void smooth_mode_array(const orc_grid *g, ocplx *array, int its, int comp
         for (int iy = 1; iy <= ny; ++iy)
           for (int ix = 1; ix <= nx; ++ix) wk[I3(g, ix, iy, m)] = array[I3(g, ix, iy, m)];
     }
-    if (it > its) alpha = (double)its * 0.5 + 1.0;
+    if (it > PLANT(43, its, its - 1)) alpha = (double)its * 0.5 + 1.0;
   }
   free(wk);
 }
@@ -820,15 +841,15 @@ void orc_deposit_from_weights(const orc_grid *g, orc_fields *f,
       exp_imtheta0 = cc(1.0, 0.0);
       exp_imdtheta = cc(1.0, 0.0);
     } else {
-      exp_imtheta0 = cmul(exp_imtheta0, exp_itheta_05);
+      exp_imtheta0 = cmul(exp_imtheta0, PLANT(35, exp_itheta_05, cc(exp_itheta_05.re, -exp_itheta_05.im)));
       exp_imdtheta = cmul(exp_imdtheta, exp_idtheta);
       if (small_angle_change) {
-        m_fac_1 = rmul(2.0, exp_imtheta0);
+        m_fac_1 = rmul(PLANT(34, 2.0, 1.0), exp_imtheta0);
         m_fac_2 = cmul(m_fac_1, cc(1.0 - sixth * m2dth2, 0.5 * mdth));
-        m_fac_3 = cmul(m_fac_1, cc(0.5 - 0.125 * m2dth2, third * mdth));
+        m_fac_3 = cmul(m_fac_1, cc(0.5 - 0.125 * m2dth2, PLANT(36, third, 0.25) * mdth));
         m_fac_4 = cmul(m_fac_1, cc(third - 0.1 * m2dth2, 0.25 * mdth));
       } else {
-        m_fac_1 = rmul(2.0 * inv_mdth, exp_imtheta0);
+        m_fac_1 = rmul(PLANT(34, 2.0, 1.0) * inv_mdth, exp_imtheta0);
         m_fac_2 = cmul(m_fac_1, cc(exp_imdtheta.im, -(exp_imdtheta.re - 1.0)));
         ocplx t = cmul(exp_imdtheta, cc(1.0, -mdth));
         m_fac_3 = cmul(m_fac_1, rmul(inv_mdth, cc(t.re - 1.0, t.im)));
@@ -861,7 +882,7 @@ void orc_deposit_from_weights(const orc_grid *g, orc_fields *f,
         else w_xt = cadd(cmulr(m_fac_2, gx[ix]), cmulr(m_fac_3, hx[ix]));
         ocplx w_xr = cadd(rmul(gx[ix], w_rt), rmul(hx[ix], ym_fac_1));
         jxh = csub(jxh, rmul(fjx * hx[ix], w_rt));
-        jyh[ix] = csub(cmulr(jyh[ix], ratio_area_xt[cy + jng]), rmul(fjy, w_xt));
+        jyh[ix] = csub(cmulr(jyh[ix], ratio_area_xt[cy + jng]), rmul(PLANT(38, fjy, -fjy), w_xt));
         ocplx jzh = rmul(fjz, w_xr);
         size_t kx = I3(g, cx + 1, cy, im), kr = I3(g, cx, cy + 1, im), kt = I3(g, cx, cy, im);
         f->jx[kx] = cadd(f->jx[kx], jxh);
@@ -979,7 +1000,7 @@ void orc_push_particles(const orc_grid *g, orc_fields *f, orc_species *sp, int n
         et_part = et_part + re_mul(e_m, patch(g, f->et, cell_x2, cell_y2, im, hx, hy));
         e_m = cmul(e_m, exp_min_itheta);
       }
-      double ey_part = er_part * exp_min_itheta.re + et_part * exp_min_itheta.im;
+      double ey_part = er_part * exp_min_itheta.re + PLANT(37, et_part, -et_part) * exp_min_itheta.im;
       double ez_part = -(er_part * exp_min_itheta.im) + et_part * exp_min_itheta.re;
       e_m = cc(1.0, 0.0);
       double bx_part = 0.0, br_part = 0.0, bt_part = 0.0;
@@ -996,18 +1017,19 @@ void orc_push_particles(const orc_grid *g, orc_fields *f, orc_species *sp, int n
       double uxm = part_ux + cmratio * ex_part;
       double uym = part_uy + cmratio * ey_part;
       double uzm = part_uz + cmratio * ez_part;
-      gamma_rel = sqrt(((uxm * uxm + uym * uym) + uzm * uzm) + 1.0);
+      gamma_rel = PLANT(31, sqrt(((uxm * uxm + uym * uym) + uzm * uzm) + 1.0),
+                        sqrt(((part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz) + 1.0));
       root = ccmratio / gamma_rel;
       double taux = bx_part * root, tauy = by_part * root, tauz = bz_part * root;
       double taux2 = taux * taux, tauy2 = tauy * tauy, tauz2 = tauz * tauz;
       double tau = 1.0 / (((1.0 + taux2) + tauy2) + tauz2);
       double uxp = ((((1.0 + taux2) - tauy2) - tauz2) * uxm +
-                    2.0 * ((taux * tauy + tauz) * uym + (taux * tauz - tauy) * uzm)) * tau;
+                    2.0 * ((taux * tauy + PLANT(33, tauz, -tauz)) * uym + (taux * tauz - tauy) * uzm)) * tau;
       double uyp = ((((1.0 - taux2) + tauy2) - tauz2) * uym +
                     2.0 * ((tauy * tauz + taux) * uzm + (tauy * taux - tauz) * uxm)) * tau;
       double uzp = ((((1.0 - taux2) - tauy2) + tauz2) * uzm +
                     2.0 * ((tauz * taux + tauy) * uxm + (tauz * tauy - taux) * uym)) * tau;
-      part_ux = uxp + cmratio * ex_part;
+      part_ux = uxp + PLANT(32, cmratio, -cmratio) * ex_part;
       part_uy = uyp + cmratio * ey_part;
       part_uz = uzp + cmratio * ez_part;
 
@@ -1133,6 +1155,7 @@ void orc_particle_bcs(const orc_grid *g, orc_species *sp, int n_species) {
           double sin_theta = s->z[ip] * inv_final_r;
           double part_pr = s->py[ip] * cos_theta + s->pz[ip] * sin_theta;
           double part_pt = -(s->py[ip] * sin_theta) + s->pz[ip] * cos_theta;
+          if (ORACLE_PLANT == 39) part_pr = -part_pr;
           s->py[ip] = -(part_pr * cos_theta) - part_pt * sin_theta;
           s->pz[ip] = -(part_pr * sin_theta) + part_pt * cos_theta;
         }
@@ -1174,7 +1197,7 @@ void orc_cell_keys(const orc_grid *g, int64_t n, const double *x, const double *
   const int ntx = (nx + shift + tile_x - 1) / tile_x;
   for (int64_t i = 0; i < n; ++i) {
     double r = sqrt(y[i] * y[i] + z[i] * z[i]);
-    int cx = (int)floor((x[i] - g->x_grid_min) / g->dx + 1.5);
+    int cx = (int)floor((x[i] - g->x_grid_min) / g->dx + PLANT(40, 1.5, 1.0));
     int cy = (int)floor((r - g->y_grid_min) / g->dy + 1.5);
     /* a particle on the domain edge within rounding maps to the edge cell */
     cx = cx < 1 ? 1 : (cx > nx ? nx : cx);
@@ -1214,7 +1237,7 @@ double orc_set_dt(double dx, double dy, double dt_multiplier,
 void orc_mode_energy(const orc_grid *g, const orc_fields *f, double *w_e, double *w_b) {
   const double mu0 = 4.0e-7 * PI;
   for (int m = 0; m < g->n_mode; ++m) {
-    double kappa = (m == 0) ? 2.0 * PI : PI;
+    double kappa = (m == 0) ? PLANT(46, 2.0 * PI, PI) : PI;
     double se = 0.0, sb = 0.0;
     for (int ir = 1; ir <= g->ny; ++ir) {
       double r_c = ((double)ir - 0.5) * g->dy;   /* er, bx, bt */
@@ -1258,7 +1281,8 @@ static void shift_fields(const orc_grid *g, orc_fields *f) {
       f->ex[I3(g, nx + 1, j, m)] = f->bnd_xmax[0][s];
       f->er[I3(g, nx, j, m)] = f->bnd_xmax[1][s];
       f->et[I3(g, nx, j, m)] = f->bnd_xmax[2][s];
-      f->ex[I3(g, nx, j, m)] = rmul(0.5, cadd(f->ex[I3(g, nx - 1, j, m)], f->ex[I3(g, nx + 1, j, m)]));
+      f->ex[I3(g, nx, j, m)] = PLANT(41, rmul(0.5, cadd(f->ex[I3(g, nx - 1, j, m)], f->ex[I3(g, nx + 1, j, m)])),
+                                     f->ex[I3(g, nx - 1, j, m)]);
       f->er[I3(g, nx - 1, j, m)] = rmul(0.5, cadd(f->er[I3(g, nx - 2, j, m)], f->er[I3(g, nx, j, m)]));
       f->et[I3(g, nx - 1, j, m)] = rmul(0.5, cadd(f->et[I3(g, nx - 2, j, m)], f->et[I3(g, nx, j, m)]));
       f->bx[I3(g, nx, j, m)] = f->bnd_xmax[3][s];
@@ -1391,7 +1415,7 @@ void orc_number_density_modes(const orc_grid *g, const orc_species *sp, int n_sp
       gy[0] = 0.5 * ((0.25 + cy2) + fy);
       gy[1] = 0.75 - cy2;
       gy[2] = 0.5 * ((0.25 + cy2) - fy);
-      if (part_r < g->dy) {
+      if (part_r < PLANT(44, g->dy, 0.5 * g->dy)) {
         gy[1] = gy[1] + gy[0];
         gy[0] = 0.0;
       }

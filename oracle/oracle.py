@@ -16,7 +16,12 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "build", "libcylpic_oracle.so")
+# CYLPIC_ORACLE_PLANT=k loads the oracle built with planted mistake k
+# (oracle/Makefile `plant`), for tests/test_oracle_planted.py, which checks
+# that every planted mistake fails a committed test.
+PLANT = int(os.environ.get("CYLPIC_ORACLE_PLANT", "0") or 0)
+_LIB_PATH = (os.path.join(_HERE, "build", "plant", f"libcylpic_oracle_{PLANT}.so") if PLANT
+             else os.path.join(_HERE, "build", "libcylpic_oracle.so"))
 _lib = None
 
 BC = dict(periodic=1, other=2, simple_laser=3, simple_outflow=4, open=5,
@@ -25,7 +30,10 @@ BC = dict(periodic=1, other=2, simple_laser=3, simple_outflow=4, open=5,
 
 
 def build():
-    subprocess.check_call(["make", "-s", "-C", _HERE])
+    if PLANT:
+        subprocess.check_call(["make", "-s", "-C", _HERE, "plant", f"PLANTS={PLANT}"])
+    else:
+        subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
 def lib():

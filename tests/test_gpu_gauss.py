@@ -73,7 +73,7 @@ def test_gauss_entry_matches_numpy(oracle):
     want = numpy_gauss(st, sim.download_field("ex"), sim.download_field("er"), species)
     log("gauss entry", got, want)
     for a, b in zip(got, want):
-        assert a == pytest.approx(b, rel=1e-12)
+        assert a == pytest.approx(b, rel=1e-12, abs=0)
     sim.close()
 
 

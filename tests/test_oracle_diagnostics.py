@@ -7,7 +7,11 @@ independently of the reference's text:
   2 e^{+i m theta} (m > 0); the periodic and reflecting ghost sums keep that
   total inside the domain; a uniform plasma gives its density in m = 0;
 * the distribution function (dist_fn.F90:118-715): numpy's histogram over
-  the same edges, gamma - 1 computed directly, arctan2 for the angles."""
+  the same edges, gamma - 1 computed directly, arctan2 for the angles;
+* the per-mode field energies: a uniform field's energy in the cylinder.
+No GPU needed."""
+import math
+
 import numpy as np
 import pytest
 
@@ -190,3 +194,35 @@ def test_empty_species_and_bad_direction(oracle):
     st2, _ = _thermal(oracle, n=10)
     with pytest.raises(ValueError):
         st2.dist_fn(0, [12], resolution=[10])
+
+
+@pytest.mark.parametrize("case", ["ex0", "e_perp1", "bx0"])
+def test_mode_energy_of_a_uniform_field(oracle, case):
+    """A uniform field of strength F fills the cylinder r < R = ny dr of
+    length L = nx dx with energy (eps0/2) F^2 pi R^2 L (electric) or
+    F^2 / (2 mu0) pi R^2 L (magnetic): along the axis it is the m = 0 mode
+    F_x,0 = F, across it (E along y) the m = 1 pair E_r,1 = F,
+    E_theta,1 = -i F, whose theta average of Re[F_m e^{-i theta}]^2 is F^2/2.
+    The per-mode energies (the r dr dx sums over each component's own
+    staggered radius) reach it up to the outermost half row, a relative
+    1/ny at most."""
+    nx, ny = 12, 40
+    st = oracle.State(nx, ny, 2, 1e-7, 1e-7, 1e-17)
+    F = 3.0e9
+    mu0 = 4.0e-7 * math.pi
+    EPS0, C_LIGHT = 8.854187817620389850536563031710750e-12, C
+    if case == "ex0":
+        st.arr["ex"][0][...] = F
+        want, got_m = EPS0 / 2 * F * F, 0
+    elif case == "e_perp1":
+        st.arr["er"][1][...] = F
+        st.arr["et"][1][...] = -1j * F
+        want, got_m = EPS0 / 2 * F * F, 1
+    else:
+        st.arr["bx"][0][...] = F / C_LIGHT
+        want, got_m = (F / C_LIGHT) ** 2 / (2 * mu0), 0
+    want *= math.pi * (ny * 1e-7) ** 2 * nx * 1e-7
+    we, wb = st.mode_energy()
+    got = (we if case[0] == "e" else wb)[got_m]
+    assert abs(got / want - 1.0) <= 1.0 / ny, got / want
+    assert not (we if case[0] == "e" else wb)[1 - got_m]

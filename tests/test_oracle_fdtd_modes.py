@@ -149,7 +149,7 @@ def mode_errors(make, kind, m, kdy, steps=20, skip=()):
     return worst, st
 
 
-SECOND_ORDER = [("TM", 0), ("TM", 1), ("TM", 2), ("TE", 0), ("TE", 1)]
+SECOND_ORDER = [("TM", 0), ("TM", 1), ("TM", 2), ("TM", 3), ("TE", 0), ("TE", 1), ("TE", 3)]
 
 
 # rows that do not follow the continuum (module docstring): the m = 1 B_r
@@ -238,6 +238,70 @@ def test_te2_first_row_rule(oracle):
     for name in ("er", "et", "bx"):
         assert fine[name] < 1e-2, (name, fine[name])
         assert coarse[name] / fine[name] > 2.0, (name, coarse[name], fine[name])
+
+
+def test_te3_first_er_row(oracle):
+    """TE_3: near the axis E_r,3 grows like r^2 (J_3(k r) / r), so setting
+    E_r's first row (r = dr/2) to a ninth of the second (r = 3 dr / 2), the
+    reference's rule for every m > 1 (fields.f90:165), is the continuum's
+    own ratio there.  The row then follows the mode within 2e-5 of the E
+    amplitude at k dr = 0.075, and its error falls about four-fold when k dr
+    halves; a ratio of a quarter leaves 8e-5."""
+    make = _oracle_grid(oracle)
+    errs = []
+    for kdy in (0.15, 0.075):
+        st = make(NX, NY, 4)
+        ng, dt = st.ng, st.dt
+        k, A = kdy / DY, 1.0e9 / C
+        j = np.arange(1 - ng, NY + ng + 1)
+        init = {n: np.zeros(st.shape, complex) for n in ("ex", "er", "et", "bx", "br", "bt")}
+        init["bx"][3][:, :] = _mode("TE", 3, k, (j - 0.5) * DY, 0.0, A)["bx"][:, None]
+        st.load(init)
+        worst = 0.0
+        for n in range(1, 21):
+            st.step()
+            want = _mode("TE", 3, k, np.array([0.5 * DY]), n * dt, A)["er"][0]
+            worst = max(worst, abs(st.get("er")[3][ng, ng] - want) / 1.0e9)
+        errs.append(worst)
+    assert errs[1] < 2e-5, errs
+    assert errs[0] / errs[1] > 3.5, errs
+
+
+def test_m1_er_reflected_row_lags_etheta_axis_row(oracle):
+    """The m = 1 E_r row at r = -dr/2 is set from E_theta's axis value
+    before that value is updated (fields.f90:144-147, in this order).  In the
+    uniform standing wave E along y (E_r,1 = E_y, E_theta,1 = -i E_y, rows
+    1.. exact, see the plane-wave test) the row therefore holds
+    2 E_y(previous half update) - E_y(now) after each E update, where the
+    other order would give E_y(now).  The two differ here by about 5 % of
+    the amplitude per half update."""
+    st = OracleGrid(oracle, 16, 48, 2)
+    ng, nx = st.ng, 16
+    k = 2 * np.pi / (nx * DX)
+    x = st.x_grid_min + (np.arange(1 - ng, nx + ng + 1) - 1) * DX
+    E0 = 1.0e9
+    init = {n: np.zeros(st.shape, complex) for n in ("ex", "er", "et", "bx", "br", "bt")}
+    init["er"][1][:, :] = E0 * np.cos(k * x)[None, :]
+    init["et"][1][:, :] = -1j * E0 * np.cos(k * x)[None, :]
+    st.load(init)
+    cols = slice(ng, ng + nx)
+    o = st.st
+    prev = o.arr["er"][1][ng, cols].copy()          # row 1 = E_y before the first update
+    lag = 0.0
+    for n in range(12):
+        for half in ("half", "final"):
+            if half == "half":
+                o.update_eb_fields_half()
+            else:
+                o.push_particles()
+                o.current_finish()
+                o.update_eb_fields_final()
+            now = o.arr["er"][1][ng, cols].copy()
+            row0 = o.arr["er"][1][ng - 1, cols]
+            assert np.abs(row0 - (2 * prev - now)).max() < 1e-13 * E0, (n, half)
+            lag = max(lag, np.abs(now - prev).max())
+            prev = now
+    assert lag > 0.03 * E0
 
 
 def test_m1_axis_br_stays_zero_without_x_dependence(oracle):

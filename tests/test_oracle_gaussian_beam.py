@@ -68,7 +68,7 @@ def axis_peak(grid):
 
 def check_focus(peak, x):
     i = int(np.argmax(peak))
-    assert peak[i] == pytest.approx(focal_field(), rel=0.03), peak[i]
+    assert peak[i] == pytest.approx(focal_field(), rel=0.03, abs=0), peak[i]
     assert 7e-6 < x[i] < 11e-6, x[i]
 
 

@@ -25,9 +25,9 @@ def _one_particle(st, x, r, theta, p=(0.0, 0.0, 0.0), q=-Q0, m=M0, w=1.0):
 # ---------------------------------------------------------------- set_dt
 def test_set_dt_closed_form(oracle):
     dx, dy = 0.02e-6, 0.1e-6
-    assert oracle.set_dt(dx, dy, 0.95) == pytest.approx(0.95 * 0.9 * dx / math.sqrt(2) / C_LIGHT, rel=1e-15)
-    assert oracle.set_dt(dx, dy, 1.0, dt_plasma=1e-18) == pytest.approx(1e-18, rel=1e-15)
-    assert oracle.set_dt(dx, dy, 1.0, dt_laser=2e-18) == pytest.approx(2e-18, rel=1e-15)
+    assert oracle.set_dt(dx, dy, 0.95) == pytest.approx(0.95 * 0.9 * dx / math.sqrt(2) / C_LIGHT, rel=1e-15, abs=0)
+    assert oracle.set_dt(dx, dy, 1.0, dt_plasma=1e-18) == pytest.approx(1e-18, rel=1e-15, abs=0)
+    assert oracle.set_dt(dx, dy, 1.0, dt_laser=2e-18) == pytest.approx(2e-18, rel=1e-15, abs=0)
 
 
 # ---------------------------------------------------------------- push
@@ -39,7 +39,7 @@ def test_uniform_ex_mode0_kick(oracle, theta):
     st.arr["ex"][0] = E0
     sp = _one_particle(st, 1.23e-6, 0.77e-6, theta)
     st.push_particles()
-    assert sp["px"][0] == pytest.approx(-Q0 * E0 * st.g.dt, rel=1e-13)
+    assert sp["px"][0] == pytest.approx(-Q0 * E0 * st.g.dt, rel=1e-13, abs=0)
     assert abs(sp["py"][0]) < 1e-40 and abs(sp["pz"][0]) < 1e-40
 
 
@@ -70,8 +70,8 @@ def test_radial_azimuthal_to_cartesian(oracle, comp):
         want = (k * math.cos(th), k * math.sin(th))
     else:
         want = (-k * math.sin(th), k * math.cos(th))
-    assert sp["py"][0] == pytest.approx(want[0], rel=1e-12)
-    assert sp["pz"][0] == pytest.approx(want[1], rel=1e-12)
+    assert sp["py"][0] == pytest.approx(want[0], rel=1e-12, abs=0)
+    assert sp["pz"][0] == pytest.approx(want[1], rel=1e-12, abs=0)
 
 
 def test_boris_rotation_uniform_bx(oracle):
@@ -84,13 +84,13 @@ def test_boris_rotation_uniform_bx(oracle):
     sp = _one_particle(st, 1.2e-6, 0.8e-6, 0.2, p=p0)
     st.push_particles()
     p1 = np.array([sp["px"][0], sp["py"][0], sp["pz"][0]])
-    assert np.linalg.norm(p1) == pytest.approx(np.linalg.norm(p0), rel=1e-14)
-    assert p1[0] == pytest.approx(p0[0], rel=1e-14)
+    assert np.linalg.norm(p1) == pytest.approx(np.linalg.norm(p0), rel=1e-14, abs=0)
+    assert p1[0] == pytest.approx(p0[0], rel=1e-14, abs=0)
     gamma = math.sqrt(1 + (np.linalg.norm(p0) / (M0 * C_LIGHT)) ** 2)
     ang = 2 * math.atan(abs(Q0 * B0 * st.g.dt / (2 * gamma * M0)))
     got = math.atan2(p0[1] * p1[2] - p0[2] * p1[1], p0[1] * p1[1] + p0[2] * p1[2])
     # electron (q<0) in +Bx rotates counter-clockwise in (y,z)
-    assert got == pytest.approx(ang, rel=1e-10)
+    assert got == pytest.approx(ang, rel=1e-10, abs=0)
 
 
 @pytest.mark.parametrize("vd", [0.05, 0.3])
@@ -122,7 +122,7 @@ def test_e_cross_b_drift(oracle, vd):
         st.push_particles()
         u = np.array([sp["px"][0], sp["py"][0], sp["pz"][0]]) / (M0 * C_LIGHT)
         dev = max(dev, np.abs(u - u0).max() / u0[0])
-    assert dev == pytest.approx(eps * eps, rel=0.05)
+    assert dev == pytest.approx(eps * eps, rel=0.05, abs=0)
     L = st.g.x_max - st.g.x_min
     travelled = (sp["x"][0] - x0) % L
     dist = vd * C_LIGHT * n * st.g.dt
@@ -555,12 +555,15 @@ def test_particle_bcs_periodic_and_reflect(oracle):
     ids = list(sp["id"])
     # staying particle first, then x_min leavers, then x_max leavers
     assert ids == [2, 0, 1]
-    assert sp["x"][1] == pytest.approx(x[0] + L)
-    assert sp["x"][2] == pytest.approx(x[1] - L)
-    # r_max reflection: radius mirrored about y_max, radial momentum reversed
-    assert sp["y"][0] == pytest.approx(g.y_max - 0.25 * g.dy, rel=1e-14)
-    assert sp["py"][0] == pytest.approx(-p[1], rel=1e-14)
-    assert sp["pz"][0] == pytest.approx(p[2], rel=1e-14)
+    assert sp["x"][1] == pytest.approx(x[0] + L, abs=0)
+    assert sp["x"][2] == pytest.approx(x[1] - L, abs=0)
+    # r_max reflection of particle 2 (at theta = 0, every momentum component
+    # p[2]): radius mirrored about y_max, radial momentum (p_y) reversed,
+    # azimuthal (p_z) and axial kept
+    assert sp["y"][0] == pytest.approx(g.y_max - 0.25 * g.dy, rel=1e-14, abs=0)
+    assert sp["py"][0] == pytest.approx(-p[2], rel=1e-14, abs=0)
+    assert sp["pz"][0] == pytest.approx(p[2], rel=1e-14, abs=0)
+    assert sp["px"][0] == p[2]
 
 
 @pytest.mark.parametrize("theta", [0.7, 2.4, -1.9])
@@ -579,13 +582,13 @@ def test_rmax_reflection_at_any_angle(oracle, theta):
     oracle.lib().orc_particle_bcs(oracle.C.byref(st.g), arr, 1)
     st._species_back(arr)
     y, z = sp["y"][0], sp["z"][0]
-    assert math.hypot(y, z) == pytest.approx(g.y_max - 0.3 * g.dy, rel=1e-13)
+    assert math.hypot(y, z) == pytest.approx(g.y_max - 0.3 * g.dy, rel=1e-13, abs=0)
     assert math.atan2(z, y) == pytest.approx(theta, abs=1e-13)
     py, pz = sp["py"][0], sp["pz"][0]
-    assert py * c + pz * s == pytest.approx(-pr, rel=1e-12)
-    assert -py * s + pz * c == pytest.approx(pt, rel=1e-12)
+    assert py * c + pz * s == pytest.approx(-pr, rel=1e-12, abs=0)
+    assert -py * s + pz * c == pytest.approx(pt, rel=1e-12, abs=0)
     assert sp["px"][0] == p[0]
-    assert math.hypot(py, pz) == pytest.approx(math.hypot(p[1], p[2]), rel=1e-14)
+    assert math.hypot(py, pz) == pytest.approx(math.hypot(p[1], p[2]), rel=1e-14, abs=0)
 
 
 def test_mode_factor_sensitivity(oracle):

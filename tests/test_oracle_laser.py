@@ -132,3 +132,125 @@ def test_rmax_outflow_absorbs(oracle, m, comp):
         return OracleBox(oracle, nx, ny, M, ybc)
     assert rmax_energy_left(make, m, comp, "simple_outflow") < 0.02
     assert rmax_energy_left(make, m, comp, "conduct") > 0.95
+
+
+# ------------------------------------------------- r_max outflow: static fields
+def static_field_run(oracle, steps=200, ny=40):
+    """Loads a uniform B_x,0 on every row (ghosts included) with an outflow
+    r_max and periodic x, steps it, and returns the largest change of any
+    component on rows 0..ny relative to the field."""
+    dt = oracle.set_dt(DX, DY, 0.95)
+    st = oracle.State(8, ny, 2, DX, DY, dt, bc_field=("periodic", "periodic", "other", "simple_outflow"),
+                      bc_particle=("periodic", "periodic", "other", "open"))
+    B0 = 1.0e9 / C
+    st.arr["bx"][0][:, :] = B0
+    st.setup_field_boundaries()
+    start = {n: st.arr[n].copy() for n in ("ex", "er", "et", "bx", "br", "bt")}
+    ng = st.g.ng
+    rows, cols = slice(ng - 1, ng + ny), slice(ng, ng + 8)
+    worst = 0.0
+    for _ in range(steps):
+        st.step()
+        for n, a in start.items():
+            scale = B0 if n[0] == "b" else B0 * C
+            worst = max(worst, np.abs(st.arr[n][:, rows, cols] - a[:, rows, cols]).max() / scale)
+    return worst
+
+
+def test_rmax_outflow_keeps_a_uniform_axial_b(oracle):
+    """A uniform B_x,0 (a solenoid's field) is a static vacuum solution, and
+    the r_max outflow's B_x update (laser.f90:664-673) is a discretised
+    one-way wave equation, so it must hold it: its four B_x terms sum to
+    (c + ly) B0 = B0 / sum_x, and the E_theta, B_r, E_r and J terms vanish.
+    Every row 0..ny of every component stays at its value to round-off over
+    200 steps.  (A uniform transverse B is not static here: the clamp of
+    the generic field boundary zeroes B_r on the r_max row, boundary.F90:
+    1449-1476.)"""
+    assert static_field_run(oracle) < 1e-13
+
+
+def test_rmax_boundary_rows_hold_their_previous_value_when_updated(oracle):
+    """update_b_field runs to row ny - 1 on a rank with r_max
+    (fields.f90:199), so row ny of B_x and B_theta is written only by the
+    boundaries.  When outflow_bcs_r_max runs, bxm(1:nx-1, ny) therefore
+    still equals its *_old copy taken in update_eb_fields_half
+    (fields.f90:316-337), and the reference's bxm_old(:, ny) term could read
+    the live array with the same result.  For btm(1:nx, ny) that holds only
+    while no x boundary writes the corner (nx, ny) first: with an x_max
+    outflow it does (test_rmax_outflow_time_levels)."""
+    dt = oracle.set_dt(DX, DY, 0.95)
+    ny = 30
+    st = oracle.State(8, ny, 2, DX, DY, dt, bc_field=("periodic", "periodic", "other", "simple_outflow"),
+                      bc_particle=("periodic", "periodic", "other", "open"))
+    rng = np.random.default_rng(3)
+    for n in ("ex", "er", "et", "bx", "br", "bt"):
+        st.arr[n][...] = (rng.standard_normal(st.shape) + 1j * rng.standard_normal(st.shape)) * (
+            1e9 if n[0] == "e" else 1e9 / C)
+    st.setup_field_boundaries()
+    ng = st.g.ng
+    cols = slice(ng, ng + 8)
+    for _ in range(3):
+        st.update_eb_fields_half()
+        st.push_particles()
+        st.current_finish()
+        st.call("orc_update_b_field")          # the first part of update_eb_fields_final
+        for n in ("bx", "bt"):
+            assert np.array_equal(st.arr[n][:, ny + ng - 1, cols], st.arr[n + "_old"][:, ny + ng - 1, cols])
+        st.call("orc_bfield_final_bcs", None, None)
+        st.call("orc_update_e_field")
+        st.call("orc_efield_bcs")
+
+
+def test_rmax_outflow_time_levels(oracle):
+    """The r_max outflow (laser.f90:637-690) is the one-way wave equation
+    discretised between the start of the step (t^n: the *_old copies taken
+    in update_eb_fields_half) and its end (t^{n+1}), with E at t^{n+1/2}.
+    For m = 0 and no current, on every column it updates,
+
+      (c + ly) Bx_N' + (c - ly) Bx_N-1' + (ly - c) Bx_N + (-c - ly) Bx_N-1
+          + c dt/r E_theta,N-1 - lx/2 (dBr' + dBr) = 0,
+      (c + ly + d) Bt_N' + (c - ly + d) Bt_N-1' + (ly - c + d) Bt_N
+          + (-c - ly + d) Bt_N-1 + lx/(2c) dEr = 0,
+
+    primes at t^{n+1}, d = dt c^2 / (4 r), dBr and dEr the x differences
+    of B_r on row N-1 and of the two-row sum of E_r.  Checked at every
+    column with outflow boundaries on x_min, x_max and r_max, so the corner
+    (nx, N), which the x_max boundary writes first in the same call, must
+    use its value from the start of the step as well."""
+    nx, ny, dx, dy = 24, 20, 1e-7, 1e-7
+    dt = oracle.set_dt(dx, dy, 0.95)
+    st = oracle.State(nx, ny, 1, dx, dy, dt, bc_field=("simple_outflow", "simple_outflow", "other", "simple_outflow"),
+                      bc_particle=("open", "open", "other", "open"))
+    ng = st.g.ng
+    j = np.arange(1 - ng, ny + ng + 1)
+    i = np.arange(1 - ng, nx + ng + 1)
+    r = j * dy
+    st.arr["ex"][0][:, :] = 1e9 * np.exp(-((r[:, None] - 15 * dy) / (3 * dy)) ** 2) * np.cos(0.2 * i)[None, :]
+    st.arr["bx"][0][:, :] = 1e9 / C * np.exp(-((r[:, None] - 14.5 * dy) / (3 * dy)) ** 2) * np.sin(0.3 * i)[None, :]
+    st.setup_field_boundaries()
+    c, lx, ly = C, dt * C * C / dx, dt * C * C / dy
+    inv_r = 1.0 / ((ny - 1.5) * dy + st.g.y_grid_min)
+    d = 0.25 * dt * C * C * inv_r
+    N, Nm = ny + ng - 1, ny + ng - 2          # array rows of r-index ny and ny - 1
+    col = lambda a, b: slice(a + ng - 1, b + ng)
+    for _ in range(12):
+        old = {n: st.arr[n][0].copy() for n in ("bx", "br", "bt")}
+        st.update_eb_fields_half()
+        half = {n: st.arr[n][0].copy() for n in ("er", "et")}
+        st.push_particles()
+        st.current_finish()
+        st.update_eb_fields_final()
+        new = {n: st.arr[n][0] for n in ("bx", "br", "bt")}
+        bx = col(1, nx - 1)
+        bxp = col(2, nx)
+        res_x = ((c + ly) * new["bx"][N, bx] + (c - ly) * new["bx"][Nm, bx] + (ly - c) * old["bx"][N, bx]
+                 + (-c - ly) * old["bx"][Nm, bx] + c * dt * inv_r * half["et"][Nm, bx]
+                 - 0.5 * lx * (new["br"][Nm, bxp] - new["br"][Nm, bx] + old["br"][Nm, bxp] - old["br"][Nm, bx]))
+        bt, btm1 = col(1, nx), col(0, nx - 1)
+        res_t = ((c + ly + d) * new["bt"][N, bt] + (c - ly + d) * new["bt"][Nm, bt] + (ly - c + d) * old["bt"][N, bt]
+                 + (-c - ly + d) * old["bt"][Nm, bt]
+                 + 0.5 * lx / c * (half["er"][N, bt] + half["er"][Nm, bt] - half["er"][N, btm1] - half["er"][Nm, btm1]))
+        scale = c * np.abs(new["bt"]).max()
+        assert np.abs(res_x).max() < 1e-12 * scale and np.abs(res_t).max() < 1e-12 * scale
+    for n in ("bx", "bt"):
+        assert np.abs(st.arr[n][0][N, col(1, nx)]).max() > 1e-3 * np.abs(st.arr[n][0]).max()

@@ -53,10 +53,10 @@ def _make(oracle):
 def test_langmuir_oracle(oracle):
     got, w_p, halves = langmuir(_make(oracle), nx=80, ny=32, ppc=4, nwaves=1)
     assert halves >= 5
-    assert got == pytest.approx(w_p, rel=0.01)
+    assert got == pytest.approx(w_p, rel=0.01, abs=0)
 
 
 def test_em_wave_in_plasma_oracle(oracle):
     got, want, halves = em_wave(_make(oracle))
     assert halves >= 6
-    assert got == pytest.approx(want, rel=0.002)
+    assert got == pytest.approx(want, rel=0.002, abs=0)
