@@ -668,6 +668,12 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
     return fail(CYLPIC_EINVAL, "precision must be CYLPIC_FP64 or CYLPIC_MIXED");
   if (precision == CYLPIC_MIXED && n_mode > 4) return fail(CYLPIC_EINVAL, "CYLPIC_MIXED supports n_mode <= 4");
   if (!bc_field || !bc_particle) return fail(CYLPIC_EINVAL, "null boundary arrays");
+  // CPML grows the grid by cpml_thickness cells per side and changes the
+  // particle extents and reflection at that edge (mpi_routines.F90:280-290,
+  // boundary.F90:1561-1569, 1756-1766); none of that is built here
+  for (int i = 0; i < 4; ++i)
+    if (bc_field[i] == BC_CPML_LASER || bc_field[i] == BC_CPML_OUTFLOW)
+      return fail(CYLPIC_EINVAL, "CPML field boundaries (cpml_laser, cpml_outflow) are not supported");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CYLPIC_EHIP, "no HIP device");
   if (device < 0 || device >= ndev) return fail(CYLPIC_EINVAL, "device index out of range");
@@ -803,11 +809,13 @@ int cylpic_set_tile(cylpic_ctx* h, int tile_x, int tile_r) {
 
 int cylpic_tune(cylpic_ctx* h, int acc_mode, int gather_lds) {
   CHECK_H(h);
+  // both arguments are checked before any state changes; a tile shape that
+  // no longer fits in LDS restores both
   if (acc_mode < 0 || acc_mode > 4) return fail(CYLPIC_EINVAL, "acc_mode must be 0..4");
-  const int old_acc = h->acc_mode, old_g = h->gather_lds;
-  h->acc_mode = acc_mode;
   if (gather_lds < 0 || gather_lds > 2 || (gather_lds == 2 && (h->prec != CYLPIC_MIXED || acc_mode != 3)))
     return fail(CYLPIC_EINVAL, "gather_lds must be 0 or 1 (2: E from LDS, mixed path with acc_mode 3 only)");
+  const int old_acc = h->acc_mode, old_g = h->gather_lds;
+  h->acc_mode = acc_mode;
   h->gather_lds = gather_lds;
   int rc = set_tiles(h, h->tl.tx, h->tl.tr);
   if (rc) {
@@ -1466,9 +1474,8 @@ int cylpic_dist_fn(cylpic_ctx* h, int species, int ndims, const int* direction, 
   return check_device_errors(h);
 }
 
-// the m = 0 charge of every mobile species at x + v dt/2 into q
-// The m = 0 charge of the mobile species at x + vs v dt/2; with `immobile`,
-// immobile species too, at x.
+// The m = 0 charge of the mobile species at x + vs v dt/2 into q; with
+// `immobile`, immobile species too, at x.
 static int charge_half(cylpic_ctx* h, double* q, bool immobile = false, double vs = 1.0) {
   HIPCHK(hipMemsetAsync(q, 0, h->g.plane * sizeof(double), h->stream));
   for (auto& s : h->sp) {

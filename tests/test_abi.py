@@ -50,6 +50,22 @@ def test_init_rejects_bad_arguments_without_gpu(libpath):
     assert not h.value
 
 
+def test_init_rejects_cpml_without_gpu(libpath):
+    """CPML field boundaries would grow the grid and change the particle
+    edges (mpi_routines.F90:280-290); they are refused, not mapped."""
+    from cylindrical_epoch_amd._lib import lib
+    L = lib()
+    h = C.c_void_p()
+    bp = (C.c_int * 4)(5, 5, 2, 5)
+    for code in (12, 13):
+        bf = (C.c_int * 4)(code, 4, 2, 4)
+        rc = L.cylpic_init(C.byref(h), 32, 16, 2, 5, 1e-7, 1e-7, 1e-17, 0.0, 3.2e-6, 0.0, 1.6e-6,
+                           C.cast(bf, C.c_void_p), C.cast(bp, C.c_void_p), 0, 0)
+        assert rc == 1
+        assert b"CPML" in L.cylpic_last_error()
+        assert not h.value
+
+
 def test_null_handle_calls_fail_cleanly(libpath):
     from cylindrical_epoch_amd._lib import lib
     L = lib()

@@ -37,7 +37,7 @@ from scipy import integrate
 
 from cylindrical_epoch_amd import deck
 
-NX, NY, M = 4, 8, 2
+NX, NY, M = 10, 8, 2
 DX = DY = 1.0e-7
 DENSITY = 1.0e24
 SUB_R = 64        # radial sub-steps per cell
@@ -119,10 +119,10 @@ def test_spline_brackets_closed_form():
         assert row_bracket(j, 0, 2) == pytest.approx(j + 0.25 / j, rel=1e-12, abs=0)
 
 
-def check_m1_drift(deposit, tol=3e-5):
-    """J_theta,1 of a transverse drift on rows 1..3 against the bracket;
-    row 1 carries the folded r < 0 ghost row."""
-    vy = 1.0e-4 * deck.C_LIGHT
+def check_m1_drift(deposit, tol=3e-5, beta=1.0e-4):
+    """J_theta,1 of a transverse drift v_y = beta c on rows 1..3 against the
+    bracket; row 1 carries the folded r < 0 ghost row."""
+    vy = beta * deck.C_LIGHT
     p = quiet_load(lambda r, t: (0.0 * r, np.full_like(r, vy), 0.0 * r))
     jt = deposit(p)["jt"]
     nqv = DENSITY * (-deck.Q0) * vy
@@ -132,10 +132,12 @@ def check_m1_drift(deposit, tol=3e-5):
         assert abs(got - want) < tol * abs(nqv), (j, got / (-1j * nqv), row_bracket(j, 1, 1))
 
 
-def check_m0_rotation(deposit, tol=1e-5):
-    """J_theta,0 of a rigid rotation v_theta = omega r on rows 1..3 against
-    the bracket with r^2; row 1 carries the folded ghost row."""
-    omega = 1.0e-4 / DT
+def check_m0_rotation(deposit, tol=1e-5, beta=1.0e-4):
+    """J_theta,0 of a rigid rotation v_theta = omega r (omega dt = beta) on
+    rows 1..3 against the bracket with r^2; row 1 carries the folded ghost
+    row.  The free particles' radii grow by (omega dt)^2 / 2 in a step, which
+    keeps beta small."""
+    omega = beta / DT
     p = quiet_load(lambda r, t: (0.0 * r, -omega * r * np.sin(t), omega * r * np.cos(t)))
     jt = deposit(p)["jt"]
     unit = DENSITY * (-deck.Q0) * omega * DY
@@ -171,7 +173,7 @@ def integral(shape, power, centre, lo, hi):
     return v
 
 
-def check_rmax_fold(deposit, tol=3e-5):
+def check_rmax_fold(deposit, tol=3e-5, beta=1.0e-4):
     """The current rows beyond a reflecting r_max = R fold back
     (particle_reflection_bcs_complex, boundary.F90:986-1013): the face of J_r
     at R + a is the face at R - a of the mirror path, whose radial velocity
@@ -186,7 +188,7 @@ def check_rmax_fold(deposit, tol=3e-5):
 
     with L the hat and S the quadratic spline; rows further in are 1."""
     ny = 16
-    vy = 1.0e-4 * deck.C_LIGHT
+    vy = beta * deck.C_LIGHT
     rk = (np.arange(ny * SUB_R) + 0.5) * (DY / SUB_R)
     th = 2.0 * math.pi * (np.arange(N_THETA) + 0.25) / N_THETA
     xs = (np.arange(2 * NX) + 0.5) * (DX / 2)
