@@ -32,6 +32,7 @@ SIGNATURES = {
     "cylpic_autotune": [_p, _i, _p, _p, _p, _p],
     "cylpic_continuity_end": [_p, _p, _p, _p],
     "cylpic_gauss_residual_m0": [_p, _p, _p, _p],
+    "cylpic_gauss_residual_m0_noalloc": [_p, _p, _p, _p],
     "cylpic_dist_fn": [_p, _i, _i, _p, _p, _p, _p, _p, _p, C.c_int64, _p],
     "cylpic_upload_field": [_p, _i, _p],
     "cylpic_download_field": [_p, _i, _p],

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -643,6 +644,40 @@ int retile(cylpic_ctx* h, int tx, int tr) {
 
 }  // namespace
 
+// ----------------------------------------------------------------------
+// Scratch of cylpic_gauss_residual_m0_noalloc: two m = 0 charge planes of at
+// most NOALLOC_MAX_PLANE cells, then the three result words.  16 MiB held by
+// every device the library runs on until the process ends.  Every call on a
+// device uses its one copy, so such calls that overlap on one device (other
+// threads, other handles' streams) race.  Host code uses the per-device
+// address from hipGetSymbolAddress, never the symbol itself.
+// ----------------------------------------------------------------------
+constexpr long NOALLOC_MAX_PLANE = 1L << 20;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_PLANE + 4];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_DEVICES];
+
+// The scratch array's address on `device`, resolved once per device with
+// that device current (cylpic_init does it first).
+int scratch_address(int device, double** out) {
+  if (device < 0 || device >= SCRATCH_DEVICES)
+    return fail(CYLPIC_EINVAL, "device index beyond the no-allocation scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CYLPIC_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (double*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return CYLPIC_OK;
+}
+}  // namespace
+
 // ======================================================================
 // C ABI
 // ======================================================================
@@ -726,6 +761,11 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
     return rc;
   };
   if (hipSetDevice(device) != hipSuccess) return bail(fail(CYLPIC_EHIP, "hipSetDevice failed"));
+  {
+    double* unused = nullptr;
+    int rc0 = scratch_address(device, &unused);
+    if (rc0) return bail(rc0);
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(CYLPIC_EHIP, "stream creation failed"));
   if (hipDeviceGetAttribute(&h->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
@@ -1475,9 +1515,12 @@ int cylpic_dist_fn(cylpic_ctx* h, int species, int ndims, const int* direction, 
 }
 
 // The m = 0 charge of the mobile species at x + vs v dt/2 into q; with
-// `immobile`, immobile species too, at x.
-static int charge_half(cylpic_ctx* h, double* q, bool immobile = false, double vs = 1.0) {
-  HIPCHK(hipMemsetAsync(q, 0, h->g.plane * sizeof(double), h->stream));
+// `immobile`, immobile species too, at x.  q is cleared first, by a fill
+// kernel when `clear_by_kernel` (the no-allocation entry) or hipMemsetAsync.
+static int charge_half(cylpic_ctx* h, double* q, bool immobile = false, double vs = 1.0,
+                       bool clear_by_kernel = false) {
+  if (clear_by_kernel) launch_zero_f64(q, h->g.plane, h->stream);
+  else HIPCHK(hipMemsetAsync(q, 0, h->g.plane * sizeof(double), h->stream));
   for (auto& s : h->sp) {
     if ((s.immobile && !immobile) || s.n <= 0) continue;
     const int* ts = s.ranged ? s.tile_start : nullptr;
@@ -1544,6 +1587,35 @@ int cylpic_gauss_residual_m0(cylpic_ctx* h, double* residual, double* q, double*
   HIPCHK(hipGetLastError());
   unsigned long long bits[3];
   HIPCHK(hipMemcpyAsync(bits, h->cont_out, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::memcpy(residual, &bits[0], sizeof(double));
+  std::memcpy(q, &bits[1], sizeof(double));
+  std::memcpy(flux, &bits[2], sizeof(double));
+  return check_device_errors(h);
+}
+
+int cylpic_gauss_residual_m0_noalloc(cylpic_ctx* h, double* residual, double* q, double* flux) {
+  CHECK_H(h);
+  if (!residual || !q || !flux) return fail(CYLPIC_EINVAL, "null output");
+  if (h->g.plane > NOALLOC_MAX_PLANE)
+    return fail(CYLPIC_EINVAL, "cylpic_gauss_residual_m0_noalloc: the grid's (nx + 2 ng) (ny + 2 ng) = " +
+                                   std::to_string(h->g.plane) + " cells exceed its scratch of " +
+                                   std::to_string(NOALLOC_MAX_PLANE) + " cells per charge plane");
+  HIPCHK(hipSetDevice(h->device));
+  double* base = nullptr;
+  int rc = scratch_address(h->device, &base);
+  if (rc) return rc;
+  double* qa = base;
+  double* qb = base + NOALLOC_MAX_PLANE;
+  unsigned long long* words = (unsigned long long*)(base + 2 * NOALLOC_MAX_PLANE);
+  rc = charge_half(h, qa, true, -1.0, true);
+  if (!rc) rc = charge_half(h, qb, true, 1.0, true);
+  if (rc) return rc;
+  launch_zero_f64((double*)words, 3, h->stream);
+  with_fields(h, [&](auto& F) { launch_gauss_m0(h->g, qa, qb, F.ex(), F.er(), geo_of(h), words, h->stream); });
+  HIPCHK(hipGetLastError());
+  unsigned long long bits[3];
+  HIPCHK(hipMemcpyAsync(bits, words, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   std::memcpy(residual, &bits[0], sizeof(double));
   std::memcpy(q, &bits[1], sizeof(double));

@@ -552,6 +552,17 @@ void launch_continuity_m0(const Grid& g, const double* q0, const double* q1, con
     hipLaunchKernelGGL(k_continuity_m0<T>, dim3(std::min(nblk(n), 1024u)), dim3(256), 0, s, g, q0, q1, jx, jr, geo,
                        out);
 }
+__global__ void k_zero_f64(double* __restrict__ p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0.0;
+}
+
+void launch_zero_f64(double* p, long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_f64, dim3(std::min(nblk(n), 8192u)), dim3(256), 0, s, p, n);
+}
+
 template <class T>
 void launch_gauss_m0(const Grid& g, const double* qa, const double* qb, const cplx<T>* ex, const cplx<T>* er,
                      const Geo& geo, unsigned long long* out, hipStream_t s) {

@@ -199,6 +199,9 @@ void launch_continuity_m0(const Grid&, const double* q0, const double* q1, const
 template <class T>
 void launch_gauss_m0(const Grid&, const double* qa, const double* qb, const cplx<T>* ex, const cplx<T>* er,
                      const Geo& geo, unsigned long long* out, hipStream_t);
+// n doubles set to zero by a kernel (vector stores), for buffers that must
+// be cleared without hipMemsetAsync
+void launch_zero_f64(double* p, long n, hipStream_t);
 
 // Distribution functions: c_dir_* numbering (constants.F90:231-247)
 enum {

@@ -222,6 +222,13 @@ class Simulation:
         check(lib().cylpic_gauss_residual_m0(self.h, C.byref(r), C.byref(q), C.byref(f)))
         return r.value, q.value, f.value
 
+    def gauss_residual_m0_noalloc(self):
+        """gauss_residual_m0 through the entry point that allocates no
+        device memory (grids of at most 1048576 cells with guards)."""
+        r, q, f = C.c_double(), C.c_double(), C.c_double()
+        check(lib().cylpic_gauss_residual_m0_noalloc(self.h, C.byref(r), C.byref(q), C.byref(f)))
+        return r.value, q.value, f.value
+
     def autotune(self, pushes=2):
         """Pick the fastest tile shape and gather source for the current
         particles (timed pushes, state restored); returns (tile_x, tile_r,

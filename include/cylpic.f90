@@ -240,6 +240,14 @@ MODULE cylpic
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_gauss_residual_m0
 
+    FUNCTION cylpic_gauss_residual_m0_noalloc(h, residual, q, flux) &
+        BIND(C, NAME='cylpic_gauss_residual_m0_noalloc') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(OUT) :: residual, q, flux
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_gauss_residual_m0_noalloc
+
     FUNCTION cylpic_number_density_modes(h, species, out) &
         BIND(C, NAME='cylpic_number_density_modes') RESULT(ierr)
       IMPORT :: C_PTR, C_INT, C_DOUBLE_COMPLEX

@@ -255,6 +255,17 @@ int cylpic_continuity_end(cylpic_ctx *h, double *residual, double *dq, double *q
  * charge stays away from the edges. */
 int cylpic_gauss_residual_m0(cylpic_ctx *h, double *residual, double *q, double *flux);
 
+/* cylpic_gauss_residual_m0 without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and result; the two charge planes and the
+ * result words live in a 16 MiB array that the library holds on each
+ * device until the process ends, so the grid's (nx + 2 ng) (ny + 2 ng) must
+ * not exceed 1048576 cells (CYLPIC_EINVAL otherwise).  Every call on one
+ * device shares that array: calls that overlap on a device (other threads,
+ * other handles) race. */
+int cylpic_gauss_residual_m0_noalloc(cylpic_ctx *h, double *residual, double *q, double *flux);
+
 int cylpic_dist_fn(cylpic_ctx *h, int species, int ndims, const int *direction, double *ranges,
                    int *resolution, const double *restrictions, const int *use_restrictions,
                    double *out, int64_t capacity, int64_t *nbins);
