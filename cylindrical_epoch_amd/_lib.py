@@ -52,6 +52,11 @@ SIGNATURES = {
     "cylpic_synchronize": [_p],
     "cylpic_profile": [_p, _i],
     "cylpic_kernel_time": [_p, C.c_char_p, _p, _p],
+    "cylpic_batch_create": [_pp, _p, _i],
+    "cylpic_batch_step": [_p, _p, _p],
+    "cylpic_batch_synchronize": [_p],
+    "cylpic_batch_stats": [_p, _p, _p, _p],
+    "cylpic_batch_destroy": [_p],
 }
 
 

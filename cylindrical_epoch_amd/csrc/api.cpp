@@ -36,7 +36,8 @@ struct Species {
   double q = 0, m = 0;
   int immobile = 0;
   long cap = 0;
-  long n = 0;             // particles held (upper bound until the next sort)
+  long n = 0;             // particles held; an upper bound while !n_exact
+  bool n_exact = true;    // false after a device-decided sort until refresh_count
   bool ids = false;
   // FP64: x, y, z, px, py, pz, w (cur / alternate)
   double* a[7] = {};
@@ -91,7 +92,7 @@ struct cylpic_ctx {
   int sort_interval = 1;
   int slot_bits = 0;
   uint32_t* mig = nullptr;  // migration sort scratch (mig_scratch_words(ntiles))
-  int* mig_host = nullptr;  // pinned copy of a species' mig_flag
+  int* d_sortstat = nullptr;  // device-decided sorts: [radix, migrate]
   bool migrate = true;      // use the neighbour migration sort when it applies
   double* stage = nullptr;  // FP64 staging of host particles for the mixed path
   long stage_cap = 0;
@@ -375,25 +376,23 @@ ExtraRange extra_of(const Species& s) {
   return ExtraRange{s.n_extra > 0 ? s.extra_start : nullptr, (int)(s.n - s.n_extra)};
 }
 
-// Neighbour migration sort of a species whose keys were tile-sorted before
-// the last push.  `counted`: that push left the direction counts (else they
-// are counted here).  *done = false (nothing changed) when some particle
-// moved beyond the 3 x 3 tile neighbourhood.  Synchronises once (reads the
-// flags).
-int migrate_species(cylpic_ctx* h, Species& s, bool counted, bool* done) {
-  *done = false;
-  if (!h->mig) {
-    HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
-    HIPCHK(hipHostMalloc((void**)&h->mig_host, 2 * sizeof(int), 0));
-  }
+// Sort of a species whose keys were tile-sorted before the last push, decided
+// on the device: the neighbour migration sort (counts, offsets, one scatter)
+// and the radix sort are queued together, guarded by the species' migration
+// flag, so that exactly one of them runs (the radix sort only when some
+// particle moved beyond the 3 x 3 tile neighbourhood), and the host waits
+// for nothing.  Both leave the sorted payload in the alternate buffers and
+// the ranges in tile_start_alt, which are then swapped in.  The radix sort
+// runs over the host's upper bound of the count, the slots a migration sort
+// left behind its kept particles first marked removed; the kept count stays
+// on the device (tile_start[ntiles]) until refresh_count reads it.
+int sort_species_device(cylpic_ctx* h, Species& s, bool counted) {
   const MigCount mc{s.mig_counts, s.mig_counts_x, s.mig_flag};
   if (!counted) {
-    HIPCHK(hipMemsetAsync(s.mig_flag, 0, 2 * sizeof(int), h->stream));
+    launch_zero_words(s.mig_flag, 2, h->stream);
     launch_mig_count(s.key[0], s.tile_start, extra_of(s), h->tl, mc, h->stream);
   }
-  HIPCHK(hipMemcpyAsync(h->mig_host, s.mig_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  if (h->mig_host[0]) return CYLPIC_OK;
+  const Guard mig{s.mig_flag, 0}, rad{s.mig_flag, 1};
   const bool f64 = h->prec == CYLPIC_FP64;
   const int narr = f64 ? 7 : 8;
   const void* src[8];
@@ -402,9 +401,20 @@ int migrate_species(cylpic_ctx* h, Species& s, bool counted, bool* done) {
     src[k] = f64 ? (const void*)s.a[k] : (const void*)s.ma[k];
     dst[k] = f64 ? (void*)s.b[k] : (void*)s.mb[k];
   }
+  const uint64_t* id_in = s.ids ? (const uint64_t*)s.id[0] : nullptr;
+  uint64_t* id_out = s.ids ? (uint64_t*)s.id[1] : nullptr;
   launch_mig_scatter(s.key[0], s.tile_start, extra_of(s), h->tl, mc, h->mig, s.tile_start_alt, f64 ? 8 : 4, src,
-                     dst, narr, s.key[1],
-                     s.ids ? (const uint64_t*)s.id[0] : nullptr, s.ids ? (uint64_t*)s.id[1] : nullptr, h->stream);
+                     dst, narr, s.key[1], id_in, id_out, h->stream, mig);
+  launch_guard_count(h->d_sortstat + 1, h->stream, mig);
+  // the fallback
+  uint32_t* perm = nullptr;
+  launch_mark_stale(s.key[0], s.tile_start, h->tl.ntiles, s.n - s.n_extra, h->stream, rad);
+  radix_sort(s.key[0], s.n, h->tl.ntiles, h->slot_bits, h->sc, s.key[1], &perm, h->stream, rad);
+  launch_tile_bounds(s.key[1], s.n, h->tl.ntiles, s.tile_start_alt, h->stream, rad);
+  if (f64) launch_gather_f64(perm, s.n, s.a, s.b, 7, s.key[1], s.tile_start_alt, 0, h->stream, rad);
+  else launch_gather_32(perm, s.n, s.ma, s.mb, 8, s.key[1], s.tile_start_alt, 0, h->stream, rad);
+  if (s.ids) launch_gather_64(perm, s.n, id_in, id_out, s.key[1], s.tile_start_alt, 0, h->stream, rad);
+  launch_guard_count(h->d_sortstat + 0, h->stream, rad);
   HIPCHK(hipGetLastError());
   for (int k = 0; k < narr; ++k) {
     if (f64) std::swap(s.a[k], s.b[k]);
@@ -414,9 +424,8 @@ int migrate_species(cylpic_ctx* h, Species& s, bool counted, bool* done) {
   if (s.ids) std::swap(s.id[0], s.id[1]);
   std::swap(s.tile_start, s.tile_start_alt);
   s.d_count = s.tile_start + h->tl.ntiles;
-  s.n = h->mig_host[1];
   s.n_extra = 0;
-  *done = true;
+  s.n_exact = false;
   return CYLPIC_OK;
 }
 
@@ -435,18 +444,15 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
     s.ranged = true;
     return CYLPIC_OK;
   }
-  if (s.ranged && !interleave && h->migrate) {
-    bool done = false;
-    int rc = migrate_species(h, s, counted, &done);
-    if (rc) return rc;
-    if (done) {
-      h->sort_migrate += 1;
-      return CYLPIC_OK;
-    }
-  }
-  h->sort_radix += 1;
   int rc = ensure_sort_scratch(h, s.n);
   if (rc) return rc;
+  if (s.ranged && !interleave && h->migrate) {
+    if (!h->mig) HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
+    return sort_species_device(h, s, counted);
+  }
+  h->sort_radix += 1;
+  // slots a device-decided sort left behind the kept particles sort last
+  if (!s.n_exact) launch_mark_stale(s.key[0], s.tile_start, h->tl.ntiles, s.n - s.n_extra, h->stream);
   uint32_t* perm = nullptr;
   // with `interleave` the keys are tile-major cell keys (upload)
   const int cells = interleave ? h->tl.tx * h->tl.tr : 0;
@@ -480,6 +486,7 @@ int refresh_count(cylpic_ctx* h, Species& s) {
   HIPCHK(hipMemcpyAsync(&c, s.tile_start + h->tl.ntiles, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   s.n = c;
+  s.n_exact = true;
   return CYLPIC_OK;
 }
 
@@ -547,7 +554,7 @@ void free_all(cylpic_ctx* h) {
   if (h->cont_q) (void)hipFree(h->cont_q);
   if (h->gauss_q) (void)hipFree(h->gauss_q);
   if (h->cont_out) (void)hipFree(h->cont_out);
-  if (h->mig_host) (void)hipHostFree(h->mig_host);
+  if (h->d_sortstat) (void)hipFree(h->d_sortstat);
   if (h->d_energy) (void)hipFree(h->d_energy);
   if (h->sc_mem) (void)hipFree(h->sc_mem);
   for (auto& kv : h->prof)
@@ -639,6 +646,106 @@ int retile(cylpic_ctx* h, int tx, int tr) {
     if (rc) return rc;
   }
   HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+
+// update_eb_fields_final (fields.f90:341-353) with the laser sources
+// already on the device (null: no laser on that side)
+int fields_final(cylpic_ctx* h, const double* dsrc_xmin, const double* dsrc_xmax) {
+  {
+    Scope s(h, "update_b");
+    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, h->stream); });
+  }
+  {
+    Scope s(h, "field_bcs");
+    with_fields(h, [&](auto& F) { launch_bfield_final_bcs(h->g, F, dsrc_xmin, dsrc_xmax, h->stream); });
+  }
+  {
+    Scope s(h, "update_e");
+    with_fields(h, [&](auto& F) { launch_update_e(h->g, F, h->stream); });
+  }
+  {
+    Scope s(h, "field_bcs");
+    with_fields(h, [&](auto& F) { launch_efield_bcs(h->g, F, h->stream); });
+  }
+  HIPCHK(hipGetLastError());
+  return CYLPIC_OK;
+}
+
+// push_particles (particles.F90:159-731) in three parts, so that a batch can
+// put the pushes of many runs into one launch between them:
+//   push_begin   jxm_old = jxm; jxm = 0 (particles.F90:163-169): the arrays
+//                swap roles and the new J is cleared;
+//   push_species the push and deposit of one species, launched, or (desc
+//                non-null, mixed path) written as a batched-push descriptor;
+//   push_end     current_bcs_r_min_final and the scheduled sort.
+int push_begin(cylpic_ctx* h) {
+  for (int k = 0; k < 3; ++k) swap_fields(h, 6 + k, 12 + k);
+  Scope s(h, "zero_j");
+  with_fields(h, [&](auto& F) {
+    for (int k = 0; k < 3; ++k) launch_fill_zero(F.f[6 + k], nfield(h), h->stream);
+  });
+  return CYLPIC_OK;
+}
+
+PushConst push_const(const cylpic_ctx* h, const Species& s) {
+  PushConst pc;
+  const double dt = h->g.dt;
+  pc.part_q = s.q;
+  pc.part_mc = C_LIGHT * s.m;
+  pc.ipart_mc = 1.0 / pc.part_mc;
+  const double dtfac = (0.5 * dt) * 0.25;
+  pc.cmratio = (pc.part_q * dtfac) * pc.ipart_mc;
+  pc.ccmratio = C_LIGHT * pc.cmratio;
+  pc.idx = 1.0 / h->g.dx;
+  pc.idy = 1.0 / h->g.dy;
+  pc.idt = 1.0 / dt;
+  pc.dtco2 = C_LIGHT * (dt / 2.0);
+  return pc;
+}
+
+int push_species(cylpic_ctx* h, Species& s, void* desc) {
+  const PushConst pc = push_const(h, s);
+  // the push counts its tiles' new keys by direction for the next
+  // migration sort (the tile ranges are those of the last sort)
+  MigCount migc{nullptr, nullptr, nullptr};
+  if (h->migrate && s.ranged) {
+    launch_zero_words(s.mig_flag, 2, h->stream);
+    migc = MigCount{s.mig_counts, s.mig_counts_x, s.mig_flag};
+  }
+  s.dirty = true;
+  s.counted = migc.counts != nullptr;
+  if (desc) {
+    fill_push_desc_mixed(desc, h->g, h->tl, pc, s.q, viewm(s), s.tile_start, extra_of(s), h->f32, h->geof,
+                         h->ntab, h->d_err, migc);
+    return CYLPIC_OK;
+  }
+  Scope sc(h, "push_deposit");
+  if (h->prec == CYLPIC_FP64) {
+    launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, extra_of(s), h->f64, geo_of(h),
+                            h->gather_lds, h->d_err, migc, h->stream);
+  } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, extra_of(s), h->f32,
+                                       h->geof, h->ntab, h->acc_mode, h->gather_lds, h->d_err, migc, h->stream)) {
+    return fail(CYLPIC_EINVAL, "mixed push: unsupported mode count");
+  }
+  return CYLPIC_OK;
+}
+
+int push_end(cylpic_ctx* h) {
+  {
+    Scope s(h, "current_axis");
+    with_fields(h, [&](auto& F) { launch_current_axis(h->g, F, h->stream); });
+  }
+  HIPCHK(hipGetLastError());
+  h->push_count += 1;
+  if (h->push_count % h->sort_interval == 0) {
+    for (auto& s : h->sp) {
+      if (s.immobile || s.n <= 0) continue;
+      int rc = sort_species(h, s);
+      if (rc) return rc;
+    }
+  }
   return CYLPIC_OK;
 }
 
@@ -810,9 +917,11 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
         return bail(fail(CYLPIC_EHIP, "event creation failed"));
   }
   if (hipMalloc((void**)&h->d_err, sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&h->d_energy, 2 * n_mode * sizeof(double)) != hipSuccess)
+      hipMalloc((void**)&h->d_energy, 2 * n_mode * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&h->d_sortstat, 2 * sizeof(int)) != hipSuccess)
     return bail(fail(CYLPIC_ENOMEM, "allocation failed"));
-  if (hipMemset(h->d_err, 0, sizeof(int)) != hipSuccess) return bail(fail(CYLPIC_EHIP, "memset failed"));
+  if (hipMemset(h->d_err, 0, sizeof(int)) != hipSuccess || hipMemset(h->d_sortstat, 0, 2 * sizeof(int)) != hipSuccess)
+    return bail(fail(CYLPIC_EHIP, "memset failed"));
   if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(CYLPIC_EHIP, "init sync failed"));
   *out = h;
   return CYLPIC_OK;
@@ -875,8 +984,12 @@ int cylpic_set_sort_method(cylpic_ctx* h, int method) {
 
 int cylpic_sort_counts(cylpic_ctx* h, int64_t* radix, int64_t* migrate) {
   CHECK_H(h);
-  if (radix) *radix = h->sort_radix;
-  if (migrate) *migrate = h->sort_migrate;
+  HIPCHK(hipSetDevice(h->device));
+  int dev[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(dev, h->d_sortstat, sizeof(dev), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (radix) *radix = h->sort_radix + dev[0];
+  if (migrate) *migrate = h->sort_migrate + dev[1];
   return CYLPIC_OK;
 }
 
@@ -926,6 +1039,7 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
     s.ids = false;
   }
   s.n = n;
+  s.n_exact = true;
   s.n_extra = 0;
   s.ranged = false;
   s.counted = false;
@@ -950,6 +1064,10 @@ int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double*
   if (!s.ranged) return fail(CYLPIC_ESTATE, "upload the species' particles before injecting");
   if ((ids != nullptr) != s.ids) return fail(CYLPIC_EINVAL, "ids must be given exactly when the upload gave them");
   HIPCHK(hipSetDevice(h->device));
+  if (!s.n_exact) {
+    int rc0 = refresh_count(h, s);
+    if (rc0) return rc0;
+  }
   Scope scope(h, "inject");
   // the extra region holds every batch injected since the last sort; this
   // batch joins it and the region is stable-sorted again by current key
@@ -1054,10 +1172,7 @@ int cylpic_move_window(cylpic_ctx* h, int ncells) {
       std::swap(s.tile_start, s.tile_start_alt);
       s.d_count = s.tile_start + h->tl.ntiles;
       if (s.n_extra > 0) {
-        if (!h->mig) {
-          HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
-          HIPCHK(hipHostMalloc((void**)&h->mig_host, 2 * sizeof(int), 0));
-        }
+        if (!h->mig) HIPCHK(hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)));
         launch_relabel_tiles(nullptr, 0, h->tl.ntiles, s.extra_start, (int*)h->mig, h->stream);
         HIPCHK(hipMemcpyAsync(s.extra_start, h->mig, (h->tl.ntiles + 2) * sizeof(int), hipMemcpyDeviceToDevice,
                               h->stream));
@@ -1192,61 +1307,14 @@ int cylpic_update_eb_fields_half(cylpic_ctx* h) {
 int cylpic_push_particles(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
-  // jxm_old = jxm; jxm = 0 (particles.F90:163-169): swap roles, clear the new J
-  for (int k = 0; k < 3; ++k) swap_fields(h, 6 + k, 12 + k);
-  {
-    Scope s(h, "zero_j");
-    with_fields(h, [&](auto& F) {
-      for (int k = 0; k < 3; ++k) launch_fill_zero(F.f[6 + k], nfield(h), h->stream);
-    });
-  }
-  const double dt = h->g.dt;
+  int rc = push_begin(h);
   for (auto& s : h->sp) {
+    if (rc) return rc;
     if (s.immobile || s.n <= 0) continue;
-    PushConst pc;
-    pc.part_q = s.q;
-    pc.part_mc = C_LIGHT * s.m;
-    pc.ipart_mc = 1.0 / pc.part_mc;
-    const double dtfac = (0.5 * dt) * 0.25;
-    pc.cmratio = (pc.part_q * dtfac) * pc.ipart_mc;
-    pc.ccmratio = C_LIGHT * pc.cmratio;
-    pc.idx = 1.0 / h->g.dx;
-    pc.idy = 1.0 / h->g.dy;
-    pc.idt = 1.0 / dt;
-    pc.dtco2 = C_LIGHT * (dt / 2.0);
-    // the push counts its tiles' new keys by direction for the next
-    // migration sort (the tile ranges are those of the last sort)
-    MigCount migc{nullptr, nullptr, nullptr};
-    if (h->migrate && s.ranged) {
-      HIPCHK(hipMemsetAsync(s.mig_flag, 0, 2 * sizeof(int), h->stream));
-      migc = MigCount{s.mig_counts, s.mig_counts_x, s.mig_flag};
-    }
-    Scope sc(h, "push_deposit");
-    s.dirty = true;
-    s.counted = migc.counts != nullptr;
-    if (h->prec == CYLPIC_FP64) {
-      launch_push_deposit_f64(h->g, h->tl, pc, view64(s), s.tile_start, extra_of(s), h->f64, geo_of(h),
-                              h->gather_lds, h->d_err, migc, h->stream);
-    } else if (launch_push_deposit_mixed(h->g, h->tl, pc, s.q, viewm(s), s.tile_start, extra_of(s), h->f32,
-                                         h->geof, h->ntab, h->acc_mode, h->gather_lds, h->d_err, migc,
-                                         h->stream)) {
-      return fail(CYLPIC_EINVAL, "mixed push: unsupported mode count");
-    }
+    rc = push_species(h, s, nullptr);
   }
-  {
-    Scope s(h, "current_axis");
-    with_fields(h, [&](auto& F) { launch_current_axis(h->g, F, h->stream); });
-  }
-  HIPCHK(hipGetLastError());
-  h->push_count += 1;
-  if (h->push_count % h->sort_interval == 0) {
-    for (auto& s : h->sp) {
-      if (s.immobile || s.n <= 0) continue;
-      int rc = sort_species(h, s);
-      if (rc) return rc;
-    }
-  }
-  return CYLPIC_OK;
+  if (rc) return rc;
+  return push_end(h);
 }
 
 int cylpic_sort_particles(cylpic_ctx* h) {
@@ -1314,24 +1382,7 @@ int cylpic_update_eb_fields_final(cylpic_ctx* h, const double* src_xmin, const d
     HIPCHK(hipEventRecord(h->src_ev[k][slot], h->stream));
     dsrc[k] = h->src_dev[k];
   }
-  {
-    Scope s(h, "update_b");
-    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, h->stream); });
-  }
-  {
-    Scope s(h, "field_bcs");
-    with_fields(h, [&](auto& F) { launch_bfield_final_bcs(h->g, F, dsrc[0], dsrc[1], h->stream); });
-  }
-  {
-    Scope s(h, "update_e");
-    with_fields(h, [&](auto& F) { launch_update_e(h->g, F, h->stream); });
-  }
-  {
-    Scope s(h, "field_bcs");
-    with_fields(h, [&](auto& F) { launch_efield_bcs(h->g, F, h->stream); });
-  }
-  HIPCHK(hipGetLastError());
-  return CYLPIC_OK;
+  return fields_final(h, dsrc[0], dsrc[1]);
 }
 
 int cylpic_step(cylpic_ctx* h, const double* src_xmin, const double* src_xmax) {
@@ -1669,6 +1720,7 @@ int cylpic_autotune(cylpic_ctx* h, int pushes, int* tile_x, int* tile_r, int* ga
       }
       if (restore) {
         s.n = n;
+        s.n_exact = true;
         s.n_extra = 0;
         s.dirty = false;
       }
@@ -1776,6 +1828,327 @@ int cylpic_kernel_time(cylpic_ctx* h, const char* name, double* ms, int64_t* cou
   }
   *ms = it->second.ms;
   *count = it->second.count;
+  return CYLPIC_OK;
+}
+
+}  // extern "C"
+
+// ======================================================================
+// Batches of runs stepped through one HIP graph
+// ======================================================================
+// Each step of a batch is captured into one graph: every run's timestep on
+// its own stream, forked from and joined to the batch's stream, and the
+// mixed pushes of all runs that share a mode count (with the default tile and
+// accumulator) in one persistent launch between the runs' field half
+// updates and the rest of their steps.  The capture runs the same host code
+// as an eager step, so the graph holds exactly the work an eager step would
+// launch and every run's host-side state (buffer swaps, push count, sort
+// schedule) advances as it would; a graph whose shape matches a cached one
+// only updates that executable's parameters.  Laser sources and push
+// descriptors go through pinned buffers in two slots used in turn, each
+// guarded by the event of the launch that read it.
+struct cylpic_batch {
+  int device = 0;
+  std::vector<cylpic_ctx*> runs;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr, pushed = nullptr;
+  std::vector<hipEvent_t> ready, done;
+  hipEvent_t launched[2] = {nullptr, nullptr};
+  int slot = 0;
+  struct Group {
+    int M = 0;
+    size_t lds = 0;
+    int nblocks = 0;
+    std::vector<std::pair<int, int>> members;  // (run, species)
+    void* d_desc = nullptr;
+    int* d_item_start = nullptr;
+    int* d_counters = nullptr;
+    char* h_desc[2] = {nullptr, nullptr};
+  };
+  std::vector<Group> groups;
+  std::vector<double*> src_pin;  // per run: [slot][side][2 (ny + 1)]
+  std::map<std::string, hipGraphExec_t> execs;
+  long steps = 0, instantiations = 0;
+};
+
+namespace {
+
+void batch_free(cylpic_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->origin) (void)hipStreamSynchronize(b->origin);
+  for (auto& kv : b->execs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& gp : b->groups) {
+    if (gp.d_desc) (void)hipFree(gp.d_desc);
+    if (gp.d_item_start) (void)hipFree(gp.d_item_start);
+    if (gp.d_counters) (void)hipFree(gp.d_counters);
+    for (auto* p : gp.h_desc)
+      if (p) (void)hipHostFree(p);
+  }
+  for (auto* p : b->src_pin)
+    if (p) (void)hipHostFree(p);
+  for (auto e : b->ready) (void)hipEventDestroy(e);
+  for (auto e : b->done) (void)hipEventDestroy(e);
+  for (auto e : {b->fork, b->pushed, b->launched[0], b->launched[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (b->origin) (void)hipStreamDestroy(b->origin);
+  delete b;
+}
+
+// The work of one batch step on the streams (captured by the caller).
+int batch_enqueue(cylpic_batch* b, const double* const* src_xmin, const double* const* src_xmax, int slot,
+                  std::string& shape) {
+  const int nr = (int)b->runs.size();
+  HIPCHK(hipEventRecord(b->fork, b->origin));
+  for (int r = 0; r < nr; ++r) HIPCHK(hipStreamWaitEvent(b->runs[r]->stream, b->fork, 0));
+  // which (run, species) pushes go into a group's launch
+  std::vector<std::vector<int>> grouped(nr);
+  for (int r = 0; r < nr; ++r) grouped[r].assign(b->runs[r]->sp.size(), -1);
+  for (int gi = 0; gi < (int)b->groups.size(); ++gi)
+    for (auto& m : b->groups[gi].members) grouped[m.first][m.second] = gi;
+  std::vector<int> ndesc(b->groups.size(), 0);
+  std::vector<std::vector<int>> items(b->groups.size());
+  for (int r = 0; r < nr; ++r) {
+    cylpic_ctx* h = b->runs[r];
+    int rc = cylpic_update_eb_fields_half(h);
+    if (!rc) rc = push_begin(h);
+    if (rc) return rc;
+    for (int k = 0; k < (int)h->sp.size(); ++k) {
+      Species& s = h->sp[k];
+      if (s.immobile || s.n <= 0) continue;
+      const int gi = grouped[r][k];
+      if (gi >= 0) {
+        auto& gp = b->groups[gi];
+        char* d = gp.h_desc[slot] + (size_t)ndesc[gi] * push_desc_mixed_bytes();
+        rc = push_species(h, s, d);
+        ndesc[gi] += 1;
+        items[gi].push_back(h->tl.ntiles);
+      } else {
+        rc = push_species(h, s, nullptr);
+      }
+      if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(b->ready[r], h->stream));
+    HIPCHK(hipStreamWaitEvent(b->origin, b->ready[r], 0));
+  }
+  for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
+    auto& gp = b->groups[gi];
+    if (!ndesc[gi]) continue;
+    // descriptor d owns work items [start[d], start[d + 1]), after the descriptors
+    std::vector<int> start(ndesc[gi] + 1, 0);
+    for (int d = 0; d < ndesc[gi]; ++d) start[d + 1] = start[d] + items[gi][d];
+    char* hd = gp.h_desc[slot];
+    const size_t db = (size_t)ndesc[gi] * push_desc_mixed_bytes();
+    std::memcpy(hd + gp.members.size() * push_desc_mixed_bytes(), start.data(), start.size() * sizeof(int));
+    HIPCHK(hipMemcpyAsync(gp.d_desc, hd, db, hipMemcpyHostToDevice, b->origin));
+    HIPCHK(hipMemcpyAsync(gp.d_item_start, hd + gp.members.size() * push_desc_mixed_bytes(),
+                          start.size() * sizeof(int), hipMemcpyHostToDevice, b->origin));
+    launch_push_deposit_mixed_batch(gp.d_desc, gp.d_item_start, ndesc[gi], gp.M, gp.lds, gp.d_counters, gp.nblocks,
+                                    b->origin);
+    shape += "g" + std::to_string(gi) + ":" + std::to_string(ndesc[gi]) + ";";
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->pushed, b->origin));
+  for (int r = 0; r < nr; ++r) {
+    cylpic_ctx* h = b->runs[r];
+    HIPCHK(hipStreamWaitEvent(h->stream, b->pushed, 0));
+    const long pc = h->push_count;
+    int rc = push_end(h);
+    if (!rc) rc = cylpic_current_finish(h);
+    if (rc) return rc;
+    h->time += h->g.dt;
+    const size_t sb = 2 * (size_t)(h->g.ny + 1) * sizeof(double);
+    const double* dsrc[2] = {nullptr, nullptr};
+    const double* const* hsrc[2] = {src_xmin, src_xmax};
+    for (int k = 0; k < 2; ++k) {
+      if (!hsrc[k] || !hsrc[k][r]) continue;
+      double* pin = b->src_pin[r] + (size_t)(2 * slot + k) * 2 * (h->g.ny + 1);
+      HIPCHK(hipMemcpyAsync(h->src_dev[k], pin, sb, hipMemcpyHostToDevice, h->stream));
+      dsrc[k] = h->src_dev[k];
+    }
+    rc = fields_final(h, dsrc[0], dsrc[1]);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b->done[r], h->stream));
+    HIPCHK(hipStreamWaitEvent(b->origin, b->done[r], 0));
+    shape += "r" + std::to_string(r) + (pc % h->sort_interval == h->sort_interval - 1 ? "s" : "") +
+             (dsrc[0] ? "a" : "") + (dsrc[1] ? "b" : "") + ";";
+  }
+  return CYLPIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cylpic_batch_create(cylpic_batch** out, cylpic_ctx* const* runs, int nruns) {
+  if (!out) return fail(CYLPIC_EINVAL, "null output");
+  *out = nullptr;
+  if (!runs || nruns < 1) return fail(CYLPIC_EINVAL, "a batch needs at least one run");
+  for (int r = 0; r < nruns; ++r) {
+    if (!runs[r]) return fail(CYLPIC_EINVAL, "null run handle");
+    if (runs[r]->device != runs[0]->device) return fail(CYLPIC_EINVAL, "the runs of a batch share one device");
+    for (int q = 0; q < r; ++q)
+      if (runs[q] == runs[r]) return fail(CYLPIC_EINVAL, "a run appears twice in the batch");
+  }
+  cylpic_batch* b = new (std::nothrow) cylpic_batch();
+  if (!b) return fail(CYLPIC_ENOMEM, "host allocation failed");
+  b->device = runs[0]->device;
+  b->runs.assign(runs, runs + nruns);
+  auto bail = [&](int rc) {
+    batch_free(b);
+    return rc;
+  };
+  if (hipSetDevice(b->device) != hipSuccess) return bail(fail(CYLPIC_EHIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(CYLPIC_EHIP, "stream creation failed"));
+  for (hipEvent_t* e : {&b->fork, &b->pushed, &b->launched[0], &b->launched[1]})
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(CYLPIC_EHIP, "event creation failed"));
+  b->ready.assign(nruns, nullptr);
+  b->done.assign(nruns, nullptr);
+  for (int r = 0; r < nruns; ++r)
+    if (hipEventCreateWithFlags(&b->ready[r], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done[r], hipEventDisableTiming) != hipSuccess)
+      return bail(fail(CYLPIC_EHIP, "event creation failed"));
+  // everything a step could allocate lazily is allocated now: a capture
+  // cannot allocate
+  for (cylpic_ctx* h : b->runs) {
+    if (!h->mig && hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)) != hipSuccess)
+      return bail(fail(CYLPIC_ENOMEM, "migration scratch allocation failed"));
+    long cap = 0;
+    for (auto& s : h->sp) cap = std::max(cap, s.cap);
+    if (int rc = ensure_sort_scratch(h, cap)) return bail(rc);
+    if (h->smooth_its + h->smooth_comp_its > 0) {
+      const long nf = nfield(h);
+      if (h->prec == CYLPIC_FP64 && !h->spare64 &&
+          hipMalloc((void**)&h->spare64, nf * sizeof(cplx<double>)) != hipSuccess)
+        return bail(fail(CYLPIC_ENOMEM, "smoothing buffer allocation failed"));
+      if (h->prec == CYLPIC_MIXED && !h->spare32 &&
+          hipMalloc((void**)&h->spare32, nf * sizeof(cplx<float>)) != hipSuccess)
+        return bail(fail(CYLPIC_ENOMEM, "smoothing buffer allocation failed"));
+    }
+    double* pin = nullptr;
+    if (hipHostMalloc((void**)&pin, 4 * 2 * (size_t)(h->g.ny + 1) * sizeof(double), 0) != hipSuccess)
+      return bail(fail(CYLPIC_ENOMEM, "pinned source allocation failed"));
+    b->src_pin.push_back(pin);
+  }
+  // the groups of the batched push: mixed species with the default tile,
+  // accumulator and gather, by mode count
+  for (int r = 0; r < nruns; ++r) {
+    cylpic_ctx* h = b->runs[r];
+    if (h->prec != CYLPIC_MIXED || !batch_push_supported(h->tl, h->g.M, h->acc_mode, h->gather_lds)) continue;
+    for (int k = 0; k < (int)h->sp.size(); ++k) {
+      if (h->sp[k].immobile || h->sp[k].cap <= 0) continue;
+      cylpic_batch::Group* gp = nullptr;
+      for (auto& gg : b->groups)
+        if (gg.M == h->g.M) gp = &gg;
+      if (!gp) {
+        b->groups.emplace_back();
+        gp = &b->groups.back();
+        gp->M = h->g.M;
+        gp->lds = push_lds_bytes_mixed(h->g, h->tl, h->acc_mode, h->gather_lds);
+      }
+      gp->members.emplace_back(r, k);
+    }
+  }
+  for (auto& gp : b->groups) {
+    const size_t nd = gp.members.size();
+    gp.nblocks = push_batch_blocks(gp.M, gp.lds);
+    if (hipMalloc(&gp.d_desc, nd * push_desc_mixed_bytes()) != hipSuccess ||
+        hipMalloc((void**)&gp.d_item_start, (nd + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&gp.d_counters, NUM_XCD * sizeof(int)) != hipSuccess)
+      return bail(fail(CYLPIC_ENOMEM, "batch allocation failed"));
+    for (auto*& p : gp.h_desc)
+      if (hipHostMalloc((void**)&p, nd * push_desc_mixed_bytes() + (nd + 1) * sizeof(int), 0) != hipSuccess)
+        return bail(fail(CYLPIC_ENOMEM, "pinned descriptor allocation failed"));
+  }
+  for (cylpic_ctx* h : b->runs)
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(CYLPIC_EHIP, "synchronize failed"));
+  *out = b;
+  return CYLPIC_OK;
+}
+
+int cylpic_batch_step(cylpic_batch* b, const double* const* src_xmin, const double* const* src_xmax) {
+  if (!b) return fail(CYLPIC_EINVAL, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  const int slot = b->slot;
+  b->slot ^= 1;
+  // the pinned slot was last read by the launch two steps ago
+  HIPCHK(hipEventSynchronize(b->launched[slot]));
+  for (int r = 0; r < (int)b->runs.size(); ++r) {
+    cylpic_ctx* h = b->runs[r];
+    const size_t nb = 2 * (size_t)(h->g.ny + 1);
+    if (src_xmin && src_xmin[r]) std::memcpy(b->src_pin[r] + (2 * slot) * nb, src_xmin[r], nb * sizeof(double));
+    if (src_xmax && src_xmax[r]) std::memcpy(b->src_pin[r] + (2 * slot + 1) * nb, src_xmax[r], nb * sizeof(double));
+  }
+  // no profiling events inside the graph
+  std::vector<bool> prof;
+  for (cylpic_ctx* h : b->runs) {
+    prof.push_back(h->profiling);
+    h->profiling = false;
+  }
+  HIPCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+  std::string shape = "s" + std::to_string(slot) + ";";
+  int rc = batch_enqueue(b, src_xmin, src_xmax, slot, shape);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->origin, &graph);
+  for (size_t r = 0; r < b->runs.size(); ++r) b->runs[r]->profiling = prof[r];
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ec != hipSuccess || !graph) return fail(CYLPIC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(ec));
+  auto it = b->execs.find(shape);
+  bool ok = false;
+  if (it != b->execs.end()) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    ok = hipGraphExecUpdate(it->second, graph, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(it->second);
+      b->execs.erase(it);
+    }
+  }
+  if (!ok) {
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(CYLPIC_EHIP, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+    }
+    b->execs[shape] = exec;
+    b->instantiations += 1;
+  }
+  (void)hipGraphDestroy(graph);
+  HIPCHK(hipGraphLaunch(b->execs[shape], b->origin));
+  HIPCHK(hipEventRecord(b->launched[slot], b->origin));
+  b->steps += 1;
+  return CYLPIC_OK;
+}
+
+int cylpic_batch_synchronize(cylpic_batch* b) {
+  if (!b) return fail(CYLPIC_EINVAL, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->origin));
+  for (cylpic_ctx* h : b->runs) {
+    int rc = check_device_errors(h);
+    if (rc) return rc;
+  }
+  return CYLPIC_OK;
+}
+
+int cylpic_batch_stats(cylpic_batch* b, int64_t* steps, int64_t* instantiations, int* push_groups) {
+  if (!b) return fail(CYLPIC_EINVAL, "null batch");
+  if (steps) *steps = b->steps;
+  if (instantiations) *instantiations = b->instantiations;
+  if (push_groups) *push_groups = (int)b->groups.size();
+  return CYLPIC_OK;
+}
+
+int cylpic_batch_destroy(cylpic_batch* b) {
+  if (!b) return fail(CYLPIC_EINVAL, "null batch");
+  batch_free(b);
   return CYLPIC_OK;
 }
 

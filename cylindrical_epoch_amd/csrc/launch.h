@@ -116,6 +116,19 @@ int launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, doubl
                               const int* tile_start, ExtraRange, FieldSet<float>&, const float* geo, int ntab,
                               int acc, int ldsf, int* err, MigCount, hipStream_t);
 void launch_keys_mixed(const Grid&, const Tiles&, const PartMixed&, long n, int cell_key, hipStream_t);
+// The batched mixed push: the arguments of one (run, species) push as an
+// opaque descriptor of push_desc_mixed_bytes(), many of which one persistent
+// launch works through (tiles of 16 x 4, the int64 box, the LDS gather: the
+// defaults).  push_batch_blocks: the launch's workgroups (resident ones)
+// for M modes and `lds` bytes; counters: NUM_XCD ints of scratch.
+size_t push_desc_mixed_bytes();
+void fill_push_desc_mixed(void* out, const Grid&, const Tiles&, const PushConst&, double charge, const PartMixed&,
+                          const int* tile_start, ExtraRange, FieldSet<float>&, const float* geo, int ntab, int* err,
+                          MigCount);
+bool batch_push_supported(const Tiles&, int M, int acc, int ldsf);
+int push_batch_blocks(int M, size_t lds);
+void launch_push_deposit_mixed_batch(const void* descs, const int* item_start, int ndesc, int M, size_t lds,
+                                     int* counters, int nblocks, hipStream_t);
 // moving window: remove the particles left of the moved x_min (key = ntiles);
 // the mixed path's cell numbers drop by ncells
 void launch_window_mixed(const PartMixed&, long n, int ncells, int ntiles, hipStream_t);
@@ -124,6 +137,17 @@ void launch_to_mixed(const Grid&, double mass, const double* const* src, const P
 void launch_from_mixed(const Grid&, double mass, const PartMixed&, double* const* dst, long n, hipStream_t);
 size_t push_lds_bytes_mixed(const Grid&, const Tiles&, int acc, int ldsf);
 size_t push_lds_bytes(const Grid&, const Tiles&, int ldsf);
+
+// A device-side condition on a launch: the kernel runs when p is null or
+// when (p[0] != 0) == (when != 0).  The neighbour migration sort and its
+// radix fallback are queued together with opposite guards on the
+// migration flag, so the device picks one and the host never waits to read
+// the flag.
+struct Guard {
+  const int* p = nullptr;
+  int when = 0;
+};
+__device__ __forceinline__ bool guard_skip(const Guard& gd) { return gd.p && ((gd.p[0] != 0) != (gd.when != 0)); }
 
 // sort.hip
 struct SortScratch {
@@ -138,8 +162,17 @@ size_t sort_scratch_words(long n);
 // = 0 sorts by tile alone.  On return perm[i] is the source index of sorted
 // position i and tiles_out[i] its tile key.
 void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, SortScratch& s,
-                uint32_t* tiles_out, uint32_t** perm_out, hipStream_t);
-void launch_tile_bounds(const uint32_t* sorted_tiles, long n, int ntiles, int* tile_start, hipStream_t);
+                uint32_t* tiles_out, uint32_t** perm_out, hipStream_t, Guard = Guard{});
+void launch_tile_bounds(const uint32_t* sorted_tiles, long n, int ntiles, int* tile_start, hipStream_t,
+                        Guard = Guard{});
+// keys of the slots [tile_start[ntiles], n_main) set to ntiles (removed): the
+// slots a migration sort left behind its kept particles, before a radix sort
+// over the host's upper bound n_main
+void launch_mark_stale(uint32_t* key, const int* tile_start, int ntiles, long n_main, hipStream_t, Guard = Guard{});
+// counter[0] += 1 when the guard passes (device-side sort statistics)
+void launch_guard_count(int* counter, hipStream_t, Guard);
+// n 32-bit words set to zero by a kernel
+void launch_zero_words(void* p, long n, hipStream_t);
 // tile columns relabelled one down (keys of n particles, null: ranges only);
 // ts_out = the ranges under the new labels
 void launch_relabel_tiles(uint32_t* key, long n, int ntiles, const int* ts_in, int* ts_out, hipStream_t);
@@ -149,11 +182,11 @@ void launch_div_keys(uint32_t* key, long n, uint32_t div, hipStream_t);
 // = the cells of a tile (> 0) the per-tile interleave of sort.hip
 // (tile_start must describe the sorted keys).
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst,
-                       int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
+                       int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t, Guard = Guard{});
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst,
-                      int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
+                      int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t, Guard = Guard{});
 void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst,
-                      const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t);
+                      const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t, Guard = Guard{});
 
 // Neighbour migration sort (keys tile-sorted before the last push).
 // launch_mig_count fills the direction counts (as the push kernels do);
@@ -168,7 +201,7 @@ void launch_mig_count(const uint32_t* key, const int* tile_start, ExtraRange xr,
 void launch_mig_scatter(const uint32_t* key, const int* tile_start, ExtraRange xr, const Tiles& tl, MigCount mc,
                         uint32_t* scratch, int* tile_start_new, int word_bytes, const void* const* src,
                         void* const* dst, int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out,
-                        hipStream_t);
+                        hipStream_t, Guard = Guard{});
 
 // diag.hip: output diagnostics
 // number density modes (complex FP64, field-shaped `out`, zeroed by the

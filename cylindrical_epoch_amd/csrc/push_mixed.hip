@@ -24,6 +24,8 @@
 // A particle whose stencils leave the tile's LDS regions (it drifted while
 // the sort ran less often than every step) gathers from and deposits to
 // global memory directly.
+#include <cstring>
+
 #include "cplx_ops.h"
 #include "launch.h"
 #include "migrate.h"
@@ -177,17 +179,18 @@ __device__ __forceinline__ float fixed_scale(float bound) {
 // TX, TR > 0: the tile shape fixed at compile time, so that the current box's
 // pitch and plane strides are constants and every LDS atomic addresses one
 // per-row base register plus an immediate offset; 0: the shape of `tl`.
-template <int M, int ACC, int LDSF, int TX = 0, int TR = 0>
-__global__ __launch_bounds__(256) void k_push_deposit_mixed(
-    Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start, ExtraRange xr,
-    const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
-    const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
-    cplx<float>* __restrict__ jx, cplx<float>* __restrict__ jr, cplx<float>* __restrict__ jt, GeoF geo,
-    int* __restrict__ err, MigCount migc) {
-  extern __shared__ double lds_raw[];
-  __shared__ float wred[8];
-  __shared__ uint32_t migs[MIG_DIRS];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+// The push and deposit of one tile by the whole workgroup: the body of the
+// per-run kernel (one workgroup per tile) and of the batched kernel (a
+// persistent workgroup loops over (run, tile) items).  lds_raw is the
+// dynamic LDS, wred and migs small static LDS arrays of the kernel.
+template <int M, int ACC, int LDSF, int TX, int TR>
+__device__ __forceinline__ void push_tile_mixed(
+    const Grid& g, const Tiles& tl, const MixConst& mc, const PartMixed& sp, const int* __restrict__ tile_start,
+    const ExtraRange& xr, const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er,
+    const cplx<float>* __restrict__ et, const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br,
+    const cplx<float>* __restrict__ bt, cplx<float>* __restrict__ jx, cplx<float>* __restrict__ jr,
+    cplx<float>* __restrict__ jt, const GeoF& geo, int* __restrict__ err, const MigCount& migc, const int tile,
+    double* lds_raw, float* wred, uint32_t* migs) {
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   Boxes<M> b;
   const int tx = TX > 0 ? TX : tl.tx, tr = TR > 0 ? TR : tl.tr;
@@ -696,6 +699,70 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   }
 }
 
+// One run: one workgroup per tile, tiles mapped XCD-contiguously.
+template <int M, int ACC, int LDSF, int TX = 0, int TR = 0>
+__global__ __launch_bounds__(256) void k_push_deposit_mixed(
+    Grid g, Tiles tl, MixConst mc, PartMixed sp, const int* __restrict__ tile_start, ExtraRange xr,
+    const cplx<float>* __restrict__ ex, const cplx<float>* __restrict__ er, const cplx<float>* __restrict__ et,
+    const cplx<float>* __restrict__ bx, const cplx<float>* __restrict__ br, const cplx<float>* __restrict__ bt,
+    cplx<float>* __restrict__ jx, cplx<float>* __restrict__ jr, cplx<float>* __restrict__ jt, GeoF geo,
+    int* __restrict__ err, MigCount migc) {
+  extern __shared__ double lds_raw[];
+  __shared__ float wred[8];
+  __shared__ uint32_t migs[MIG_DIRS];
+  push_tile_mixed<M, ACC, LDSF, TX, TR>(g, tl, mc, sp, tile_start, xr, ex, er, et, bx, br, bt, jx, jr, jt, geo, err,
+                                        migc, xcd_tile(blockIdx.x, gridDim.x), lds_raw, wred, migs);
+}
+
+// The arguments of one (run, species) push for the batched kernel.
+struct PushArgsMixed {
+  Grid g;
+  Tiles tl;
+  MixConst mc;
+  PartMixed sp;
+  const int* tile_start;
+  ExtraRange xr;
+  const cplx<float>* f[6];  // ex er et bx br bt
+  cplx<float>* j[3];        // jx jr jt
+  GeoF geo;
+  int* err;
+  MigCount migc;
+};
+
+// Many runs (or species) that share the mode count and the tile shape, in one
+// persistent launch: the work items are their tiles, descriptor d owning
+// items [item_start[d], item_start[d + 1]).  The item range is cut into one
+// contiguous chunk per XCD (workgroup b runs on XCD b mod 8), whose
+// workgroups take items in order from the XCD's counter until the chunk is
+// exhausted: neighbouring tiles share an L2, and every workgroup exits once
+// its chunk's counter passes the end.  counters[0..7] must be zero at launch.
+template <int M, int ACC, int LDSF, int TX, int TR>
+__global__ __launch_bounds__(256) void k_push_deposit_mixed_batch(const PushArgsMixed* __restrict__ descs,
+                                                                  const int* __restrict__ item_start, int ndesc,
+                                                                  int* __restrict__ counters) {
+  extern __shared__ double lds_raw[];
+  __shared__ float wred[8];
+  __shared__ uint32_t migs[MIG_DIRS];
+  __shared__ int s_item;
+  const int total = item_start[ndesc];
+  const int x = (int)(blockIdx.x % NUM_XCD);
+  const int lo = (int)((long)total * x / NUM_XCD), hi = (int)((long)total * (x + 1) / NUM_XCD);
+  for (;;) {
+    __syncthreads();  // every work-item is done with the previous tile's LDS
+    if (threadIdx.x == 0) s_item = lo + atomicAdd(counters + x, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= hi) break;
+    int d = 0;
+    while (d + 1 < ndesc && item_start[d + 1] <= it) ++d;
+    d = __builtin_amdgcn_readfirstlane(d);
+    const PushArgsMixed& a = descs[d];
+    push_tile_mixed<M, ACC, LDSF, TX, TR>(a.g, a.tl, a.mc, a.sp, a.tile_start, a.xr, a.f[0], a.f[1], a.f[2], a.f[3],
+                                          a.f[4], a.f[5], a.j[0], a.j[1], a.j[2], a.geo, a.err, a.migc,
+                                          it - item_start[d], lds_raw, wred, migs);
+  }
+}
+
 // ------------------------------------------------------------------------
 // continuity check: the m = 0 charge q w hx hy / 4 of each particle at its
 // half-step position, with the push's own FP32 half step and staggered
@@ -915,6 +982,62 @@ int launch_push_deposit_mixed(const Grid& g, const Tiles& tl, const PushConst& p
 #undef LAUNCH
 #undef LAUNCH_T
   return 0;
+}
+
+// ---------------------------------------------------------------- batched push
+size_t push_desc_mixed_bytes() { return sizeof(PushArgsMixed); }
+
+void fill_push_desc_mixed(void* out, const Grid& g, const Tiles& tl, const PushConst& pc, double charge,
+                          const PartMixed& sp, const int* tile_start, ExtraRange xr, FieldSet<float>& F,
+                          const float* geo, int ntab, int* err, MigCount migc) {
+  PushArgsMixed a;
+  a.g = g;
+  a.tl = tl;
+  a.mc = mix_const(g, pc, charge);
+  a.sp = sp;
+  a.tile_start = tile_start;
+  a.xr = xr;
+  for (int k = 0; k < 6; ++k) a.f[k] = F.f[k];
+  for (int k = 0; k < 3; ++k) a.j[k] = F.f[6 + k];
+  a.geo = GeoF{geo, geo + ntab, geo + 2 * ntab, geo + 3 * ntab};
+  a.err = err;
+  a.migc = migc;
+  std::memcpy(out, &a, sizeof(a));
+}
+
+bool batch_push_supported(const Tiles& tl, int M, int acc, int ldsf) {
+  return tl.tx == 16 && tl.tr == 4 && acc == 3 && ldsf == 1 && M >= 1 && M <= 4;
+}
+
+template <int M>
+static const void* batch_kernel() {
+  return reinterpret_cast<const void*>(&k_push_deposit_mixed_batch<M, 3, 1, 16, 4>);
+}
+
+int push_batch_blocks(int M, size_t lds) {
+  const void* k = M == 1 ? batch_kernel<1>() : (M == 2 ? batch_kernel<2>() : (M == 3 ? batch_kernel<3>() : batch_kernel<4>()));
+  if (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  return per_cu * cus;
+}
+
+void launch_push_deposit_mixed_batch(const void* descs, const int* item_start, int ndesc, int M, size_t lds,
+                                     int* counters, int nblocks, hipStream_t s) {
+  launch_zero_words(counters, NUM_XCD, s);
+  const PushArgsMixed* d = (const PushArgsMixed*)descs;
+#define LB(MM)                                                                                          \
+  hipLaunchKernelGGL((k_push_deposit_mixed_batch<MM, 3, 1, 16, 4>), dim3(nblocks), dim3(256), lds, s, d, \
+                     item_start, ndesc, counters)
+  switch (M) {
+    case 1: LB(1); break;
+    case 2: LB(2); break;
+    case 3: LB(3); break;
+    default: LB(4); break;
+  }
+#undef LB
 }
 
 void launch_keys_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, long n, int cell_key, hipStream_t s) {

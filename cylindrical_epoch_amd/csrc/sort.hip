@@ -11,6 +11,8 @@
 //             one ballot per distinct digit in the row (few for nearly sorted
 //             keys), and the per-digit running counts live in registers
 //             spread across the wavefront's lanes.
+#include <algorithm>
+
 #include "launch.h"
 #include "migrate.h"
 
@@ -31,7 +33,8 @@ __device__ __forceinline__ uint32_t comp_key(const uint32_t* __restrict__ keys, 
 
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_count(const uint32_t* __restrict__ keys, long n,
                                                              int shift, int compose_bits,
-                                                             uint32_t* __restrict__ counts, int nb) {
+                                                             uint32_t* __restrict__ counts, int nb, Guard gd) {
+  if (guard_skip(gd)) return;
   __shared__ uint32_t hist[RADIX];
   for (int d = threadIdx.x; d < RADIX; d += SORT_BLOCK) hist[d] = 0;
   __syncthreads();
@@ -55,7 +58,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __
                                                                int shift, int compose_bits, int out_shift,
                                                                const uint32_t* __restrict__ offs,
                                                                int nb, uint32_t* __restrict__ kout,
-                                                               uint32_t* __restrict__ vout) {
+                                                               uint32_t* __restrict__ vout, Guard gd) {
+  if (guard_skip(gd)) return;
   __shared__ uint32_t wcnt[WAVES][RADIX];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -147,7 +151,8 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
 }
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ in, long n,
-                                                            uint32_t* __restrict__ partial) {
+                                                            uint32_t* __restrict__ partial, Guard gd) {
+  if (guard_skip(gd)) return;
   const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_IPT;
   uint32_t s = 0;
   for (int k = 0; k < SCAN_IPT; ++k)
@@ -158,7 +163,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __re
 }
 
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict__ data, long n,
-                                                           const uint32_t* __restrict__ partial_scanned) {
+                                                           const uint32_t* __restrict__ partial_scanned, Guard gd) {
+  if (guard_skip(gd)) return;
   const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_IPT;
   uint32_t v[SCAN_IPT];
   uint32_t s = 0;
@@ -174,15 +180,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(uint32_t* __restrict_
   }
 }
 
-static void exclusive_scan(uint32_t* data, long n, uint32_t* tmp, hipStream_t s) {
+static void exclusive_scan(uint32_t* data, long n, uint32_t* tmp, hipStream_t s, Guard gd) {
   const long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nb <= 1) {
-    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_BLOCK), 0, s, data, n, (const uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SCAN_BLOCK), 0, s, data, n, (const uint32_t*)nullptr, gd);
     return;
   }
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, data, n, tmp);
-  exclusive_scan(tmp, nb, tmp + nb, s);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, data, n, tmp);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, data, n, tmp, gd);
+  exclusive_scan(tmp, nb, tmp + nb, s, gd);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, data, n, tmp, gd);
 }
 
 size_t sort_scratch_words(long n) {
@@ -197,7 +203,7 @@ size_t sort_scratch_words(long n) {
 }
 
 void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, SortScratch& sc,
-                uint32_t* tiles_out, uint32_t** perm_out, hipStream_t s) {
+                uint32_t* tiles_out, uint32_t** perm_out, hipStream_t s, Guard gd) {
   int bits = slot_bits;
   while ((1l << (bits - slot_bits)) <= (long)ntiles) ++bits;
   const int passes = (bits + 7) / 8;
@@ -211,10 +217,10 @@ void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, Sor
     const bool last = p == passes - 1;
     uint32_t* kout = last ? tiles_out : sc.k[cur];
     uint32_t* vout = sc.v[cur];
-    hipLaunchKernelGGL(k_radix_count, dim3(nb), dim3(SORT_BLOCK), 0, s, kin, n, shift, compose, sc.counts, nb);
-    exclusive_scan(sc.counts, (long)nb * RADIX, sc.scan_tmp, s);
+    hipLaunchKernelGGL(k_radix_count, dim3(nb), dim3(SORT_BLOCK), 0, s, kin, n, shift, compose, sc.counts, nb, gd);
+    exclusive_scan(sc.counts, (long)nb * RADIX, sc.scan_tmp, s, gd);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(SORT_BLOCK), 0, s, kin, vin, n, shift, compose,
-                       last ? slot_bits : -1, sc.counts, nb, kout, vout);
+                       last ? slot_bits : -1, sc.counts, nb, kout, vout, gd);
     kin = kout;
     vin = vout;
     cur ^= 1;
@@ -224,7 +230,9 @@ void radix_sort(const uint32_t* tiles_in, long n, int ntiles, int slot_bits, Sor
 
 // tile_start[t] = first sorted index with key >= t, t in 0..ntiles, and
 // tile_start[ntiles + 1] = n
-__global__ void k_tile_bounds(const uint32_t* __restrict__ key, long n, int ntiles, int* __restrict__ tile_start) {
+__global__ void k_tile_bounds(const uint32_t* __restrict__ key, long n, int ntiles, int* __restrict__ tile_start,
+                              Guard gd) {
+  if (guard_skip(gd)) return;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   const long lo = (i == 0) ? -1 : (long)key[i - 1];
@@ -233,9 +241,42 @@ __global__ void k_tile_bounds(const uint32_t* __restrict__ key, long n, int ntil
   if (i == n) tile_start[ntiles + 1] = (int)n;  // sentinel past the removed ones
 }
 
-void launch_tile_bounds(const uint32_t* sorted_keys, long n, int ntiles, int* tile_start, hipStream_t s) {
+void launch_tile_bounds(const uint32_t* sorted_keys, long n, int ntiles, int* tile_start, hipStream_t s, Guard gd) {
   hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sorted_keys, n,
-                     ntiles, tile_start);
+                     ntiles, tile_start, gd);
+}
+
+__global__ void k_mark_stale(uint32_t* __restrict__ key, const int* __restrict__ ts, int ntiles, long n_main,
+                             Guard gd) {
+  if (guard_skip(gd)) return;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_main && i >= ts[ntiles]) key[i] = (uint32_t)ntiles;
+}
+
+void launch_mark_stale(uint32_t* key, const int* ts, int ntiles, long n_main, hipStream_t s, Guard gd) {
+  if (n_main > 0)
+    hipLaunchKernelGGL(k_mark_stale, dim3((unsigned)((n_main + 255) / 256)), dim3(256), 0, s, key, ts, ntiles,
+                       n_main, gd);
+}
+
+__global__ void k_guard_count(int* counter, Guard gd) {
+  if (!guard_skip(gd)) counter[0] += 1;
+}
+
+void launch_guard_count(int* counter, hipStream_t s, Guard gd) {
+  hipLaunchKernelGGL(k_guard_count, dim3(1), dim3(1), 0, s, counter, gd);
+}
+
+__global__ void k_zero_words(uint32_t* __restrict__ p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0u;
+}
+
+void launch_zero_words(void* p, long n, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<long>((n + 255) / 256, 4096)), dim3(256), 0, s,
+                       (uint32_t*)p, n);
 }
 
 // tile columns relabelled one down: keys k < ntiles -> k - 1, and range
@@ -291,7 +332,8 @@ __device__ __forceinline__ long dest_of(long i, const uint32_t* __restrict__ key
 
 template <class W>
 __global__ void k_gather(const uint32_t* __restrict__ perm, long n, Ptr8 p, int narr,
-                         const uint32_t* __restrict__ key, const int* __restrict__ ts, int inter) {
+                         const uint32_t* __restrict__ key, const int* __restrict__ ts, int inter, Guard gd) {
+  if (guard_skip(gd)) return;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t j = perm[i];
@@ -300,7 +342,7 @@ __global__ void k_gather(const uint32_t* __restrict__ perm, long n, Ptr8 p, int 
 }
 
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst, int narr,
-                       const uint32_t* key, const int* ts, int inter, hipStream_t s) {
+                       const uint32_t* key, const int* ts, int inter, hipStream_t s, Guard gd) {
   Ptr8 p;
   for (int a = 0; a < narr; ++a) {
     p.src[a] = src[a];
@@ -308,10 +350,10 @@ void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, d
   }
   if (n > 0)
     hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr,
-                       key, ts, inter);
+                       key, ts, inter, gd);
 }
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst, int narr,
-                      const uint32_t* key, const int* ts, int inter, hipStream_t s) {
+                      const uint32_t* key, const int* ts, int inter, hipStream_t s, Guard gd) {
   Ptr8 p;
   for (int a = 0; a < narr; ++a) {
     p.src[a] = src[a];
@@ -319,16 +361,16 @@ void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, 
   }
   if (n > 0)
     hipLaunchKernelGGL(k_gather<uint32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr,
-                       key, ts, inter);
+                       key, ts, inter, gd);
 }
 void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, const uint32_t* key,
-                      const int* ts, int inter, hipStream_t s) {
+                      const int* ts, int inter, hipStream_t s, Guard gd) {
   Ptr8 p;
   p.src[0] = src;
   p.dst[0] = dst;
   if (n > 0)
     hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, 1, key,
-                       ts, inter);
+                       ts, inter, gd);
 }
 
 // ---------------- neighbour migration sort -------------------------------
@@ -363,7 +405,8 @@ __global__ __launch_bounds__(MIG_BLOCK) void k_mig_count(const uint32_t* __restr
 // extra arrivals from sources before S.  tot[ntiles] = 0.
 __global__ void k_mig_offsets(const uint32_t* __restrict__ counts, const uint32_t* __restrict__ counts_x, int ntx,
                               int ntr, int ntiles, uint32_t* __restrict__ tot, uint32_t* __restrict__ pre,
-                              uint32_t* __restrict__ pre_x) {
+                              uint32_t* __restrict__ pre_x, Guard gd) {
+  if (guard_skip(gd)) return;
   const int D = blockIdx.x * blockDim.x + threadIdx.x;
   if (D > ntiles) return;
   if (D == ntiles) {
@@ -409,7 +452,9 @@ __global__ void k_mig_offsets(const uint32_t* __restrict__ counts, const uint32_
   tot[D] = total + total_x;
 }
 
-__global__ void k_mig_sentinel(int* ts, int ntiles) { ts[ntiles + 1] = ts[ntiles]; }
+__global__ void k_mig_sentinel(int* ts, int ntiles, Guard gd) {
+  if (!guard_skip(gd)) ts[ntiles + 1] = ts[ntiles];
+}
 
 // Stable rank of each particle of the range [p0, p1) among those with the
 // same direction, then payload copy to its sorted position.
@@ -464,7 +509,8 @@ __global__ __launch_bounds__(MIG_BLOCK) void k_mig_scatter(const uint32_t* __res
                                                            const int* __restrict__ ts_new, Ptr8 p, int narr,
                                                            uint32_t* __restrict__ key_out,
                                                            const uint64_t* __restrict__ id_in,
-                                                           uint64_t* __restrict__ id_out) {
+                                                           uint64_t* __restrict__ id_out, Guard gd) {
+  if (guard_skip(gd)) return;
   __shared__ uint32_t run[9];
   __shared__ uint32_t wc[MIG_WAVES][9];
   const int S = blockIdx.x;
@@ -490,16 +536,17 @@ void launch_mig_count(const uint32_t* key, const int* ts, ExtraRange xr, const T
 
 void launch_mig_scatter(const uint32_t* key, const int* ts, ExtraRange xr, const Tiles& tl, MigCount mc,
                         uint32_t* scratch, int* ts_new, int word_bytes, const void* const* src, void* const* dst,
-                        int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out, hipStream_t s) {
+                        int narr, uint32_t* key_out, const uint64_t* id_in, uint64_t* id_out, hipStream_t s,
+                        Guard gd) {
   const int nt = tl.ntiles;
   uint32_t* pre = scratch;
   uint32_t* pre_x = pre + (size_t)nt * 9;
   uint32_t* tmp = pre_x + (size_t)nt * 9;
   hipLaunchKernelGGL(k_mig_offsets, dim3((nt + 1 + 255) / 256), dim3(256), 0, s, mc.counts,
                      xr.start ? (const uint32_t*)mc.counts_x : nullptr, tl.ntx, tl.ntr, nt, (uint32_t*)ts_new, pre,
-                     pre_x);
-  exclusive_scan((uint32_t*)ts_new, nt + 1, tmp, s);
-  hipLaunchKernelGGL(k_mig_sentinel, dim3(1), dim3(1), 0, s, ts_new, nt);
+                     pre_x, gd);
+  exclusive_scan((uint32_t*)ts_new, nt + 1, tmp, s, gd);
+  hipLaunchKernelGGL(k_mig_sentinel, dim3(1), dim3(1), 0, s, ts_new, nt, gd);
   Ptr8 p;
   for (int a = 0; a < narr; ++a) {
     p.src[a] = src[a];
@@ -507,10 +554,10 @@ void launch_mig_scatter(const uint32_t* key, const int* ts, ExtraRange xr, const
   }
   if (word_bytes == 8)
     hipLaunchKernelGGL(k_mig_scatter<uint64_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, xr, tl.ntx, nt, pre, pre_x,
-                       ts_new, p, narr, key_out, id_in, id_out);
+                       ts_new, p, narr, key_out, id_in, id_out, gd);
   else
     hipLaunchKernelGGL(k_mig_scatter<uint32_t>, dim3(nt), dim3(MIG_BLOCK), 0, s, key, ts, xr, tl.ntx, nt, pre, pre_x,
-                       ts_new, p, narr, key_out, id_in, id_out);
+                       ts_new, p, narr, key_out, id_in, id_out, gd);
 }
 
 }  // namespace cylpic

@@ -302,3 +302,53 @@ class Simulation:
         cnt = C.c_int64()
         check(lib().cylpic_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+
+class Batch:
+    """Several runs stepped together through one captured HIP graph per
+    step, their mixed pushes sharing persistent launches (cylpic_batch_*).
+    The runs stay ordinary Simulations for set-up, downloads and
+    diagnostics; between batch steps call synchronize() before touching
+    them directly."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h for s in self.sims])
+        self._arr = arr
+        b = C.c_void_p()
+        check(lib().cylpic_batch_create(C.byref(b), C.cast(arr, C.c_void_p), len(self.sims)))
+        self.b = b
+
+    def step(self, src_xmin=None, src_xmax=None):
+        """One timestep of every run; src_xmin / src_xmax: per-run laser
+        sources (each None or 2 (ny + 1) doubles), or None for no laser."""
+        keep = []
+
+        def table(srcs):
+            if srcs is None:
+                return None
+            t = (C.c_void_p * len(self.sims))()
+            for k, v in enumerate(srcs):
+                if v is not None:
+                    a = np.ascontiguousarray(v, np.float64)
+                    keep.append(a)
+                    t[k] = a.ctypes.data
+            keep.append(t)
+            return C.cast(t, C.c_void_p)
+
+        check(lib().cylpic_batch_step(self.b, table(src_xmin), table(src_xmax)))
+        for s in self.sims:
+            s.time += s.dt
+
+    def synchronize(self):
+        check(lib().cylpic_batch_synchronize(self.b))
+
+    def stats(self):
+        steps, inst, groups = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().cylpic_batch_stats(self.b, C.byref(steps), C.byref(inst), C.byref(groups)))
+        return {"steps": steps.value, "instantiations": inst.value, "push_groups": groups.value}
+
+    def close(self):
+        if self.b:
+            check(lib().cylpic_batch_destroy(self.b))
+            self.b = None

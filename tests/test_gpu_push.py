@@ -214,6 +214,54 @@ def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, te
     sim.close()
 
 
+def test_device_decided_sorts_in_sequence(oracle):
+    """The sort is chosen on the device (migration, or radix after a move
+    beyond the tile neighbourhood) and the kept count stays there: a
+    migration sort that drops particles leaves stale slots behind the kept
+    ones, and a later radix fallback over the host's upper bound must not
+    bring them back.  Open boundaries, 4 x 1 tiles: one push and a sort
+    (migration; a window shift of two cells, within a tile's width, removes
+    the first two columns), five pushes and a sort (radix: some particles
+    move two or more tiles in r), against the CPU stable sort of the
+    oracle's keys after each."""
+    log("device-decided sorts in sequence")
+    nx, ny, dx, dy, tile = 24, 12, 1e-7, 1e-7, (4, 1)
+    bcf = ("simple_outflow", "simple_outflow", "other", "conduct")
+    bcp = ("open", "open", "other", "open")
+    st, sim = make_pair(oracle, nx, ny, 1, dx, dy, bc_field=bcf, bc_particle=bcp, precision="fp64", tile=tile,
+                        sort_interval=1000)
+    add_plasma(st, sim, nx, ny, dx, dy, ppc=6, temp_ev=2.0e5, seed=33, shuffle=True)
+    ntiles = 1 << 16                                 # beyond every tile key of the offset tiling
+    order = sim.download_particles(0)["id"]          # the GPU's order after the upload sort
+    shifted = 0
+    c0 = sim.sort_counts()
+    # no download (which reads the count back) between the two sorts
+    for pushes, shift in ((1, 2), (5, 0)):
+        for _ in range(pushes):
+            sim.push_particles()
+            st.push_particles()
+        if shift:
+            sim.move_window(shift)
+            st.shift_window(shift)
+            shifted += shift
+        sim.sort_particles()
+        sp = st.species[0]
+        pos = {i: k for k, i in enumerate(sp["id"])}
+        # the tiles move with the plasma: a window cell is a tile offset of 1
+        keys_alive = oracle.cell_keys(st, sp["x"], sp["y"], sp["z"], *tile, shift=shifted) // (tile[0] * tile[1])
+        keys = np.array([keys_alive[pos[i]] if i in pos else ntiles for i in order], np.int32)
+        perm = oracle.stable_sort(keys, ntiles + 1)
+        kept = order[perm][: int(np.sum(keys < ntiles))]
+        log(f"  after {pushes} push(es), shift {shift}: {len(order)} -> {len(kept)} particles")
+        assert len(kept) < len(order)
+        order = kept
+    c1 = sim.sort_counts()
+    assert (c1["migrate"] - c0["migrate"], c1["radix"] - c0["radix"]) == (1, 1), (c0, c1)
+    got = sim.download_particles(0)
+    assert np.array_equal(got["id"], order)
+    sim.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_autotune_restores_state(oracle, precision):
     """The autotuner times pushes under each candidate tile shape and puts
