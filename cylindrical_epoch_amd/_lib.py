@@ -54,8 +54,9 @@ SIGNATURES = {
     "cylpic_kernel_time": [_p, C.c_char_p, _p, _p],
     "cylpic_batch_create": [_pp, _p, _i],
     "cylpic_batch_step": [_p, _p, _p],
+    "cylpic_batch_steps": [_p, _i, _p],
     "cylpic_batch_synchronize": [_p],
-    "cylpic_batch_stats": [_p, _p, _p, _p],
+    "cylpic_batch_stats": [_p, _p, _p, _p, _p],
     "cylpic_batch_destroy": [_p],
 }
 

@@ -6,9 +6,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -655,7 +657,7 @@ int retile(cylpic_ctx* h, int tx, int tr) {
 int fields_final(cylpic_ctx* h, const double* dsrc_xmin, const double* dsrc_xmax) {
   {
     Scope s(h, "update_b");
-    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, h->stream); });
+    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, 0, h->stream); });
   }
   {
     Scope s(h, "field_bcs");
@@ -683,9 +685,7 @@ int fields_final(cylpic_ctx* h, const double* dsrc_xmin, const double* dsrc_xmax
 int push_begin(cylpic_ctx* h) {
   for (int k = 0; k < 3; ++k) swap_fields(h, 6 + k, 12 + k);
   Scope s(h, "zero_j");
-  with_fields(h, [&](auto& F) {
-    for (int k = 0; k < 3; ++k) launch_fill_zero(F.f[6 + k], nfield(h), h->stream);
-  });
+  with_fields(h, [&](auto& F) { launch_zero_j(F, nfield(h), h->stream); });
   return CYLPIC_OK;
 }
 
@@ -1278,7 +1278,6 @@ int cylpic_setup_field_boundaries(cylpic_ctx* h) {
 int cylpic_update_eb_fields_half(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
-  const size_t bytes = nfield(h) * cplx_bytes(h);
   {
     Scope s(h, "update_e");
     with_fields(h, [&](auto& F) { launch_update_e(h->g, F, h->stream); });
@@ -1288,13 +1287,9 @@ int cylpic_update_eb_fields_half(cylpic_ctx* h) {
     with_fields(h, [&](auto& F) { launch_efield_bcs(h->g, F, h->stream); });
   }
   {
-    Scope s(h, "copy_old");
-    for (int k = 0; k < 3; ++k)
-      HIPCHK(hipMemcpyAsync(field_ptr(h, 9 + k), field_ptr(h, 3 + k), bytes, hipMemcpyDeviceToDevice, h->stream));
-  }
-  {
+    // bxm_old = bxm (fields.f90:326-328) in the update's own pass
     Scope s(h, "update_b");
-    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, h->stream); });
+    with_fields(h, [&](auto& F) { launch_update_b(h->g, F, 1, h->stream); });
   }
   {
     Scope s(h, "field_bcs");
@@ -1834,19 +1829,28 @@ int cylpic_kernel_time(cylpic_ctx* h, const char* name, double* ms, int64_t* cou
 }  // extern "C"
 
 // ======================================================================
-// Batches of runs stepped through one HIP graph
+// Batches of runs stepped through HIP graphs
 // ======================================================================
-// Each step of a batch is captured into one graph: every run's timestep on
-// its own stream, forked from and joined to the batch's stream, and the
-// mixed pushes of all runs that share a mode count (with the default tile and
-// accumulator) in one persistent launch between the runs' field half
-// updates and the rest of their steps.  The capture runs the same host code
-// as an eager step, so the graph holds exactly the work an eager step would
-// launch and every run's host-side state (buffer swaps, push count, sort
-// schedule) advances as it would; a graph whose shape matches a cached one
-// only updates that executable's parameters.  Laser sources and push
-// descriptors go through pinned buffers in two slots used in turn, each
-// guarded by the event of the launch that read it.
+// A batch step is every run's timestep on its own stream, forked from and
+// joined to the batch's stream, with the mixed pushes of all runs that share
+// a mode count (with the default tile, accumulator and gather) in one
+// persistent launch between the runs' field half updates and the rest of
+// their steps.  It runs the same host code as an eager step, so every run's
+// host-side state (buffer swaps, push count, sort schedule) advances as it
+// would.
+//
+// cylpic_batch_steps goes in chunks of P steps, P the period of the runs'
+// buffer parities (the least common multiple of twice their sort intervals).
+// A chunk that starts from the state the previous chunk started and ended in
+// (a periodic run: no window move, injection or download in between) is
+// captured into one HIP graph, which is kept under that state's signature
+// and relaunched whenever a chunk starts from the same state, with only the
+// push counts and times advanced on the host.  Other chunks, and single
+// steps, run eagerly.  Laser sources go through a device ring of P steps that
+// the chunk's kernels read, filled by one copy on the batch's stream before
+// the chunk from one of two pinned buffers used in turn, each guarded by the
+// event of the chunk that last used it; push descriptors likewise through
+// pinned blocks (the graphs own theirs).
 struct cylpic_batch {
   int device = 0;
   std::vector<cylpic_ctx*> runs;
@@ -1863,30 +1867,67 @@ struct cylpic_batch {
     void* d_desc = nullptr;
     int* d_item_start = nullptr;
     int* d_counters = nullptr;
-    char* h_desc[2] = {nullptr, nullptr};
   };
   std::vector<Group> groups;
-  std::vector<double*> src_pin;  // per run: [slot][side][2 (ny + 1)]
-  std::map<std::string, hipGraphExec_t> execs;
-  long steps = 0, instantiations = 0;
+  std::vector<size_t> src_off;  // run r's sources start at src_off[r] within a step's block
+  size_t src_len = 0;           // doubles per step (both sides, every run)
+  size_t chunk_cap = 0;         // steps the buffers below hold
+  double* src_pin[2] = {nullptr, nullptr};
+  double* src_ring = nullptr;   // device
+  char* eager_desc[2] = {nullptr, nullptr};
+  struct Entry {
+    hipGraphExec_t exec = nullptr;
+    char* desc = nullptr;     // pinned descriptors the graph reads
+    std::vector<long> radix;  // host-decided radix sorts per run in the chunk
+    long chunk = 0;           // the chunk that last launched it
+  };
+  std::map<std::string, Entry> cache;  // periodic chunks, by start state
+  std::vector<Entry> retired;          // launched once, freed once that launch is done
+  std::string periodic;                // start state of the last chunk that ended where it started
+  long steps = 0, chunks = 0, instantiations = 0, replays = 0;
 };
 
 namespace {
+
+size_t desc_block_bytes(const cylpic_batch* b) {
+  size_t n = 0;
+  for (auto& gp : b->groups) n += gp.members.size() * push_desc_mixed_bytes() + (gp.members.size() + 1) * sizeof(int);
+  return n + 64;
+}
+
+void free_entry(cylpic_batch::Entry& e) {
+  if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  if (e.desc) (void)hipHostFree(e.desc);
+  e.exec = nullptr;
+  e.desc = nullptr;
+}
+
+void free_chunk_buffers(cylpic_batch* b) {
+  for (auto*& p : b->src_pin) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+  }
+  for (auto*& p : b->eager_desc) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+  }
+  if (b->src_ring) (void)hipFree(b->src_ring);
+  b->src_ring = nullptr;
+  b->chunk_cap = 0;
+}
 
 void batch_free(cylpic_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->origin) (void)hipStreamSynchronize(b->origin);
-  for (auto& kv : b->execs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : b->cache) free_entry(kv.second);
+  for (auto& e : b->retired) free_entry(e);
   for (auto& gp : b->groups) {
     if (gp.d_desc) (void)hipFree(gp.d_desc);
     if (gp.d_item_start) (void)hipFree(gp.d_item_start);
     if (gp.d_counters) (void)hipFree(gp.d_counters);
-    for (auto* p : gp.h_desc)
-      if (p) (void)hipHostFree(p);
   }
-  for (auto* p : b->src_pin)
-    if (p) (void)hipHostFree(p);
+  free_chunk_buffers(b);
   for (auto e : b->ready) (void)hipEventDestroy(e);
   for (auto e : b->done) (void)hipEventDestroy(e);
   for (auto e : {b->fork, b->pushed, b->launched[0], b->launched[1]})
@@ -1895,17 +1936,28 @@ void batch_free(cylpic_batch* b) {
   delete b;
 }
 
-// The work of one batch step on the streams (captured by the caller).
-int batch_enqueue(cylpic_batch* b, const double* const* src_xmin, const double* const* src_xmax, int slot,
-                  std::string& shape) {
+// The work of one batch step on the streams (eager, or captured by the
+// caller).  desc: pinned space for this step's push descriptors; src: this
+// step's sources in device memory (src_len doubles: per run, x_min then
+// x_max), null for none.
+int batch_enqueue(cylpic_batch* b, char* desc, const double* src) {
   const int nr = (int)b->runs.size();
   HIPCHK(hipEventRecord(b->fork, b->origin));
   for (int r = 0; r < nr; ++r) HIPCHK(hipStreamWaitEvent(b->runs[r]->stream, b->fork, 0));
-  // which (run, species) pushes go into a group's launch
   std::vector<std::vector<int>> grouped(nr);
   for (int r = 0; r < nr; ++r) grouped[r].assign(b->runs[r]->sp.size(), -1);
   for (int gi = 0; gi < (int)b->groups.size(); ++gi)
     for (auto& m : b->groups[gi].members) grouped[m.first][m.second] = gi;
+  // descriptor space of group gi: [members][desc], then the item starts
+  std::vector<char*> gdesc(b->groups.size());
+  {
+    char* p = desc;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+      gdesc[gi] = p;
+      const size_t nm = b->groups[gi].members.size();
+      p += nm * push_desc_mixed_bytes() + (nm + 1) * sizeof(int);
+    }
+  }
   std::vector<int> ndesc(b->groups.size(), 0);
   std::vector<std::vector<int>> items(b->groups.size());
   for (int r = 0; r < nr; ++r) {
@@ -1918,9 +1970,7 @@ int batch_enqueue(cylpic_batch* b, const double* const* src_xmin, const double* 
       if (s.immobile || s.n <= 0) continue;
       const int gi = grouped[r][k];
       if (gi >= 0) {
-        auto& gp = b->groups[gi];
-        char* d = gp.h_desc[slot] + (size_t)ndesc[gi] * push_desc_mixed_bytes();
-        rc = push_species(h, s, d);
+        rc = push_species(h, s, gdesc[gi] + (size_t)ndesc[gi] * push_desc_mixed_bytes());
         ndesc[gi] += 1;
         items[gi].push_back(h->tl.ntiles);
       } else {
@@ -1934,46 +1984,125 @@ int batch_enqueue(cylpic_batch* b, const double* const* src_xmin, const double* 
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     auto& gp = b->groups[gi];
     if (!ndesc[gi]) continue;
-    // descriptor d owns work items [start[d], start[d + 1]), after the descriptors
-    std::vector<int> start(ndesc[gi] + 1, 0);
+    // descriptor d owns work items [start[d], start[d + 1])
+    int* start = (int*)(gdesc[gi] + gp.members.size() * push_desc_mixed_bytes());
+    start[0] = 0;
     for (int d = 0; d < ndesc[gi]; ++d) start[d + 1] = start[d] + items[gi][d];
-    char* hd = gp.h_desc[slot];
-    const size_t db = (size_t)ndesc[gi] * push_desc_mixed_bytes();
-    std::memcpy(hd + gp.members.size() * push_desc_mixed_bytes(), start.data(), start.size() * sizeof(int));
-    HIPCHK(hipMemcpyAsync(gp.d_desc, hd, db, hipMemcpyHostToDevice, b->origin));
-    HIPCHK(hipMemcpyAsync(gp.d_item_start, hd + gp.members.size() * push_desc_mixed_bytes(),
-                          start.size() * sizeof(int), hipMemcpyHostToDevice, b->origin));
+    HIPCHK(hipMemcpyAsync(gp.d_desc, gdesc[gi], (size_t)ndesc[gi] * push_desc_mixed_bytes(), hipMemcpyHostToDevice,
+                          b->origin));
+    HIPCHK(hipMemcpyAsync(gp.d_item_start, start, (ndesc[gi] + 1) * sizeof(int), hipMemcpyHostToDevice, b->origin));
     launch_push_deposit_mixed_batch(gp.d_desc, gp.d_item_start, ndesc[gi], gp.M, gp.lds, gp.d_counters, gp.nblocks,
                                     b->origin);
-    shape += "g" + std::to_string(gi) + ":" + std::to_string(ndesc[gi]) + ";";
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->pushed, b->origin));
   for (int r = 0; r < nr; ++r) {
     cylpic_ctx* h = b->runs[r];
     HIPCHK(hipStreamWaitEvent(h->stream, b->pushed, 0));
-    const long pc = h->push_count;
     int rc = push_end(h);
     if (!rc) rc = cylpic_current_finish(h);
     if (rc) return rc;
     h->time += h->g.dt;
-    const size_t sb = 2 * (size_t)(h->g.ny + 1) * sizeof(double);
-    const double* dsrc[2] = {nullptr, nullptr};
-    const double* const* hsrc[2] = {src_xmin, src_xmax};
-    for (int k = 0; k < 2; ++k) {
-      if (!hsrc[k] || !hsrc[k][r]) continue;
-      double* pin = b->src_pin[r] + (size_t)(2 * slot + k) * 2 * (h->g.ny + 1);
-      HIPCHK(hipMemcpyAsync(h->src_dev[k], pin, sb, hipMemcpyHostToDevice, h->stream));
-      dsrc[k] = h->src_dev[k];
-    }
-    rc = fields_final(h, dsrc[0], dsrc[1]);
+    const size_t sb = 2 * (size_t)(h->g.ny + 1);
+    const double* lo = src ? src + b->src_off[r] : nullptr;
+    rc = fields_final(h, lo, src ? lo + sb : nullptr);
     if (rc) return rc;
     HIPCHK(hipEventRecord(b->done[r], h->stream));
     HIPCHK(hipStreamWaitEvent(b->origin, b->done[r], 0));
-    shape += "r" + std::to_string(r) + (pc % h->sort_interval == h->sort_interval - 1 ? "s" : "") +
-             (dsrc[0] ? "a" : "") + (dsrc[1] ? "b" : "") + ";";
   }
   return CYLPIC_OK;
+}
+
+// Captures `nsteps` steps into an instantiated graph (descriptors in `desc`,
+// the sources of step j at src + j * src_len in device memory, or none).
+int batch_capture(cylpic_batch* b, int nsteps, char* desc, const double* src, hipGraphExec_t* exec) {
+  std::vector<bool> prof;
+  for (cylpic_ctx* h : b->runs) {
+    prof.push_back(h->profiling);
+    h->profiling = false;  // no profiling events inside a graph
+  }
+  HIPCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+  int rc = CYLPIC_OK;
+  const size_t db = desc_block_bytes(b);
+  for (int j = 0; j < nsteps && !rc; ++j) rc = batch_enqueue(b, desc + j * db, src ? src + j * b->src_len : nullptr);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->origin, &graph);
+  for (size_t r = 0; r < b->runs.size(); ++r) b->runs[r]->profiling = prof[r];
+  if (rc || ec != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc ? rc : fail(CYLPIC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(ec));
+  }
+  const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(CYLPIC_EHIP, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+  b->instantiations += 1;
+  return CYLPIC_OK;
+}
+
+template <class T>
+void put(std::string& s, const T& v) {
+  s.append(reinterpret_cast<const char*>(&v), sizeof(v));
+}
+
+// Everything of a run that a step's launches or their parameters depend on.
+void run_signature(std::string& s, const cylpic_ctx* h, int period) {
+  const Grid& g = h->g;
+  for (int v : {g.nx, g.ny, g.M, g.ng}) put(s, v);
+  for (double v : {g.dx, g.dy, g.dt, g.x_min, g.x_max, g.y_min, g.y_max, g.x_grid_min, g.y_grid_min}) put(s, v);
+  for (int k = 0; k < 4; ++k) {
+    put(s, g.bc_field[k]);
+    put(s, g.bc_particle[k]);
+  }
+  for (int v : {h->tl.tx, h->tl.tr, h->tl.ntx, h->tl.ntr, h->tl.ntiles, h->tl.shift}) put(s, v);
+  for (int k = 0; k < 15; ++k) {
+    put(s, (const void*)h->f64.f[k]);
+    put(s, (const void*)h->f32.f[k]);
+  }
+  for (int k = 0; k < 6; ++k) {
+    put(s, (const void*)h->f64.bnd_xmin[k]);
+    put(s, (const void*)h->f64.bnd_xmax[k]);
+    put(s, (const void*)h->f32.bnd_xmin[k]);
+    put(s, (const void*)h->f32.bnd_xmax[k]);
+  }
+  for (const void* p : {(const void*)h->spare64, (const void*)h->spare32, (const void*)h->mig, (const void*)h->sc_mem,
+                        (const void*)h->src_dev[0], (const void*)h->src_dev[1], (const void*)h->geo,
+                        (const void*)h->geof, (const void*)h->d_err, (const void*)h->d_sortstat,
+                        (const void*)h->sc.k[0], (const void*)h->sc.k[1], (const void*)h->sc.v[0],
+                        (const void*)h->sc.v[1], (const void*)h->sc.counts, (const void*)h->sc.scan_tmp})
+    put(s, p);
+  put(s, h->sc.cap);
+  for (int v : {h->prec, h->ntab, h->sort_interval, h->slot_bits, (int)h->migrate, h->acc_mode, h->gather_lds,
+                h->smooth_its, h->smooth_comp_its, (int)h->cont_armed, (int)(h->push_count % period)})
+    put(s, v);
+  for (int v : h->smooth_strides) put(s, v);
+  for (const Species& sp : h->sp) {
+    put(s, sp.q);
+    put(s, sp.m);
+    for (long v : {sp.cap, sp.n, sp.n_extra}) put(s, v);
+    for (int v : {sp.immobile, (int)sp.n_exact, (int)sp.ids, (int)sp.dirty, (int)sp.ranged, (int)sp.counted}) put(s, v);
+    for (int k = 0; k < 7; ++k) {
+      put(s, (const void*)sp.a[k]);
+      put(s, (const void*)sp.b[k]);
+    }
+    for (int k = 0; k < 8; ++k) {
+      put(s, (const void*)sp.ma[k]);
+      put(s, (const void*)sp.mb[k]);
+    }
+    for (const void* p : {(const void*)sp.id[0], (const void*)sp.id[1], (const void*)sp.key[0], (const void*)sp.key[1],
+                          (const void*)sp.tile_start, (const void*)sp.tile_start_alt, (const void*)sp.extra_start,
+                          (const void*)sp.d_count, (const void*)sp.mig_counts, (const void*)sp.mig_counts_x,
+                          (const void*)sp.mig_flag})
+      put(s, p);
+  }
+}
+
+// The least common multiple of the runs' doubled sort intervals, the period
+// of every run's buffer parities, capped at 64 steps (a longer period only
+// means chunks that never repeat: each is captured afresh).
+int batch_period(const cylpic_batch* b) {
+  long p = 2;
+  for (const cylpic_ctx* h : b->runs) p = std::min(64L, std::lcm(p, 2L * std::max(1, h->sort_interval)));
+  return (int)p;
 }
 
 }  // namespace
@@ -2016,7 +2145,7 @@ int cylpic_batch_create(cylpic_batch** out, cylpic_ctx* const* runs, int nruns) 
     if (!h->mig && hipMalloc((void**)&h->mig, mig_scratch_words(h->tl.ntiles) * sizeof(uint32_t)) != hipSuccess)
       return bail(fail(CYLPIC_ENOMEM, "migration scratch allocation failed"));
     long cap = 0;
-    for (auto& s : h->sp) cap = std::max(cap, s.cap);
+    for (auto& sp : h->sp) cap = std::max(cap, sp.cap);
     if (int rc = ensure_sort_scratch(h, cap)) return bail(rc);
     if (h->smooth_its + h->smooth_comp_its > 0) {
       const long nf = nfield(h);
@@ -2027,10 +2156,8 @@ int cylpic_batch_create(cylpic_batch** out, cylpic_ctx* const* runs, int nruns) 
           hipMalloc((void**)&h->spare32, nf * sizeof(cplx<float>)) != hipSuccess)
         return bail(fail(CYLPIC_ENOMEM, "smoothing buffer allocation failed"));
     }
-    double* pin = nullptr;
-    if (hipHostMalloc((void**)&pin, 4 * 2 * (size_t)(h->g.ny + 1) * sizeof(double), 0) != hipSuccess)
-      return bail(fail(CYLPIC_ENOMEM, "pinned source allocation failed"));
-    b->src_pin.push_back(pin);
+    b->src_off.push_back(b->src_len);
+    b->src_len += 4 * (size_t)(h->g.ny + 1);
   }
   // the groups of the batched push: mixed species with the default tile,
   // accumulator and gather, by mode count
@@ -2058,9 +2185,6 @@ int cylpic_batch_create(cylpic_batch** out, cylpic_ctx* const* runs, int nruns) 
         hipMalloc((void**)&gp.d_item_start, (nd + 1) * sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&gp.d_counters, NUM_XCD * sizeof(int)) != hipSuccess)
       return bail(fail(CYLPIC_ENOMEM, "batch allocation failed"));
-    for (auto*& p : gp.h_desc)
-      if (hipHostMalloc((void**)&p, nd * push_desc_mixed_bytes() + (nd + 1) * sizeof(int), 0) != hipSuccess)
-        return bail(fail(CYLPIC_ENOMEM, "pinned descriptor allocation failed"));
   }
   for (cylpic_ctx* h : b->runs)
     if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(CYLPIC_EHIP, "synchronize failed"));
@@ -2068,62 +2192,138 @@ int cylpic_batch_create(cylpic_batch** out, cylpic_ctx* const* runs, int nruns) 
   return CYLPIC_OK;
 }
 
+// Chunk buffers for `steps` steps (outside any capture).  The kept graphs
+// read the ring being replaced: they go too.
+static int ensure_chunk_buffers(cylpic_batch* b, size_t steps) {
+  if (steps <= b->chunk_cap) return CYLPIC_OK;
+  HIPCHK(hipStreamSynchronize(b->origin));
+  for (auto& kv : b->cache) free_entry(kv.second);
+  b->cache.clear();
+  for (auto& e : b->retired) free_entry(e);
+  b->retired.clear();
+  free_chunk_buffers(b);
+  const size_t sbytes = steps * std::max<size_t>(b->src_len, 1) * sizeof(double);
+  for (int k = 0; k < 2; ++k) {
+    if (hipHostMalloc((void**)&b->src_pin[k], sbytes, 0) != hipSuccess ||
+        hipHostMalloc((void**)&b->eager_desc[k], steps * desc_block_bytes(b), 0) != hipSuccess)
+      return fail(CYLPIC_ENOMEM, "pinned batch buffer allocation failed");
+  }
+  if (hipMalloc((void**)&b->src_ring, sbytes) != hipSuccess)
+    return fail(CYLPIC_ENOMEM, "batch source ring allocation failed");
+  b->chunk_cap = steps;
+  return CYLPIC_OK;
+}
+
+static std::string batch_signature(const cylpic_batch* b, int nsteps, bool src) {
+  const int period = batch_period(b);
+  std::string key;
+  put(key, nsteps);
+  put(key, (int)src);
+  for (const cylpic_ctx* h : b->runs) run_signature(key, h, period);
+  return key;
+}
+
+// One chunk of `nsteps` steps: a kept graph when the runs start from a state
+// seen before and periodic; a new graph (kept if the chunk turns out
+// periodic) when the previous chunk started and ended in this start state;
+// else eagerly.  src: host sources, nsteps blocks of src_len doubles, or null.
+static int batch_chunk(cylpic_batch* b, int nsteps, const double* src, bool graphs) {
+  const int parity = b->slot;
+  b->slot ^= 1;
+  const long chunk = b->chunks++;
+  // the buffers of this parity were last used by the chunk before last;
+  // chunks complete in order, so every graph retired before it is free
+  HIPCHK(hipEventSynchronize(b->launched[parity]));
+  {
+    size_t keep = 0;
+    for (size_t i = 0; i < b->retired.size(); ++i) {
+      if (b->retired[i].chunk <= chunk - 2) free_entry(b->retired[i]);
+      else b->retired[keep++] = b->retired[i];
+    }
+    b->retired.resize(keep);
+  }
+  if (int rc = ensure_chunk_buffers(b, (size_t)nsteps)) return rc;
+  if (src) {
+    std::memcpy(b->src_pin[parity], src, (size_t)nsteps * b->src_len * sizeof(double));
+    HIPCHK(hipMemcpyAsync(b->src_ring, b->src_pin[parity], (size_t)nsteps * b->src_len * sizeof(double),
+                          hipMemcpyHostToDevice, b->origin));
+  }
+  const double* ring = src ? b->src_ring : nullptr;
+  const std::string key = graphs ? batch_signature(b, nsteps, src != nullptr) : std::string();
+  if (graphs) {
+    auto it = b->cache.find(key);
+    if (it != b->cache.end()) {
+      HIPCHK(hipGraphLaunch(it->second.exec, b->origin));
+      HIPCHK(hipEventRecord(b->launched[parity], b->origin));
+      it->second.chunk = chunk;
+      for (size_t r = 0; r < b->runs.size(); ++r) {
+        cylpic_ctx* h = b->runs[r];
+        h->push_count += nsteps;
+        h->time += nsteps * h->g.dt;
+        h->sort_radix += it->second.radix[r];
+      }
+      b->steps += nsteps;
+      b->replays += 1;
+      return CYLPIC_OK;
+    }
+  }
+  if (graphs && key == b->periodic) {
+    cylpic_batch::Entry e;
+    e.chunk = chunk;
+    for (const cylpic_ctx* h : b->runs) e.radix.push_back(-h->sort_radix);
+    HIPCHK(hipHostMalloc((void**)&e.desc, (size_t)nsteps * desc_block_bytes(b), 0));
+    int rc = batch_capture(b, nsteps, e.desc, ring, &e.exec);
+    if (rc) {
+      free_entry(e);
+      return rc;
+    }
+    HIPCHK(hipGraphLaunch(e.exec, b->origin));
+    HIPCHK(hipEventRecord(b->launched[parity], b->origin));
+    b->steps += nsteps;
+    for (size_t r = 0; r < b->runs.size(); ++r) e.radix[r] += b->runs[r]->sort_radix;
+    if (batch_signature(b, nsteps, src != nullptr) == key && b->cache.size() < 8) b->cache[key] = e;
+    else b->retired.push_back(e);
+    return CYLPIC_OK;
+  }
+  const size_t db = desc_block_bytes(b);
+  for (int j = 0; j < nsteps; ++j) {
+    int rc = batch_enqueue(b, b->eager_desc[parity] + j * db, ring ? ring + j * b->src_len : nullptr);
+    if (rc) return rc;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->launched[parity], b->origin));
+  b->steps += nsteps;
+  if (graphs && batch_signature(b, nsteps, src != nullptr) == key) b->periodic = key;
+  return CYLPIC_OK;
+}
+
 int cylpic_batch_step(cylpic_batch* b, const double* const* src_xmin, const double* const* src_xmax) {
   if (!b) return fail(CYLPIC_EINVAL, "null batch");
   HIPCHK(hipSetDevice(b->device));
-  const int slot = b->slot;
-  b->slot ^= 1;
-  // the pinned slot was last read by the launch two steps ago
-  HIPCHK(hipEventSynchronize(b->launched[slot]));
-  for (int r = 0; r < (int)b->runs.size(); ++r) {
-    cylpic_ctx* h = b->runs[r];
-    const size_t nb = 2 * (size_t)(h->g.ny + 1);
-    if (src_xmin && src_xmin[r]) std::memcpy(b->src_pin[r] + (2 * slot) * nb, src_xmin[r], nb * sizeof(double));
-    if (src_xmax && src_xmax[r]) std::memcpy(b->src_pin[r] + (2 * slot + 1) * nb, src_xmax[r], nb * sizeof(double));
-  }
-  // no profiling events inside the graph
-  std::vector<bool> prof;
-  for (cylpic_ctx* h : b->runs) {
-    prof.push_back(h->profiling);
-    h->profiling = false;
-  }
-  HIPCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
-  std::string shape = "s" + std::to_string(slot) + ";";
-  int rc = batch_enqueue(b, src_xmin, src_xmax, slot, shape);
-  hipGraph_t graph = nullptr;
-  const hipError_t ec = hipStreamEndCapture(b->origin, &graph);
-  for (size_t r = 0; r < b->runs.size(); ++r) b->runs[r]->profiling = prof[r];
-  if (rc) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
-  }
-  if (ec != hipSuccess || !graph) return fail(CYLPIC_EHIP, std::string("graph capture failed: ") + hipGetErrorString(ec));
-  auto it = b->execs.find(shape);
-  bool ok = false;
-  if (it != b->execs.end()) {
-    hipGraphNode_t err_node = nullptr;
-    hipGraphExecUpdateResult res;
-    ok = hipGraphExecUpdate(it->second, graph, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
-    if (!ok) {
-      (void)hipGetLastError();
-      (void)hipGraphExecDestroy(it->second);
-      b->execs.erase(it);
+  std::vector<double> src;
+  if (src_xmin || src_xmax) {
+    src.assign(b->src_len, 0.0);
+    for (size_t r = 0; r < b->runs.size(); ++r) {
+      const size_t nb = 2 * (size_t)(b->runs[r]->g.ny + 1);
+      if (src_xmin && src_xmin[r]) std::memcpy(&src[b->src_off[r]], src_xmin[r], nb * sizeof(double));
+      if (src_xmax && src_xmax[r]) std::memcpy(&src[b->src_off[r] + nb], src_xmax[r], nb * sizeof(double));
     }
   }
-  if (!ok) {
-    hipGraphExec_t exec = nullptr;
-    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ei != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return fail(CYLPIC_EHIP, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
-    }
-    b->execs[shape] = exec;
-    b->instantiations += 1;
+  return batch_chunk(b, 1, src.empty() ? nullptr : src.data(), false);
+}
+
+int cylpic_batch_steps(cylpic_batch* b, int nsteps, const double* src) {
+  if (!b) return fail(CYLPIC_EINVAL, "null batch");
+  if (nsteps < 0) return fail(CYLPIC_EINVAL, "negative step count");
+  HIPCHK(hipSetDevice(b->device));
+  const int period = batch_period(b);
+  int done = 0;
+  while (done < nsteps) {
+    const int n = std::min(period, nsteps - done);
+    int rc = batch_chunk(b, n, src ? src + (size_t)done * b->src_len : nullptr, n == period);
+    if (rc) return rc;
+    done += n;
   }
-  (void)hipGraphDestroy(graph);
-  HIPCHK(hipGraphLaunch(b->execs[shape], b->origin));
-  HIPCHK(hipEventRecord(b->launched[slot], b->origin));
-  b->steps += 1;
   return CYLPIC_OK;
 }
 
@@ -2131,6 +2331,8 @@ int cylpic_batch_synchronize(cylpic_batch* b) {
   if (!b) return fail(CYLPIC_EINVAL, "null batch");
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->origin));
+  for (auto& e : b->retired) free_entry(e);
+  b->retired.clear();
   for (cylpic_ctx* h : b->runs) {
     int rc = check_device_errors(h);
     if (rc) return rc;
@@ -2138,10 +2340,11 @@ int cylpic_batch_synchronize(cylpic_batch* b) {
   return CYLPIC_OK;
 }
 
-int cylpic_batch_stats(cylpic_batch* b, int64_t* steps, int64_t* instantiations, int* push_groups) {
+int cylpic_batch_stats(cylpic_batch* b, int64_t* steps, int64_t* instantiations, int64_t* replays, int* push_groups) {
   if (!b) return fail(CYLPIC_EINVAL, "null batch");
   if (steps) *steps = b->steps;
   if (instantiations) *instantiations = b->instantiations;
+  if (replays) *replays = b->replays;
   if (push_groups) *push_groups = (int)b->groups.size();
   return CYLPIC_OK;
 }

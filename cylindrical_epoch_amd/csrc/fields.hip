@@ -129,25 +129,36 @@ __global__ __launch_bounds__(256) void k_e_axis(Grid g, cplx<T>* __restrict__ ex
 
 // ------------------------------------------------------------------------
 // update_b_field interior, fields.f90:203-241: ix in 0..nx, ir in 1..ny-1
-// (r_max local).
+// (r_max local).  COPY = 1 also copies every element of B, ghosts included,
+// to B_old before updating it (the half step's bxm_old = bxm,
+// fields.f90:326-328, in the same pass): the grid then spans the whole
+// array.
 // ------------------------------------------------------------------------
-template <class T>
+template <class T, int COPY>
 __global__ __launch_bounds__(256) void k_update_b(Grid g, cplx<T>* __restrict__ bx,
                                                   cplx<T>* __restrict__ br, cplx<T>* __restrict__ bt,
                                                   const cplx<T>* __restrict__ ex,
                                                   const cplx<T>* __restrict__ er,
-                                                  const cplx<T>* __restrict__ et) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ir = blockIdx.y + 1;
+                                                  const cplx<T>* __restrict__ et, cplx<T>* __restrict__ bx_old,
+                                                  cplx<T>* __restrict__ br_old, cplx<T>* __restrict__ bt_old) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x + (COPY ? 1 - g.ng : 0);
+  const int ir = blockIdx.y + (COPY ? 1 - g.ng : 1);
   const int m = blockIdx.z;
-  if (ix > g.nx) return;
+  if (ix > (COPY ? g.nx + g.ng : g.nx)) return;
+  const long k = g.idx(ix, ir, m);
+  const cplx<T> bx0 = bx[k], br0 = br[k], bt0 = bt[k];
+  if (COPY) {
+    bx_old[k] = bx0;
+    br_old[k] = br0;
+    bt_old[k] = bt0;
+    if (ix < 0 || ix > g.nx || ir < 1 || ir > g.ny - 1) return;
+  }
   const double r_dd = fabs((double)(ir - 1) * g.dy + g.y_grid_min);
   const double r_pd = r_dd + 0.5 * g.dy;
   const T r_d = (T)r_dd;
   const T a_x = (T)((double)m / r_dd);
   const T a_r = (T)((double)m / r_pd);
   const T dx = (T)g.dx, dy = (T)g.dy, dt = (T)g.dt, half = (T)0.5;
-  const long k = g.idx(ix, ir, m);
   const long krm = k - g.nx2;
   const long kxm = k - 1;
   const cplx<T> et0 = et[k], et_rm = et[krm], et_xm = et[kxm];
@@ -158,19 +169,19 @@ __global__ __launch_bounds__(256) void k_update_b(Grid g, cplx<T>* __restrict__ 
     cplx<T> t2 = cdivr(rmul(half, cadd(et0, et_rm)), r_d);
     cplx<T> t3 = cdivr(csub(et0, et_rm), dy);
     cplx<T> s = cadd(cadd(t1, t2), t3);
-    bx[k] = csub(bx[k], cmulr(cmulr(s, half), dt));
+    bx[k] = csub(bx0, cmulr(cmulr(s, half), dt));
   }
   {  // (6) brm
     cplx<T> t1 = imul(a_r, ex0);
     cplx<T> t2 = cdivr(csub(et0, et_xm), dx);
     cplx<T> s = cadd(t1, t2);
-    br[k] = cadd(br[k], cmulr(cmulr(s, half), dt));
+    br[k] = cadd(br0, cmulr(cmulr(s, half), dt));
   }
   {  // (7) btm
     cplx<T> t1 = cneg(cdivr(csub(er0, er_xm), dx));
     cplx<T> t2 = cdivr(csub(ex0, ex_rm), dy);
     cplx<T> s = cadd(t1, t2);
-    bt[k] = cadd(bt[k], cmulr(cmulr(s, half), dt));
+    bt[k] = cadd(bt0, cmulr(cmulr(s, half), dt));
   }
 }
 
@@ -238,9 +249,8 @@ __global__ __launch_bounds__(256) void k_b_axis(Grid g, cplx<T>* __restrict__ bx
 // row_lo..row_hi.  One thread per (row, mode), ng columns each side.
 // ------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_halo_x(Grid g, cplx<T>* __restrict__ f, int row_lo,
-                                                int row_hi, int do_max, int do_min) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void halo_x_body(const Grid& g, cplx<T>* __restrict__ f, int t, int row_lo, int row_hi,
+                                            int do_max, int do_min) {
   const int nrow = row_hi - row_lo + 1;
   if (t >= nrow * g.M) return;
   const int j = row_lo + t % nrow, m = t / nrow;
@@ -251,13 +261,18 @@ __global__ __launch_bounds__(256) void k_halo_x(Grid g, cplx<T>* __restrict__ f,
     for (int i = 1; i <= ng; ++i) f[g.idx(i - ng, j, m)] = f[g.idx(nx - ng + i, j, m)];
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_halo_x(Grid g, cplx<T>* __restrict__ f, int row_lo,
+                                                int row_hi, int do_max, int do_min) {
+  halo_x_body(g, f, (int)(blockIdx.x * blockDim.x + threadIdx.x), row_lo, row_hi, do_max, do_min);
+}
+
 // field_mode_clamp_zero / field_mode_zero_gradient, boundary.F90:654-829,
 // for x_min, x_max (one thread per (row, mode)) or y_max (one thread per
 // (column, mode)).
 template <class T>
-__global__ __launch_bounds__(256) void k_edge(Grid g, cplx<T>* __restrict__ f, int stag_x,
-                                              int stag_y, int bnd, int clamp) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void edge_body(const Grid& g, cplx<T>* __restrict__ f, int t, int stag_x, int stag_y,
+                                          int bnd, int clamp) {
   const int nx = g.nx, ny = g.ny, ng = g.ng;
   auto setv = [&](long a, long b) { f[a] = clamp ? cneg(f[b]) : f[b]; };
   if (bnd == XMIN || bnd == XMAX) {
@@ -290,6 +305,35 @@ __global__ __launch_bounds__(256) void k_edge(Grid g, cplx<T>* __restrict__ f, i
       for (int i = 1; i <= ng; ++i) setv(g.idx(ix, ny + i, m), g.idx(ix, ny + 1 - i, m));
     }
   }
+}
+
+// The halo copies and edge rules of one boundary call, as a list of
+// operations on whole fields: one launch runs a group of operations that
+// commute (blockIdx.y picks the operation), and a group ends where the next
+// operation does not commute with one in it.  Operations on different
+// fields commute; on one field only the x_min and x_max edges do (they touch
+// disjoint columns, nx >= 2 ng); an x operation and a y_max edge of one field
+// meet at the corners, so they keep their order.
+enum { BC_OP_HALO = 0, BC_OP_EDGE = 1 };
+template <class T>
+struct BcOp {
+  cplx<T>* f;
+  int kind, n;              // n: threads
+  int a, b, c, d;           // halo: row_lo, row_hi, do_max, do_min; edge: stag_x, stag_y, bnd, clamp
+};
+constexpr int MAX_BC_OPS = 12;
+template <class T>
+struct BcOpList {
+  BcOp<T> op[MAX_BC_OPS];
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bc_ops(Grid g, BcOpList<T> L) {
+  const BcOp<T>& o = L.op[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= o.n) return;
+  if (o.kind == BC_OP_HALO) halo_x_body(g, o.f, t, o.a, o.b, o.c, o.d);
+  else edge_body(g, o.f, t, o.a, o.b, o.c, o.d);
 }
 
 // ------------------------------------------------------------------------
@@ -499,11 +543,20 @@ __global__ __launch_bounds__(256) void k_current_axis(Grid g, cplx<T>* __restric
   }
 }
 
+// The three current components of one kernel's launch: blockIdx.y = 0, 1, 2
+// picks jx, jr, jt (flip = blockIdx.y + 1); the components are independent.
+template <class T>
+struct Jxrt {
+  cplx<T>* j[3];
+};
+
 // particle_reflection_bcs_complex (boundary.F90:918-1015) + the one-rank
-// particle_periodic_bcs_complex (:1133-1245) for one component (flip = 1, 2,
+// particle_periodic_bcs_complex (:1133-1245), each component (flip = 1, 2,
 // 3 for jx, jr, jt), all modes.  x part: one thread per (row, mode).
 template <class T>
-__global__ void k_current_x(Grid g, cplx<T>* __restrict__ J, int flip) {
+__global__ void k_current_x(Grid g, Jxrt<T> jj) {
+  cplx<T>* __restrict__ J = jj.j[blockIdx.y];
+  const int flip = blockIdx.y + 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nrow = g.ny2;
   if (t >= nrow * g.M) return;
@@ -541,7 +594,9 @@ __global__ void k_current_x(Grid g, cplx<T>* __restrict__ J, int flip) {
 // (column, mode); runs after the x part (the reference applies x_min, x_max,
 // then y_max within one call).
 template <class T>
-__global__ void k_current_rmax(Grid g, cplx<T>* __restrict__ J, int flip) {
+__global__ void k_current_rmax(Grid g, Jxrt<T> jj) {
+  cplx<T>* __restrict__ J = jj.j[blockIdx.y];
+  const int flip = blockIdx.y + 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.nx2 * g.M) return;
   const int ix = 1 - g.ng + t % g.nx2, m = t / g.nx2;
@@ -566,7 +621,8 @@ __global__ void k_current_rmax(Grid g, cplx<T>* __restrict__ J, int flip) {
 
 // periodic ghost sum on one rank, one thread per (row, mode)
 template <class T>
-__global__ void k_current_periodic(Grid g, cplx<T>* __restrict__ J) {
+__global__ void k_current_periodic(Grid g, Jxrt<T> jj) {
+  cplx<T>* __restrict__ J = jj.j[blockIdx.y];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nrow = g.ny2;
   if (t >= nrow * g.M) return;
@@ -649,29 +705,64 @@ void launch_update_e(const Grid& g, FieldSet<T>& F, hipStream_t s) {
 }
 
 template <class T>
-void launch_update_b(const Grid& g, FieldSet<T>& F, hipStream_t s) {
-  dim3 grid(nblk(g.nx + 1), g.ny - 1, g.M);
-  if (g.ny > 1)
-    hipLaunchKernelGGL(k_update_b<T>, grid, dim3(256), 0, s, g, F.bx(), F.br(), F.bt(), F.ex(), F.er(),
-                       F.et());
+void launch_update_b(const Grid& g, FieldSet<T>& F, int copy_old, hipStream_t s) {
+  if (copy_old)
+    hipLaunchKernelGGL((k_update_b<T, 1>), dim3(nblk(g.nx2), g.ny2, g.M), dim3(256), 0, s, g, F.bx(), F.br(),
+                       F.bt(), F.ex(), F.er(), F.et(), F.bx_old(), F.br_old(), F.bt_old());
+  else if (g.ny > 1)
+    hipLaunchKernelGGL((k_update_b<T, 0>), dim3(nblk(g.nx + 1), g.ny - 1, g.M), dim3(256), 0, s, g, F.bx(),
+                       F.br(), F.bt(), F.ex(), F.er(), F.et(), nullptr, nullptr, nullptr);
   hipLaunchKernelGGL(k_b_axis<T>, dim3(nblk(g.nx2)), dim3(256), 0, s, g, F.bx(), F.br(), F.bt(), F.ex(),
                      F.et());
 }
 
 template <class T>
-static void halo(const Grid& g, cplx<T>* f, int row_lo, int row_hi, hipStream_t s) {
-  int do_max = g.bc_field[XMAX] == BC_PERIODIC, do_min = g.bc_field[XMIN] == BC_PERIODIC;
-  if (!do_max && !do_min) return;
-  long n = (long)(row_hi - row_lo + 1) * g.M;
-  hipLaunchKernelGGL(k_halo_x<T>, dim3(nblk(n)), dim3(256), 0, s, g, f, row_lo, row_hi, do_max, do_min);
-}
+class BcQueue {
+ public:
+  BcQueue(const Grid& g, hipStream_t s) : g_(g), s_(s) {}
+  ~BcQueue() { flush(); }
+  // halo rows row_lo..row_hi of f when x is periodic
+  void halo(cplx<T>* f, int row_lo, int row_hi) {
+    const int do_max = g_.bc_field[XMAX] == BC_PERIODIC, do_min = g_.bc_field[XMIN] == BC_PERIODIC;
+    if (!do_max && !do_min) return;
+    add(BcOp<T>{f, BC_OP_HALO, (row_hi - row_lo + 1) * g_.M, row_lo, row_hi, do_max, do_min});
+  }
+  void edge(cplx<T>* f, int sx, int sy, int bnd, int clamp) {
+    if (g_.bc_field[bnd] == BC_PERIODIC) return;
+    const int n = (bnd == YMAX ? g_.nx2 : g_.ny2) * g_.M;
+    add(BcOp<T>{f, BC_OP_EDGE, n, sx, sy, bnd, clamp});
+  }
+  void flush() {
+    if (!n_) return;
+    BcOpList<T> L{};
+    int most = 0;
+    for (int i = 0; i < n_; ++i) {
+      L.op[i] = ops_[i];
+      most = std::max(most, ops_[i].n);
+    }
+    hipLaunchKernelGGL(k_bc_ops<T>, dim3(nblk(most), n_), dim3(256), 0, s_, g_, L);
+    n_ = 0;
+  }
 
-template <class T>
-static void edge(const Grid& g, cplx<T>* f, int sx, int sy, int bnd, int clamp, hipStream_t s) {
-  if (g.bc_field[bnd] == BC_PERIODIC) return;
-  long n = (bnd == YMAX ? (long)g.nx2 : (long)g.ny2) * g.M;
-  hipLaunchKernelGGL(k_edge<T>, dim3(nblk(n)), dim3(256), 0, s, g, f, sx, sy, bnd, clamp);
-}
+ private:
+  static bool commute(const BcOp<T>& x, const BcOp<T>& y) {
+    if (x.f != y.f) return true;
+    return x.kind == BC_OP_EDGE && y.kind == BC_OP_EDGE && x.c != YMAX && y.c != YMAX && x.c != y.c;
+  }
+  void add(const BcOp<T>& o) {
+    for (int i = 0; i < n_; ++i)
+      if (!commute(ops_[i], o)) {
+        flush();
+        break;
+      }
+    if (n_ == MAX_BC_OPS) flush();
+    ops_[n_++] = o;
+  }
+  const Grid& g_;
+  hipStream_t s_;
+  BcOp<T> ops_[MAX_BC_OPS];
+  int n_ = 0;
+};
 
 // stagger flags (x, y) of each component, constants.F90:291-299
 enum { S_EX, S_ER, S_ET, S_BX, S_BR, S_BT };
@@ -679,17 +770,17 @@ static const int STX[6] = {0, 1, 1, 1, 0, 0};
 static const int STY[6] = {1, 0, 1, 0, 1, 0};
 
 template <class T>
-static void generic_edges(const Grid& g, cplx<T>* a, cplx<T>* b, cplx<T>* c, int sa, int sb, int sc,
-                          hipStream_t s) {
+static void generic_edges(BcQueue<T>& q, const Grid& g, cplx<T>* a, cplx<T>* b, cplx<T>* c, int sa, int sb,
+                          int sc) {
   for (int i = XMIN; i <= YMAX; ++i) {
     if (i == YMIN) continue;
     int bc = g.bc_field[i];
     int clamp = (bc == BC_CLAMP || bc == BC_SIMPLE_LASER || bc == BC_SIMPLE_OUTFLOW);
     int zg = (bc == BC_ZERO_GRADIENT || bc == BC_CPML_LASER || bc == BC_CPML_OUTFLOW);
     if (clamp || zg) {
-      edge(g, a, STX[sa], STY[sa], i, clamp, s);
-      edge(g, b, STX[sb], STY[sb], i, clamp, s);
-      edge(g, c, STX[sc], STY[sc], i, clamp, s);
+      q.edge(a, STX[sa], STY[sa], i, clamp);
+      q.edge(b, STX[sb], STY[sb], i, clamp);
+      q.edge(c, STX[sc], STY[sc], i, clamp);
     }
   }
 }
@@ -698,43 +789,45 @@ static void generic_edges(const Grid& g, cplx<T>* a, cplx<T>* b, cplx<T>* c, int
 template <class T>
 void launch_efield_bcs(const Grid& g, FieldSet<T>& F, hipStream_t s) {
   const int ny = g.ny, ng = g.ng;
-  halo(g, F.ex(), 1 - ng, ny + ng - 1, s);
-  halo(g, F.er(), 1 - ng, ny + ng, s);
-  halo(g, F.et(), 1 - ng, ny + ng - 1, s);
+  BcQueue<T> q(g, s);
+  q.halo(F.ex(), 1 - ng, ny + ng - 1);
+  q.halo(F.er(), 1 - ng, ny + ng);
+  q.halo(F.et(), 1 - ng, ny + ng - 1);
   for (int i = XMIN; i <= XMAX; ++i)
     if (g.bc_field[i] == BC_CONDUCT) {
-      edge(g, F.ex(), STX[S_EX], STY[S_EX], i, 1, s);
-      edge(g, F.er(), STX[S_ER], STY[S_ER], i, 0, s);
-      edge(g, F.et(), STX[S_ET], STY[S_ET], i, 0, s);
+      q.edge(F.ex(), STX[S_EX], STY[S_EX], i, 1);
+      q.edge(F.er(), STX[S_ER], STY[S_ER], i, 0);
+      q.edge(F.et(), STX[S_ET], STY[S_ET], i, 0);
     }
   if (g.bc_field[YMAX] == BC_CONDUCT) {
-    edge(g, F.ex(), STX[S_EX], STY[S_EX], YMAX, 0, s);
-    edge(g, F.er(), STX[S_ER], STY[S_ER], YMAX, 1, s);
-    edge(g, F.et(), STX[S_ET], STY[S_ET], YMAX, 0, s);
+    q.edge(F.ex(), STX[S_EX], STY[S_EX], YMAX, 0);
+    q.edge(F.er(), STX[S_ER], STY[S_ER], YMAX, 1);
+    q.edge(F.et(), STX[S_ET], STY[S_ET], YMAX, 0);
   }
-  generic_edges(g, F.ex(), F.er(), F.et(), S_EX, S_ER, S_ET, s);
+  generic_edges(q, g, F.ex(), F.er(), F.et(), S_EX, S_ER, S_ET);
 }
 
 // bfield_bcs, boundary.F90:1417-1476
 template <class T>
 void launch_bfield_bcs(const Grid& g, FieldSet<T>& F, int mpi_only, hipStream_t s) {
   const int ny = g.ny, ng = g.ng;
-  halo(g, F.bx(), 1 - ng, ny + ng, s);
-  halo(g, F.br(), 1 - ng, ny + ng - 1, s);
-  halo(g, F.bt(), 1 - ng, ny + ng, s);
+  BcQueue<T> q(g, s);
+  q.halo(F.bx(), 1 - ng, ny + ng);
+  q.halo(F.br(), 1 - ng, ny + ng - 1);
+  q.halo(F.bt(), 1 - ng, ny + ng);
   if (mpi_only) return;
   for (int i = XMIN; i <= XMAX; ++i)
     if (g.bc_field[i] == BC_CONDUCT) {
-      edge(g, F.bx(), STX[S_BX], STY[S_BX], i, 0, s);
-      edge(g, F.br(), STX[S_BR], STY[S_BR], i, 1, s);
-      edge(g, F.bt(), STX[S_BT], STY[S_BT], i, 1, s);
+      q.edge(F.bx(), STX[S_BX], STY[S_BX], i, 0);
+      q.edge(F.br(), STX[S_BR], STY[S_BR], i, 1);
+      q.edge(F.bt(), STX[S_BT], STY[S_BT], i, 1);
     }
   if (g.bc_field[YMAX] == BC_CONDUCT) {
-    edge(g, F.bx(), STX[S_BX], STY[S_BX], YMAX, 1, s);
-    edge(g, F.br(), STX[S_BR], STY[S_BR], YMAX, 0, s);
-    edge(g, F.bt(), STX[S_BT], STY[S_BT], YMAX, 1, s);
+    q.edge(F.bx(), STX[S_BX], STY[S_BX], YMAX, 1);
+    q.edge(F.br(), STX[S_BR], STY[S_BR], YMAX, 0);
+    q.edge(F.bt(), STX[S_BT], STY[S_BT], YMAX, 1);
   }
-  generic_edges(g, F.bx(), F.br(), F.bt(), S_BX, S_BR, S_BT, s);
+  generic_edges(q, g, F.bx(), F.br(), F.bt(), S_BX, S_BR, S_BT);
 }
 
 template <class T>
@@ -783,19 +876,33 @@ void launch_current_axis(const Grid& g, FieldSet<T>& F, hipStream_t s) {
 // current_finish, current_smooth.F90:29-46: current_bcs then J halos
 template <class T>
 void launch_current_finish(const Grid& g, FieldSet<T>& F, hipStream_t s) {
-  cplx<T>* J[3] = {F.jx(), F.jr(), F.jt()};
+  const Jxrt<T> J{{F.jx(), F.jr(), F.jt()}};
   const bool xref = g.bc_particle[XMIN] == BC_REFLECT || g.bc_particle[XMAX] == BC_REFLECT;
   const bool rref = g.bc_particle[YMAX] == BC_REFLECT;
   const bool per = g.bc_particle[XMIN] == BC_PERIODIC && g.bc_particle[XMAX] == BC_PERIODIC;
-  for (int c = 0; c < 3; ++c) {
-    if (xref)
-      hipLaunchKernelGGL(k_current_x<T>, dim3(nblk((long)g.ny2 * g.M)), dim3(256), 0, s, g, J[c], c + 1);
-    if (rref)
-      hipLaunchKernelGGL(k_current_rmax<T>, dim3(nblk((long)g.nx2 * g.M)), dim3(256), 0, s, g, J[c], c + 1);
-    if (per)
-      hipLaunchKernelGGL(k_current_periodic<T>, dim3(nblk((long)g.ny2 * g.M)), dim3(256), 0, s, g, J[c]);
-  }
-  for (int c = 0; c < 3; ++c) halo(g, J[c], 1 - g.ng, g.ny + g.ng, s);
+  // per component: x reflection, r_max reflection, periodic sum, halos (the
+  // reference's order); the components run side by side
+  if (xref) hipLaunchKernelGGL(k_current_x<T>, dim3(nblk((long)g.ny2 * g.M), 3), dim3(256), 0, s, g, J);
+  if (rref) hipLaunchKernelGGL(k_current_rmax<T>, dim3(nblk((long)g.nx2 * g.M), 3), dim3(256), 0, s, g, J);
+  if (per) hipLaunchKernelGGL(k_current_periodic<T>, dim3(nblk((long)g.ny2 * g.M), 3), dim3(256), 0, s, g, J);
+  BcQueue<T> q(g, s);
+  for (int c = 0; c < 3; ++c) q.halo(J.j[c], 1 - g.ng, g.ny + g.ng);
+}
+
+// the three current arrays of a push, one launch
+template <class T>
+__global__ void k_zero_j(Jxrt<T> jj, long n) {
+  cplx<T>* __restrict__ p = jj.j[blockIdx.y];
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = czero<T>();
+}
+
+template <class T>
+void launch_zero_j(FieldSet<T>& F, long n, hipStream_t s) {
+  unsigned b = nblk(n);
+  if (b > 16384) b = 16384;
+  hipLaunchKernelGGL(k_zero_j<T>, dim3(b, 3), dim3(256), 0, s, Jxrt<T>{{F.jx(), F.jr(), F.jt()}}, n);
 }
 
 template <class T>
@@ -921,7 +1028,7 @@ void launch_energy(const Grid& g, FieldSet<T>& F, double* out, hipStream_t s) {
 
 #define INST(T)                                                                               \
   template void launch_update_e<T>(const Grid&, FieldSet<T>&, hipStream_t);                   \
-  template void launch_update_b<T>(const Grid&, FieldSet<T>&, hipStream_t);                   \
+  template void launch_update_b<T>(const Grid&, FieldSet<T>&, int, hipStream_t);              \
   template void launch_efield_bcs<T>(const Grid&, FieldSet<T>&, hipStream_t);                 \
   template void launch_bfield_bcs<T>(const Grid&, FieldSet<T>&, int, hipStream_t);            \
   template void launch_setup_bnd<T>(const Grid&, FieldSet<T>&, hipStream_t);                  \
@@ -930,6 +1037,7 @@ void launch_energy(const Grid& g, FieldSet<T>& F, double* out, hipStream_t s) {
   template void launch_current_axis<T>(const Grid&, FieldSet<T>&, hipStream_t);               \
   template void launch_current_finish<T>(const Grid&, FieldSet<T>&, hipStream_t);             \
   template void launch_fill_zero<T>(cplx<T>*, long, hipStream_t);                             \
+  template void launch_zero_j<T>(FieldSet<T>&, long, hipStream_t);                            \
   template void launch_energy<T>(const Grid&, FieldSet<T>&, double*, hipStream_t);            \
   template void launch_shift_window_fields<T>(const Grid&, FieldSet<T>&, cplx<T>*&, hipStream_t); \
   template void launch_smooth_current<T>(const Grid&, FieldSet<T>&, cplx<T>*, int, int, const int*, int,  \

@@ -62,7 +62,8 @@ struct PartMixed {
 
 // fields.hip
 template <class T> void launch_update_e(const Grid&, FieldSet<T>&, hipStream_t);
-template <class T> void launch_update_b(const Grid&, FieldSet<T>&, hipStream_t);
+// copy_old: also B_old = B (every element) before the update, in the same pass
+template <class T> void launch_update_b(const Grid&, FieldSet<T>&, int copy_old, hipStream_t);
 template <class T> void launch_efield_bcs(const Grid&, FieldSet<T>&, hipStream_t);
 template <class T> void launch_bfield_bcs(const Grid&, FieldSet<T>&, int mpi_only, hipStream_t);
 template <class T> void launch_setup_bnd(const Grid&, FieldSet<T>&, hipStream_t);
@@ -72,6 +73,7 @@ void launch_bfield_final_bcs(const Grid&, FieldSet<T>&, const double* src_xmin, 
 template <class T> void launch_current_axis(const Grid&, FieldSet<T>&, hipStream_t);
 template <class T> void launch_current_finish(const Grid&, FieldSet<T>&, hipStream_t);
 template <class T> void launch_fill_zero(cplx<T>*, long n, hipStream_t);
+template <class T> void launch_zero_j(FieldSet<T>&, long n, hipStream_t);  // jx, jr, jt = 0
 template <class T> void launch_energy(const Grid&, FieldSet<T>&, double* out, hipStream_t);
 // moving window: shift E, B, J one cell towards x_min (through `spare`, a
 // field-sized buffer that is swapped with each array) and fix the x_max

@@ -305,11 +305,12 @@ class Simulation:
 
 
 class Batch:
-    """Several runs stepped together through one captured HIP graph per
-    step, their mixed pushes sharing persistent launches (cylpic_batch_*).
-    The runs stay ordinary Simulations for set-up, downloads and
-    diagnostics; between batch steps call synchronize() before touching
-    them directly."""
+    """Several runs stepped together through captured HIP graphs, their
+    mixed pushes sharing persistent launches (cylpic_batch_*).  run() steps
+    in chunks whose graphs are kept and relaunched while the runs' state
+    repeats; step() captures a single step each call.  The runs stay
+    ordinary Simulations for set-up, downloads and diagnostics; between
+    batch steps call synchronize() before touching them directly."""
 
     def __init__(self, sims):
         self.sims = list(sims)
@@ -340,13 +341,39 @@ class Batch:
         for s in self.sims:
             s.time += s.dt
 
+    def run(self, nsteps, src_xmin=None, src_xmax=None):
+        """nsteps timesteps of every run.  src_xmin / src_xmax: None, or a
+        function of the step index j (0 .. nsteps - 1) returning the per-run
+        laser sources of that step (a list with None or 2 (ny + 1) doubles
+        per run)."""
+        src = None
+        if src_xmin is not None or src_xmax is not None:
+            widths = [2 * (s.ny + 1) for s in self.sims]
+            block = 2 * sum(widths)
+            src = np.zeros((nsteps, block), np.float64)
+            for j in range(nsteps):
+                off = 0
+                lo = src_xmin(j) if src_xmin is not None else None
+                hi = src_xmax(j) if src_xmax is not None else None
+                for r, wdt in enumerate(widths):
+                    if lo is not None and lo[r] is not None:
+                        src[j, off:off + wdt] = lo[r]
+                    if hi is not None and hi[r] is not None:
+                        src[j, off + wdt:off + 2 * wdt] = hi[r]
+                    off += 2 * wdt
+        ptr = src.ctypes.data if src is not None else None
+        check(lib().cylpic_batch_steps(self.b, int(nsteps), ptr))
+        for s in self.sims:
+            s.time += nsteps * s.dt
+
     def synchronize(self):
         check(lib().cylpic_batch_synchronize(self.b))
 
     def stats(self):
-        steps, inst, groups = C.c_int64(), C.c_int64(), C.c_int()
-        check(lib().cylpic_batch_stats(self.b, C.byref(steps), C.byref(inst), C.byref(groups)))
-        return {"steps": steps.value, "instantiations": inst.value, "push_groups": groups.value}
+        steps, inst, rep, groups = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().cylpic_batch_stats(self.b, C.byref(steps), C.byref(inst), C.byref(rep), C.byref(groups)))
+        return {"steps": steps.value, "instantiations": inst.value, "replays": rep.value,
+                "push_groups": groups.value}
 
     def close(self):
         if self.b:

@@ -282,25 +282,35 @@ int cylpic_kernel_time(cylpic_ctx *h, const char *name, double *ms, int64_t *cou
 /* Batches: many independent runs (a parameter scan) stepped together.  No
  * counterpart in the reference, which runs one simulation per job; each
  * batch step is one cylpic_step (src/epoch2d.F90:213-265) of every run.
- * The step of all runs is captured into one HIP graph and launched once:
- * each run's kernels on its own stream, and the mixed-precision pushes of
- * all runs with the same mode count (default 16 x 4 tiles, accumulator and
- * gather) in one persistent launch over their (run, tile) items.  A run in
- * a batch must not be stepped on its own or have calls queued while a batch
- * step is in flight (cylpic_batch_synchronize first).
+ * Steps are captured into HIP graphs and launched whole: each run's kernels
+ * on its own stream, and the mixed-precision pushes of all runs with the
+ * same mode count (default 16 x 4 tiles, accumulator and gather) in one
+ * persistent launch over their (run, tile) items.  A run in a batch must
+ * not be stepped on its own or have calls queued while batch steps are in
+ * flight (cylpic_batch_synchronize first).
  *   cylpic_batch_create   the runs share one device; allocates what a step
  *                         could allocate lazily
- *   cylpic_batch_step     src_xmin[r], src_xmax[r]: run r's laser sources
- *                         as in cylpic_update_eb_fields_final (either array
- *                         or any entry may be null)
- *   cylpic_batch_stats    steps taken, graphs instantiated (the others only
- *                         updated an executable), batched push launches
- *                         per step */
+ *   cylpic_batch_step     one step; src_xmin[r], src_xmax[r]: run r's laser
+ *                         sources as in cylpic_update_eb_fields_final
+ *                         (either array or any entry may be null); captured
+ *                         and instantiated afresh every call
+ *   cylpic_batch_steps    nsteps steps in chunks of P = lcm of the runs'
+ *                         2 sort_interval (at most 64).  A chunk that starts
+ *                         from a state seen before, and ends in the state it
+ *                         started from (no window move, injection, upload or
+ *                         download in between), relaunches the graph kept
+ *                         for it: no host work per step.  src: null (no
+ *                         laser) or nsteps blocks, one per step, each holding
+ *                         per run in batch order 2 (ny + 1) doubles of x_min
+ *                         sources then 2 (ny + 1) of x_max sources
+ *   cylpic_batch_stats    steps taken, graphs instantiated, chunks that
+ *                         relaunched a kept graph, batched push groups */
 typedef struct cylpic_batch cylpic_batch;
 int cylpic_batch_create(cylpic_batch **out, cylpic_ctx *const *runs, int nruns);
 int cylpic_batch_step(cylpic_batch *b, const double *const *src_xmin, const double *const *src_xmax);
+int cylpic_batch_steps(cylpic_batch *b, int nsteps, const double *src);
 int cylpic_batch_synchronize(cylpic_batch *b);
-int cylpic_batch_stats(cylpic_batch *b, int64_t *steps, int64_t *instantiations, int *push_groups);
+int cylpic_batch_stats(cylpic_batch *b, int64_t *steps, int64_t *instantiations, int64_t *replays, int *push_groups);
 int cylpic_batch_destroy(cylpic_batch *b);
 
 #ifdef __cplusplus

@@ -223,43 +223,82 @@ def test_device_decided_sorts_in_sequence(oracle):
     (migration; a window shift of two cells, within a tile's width, removes
     the first two columns), five pushes and a sort (radix: some particles
     move two or more tiles in r), against the CPU stable sort of the
-    oracle's keys after each."""
+    oracle's keys after each.  The keys are those the last push left: a
+    window shift removes the particles left of the new x_min and keeps the
+    others' keys (a particle that the push found beyond x_max, where the
+    cell clamps to nx, keeps that tile until the next push).  A twin run,
+    identical up to the first sort, is downloaded there (a download reads
+    the count back, which the run under test must not do before its second
+    sort)."""
     log("device-decided sorts in sequence")
     nx, ny, dx, dy, tile = 24, 12, 1e-7, 1e-7, (4, 1)
     bcf = ("simple_outflow", "simple_outflow", "other", "conduct")
     bcp = ("open", "open", "other", "open")
     st, sim = make_pair(oracle, nx, ny, 1, dx, dy, bc_field=bcf, bc_particle=bcp, precision="fp64", tile=tile,
                         sort_interval=1000)
-    add_plasma(st, sim, nx, ny, dx, dy, ppc=6, temp_ev=2.0e5, seed=33, shuffle=True)
+    _, twin = make_pair(oracle, nx, ny, 1, dx, dy, bc_field=bcf, bc_particle=bcp, precision="fp64", tile=tile,
+                        sort_interval=1000)
+    p = add_plasma(st, sim, nx, ny, dx, dy, ppc=6, temp_ev=2.0e5, seed=33, shuffle=True)
+    sid = twin.set_species(-Q0, M0, False, capacity=len(p["x"]))
+    twin.upload_particles(sid, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"], ids=p["id"])
     ntiles = 1 << 16                                 # beyond every tile key of the offset tiling
     order = sim.download_particles(0)["id"]          # the GPU's order after the upload sort
+    assert np.array_equal(twin.download_particles(0)["id"], order)
     shifted = 0
     c0 = sim.sort_counts()
-    # no download (which reads the count back) between the two sorts
-    for pushes, shift in ((1, 2), (5, 0)):
+    for stage, (pushes, shift) in enumerate(((1, 2), (5, 0))):
         for _ in range(pushes):
             sim.push_particles()
             st.push_particles()
+            if stage == 0:
+                twin.push_particles()
+        # the keys of the last push (the tiles move with the plasma: a window
+        # cell is a tile offset of 1)
+        sp = st.species[0]
+        pushed = oracle.cell_keys(st, sp["x"], sp["y"], sp["z"], *tile, shift=shifted) // (tile[0] * tile[1])
+        key_of = dict(zip(sp["id"].tolist(), pushed.tolist()))
         if shift:
             sim.move_window(shift)
             st.shift_window(shift)
             shifted += shift
+            if stage == 0:
+                twin.move_window(shift)
         sim.sort_particles()
-        sp = st.species[0]
-        pos = {i: k for k, i in enumerate(sp["id"])}
-        # the tiles move with the plasma: a window cell is a tile offset of 1
-        keys_alive = oracle.cell_keys(st, sp["x"], sp["y"], sp["z"], *tile, shift=shifted) // (tile[0] * tile[1])
-        keys = np.array([keys_alive[pos[i]] if i in pos else ntiles for i in order], np.int32)
+        alive = set(st.species[0]["id"].tolist())
+        key_of = {i: k for i, k in key_of.items() if i in alive}
+        keys = np.array([key_of.get(int(i), ntiles) for i in order], np.int32)
         perm = oracle.stable_sort(keys, ntiles + 1)
         kept = order[perm][: int(np.sum(keys < ntiles))]
         log(f"  after {pushes} push(es), shift {shift}: {len(order)} -> {len(kept)} particles")
         assert len(kept) < len(order)
+        if stage == 0:
+            twin.sort_particles()
+            c = twin.sort_counts()
+            got = twin.download_particles(0)["id"]
+            log(f"  twin after the first sort: counts {c}")
+            report_order(got, kept, key_of)
+            twin.close()
         order = kept
     c1 = sim.sort_counts()
+    got = sim.download_particles(0)["id"]
+    log(f"  counts {c0} -> {c1}")
+    report_order(got, order, key_of)
     assert (c1["migrate"] - c0["migrate"], c1["radix"] - c0["radix"]) == (1, 1), (c0, c1)
-    got = sim.download_particles(0)
-    assert np.array_equal(got["id"], order)
     sim.close()
+
+
+def report_order(got, want, key_of):
+    """Asserts equal id orders; on a mismatch logs where and the keys there."""
+    if len(got) == len(want) and np.array_equal(got, want):
+        return
+    n = min(len(got), len(want))
+    bad = np.nonzero(got[:n] != want[:n])[0]
+    k = int(bad[0]) if len(bad) else n
+    lo, hi = max(0, k - 3), min(n, k + 6)
+    log(f"  order differs: {len(got)} vs {len(want)} ids, {len(bad)} positions differ, first at {k}")
+    log("  got  ", [(int(i), int(key_of.get(int(i), -1))) for i in got[lo:hi]])
+    log("  want ", [(int(i), int(key_of.get(int(i), -1))) for i in want[lo:hi]])
+    assert False, "sorted order differs from the stable sort of the oracle's keys"
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
