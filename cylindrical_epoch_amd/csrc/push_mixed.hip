@@ -733,9 +733,11 @@ struct PushArgsMixed {
 // persistent launch: the work items are their tiles, descriptor d owning
 // items [item_start[d], item_start[d + 1]).  The item range is cut into one
 // contiguous chunk per XCD (workgroup b runs on XCD b mod 8), whose
-// workgroups take items in order from the XCD's counter until the chunk is
-// exhausted: neighbouring tiles share an L2, and every workgroup exits once
-// its chunk's counter passes the end.  counters[0..7] must be zero at launch.
+// workgroups take items in order from the XCD's counter: neighbouring tiles
+// share an L2.  A workgroup whose chunk is exhausted moves on to the next
+// XCD's chunk (x + 1, x + 2, ... mod 8), so that uneven chunks (runs of
+// different loads) do not leave XCDs idle; it exits once all eight counters
+// have passed their ends.  counters[0..7] must be zero at launch.
 template <int M, int ACC, int LDSF, int TX, int TR>
 __global__ __launch_bounds__(256) void k_push_deposit_mixed_batch(const PushArgsMixed* __restrict__ descs,
                                                                   const int* __restrict__ item_start, int ndesc,
@@ -745,14 +747,27 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed_batch(const PushArgs
   __shared__ uint32_t migs[MIG_DIRS];
   __shared__ int s_item;
   const int total = item_start[ndesc];
-  const int x = (int)(blockIdx.x % NUM_XCD);
-  const int lo = (int)((long)total * x / NUM_XCD), hi = (int)((long)total * (x + 1) / NUM_XCD);
+  const int x0 = (int)(blockIdx.x % NUM_XCD);
+  int tried = 0;  // chunks found exhausted (thread 0's count)
   for (;;) {
     __syncthreads();  // every work-item is done with the previous tile's LDS
-    if (threadIdx.x == 0) s_item = lo + atomicAdd(counters + x, 1);
+    if (threadIdx.x == 0) {
+      int got = -1;
+      while (tried < NUM_XCD) {
+        const int x = (x0 + tried) % NUM_XCD;
+        const int hi = (int)((long)total * (x + 1) / NUM_XCD);
+        const int it = (int)((long)total * x / NUM_XCD) + atomicAdd(counters + x, 1);
+        if (it < hi) {
+          got = it;
+          break;
+        }
+        ++tried;
+      }
+      s_item = got;
+    }
     __syncthreads();
     const int it = s_item;
-    if (it >= hi) break;
+    if (it < 0) break;
     int d = 0;
     while (d + 1 < ndesc && item_start[d + 1] <= it) ++d;
     d = __builtin_amdgcn_readfirstlane(d);
