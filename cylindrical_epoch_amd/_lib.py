@@ -20,6 +20,8 @@ SIGNATURES = {
     "cylpic_finalize": [_p],
     "cylpic_set_dt": [_p, _d],
     "cylpic_set_time": [_p, _d],
+    "cylpic_get_restart_state": [_p, _p, _p, _p, _p, _p],
+    "cylpic_set_restart_state": [_p, _d, _d, _d, _l, _i],
     "cylpic_set_species": [_p, _i, _d, _d, _i, _l],
     "cylpic_upload_particles": [_p, _i, _l, _p, _p, _p, _p, _p, _p, _p, _p],
     "cylpic_download_particles": [_p, _i, _l, _p, _p, _p, _p, _p, _p, _p, _p, _p],

@@ -272,7 +272,9 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   Tiles t;
   t.tx = tx;
   t.tr = tr;
-  t.shift = 0;
+  // the moving window's tile offset survives a change that keeps the width
+  // (cylpic_tune); a new width starts from offset 0 (the callers re-key)
+  t.shift = tx == h->tl.tx ? h->tl.shift : 0;
   // a spare column for the moving window's tile offset (non-periodic x only)
   const bool periodic_x = h->g.bc_field[XMIN] == BC_PERIODIC || h->g.bc_field[XMAX] == BC_PERIODIC ||
                           h->g.bc_particle[XMIN] == BC_PERIODIC || h->g.bc_particle[XMAX] == BC_PERIODIC;
@@ -946,6 +948,39 @@ int cylpic_set_dt(cylpic_ctx* h, double dt) {
 int cylpic_set_time(cylpic_ctx* h, double time) {
   CHECK_H(h);
   h->time = time;
+  return CYLPIC_OK;
+}
+
+int cylpic_get_restart_state(cylpic_ctx* h, double* time, double* x_min, double* x_grid_min, int64_t* push_count,
+                             int* tile_shift) {
+  CHECK_H(h);
+  if (time) *time = h->time;
+  if (x_min) *x_min = h->g.x_min;
+  if (x_grid_min) *x_grid_min = h->g.x_grid_min;
+  if (push_count) *push_count = h->push_count;
+  if (tile_shift) *tile_shift = h->tl.shift;
+  return CYLPIC_OK;
+}
+
+int cylpic_set_restart_state(cylpic_ctx* h, double time, double x_min, double x_grid_min, int64_t push_count,
+                             int tile_shift) {
+  CHECK_H(h);
+  for (auto& s : h->sp)
+    if (s.n > 0) return fail(CYLPIC_ESTATE, "set the restart state before uploading particles");
+  if (push_count < 0) return fail(CYLPIC_EINVAL, "push_count must be >= 0");
+  const bool periodic_x = h->g.bc_field[XMIN] == BC_PERIODIC || h->g.bc_field[XMAX] == BC_PERIODIC ||
+                          h->g.bc_particle[XMIN] == BC_PERIODIC || h->g.bc_particle[XMAX] == BC_PERIODIC;
+  if (tile_shift < 0 || tile_shift >= h->tl.tx || (periodic_x && tile_shift != 0))
+    return fail(CYLPIC_EINVAL, "tile_shift must be in [0, tile_x), and 0 for periodic x");
+  if (!(std::isfinite(x_min) && std::isfinite(x_grid_min) && std::isfinite(time)))
+    return fail(CYLPIC_EINVAL, "non-finite restart state");
+  h->time = time;
+  // the window's position as shift_window leaves it (window.F90:70-85)
+  h->g.x_min = x_min;
+  h->g.x_grid_min = x_grid_min;
+  h->g.x_max = x_min + (double)h->g.nx * h->g.dx;
+  h->push_count = push_count;
+  h->tl.shift = tile_shift;
   return CYLPIC_OK;
 }
 

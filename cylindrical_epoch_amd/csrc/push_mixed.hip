@@ -497,7 +497,9 @@ __device__ __forceinline__ void push_tile_mixed(
     };
     auto lds_add_c = [&](int c, float re, float im, bool has_im) {
       if (ACC == 4) {
-        const int qr = (int)re, qi = has_im ? (int)im : 0;
+        // round to nearest: the 32-bit unit is coarse (sized for the tile's
+        // particle count), and rounding keeps the per-add errors unbiased
+        const int qr = (int)rintf(re), qi = has_im ? (int)rintf(im) : 0;
         atomicAdd(jq + c, ((unsigned long long)(unsigned)qr << 32) + (unsigned long long)(long long)qi);
       } else {
         lds_add(c, 0, re);

@@ -145,6 +145,46 @@ class Simulation:
     def setup_field_boundaries(self):
         check(lib().cylpic_setup_field_boundaries(self.h))
 
+    # ------------------------------------------------------------ restart
+    def restart_state(self):
+        """time, the window's position (x_min, x_grid_min), the push count
+        (the sort phase) and the tile columns' offset."""
+        t, x0, xg = C.c_double(), C.c_double(), C.c_double()
+        pc, sh = C.c_int64(), C.c_int()
+        check(lib().cylpic_get_restart_state(self.h, C.byref(t), C.byref(x0), C.byref(xg), C.byref(pc),
+                                             C.byref(sh)))
+        return {"time": t.value, "x_min": x0.value, "x_grid_min": xg.value, "push_count": pc.value,
+                "tile_shift": sh.value}
+
+    def set_restart_state(self, st):
+        check(lib().cylpic_set_restart_state(self.h, float(st["time"]), float(st["x_min"]),
+                                             float(st["x_grid_min"]), int(st["push_count"]),
+                                             int(st["tile_shift"])))
+        self.time = st["time"]
+        self.x_min = st["x_min"]
+        self.x_max = self.x_min + self.nx * self.dx
+
+    def checkpoint(self):
+        """What a restart dump holds: the fifteen field arrays (E, B, J and
+        the *_old copies), every species' particles with their ids, and
+        restart_state()."""
+        return {"fields": {n: self.download_field(n) for n in FIELDS},
+                "particles": [self.download_particles(k) for k in range(len(self.species))],
+                "state": self.restart_state()}
+
+    def restore(self, ckpt):
+        """Continue from checkpoint() on a new handle made with the same grid,
+        boundaries, tile and tuning, and with its species registered: the
+        boundary snapshots are taken from the still-zero fields (the
+        reference's after_deck_last runs before restart_data reads the
+        dump), then the state, fields and particles are loaded."""
+        self.setup_field_boundaries()
+        self.set_restart_state(ckpt["state"])
+        for n, a in ckpt["fields"].items():
+            self.upload_field(n, a)
+        for k, p in enumerate(ckpt["particles"]):
+            self.upload_particles(k, p["x"], p["y"], p["z"], p["px"], p["py"], p["pz"], p["w"], ids=p["id"])
+
     # ------------------------------------------------------------ hot path
     def update_eb_fields_half(self):
         check(lib().cylpic_update_eb_fields_half(self.h))

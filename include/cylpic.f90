@@ -65,6 +65,26 @@ MODULE cylpic
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_set_time
 
+    FUNCTION cylpic_get_restart_state(h, time, x_min, x_grid_min, push_count, tile_shift) &
+        BIND(C, NAME='cylpic_get_restart_state') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_DOUBLE, C_INT64_T
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(OUT) :: time, x_min, x_grid_min
+      INTEGER(C_INT64_T), INTENT(OUT) :: push_count
+      INTEGER(C_INT), INTENT(OUT) :: tile_shift
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_get_restart_state
+
+    FUNCTION cylpic_set_restart_state(h, time, x_min, x_grid_min, push_count, tile_shift) &
+        BIND(C, NAME='cylpic_set_restart_state') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_DOUBLE, C_INT64_T
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), VALUE :: time, x_min, x_grid_min
+      INTEGER(C_INT64_T), VALUE :: push_count
+      INTEGER(C_INT), VALUE :: tile_shift
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_set_restart_state
+
     FUNCTION cylpic_set_species(h, id, charge, mass, immobile, capacity) &
         BIND(C, NAME='cylpic_set_species') RESULT(ierr)
       IMPORT :: C_PTR, C_INT, C_DOUBLE, C_INT64_T

@@ -73,6 +73,24 @@ const char *cylpic_last_error(void);
 int cylpic_set_dt(cylpic_ctx *h, double dt);
 int cylpic_set_time(cylpic_ctx *h, double time);
 
+/* Restart.  The reference's restart dump holds the fields (E, B, B_old, J,
+ * J_old: restart_data, src/housekeeping/setup.F90:1199-1362), the
+ * particles, time and step (the dump's header, :815), x_grid_min and the
+ * window's shift fraction (:1094-1098).  Here the fields and particles go through
+ * cylpic_download_field / cylpic_download_particles and back through the
+ * uploads; the host keeps the shift fraction; and these two calls carry the
+ * rest: time, the window's position (x_min, x_grid_min; x_max follows), the
+ * push count (its remainder modulo the sort interval is the sort phase) and
+ * the tile columns' offset.  On the new handle: cylpic_init with the same
+ * grid, tile and tuning calls, cylpic_setup_field_boundaries while the
+ * fields are still zero (the reference snapshots them in after_deck_last
+ * before restart_data reads the dump, src/epoch2d.F90:113-117),
+ * cylpic_set_restart_state before any particle upload, then the uploads. */
+int cylpic_get_restart_state(cylpic_ctx *h, double *time, double *x_min, double *x_grid_min,
+                             int64_t *push_count, int *tile_shift);
+int cylpic_set_restart_state(cylpic_ctx *h, double time, double x_min, double x_grid_min,
+                             int64_t push_count, int tile_shift);
+
 /* Register species `id` (0-based, ids dense): charge, mass, immobile flag
  * (src/shared_data.F90:190-280); capacity = maximum particle count. */
 int cylpic_set_species(cylpic_ctx *h, int id, double charge, double mass,
