@@ -154,6 +154,7 @@ def accuracy_step(sim, laser, t, window=None):
     sim.continuity_begin()
     sim.push_particles()
     res, dq, q = sim.continuity_end()
+    cm = sim.continuity_modes()
     sim.current_finish()
     sim.update_eb_fields_final(laser.sources(t))
     if window:
@@ -161,8 +162,13 @@ def accuracy_step(sim, laser, t, window=None):
     ng = 5
     ex = sim.download_field("ex")
     axis = ex[0, ng - 1, ng:ng + sim.nx].real.copy()  # iy = 0 is r = 0
+    rel = lambda a, b: [float(x / y) if y > 0 else None for x, y in zip(a, b)]
     return {"residual_over_max_dq": res / dq if dq > 0 else None,
-            "residual_over_max_q": res / q if q > 0 else None}, axis
+            "residual_over_max_q": res / q if q > 0 else None,
+            # every mode: the x-r identity's residual, and the part of the
+            # charge change the deposited J_theta misses, both over max |dQ_m|
+            "modes_xr_identity_residual_over_max_dq": rel(cm["residual"], cm["dq"]),
+            "modes_jtheta_part_over_max_dq": rel(cm["jtheta"], cm["dq"])}, axis
 
 
 def measured_traffic(precision):
@@ -504,7 +510,7 @@ def main():
         "roofline": roofline(w, n, r["prof"]["push_deposit"], main_prec),
     }
     acc = {"after_steps": a.warmup + a.steps + 1,
-           "continuity_m0": {k: v["cont"] for k, v in results.items()}}
+           "continuity": {k: v["cont"] for k, v in results.items()}}
     if "fp64" in results and "mixed" in results:
         out["mixed_over_fp64"] = round(results["mixed"]["value"] / results["fp64"]["value"], 3)
         ref = results["fp64"]["axis"]

@@ -33,6 +33,7 @@ SIGNATURES = {
     "cylpic_continuity_begin": [_p],
     "cylpic_autotune": [_p, _i, _p, _p, _p, _p],
     "cylpic_continuity_end": [_p, _p, _p, _p],
+    "cylpic_continuity_modes": [_p, _p, _p, _p, _p],
     "cylpic_gauss_residual_m0": [_p, _p, _p, _p],
     "cylpic_gauss_residual_m0_noalloc": [_p, _p, _p, _p],
     "cylpic_dist_fn": [_p, _i, _i, _p, _p, _p, _p, _p, _p, C.c_int64, _p],

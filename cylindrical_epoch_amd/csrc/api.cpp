@@ -108,6 +108,9 @@ struct cylpic_ctx {
   double* gauss_q = nullptr;              // two m = 0 charge planes of the Gauss check
   unsigned long long* cont_out = nullptr;  // the three maxima of either check
   bool cont_armed = false;
+  double* cm_planes = nullptr;             // continuity of every mode: Q0, Q1, Z, Zp (M complex planes each)
+  unsigned long long* cm_out = nullptr;    // its per-mode maxima (4 per mode)
+  std::vector<double> cm_last;             // the last check's per-mode maxima, empty before one
   int smooth_its = 0, smooth_comp_its = 0;  // current smoothing (current_smooth.F90)
   std::vector<int> smooth_strides{1};
   long sort_radix = 0, sort_migrate = 0;
@@ -556,6 +559,8 @@ void free_all(cylpic_ctx* h) {
   if (h->diag_part) (void)hipFree(h->diag_part);
   if (h->diag_hist) (void)hipFree(h->diag_hist);
   if (h->cont_q) (void)hipFree(h->cont_q);
+  if (h->cm_planes) (void)hipFree(h->cm_planes);
+  if (h->cm_out) (void)hipFree(h->cm_out);
   if (h->gauss_q) (void)hipFree(h->gauss_q);
   if (h->cont_out) (void)hipFree(h->cont_out);
   if (h->d_sortstat) (void)hipFree(h->d_sortstat);
@@ -1614,13 +1619,30 @@ static int charge_half(cylpic_ctx* h, double* q, bool immobile = false, double v
   return CYLPIC_OK;
 }
 
+// words of one set of M complex planes of the mode check
+static long cm_words(const cylpic_ctx* h) { return 2 * (long)h->g.M * h->g.plane; }
+
 int cylpic_continuity_begin(cylpic_ctx* h) {
   CHECK_H(h);
   HIPCHK(hipSetDevice(h->device));
   if (!h->cont_q) HIPCHK(hipMalloc((void**)&h->cont_q, 2 * h->g.plane * sizeof(double)));
   if (!h->cont_out) HIPCHK(hipMalloc((void**)&h->cont_out, 3 * sizeof(unsigned long long)));
+  if (!h->cm_planes) HIPCHK(hipMalloc((void**)&h->cm_planes, 4 * cm_words(h) * sizeof(double)));
+  if (!h->cm_out) HIPCHK(hipMalloc((void**)&h->cm_out, 4 * h->g.M * sizeof(unsigned long long)));
   int rc = charge_half(h, h->cont_q);
   if (rc) return rc;
+  // every mode: the charge at the push's half step (mobile species; an
+  // immobile one adds the same charge before and after)
+  launch_zero_f64(h->cm_planes, cm_words(h), h->stream);
+  for (auto& s : h->sp) {
+    if (s.immobile || s.n <= 0) continue;
+    const int* ts = s.ranged ? s.tile_start : nullptr;
+    if (h->prec == CYLPIC_FP64)
+      launch_cont_modes_begin_f64(h->g, view64(s), s.n, h->tl.ntiles, ts, extra_of(s), s.m, s.q, h->cm_planes,
+                                  h->stream);
+    else launch_cont_modes_begin_mixed(h->g, viewm(s), s.n, h->tl.ntiles, ts, extra_of(s), s.q, h->cm_planes, h->stream);
+  }
+  HIPCHK(hipGetLastError());
   h->cont_armed = true;
   return CYLPIC_OK;
 }
@@ -1638,14 +1660,50 @@ int cylpic_continuity_end(cylpic_ctx* h, double* residual, double* dq, double* q
   with_fields(h, [&](auto& F) {
     launch_continuity_m0(h->g, h->cont_q, q1, F.jx(), F.jr(), geo_of(h), h->cont_out, h->stream);
   });
+  // every mode: charge at t + 3dt/2 and the azimuthal terms, then the maxima
+  const long w = cm_words(h);
+  double* cq0 = h->cm_planes;
+  double* cq1 = cq0 + w;
+  double* cz = cq1 + w;
+  double* czp = cz + w;
+  launch_zero_f64(cq1, 3 * w, h->stream);
+  for (auto& s : h->sp) {
+    if (s.immobile || s.n <= 0) continue;
+    const int* ts = s.ranged ? s.tile_start : nullptr;
+    if (h->prec == CYLPIC_FP64)
+      launch_cont_modes_end_f64(h->g, view64(s), s.n, h->tl.ntiles, ts, extra_of(s), s.m, s.q, cq1, cz, czp,
+                                h->stream);
+    else launch_cont_modes_end_mixed(h->g, viewm(s), s.n, h->tl.ntiles, ts, extra_of(s), s.q, cq1, cz, czp, h->stream);
+  }
+  launch_zero_words((uint32_t*)h->cm_out, 8L * h->g.M, h->stream);
+  with_fields(h, [&](auto& F) {
+    launch_continuity_modes(h->g, cq0, cq1, cz, czp, F.jx(), F.jr(), geo_of(h), h->cm_out, h->stream);
+  });
   HIPCHK(hipGetLastError());
   unsigned long long bits[3];
+  std::vector<unsigned long long> mbits(4 * h->g.M);
   HIPCHK(hipMemcpyAsync(bits, h->cont_out, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(mbits.data(), h->cm_out, mbits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   std::memcpy(residual, &bits[0], sizeof(double));
   std::memcpy(dq, &bits[1], sizeof(double));
   std::memcpy(q, &bits[2], sizeof(double));
+  h->cm_last.assign(mbits.size(), 0.0);
+  std::memcpy(h->cm_last.data(), mbits.data(), mbits.size() * sizeof(double));
   return check_device_errors(h);
+}
+
+int cylpic_continuity_modes(cylpic_ctx* h, double* residual, double* dq, double* jtheta, double* z) {
+  CHECK_H(h);
+  if (h->cm_last.empty()) return fail(CYLPIC_ESTATE, "no continuity check has run on this handle");
+  for (int m = 0; m < h->g.M; ++m) {
+    if (residual) residual[m] = h->cm_last[4 * m];
+    if (dq) dq[m] = h->cm_last[4 * m + 1];
+    if (jtheta) jtheta[m] = h->cm_last[4 * m + 2];
+    if (z) z[m] = h->cm_last[4 * m + 3];
+  }
+  return CYLPIC_OK;
 }
 
 int cylpic_gauss_residual_m0(cylpic_ctx* h, double* residual, double* q, double* flux) {

@@ -9,6 +9,7 @@
 // downloaded particles.
 #include <algorithm>
 
+#include "cont_modes.h"
 #include "cplx_ops.h"
 #include "launch.h"
 #include "tile_scatter.h"
@@ -482,6 +483,92 @@ __global__ __launch_bounds__(256) void k_gauss_m0(Grid g, const double* __restri
   }
 }
 
+// Continuity of every mode, FP64 path: the charge modes at the push's half
+// step (the arithmetic of k_charge_half) before the push ...
+__global__ __launch_bounds__(256) void k_cont_modes_begin_f64(Grid g, PartF64 p, long n, int ntiles,
+                                                              const int* __restrict__ ts, ExtraRange xr, double mc,
+                                                              double q, double* __restrict__ q0) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)p.key[i] >= (unsigned)ntiles) return;
+  const double ux = p.px[i] / mc, uy = p.py[i] / mc, uz = p.pz[i] / mc;
+  const double root = (C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+  const double xh = p.x[i] + ux * root, yh = p.y[i] + uy * root, zh = p.z[i] + uz * root;
+  const double rh = sqrt(yh * yh + zh * zh);
+  const ContPos a{(xh - g.x_grid_min) / g.dx, (rh - g.y_grid_min) / g.dy, rh > 0.0 ? yh / rh : 1.0,
+                  rh > 0.0 ? zh / rh : 0.0, false};
+  cm_charge(g, q * p.w[i], a, q0);
+}
+
+// ... and after it: the charge modes at t + 3dt/2 = x(t+dt) + v dt/2 and the
+// azimuthal terms of the move from x(t+dt) - v dt/2 to there
+__global__ __launch_bounds__(256) void k_cont_modes_end_f64(Grid g, PartF64 p, long n, int ntiles,
+                                                            const int* __restrict__ ts, ExtraRange xr, double mc,
+                                                            double q, double* __restrict__ q1, double* __restrict__ z,
+                                                            double* __restrict__ zp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)p.key[i] >= (unsigned)ntiles) return;
+  const double ux = p.px[i] / mc, uy = p.py[i] / mc, uz = p.pz[i] / mc;
+  const double gam = sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+  const double root = (C_LIGHT * (g.dt / 2.0)) / gam;
+  const double x = p.x[i], y = p.y[i], zz = p.z[i];
+  ContPos pos[2];
+  for (int k = 0; k < 2; ++k) {
+    const double sg = k ? 1.0 : -1.0;
+    const double xs = x + sg * (ux * root), ys = y + sg * (uy * root), zs = zz + sg * (uz * root);
+    const double rs = sqrt(ys * ys + zs * zs);
+    pos[k] = ContPos{(xs - g.x_grid_min) / g.dx, (rs - g.y_grid_min) / g.dy, rs > 0.0 ? ys / rs : 1.0,
+                     rs > 0.0 ? zs / rs : 0.0, false};
+  }
+  const double qw = q * p.w[i];
+  cm_charge(g, qw, pos[1], q1);
+  const double r1 = sqrt(y * y + zz * zz);
+  const double vtdt_r = r1 > 0.0 ? ((-(uy * zz) + uz * y) / r1 / gam) * (C_LIGHT * g.dt) / r1 : 0.0;
+  cm_azimuthal(g, qw, pos[0], pos[1], vtdt_r, z, zp);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_continuity_modes(Grid g, const double* __restrict__ q0,
+                                                          const double* __restrict__ q1, const double* __restrict__ z,
+                                                          const double* __restrict__ zp, const cplx<T>* __restrict__ jx,
+                                                          const cplx<T>* __restrict__ jr, Geo geo,
+                                                          unsigned long long* __restrict__ out) {
+  __shared__ double red[4][4];
+  const int m = blockIdx.y;
+  const int nxi = g.nx - 2 * g.ng, nyi = g.ny - 2 * g.ng;
+  const long n = (long)nxi * nyi;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int cx = g.ng + 1 + (int)(e % nxi), cy = g.ng + 1 + (int)(e / nxi);
+    const int jng = g.ng;
+    const cplx<T> a = jx[g.idx(cx + 1, cy, m)], b = jx[g.idx(cx, cy, m)];
+    const cplx<T> c = jr[g.idx(cx, cy + 1, m)], d = jr[g.idx(cx, cy, m)];
+    const double irt = geo.inv_area_rt[cy + jng], ixt1 = geo.inv_area_xt[cy + jng], ixt0 = geo.inv_area_xt[cy - 1 + jng];
+    const double fr = ((double)a.re - (double)b.re) / irt + (double)c.re / ixt1 - (double)d.re / ixt0;
+    const double fi = ((double)a.im - (double)b.im) / irt + (double)c.im / ixt1 - (double)d.im / ixt0;
+    const long k = 2 * g.idx(cx, cy, m);
+    const double dr = q1[k] - q0[k], di = q1[k + 1] - q0[k + 1];
+    const double rr = dr + g.dt * fr - z[k], ri = di + g.dt * fi - z[k + 1];
+    v[0] = fmax(v[0], sqrt(rr * rr + ri * ri));
+    v[1] = fmax(v[1], sqrt(dr * dr + di * di));
+    const double er = z[k] - zp[k], ei = z[k + 1] - zp[k + 1];
+    v[2] = fmax(v[2], sqrt(er * er + ei * ei));
+    v[3] = fmax(v[3], sqrt(z[k] * z[k] + z[k + 1] * z[k + 1]));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) v[t] = fmax(v[t], __shfl_xor(v[t], o));
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int t = 0; t < 4; ++t) red[t][wv] = v[t];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, red[threadIdx.x][w]);
+    atomicMax(out + 4 * m + threadIdx.x, (unsigned long long)__double_as_longlong(r));
+  }
+}
+
 inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 template <class P>
@@ -552,6 +639,34 @@ void launch_continuity_m0(const Grid& g, const double* q0, const double* q1, con
     hipLaunchKernelGGL(k_continuity_m0<T>, dim3(std::min(nblk(n), 1024u)), dim3(256), 0, s, g, q0, q1, jx, jr, geo,
                        out);
 }
+void launch_cont_modes_begin_f64(const Grid& g, const PartF64& p, long n, int ntiles, const int* ts, ExtraRange xr,
+                                 double mass, double q, double* q0, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cont_modes_begin_f64, dim3(nblk(n)), dim3(256), 0, s, g, p, n, ntiles, ts, xr,
+                       mass * C_LIGHT, q, q0);
+}
+void launch_cont_modes_end_f64(const Grid& g, const PartF64& p, long n, int ntiles, const int* ts, ExtraRange xr,
+                               double mass, double q, double* q1, double* z, double* zp, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cont_modes_end_f64, dim3(nblk(n)), dim3(256), 0, s, g, p, n, ntiles, ts, xr,
+                       mass * C_LIGHT, q, q1, z, zp);
+}
+template <class T>
+void launch_continuity_modes(const Grid& g, const double* q0, const double* q1, const double* z, const double* zp,
+                             const cplx<T>* jx, const cplx<T>* jr, const Geo& geo, unsigned long long* out,
+                             hipStream_t s) {
+  const long n = (long)std::max(0, g.nx - 2 * g.ng) * std::max(0, g.ny - 2 * g.ng);
+  if (n > 0)
+    hipLaunchKernelGGL(k_continuity_modes<T>, dim3(std::min(nblk(n), 512u), g.M), dim3(256), 0, s, g, q0, q1, z, zp,
+                       jx, jr, geo, out);
+}
+template void launch_continuity_modes<float>(const Grid&, const double*, const double*, const double*, const double*,
+                                             const cplx<float>*, const cplx<float>*, const Geo&, unsigned long long*,
+                                             hipStream_t);
+template void launch_continuity_modes<double>(const Grid&, const double*, const double*, const double*,
+                                              const double*, const cplx<double>*, const cplx<double>*, const Geo&,
+                                              unsigned long long*, hipStream_t);
+
 __global__ void k_zero_f64(double* __restrict__ p, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long stride = (long)gridDim.x * blockDim.x;

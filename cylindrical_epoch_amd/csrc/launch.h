@@ -228,6 +228,26 @@ void launch_charge_half_mixed(const Grid&, const Tiles&, const PartMixed&, const
 template <class T>
 void launch_continuity_m0(const Grid&, const double* q0, const double* q1, const cplx<T>* jx, const cplx<T>* jr,
                           const Geo& geo, unsigned long long* out, hipStream_t);
+// Continuity of every mode (cont_modes.h): the charge modes at the push's
+// half step before it (Q0: M complex planes), then after it the charge modes
+// at t + 3dt/2 (Q1) and the azimuthal terms Z, Zp (M complex planes each),
+// all zeroed by the caller; then per mode m the maxima over the same cells
+// of |Q1 - Q0 + dt (x-r flux of J_m) - Z|, |Q1 - Q0|, |Z - Zp| and |Z| into
+// out[4 m .. 4 m + 3] (bit patterns, zeroed by the caller)
+// (ts, xr: the species' tile ranges and appended ranges, which hold its
+// particles; ts null: every slot below n is a particle)
+void launch_cont_modes_begin_f64(const Grid&, const PartF64&, long n, int ntiles, const int* ts, ExtraRange xr,
+                                 double mass, double q, double* q0, hipStream_t);
+void launch_cont_modes_end_f64(const Grid&, const PartF64&, long n, int ntiles, const int* ts, ExtraRange xr,
+                               double mass, double q, double* q1, double* z, double* zp, hipStream_t);
+void launch_cont_modes_begin_mixed(const Grid&, const PartMixed&, long n, int ntiles, const int* ts, ExtraRange xr,
+                                   double q, double* q0, hipStream_t);
+void launch_cont_modes_end_mixed(const Grid&, const PartMixed&, long n, int ntiles, const int* ts, ExtraRange xr,
+                                 double q, double* q1, double* z, double* zp, hipStream_t);
+template <class T>
+void launch_continuity_modes(const Grid&, const double* q0, const double* q1, const double* z, const double* zp,
+                             const cplx<T>* jx, const cplx<T>* jr, const Geo& geo, unsigned long long* out,
+                             hipStream_t);
 // Gauss's law for m = 0 on the same cells: max |eps0 (net flux of Ex, Er out
 // of the cell) - (qa + qb) / 2|, max |(qa + qb) / 2|, max |eps0 net flux|
 // (out[3] as above)

@@ -253,6 +253,14 @@ class Simulation:
         check(lib().cylpic_continuity_end(self.h, C.byref(r), C.byref(dq), C.byref(q)))
         return r.value, dq.value, q.value
 
+    def continuity_modes(self):
+        """Per-mode maxima of the last continuity check: the x-r identity's
+        residual, the charge change, the J_theta difference |Z - Zp| and |Z|
+        (arrays of n_mode)."""
+        out = [np.zeros(self.n_mode) for _ in range(4)]
+        check(lib().cylpic_continuity_modes(self.h, *[_ptr(a) for a in out]))
+        return dict(zip(("residual", "dq", "jtheta", "z"), out))
+
     def gauss_residual_m0(self):
         """Between steps: (max |eps0 net flux of Ex, Er - Q|, max |Q|, max
         |eps0 net flux|) of the m = 0 Gauss law over the cells at least ng

@@ -252,6 +252,14 @@ MODULE cylpic
       INTEGER(C_INT) :: ierr
     END FUNCTION cylpic_continuity_end
 
+    FUNCTION cylpic_continuity_modes(h, residual, dq, jtheta, z) &
+        BIND(C, NAME='cylpic_continuity_modes') RESULT(ierr)
+      IMPORT :: C_PTR, C_INT, C_DOUBLE
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(OUT) :: residual(*), dq(*), jtheta(*), z(*)
+      INTEGER(C_INT) :: ierr
+    END FUNCTION cylpic_continuity_modes
+
     FUNCTION cylpic_gauss_residual_m0(h, residual, q, flux) &
         BIND(C, NAME='cylpic_gauss_residual_m0') RESULT(ierr)
       IMPORT :: C_PTR, C_INT, C_DOUBLE

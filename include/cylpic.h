@@ -255,10 +255,30 @@ enum {
  * cylpic_continuity_end, right after the push and before
  * cylpic_current_finish, returns max |Q(t+3dt/2) - Q(t+dt/2) + dt (net Jx, Jr
  * current out of the cell)|, max |Q(t+3dt/2) - Q(t+dt/2)| and max
- * |Q(t+3dt/2)| in coulombs over the cells at least ng from every edge.  For m > 0 the reference's scheme
- * has no such identity (J_theta is deposited with v_theta(t+dt)). */
+ * |Q(t+3dt/2)| in coulombs over the cells at least ng from every edge.
+ *
+ * The same pair also checks every mode m; cylpic_continuity_modes returns
+ * the last check's per-mode maxima (arrays of n_mode, each may be null).
+ * A particle's charge in mode m is q w Sx Sy c_m e^{i m theta} (c_0 = 1,
+ * c_m = 2), the deposit moves it from t+dt/2 to t+3dt/2 along straight
+ * weights and an angle theta + s dtheta (dtheta as the deposit takes it),
+ * and its x-r currents carry all of the change except
+ *   Z_m = i m dtheta q w c_m e^{i m theta_a} (f2 A + f3 B + f4 C)
+ * (f_k(x) = int_0^1 s^(k-2) e^{i x s} ds at x = m dtheta, the reference's
+ * m-factors; A, B, C the shape products of the deposit).  So
+ *   residual[m] = max |Q_m(t+3dt/2) - Q_m(t+dt/2) + dt (net Jx, Jr) - Z_m|
+ * is round-off for every mode (Z_0 = 0: the m = 0 identity above), with
+ *   dq[m] = max |Q_m(t+3dt/2) - Q_m(t+dt/2)| its scale.  The deposited
+ * J_theta carries v_theta(t+dt), not r dtheta / dt: Zp_m, Z_m with dtheta
+ * replaced by v_theta dt / r at t+dt, is that current's azimuthal transport
+ * particle by particle, and
+ *   jtheta[m] = max |Z_m - Zp_m|,  z[m] = max |Z_m|
+ * measure the difference; per particle it is r dtheta / (v_theta dt) - 1 =
+ * (dt / r)^2 (3 v_r^2 - v_theta^2) / 12 + ..., at most (v_perp dt / r)^2 / 4
+ * to leading order. */
 int cylpic_continuity_begin(cylpic_ctx *h);
 int cylpic_continuity_end(cylpic_ctx *h, double *residual, double *dq, double *q);
+int cylpic_continuity_modes(cylpic_ctx *h, double *residual, double *dq, double *jtheta, double *z);
 
 /* Gauss's law for m = 0 (no counterpart in the reference; the quantity its
  * charge-conserving deposit keeps, src/particles.F90:508-665, with the E
