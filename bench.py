@@ -171,6 +171,74 @@ def accuracy_step(sim, laser, t, window=None):
             "modes_jtheta_part_over_max_dq": rel(cm["jtheta"], cm["dq"])}, axis
 
 
+def wake_accuracy(steps, every=100, sort_interval=8):
+    """The wakefield deck at its definition, mixed against FP64 on the GPU
+    from the same particles: 2048 x 256 cells (41 um x 25.6 um), 2 modes, 16
+    cold electrons per cell, n_e = 1e24 m^-3 reached over a 50 um linear
+    up-ramp from x = 0, the x_min laser (a0 = 2, 0.8 um, 10 um waist, 30 fs
+    FWHM, peak entering at 40 fs), open particle boundaries, and a window
+    that starts at 80 fs (the pulse's peak 12 um into the box) and then
+    moves at c, injecting the ramp's columns.  After `steps` steps (3000:
+    121 fs, the pulse in the plasma with its wake behind it) compares the
+    on-axis Ez (Re exm(1:nx, r = 0, m = 0)), the per-mode field energies
+    sampled every `every` steps, and the electrons' kinetic-energy spectrum
+    (cylpic_dist_fn over "en", 64 bins on the FP64 run's range)."""
+    from cylindrical_epoch_amd.window import MovingWindow, apply_to_sim
+    w = deck.WORKLOADS["wakefield_2048x256_m2_ppc16"]
+    nx, ny, M, dx, dy = w["nx"], w["ny"], w["n_mode"], w["dx"], w["dy"]
+    profile = deck.ramp_profile(0.0, 50e-6)
+    parts = deck.profiled_plasma(nx, ny, dx, dy, w["ppc"], w["density"], profile, seed=20240)
+    laser = Laser(ny, dy, a0=2.0, waist=10e-6, fwhm=30e-15)
+    laser.t0 = 40e-15
+    dt = deck.set_dt(dx, dy, 0.95)
+    cols = int(math.ceil(steps * dt * deck.C_LIGHT / dx)) + 2
+    res = {}
+    # "fp64_reordered": FP64 with another tile shape, so that only the order
+    # of the deposit's atomic sums differs from "fp64": how far two FP64 runs
+    # drift apart by round-off alone (the trajectories are chaotic)
+    for run in ("fp64", "mixed", "fp64_reordered"):
+        prec = "mixed" if run == "mixed" else "fp64"
+        sim = Simulation(nx, ny, M, dx, dy, dt, bc_field=("simple_laser", "simple_outflow", "other", "open"),
+                         bc_particle=("open", "open", "other", "open"), precision=prec, sort_interval=sort_interval,
+                         tile=(8, 8) if run == "fp64_reordered" else None)
+        sid = sim.set_species(-deck.Q0, deck.M0, False, capacity=len(parts["x"]) + cols * ny * w["ppc"])
+        sim.upload_particles(sid, parts["x"], parts["y"], parts["z"], parts["px"], parts["py"], parts["pz"],
+                             parts["w"], ids=parts["id"])
+        sim.setup_field_boundaries()
+        win = MovingWindow(ny, dx, dy, dt, w["ppc"], w["density"], profile, start=80e-15)
+        hist, axes = [], []
+        t0 = time.perf_counter()
+        for k in range(steps):
+            sim.step(laser.sources((k + 1) * dt))
+            apply_to_sim(sim, sid, *win.advance(sim.x_max, (k + 1) * dt))
+            if (k + 1) % every == 0:
+                hist.append(np.concatenate(sim.mode_energy()))
+            if (k + 1) % (5 * every) == 0 or k + 1 == steps:
+                axes.append(sim.download_field("ex")[0, 4, 5:5 + nx].real.copy())
+        spec = sim.dist_fn(sid, ["en"], ranges=res["fp64"]["range"] if run != "fp64" else [(0.0, 0.0)],
+                           resolution=[64])
+        res[run] = dict(axes=axes, hist=np.array(hist), spec=spec[0].ravel(), range=spec[1].tolist(),
+                        n=sim.particle_count(sid), s=time.perf_counter() - t0)
+        log(f"wake accuracy {run}: {steps} steps in {res[run]['s']:.1f} s, {res[run]['n']} electrons, "
+            f"window moved {win.columns} cells")
+        sim.close()
+    a = res["fp64"]
+    floor = 1e-6 * a["hist"].max()
+    big = a["hist"] > floor
+
+    def against(b):
+        ez = [float(np.max(np.abs(y - x)) / np.max(np.abs(x))) for x, y in zip(a["axes"], b["axes"])]
+        return {"on_axis_ez_rel_err": ez[-1], "on_axis_ez_rel_err_every_500_steps": ez,
+                "field_energy_history_max_rel_err": float(np.max(np.abs(b["hist"] - a["hist"])[big] / a["hist"][big])),
+                "electron_spectrum_l1_rel_diff": float(np.sum(np.abs(b["spec"] - a["spec"])) / np.sum(a["spec"])),
+                "electrons": b["n"]}
+    return {"workload": "wakefield_2048x256_m2_ppc16, 50 um up-ramp from x = 0, a0 = 2 peaking at 40 fs, "
+                        "window at c from 80 fs",
+            "steps": steps, "max_on_axis_ez_v_per_m": float(np.max(np.abs(a["axes"][-1]))), "electrons": a["n"],
+            "field_energy_samples": int(len(a["hist"])),
+            "mixed_vs_fp64": against(res["mixed"]), "fp64_reordered_vs_fp64": against(res["fp64_reordered"])}
+
+
 def measured_traffic(precision):
     """HBM bytes per push_deposit launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
@@ -402,6 +470,10 @@ def main():
                     help="with --scan: run only one of its two ways of stepping (for profiling)")
     ap.add_argument("--graph-single", action="store_true",
                     help="only time one run of --workload stepped eagerly and as a one-run graph batch")
+    ap.add_argument("--wake-steps", type=int, default=3000,
+                    help="steps of the wakefield accuracy study (mixed vs FP64 with the pulse in the plasma) "
+                         "reported in the accuracy block; 0 skips it")
+    ap.add_argument("--wake-only", action="store_true", help="only run the wakefield accuracy study")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -429,6 +501,9 @@ def main():
                        "parallelism": "single-gpu, one graph per step, persistent push per mode count"},
             "batch_stats": r["stats"],
             "interleaved_handles_particle_steps_per_s": round(r["value_int"], 1)}), flush=True)
+        return
+    if a.wake_only:
+        print(json.dumps({"metric": "wakefield accuracy, mixed vs FP64", **wake_accuracy(a.wake_steps)}), flush=True)
         return
     if a.graph_single:
         w = deck.WORKLOADS[a.workload]
@@ -516,6 +591,8 @@ def main():
         ref = results["fp64"]["axis"]
         acc["on_axis_ex_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(results["mixed"]["axis"] - ref)) /
                                                         max(np.max(np.abs(ref)), 1e-300))
+    if a.wake_steps > 0 and "fp64" in results and "mixed" in results:
+        acc["wakefield"] = wake_accuracy(a.wake_steps)
     out["accuracy"] = acc
     if not a.no_cpu_baseline:
         log("cpu baseline ...")

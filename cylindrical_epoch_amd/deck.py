@@ -71,6 +71,28 @@ def uniform_plasma(nx, ny, dx, dy, ppc, density, temp_ev=0.0, mass=M0, x_min=0.0
     return dict(x=x, y=y, z=z, px=p[0], py=p[1], pz=p[2], w=w, id=np.arange(n, dtype=np.int64))
 
 
+def ramp_profile(x_start, ramp):
+    """A density profile 0 before x_start, rising linearly over `ramp`
+    metres to 1, then flat (the wakefield deck's up-ramp)."""
+    def f(x):
+        return np.clip((np.asarray(x) - x_start) / ramp, 0.0, 1.0)
+    return f
+
+
+def profiled_plasma(nx, ny, dx, dy, ppc, density, profile, temp_ev=0.0, mass=M0, x_min=0.0, seed=7842432,
+                    cells_x=None, id0=0):
+    """uniform_plasma with each weight scaled by profile(x) (the deck's
+    density function at the particle) and the particles of zero weight
+    dropped; ids from id0."""
+    p = uniform_plasma(nx, ny, dx, dy, ppc, density, temp_ev, mass=mass, x_min=x_min, seed=seed, cells_x=cells_x)
+    f = profile(p["x"])
+    keep = f > 0
+    out = {k: v[keep] for k, v in p.items()}
+    out["w"] = out["w"] * f[keep]
+    out["id"] = np.arange(int(keep.sum()), dtype=np.int64) + id0
+    return out
+
+
 # The workloads of the benchmark table (grid, modes, load).  dx = dy where the
 # table gives one spacing; the large single run uses the wakefield spacing.
 WORKLOADS = {
