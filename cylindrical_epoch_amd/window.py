@@ -2,9 +2,9 @@
 (src/housekeeping/window.F90:353-376: the shift fraction accumulates
 window_v_x dt / dx and each whole cell moves the window) and insert_particles
 (window.F90:157-325: a column of new particles beyond x_max per cell, from
-the deck's density), driving any number of targets the same way: a library
-handle (cylindrical_epoch_amd.host.Simulation) and, in tests, the oracle's
-state, so that both see identical columns."""
+the deck's density), producing the columns and shifts once so that any
+number of targets (library handles, cylindrical_epoch_amd.host.Simulation,
+or a CPU state a test compares with) see identical columns."""
 import math
 
 from . import deck
