@@ -239,16 +239,25 @@ def wake_accuracy(steps, every=100, sort_interval=8):
             "mixed_vs_fp64": against(res["mixed"]), "fp64_reordered_vs_fp64": against(res["fp64_reordered"])}
 
 
+def _profile_summary(precision):
+    path = os.path.join(ROOT, "profiles", f"push_deposit_{precision}_{WORKLOAD}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
 def measured_traffic(precision):
     """HBM bytes per push_deposit launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
     or None when no summary for this kernel and workload exists."""
-    path = os.path.join(ROOT, "profiles", f"push_deposit_{precision}_{WORKLOAD}_traffic.json")
-    try:
-        with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
-        return None
+    return _profile_summary(precision).get("hbm_bytes_per_launch")
+
+
+# ds_add_u64 lane-operations per second over the 256 CUs, random addresses in
+# a 64 KB box (the per-tile current box's atomic), as measured on gfx950
+LDS_ATOMIC_PEAK = 3208e9
 
 
 def roofline(w, n, prof_push, precision):
@@ -266,10 +275,20 @@ def roofline(w, n, prof_push, precision):
     bytes_ = per_particle * n + per_cell_mode * w["n_mode"] * cells
     avg_s = ms / cnt / 1e3
     achieved = bytes_ / avg_s / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(precision),
-            "kernel": "push_deposit", "avg_launch_ms": round(ms / cnt, 4),
-            "algorithmic_bytes_per_launch": bytes_}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(precision),
+           "kernel": "push_deposit", "avg_launch_ms": round(ms / cnt, 4),
+           "algorithmic_bytes_per_launch": bytes_}
+    # the deposit's LDS atomics (the kernel's measured bound): lane-operations
+    # per launch from the committed SQ_INSTS_LDS_ATOMIC reading (x 64 lanes)
+    # over this run's launch time, against the ds_add_u64 rate
+    ops = _profile_summary(precision).get("lds_atomic_lane_ops_per_launch")
+    if ops:
+        rate = ops / avg_s
+        out["lds_atomic"] = {"achieved": round(rate / 1e9, 1), "peak": LDS_ATOMIC_PEAK / 1e9, "unit": "Gops/s",
+                             "frac": round(rate / LDS_ATOMIC_PEAK, 4), "ops_per_launch": ops,
+                             "ops_per_particle": round(ops / n, 1)}
+    return out
 
 
 def cpu_baseline(w, sample_cols):
