@@ -102,7 +102,9 @@ void orc_update_e_field(const orc_grid *g, orc_fields *f) {
   const ocplx *bx = f->bx, *br = f->br, *bt = f->bt;
   const ocplx *jx = f->jx, *jr = f->jr, *jt = f->jt;
 
-  /* interior, ir from 1 (r=0 axis is local) to ny, ix from 0 to nx  :62-113 */
+  /* interior, ir from 1 (r=0 axis is local) to ny, ix from 0 to nx  :62-113
+   * (every element is independent: the rows run in parallel) */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int m = 0; m < M; ++m) {
     for (int ir = 1; ir <= ny; ++ir) {
       double r_d = fabs((double)(ir - 1) * dy + g->y_grid_min);
@@ -197,7 +199,9 @@ void orc_update_b_field(const orc_grid *g, orc_fields *f) {
   ocplx *bx = f->bx, *br = f->br, *bt = f->bt;
   const ocplx *ex = f->ex, *er = f->er, *et = f->et;
 
-  /* interior, ir from 1 to ny-1 (r_max is local), ix from 0 to nx  :203-241 */
+  /* interior, ir from 1 to ny-1 (r_max is local), ix from 0 to nx  :203-241
+   * (every element is independent: the rows run in parallel) */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int m = 0; m < M; ++m) {
     for (int ir = 1; ir <= ny - 1; ++ir) {
       double r_d = fabs((double)(ir - 1) * dy + g->y_grid_min);
@@ -911,8 +915,235 @@ static inline double re_mul(ocplx a, ocplx b) { return a.re * b.re - a.im * b.im
 /* push_particles, particles.F90:159-731 (probes, tracers, photons,
  * HC_PUSH, delta-f and work-done options are not part of the default build
  * and are not restated). */
+/* The constants of one species' push (particles.F90:159-261) and the
+ * geometry tables, shared by the particles of a push. */
+typedef struct {
+  double part_q, part_mc, ipart_mc, cmratio, ccmratio;
+  double c, dtco2, idx, idy, idt, fac;
+  const double *inv_area_rt, *inv_area_xt, *inv_volume, *ratio_area_xt;
+} push_consts;
+
+/* One particle's push and deposit (particles.F90:268-665): the gathers read
+ * f on grid g; the current goes into fj on grid gj, whose column ix is the
+ * global column ix + xoff (gj = g, fj = f, xoff = 0 for the plain push).
+ * Returns 0, or the error bit of a skipped particle (1: a move of more than
+ * one cell, 2: a stencil outside the arrays). */
+static int push_one(const orc_grid *g, const orc_fields *f, const orc_grid *gj, orc_fields *fj, int xoff,
+                    orc_species *s, int64_t ip, const push_consts *k) {
+  const int M = g->n_mode;
+  const double part_q = k->part_q, part_mc = k->part_mc, ipart_mc = k->ipart_mc;
+  const double cmratio = k->cmratio, ccmratio = k->ccmratio;
+  const double c = k->c, dtco2 = k->dtco2, idx = k->idx, idy = k->idy, idt = k->idt, fac = k->fac;
+  double part_weight = s->w[ip];
+  double part_x = s->x[ip], part_y = s->y[ip], part_z = s->z[ip];
+  double part_ux = s->px[ip] * ipart_mc;
+  double part_uy = s->py[ip] * ipart_mc;
+  double part_uz = s->pz[ip] * ipart_mc;
+
+  double gamma_rel = sqrt(((part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz) + 1.0);
+  double root = dtco2 / gamma_rel;
+  part_x = part_x + part_ux * root;
+  part_y = part_y + part_uy * root;
+  part_z = part_z + part_uz * root;
+
+  double part_x_local = part_x - g->x_grid_min;
+  double part_r = sqrt(part_y * part_y + part_z * part_z);
+  double part_r_local = part_r - g->y_grid_min;
+  ocplx exp_min_itheta = cc(part_y / part_r, (-part_z) / part_r);
+  double theta_05 = atan2(part_z, part_y);
+  ocplx exp_min_itheta_05 = exp_min_itheta;
+  ocplx exp_itheta_05 = crecip(exp_min_itheta_05);
+
+  double cell_x_r = part_x_local * idx;
+  double cell_y_r = part_r_local * idy;
+  int cell_x1 = (int)floor(cell_x_r + 0.5);
+  double cell_frac_x = (double)cell_x1 - cell_x_r;
+  cell_x1 += 1;
+  int cell_y1 = (int)floor(cell_y_r + 0.5);
+  double cell_frac_y = (double)cell_y1 - cell_y_r;
+  cell_y1 += 1;
+  double gx_[5] = {0}, gy_[5] = {0}, hx_[5] = {0}, hy_[5] = {0};
+  double *gx = gx_ + 2, *gy = gy_ + 2, *hx = hx_ + 2, *hy = hy_ + 2;
+  weights3(cell_frac_x, gx);
+  weights3(cell_frac_y, gy);
+
+  int cell_x2 = (int)floor(cell_x_r);
+  cell_frac_x = ((double)cell_x2 - cell_x_r) + 0.5;
+  cell_x2 += 1;
+  int cell_y2 = (int)floor(cell_y_r);
+  cell_frac_y = ((double)cell_y2 - cell_y_r) + 0.5;
+  cell_y2 += 1;
+  weights3(cell_frac_x, hx);
+  weights3(cell_frac_y, hy);
+  if (!(fabs(cell_x_r) < 1e9 && fabs(cell_y_r) < 1e9) || cell_x1 < 2 - g->ng || cell_x1 > g->nx + g->ng - 1 ||
+      cell_y1 < 2 - g->ng || cell_y1 > g->ny + g->ng - 1 || cell_x2 < 2 - g->ng || cell_x2 > g->nx + g->ng - 1 ||
+      cell_y2 < 2 - g->ng || cell_y2 > g->ny + g->ng - 1) {
+    /* a stencil outside the arrays (a position the reference's grid
+     * cannot hold, or not finite): flag and skip */
+    return 2;
+  }
+
+  /* e_part.inc / b_part.inc */
+  ocplx e_m = cc(1.0, 0.0);
+  double ex_part = 0.0, er_part = 0.0, et_part = 0.0;
+  for (int im = 0; im < M; ++im) {
+    ex_part = ex_part + re_mul(e_m, patch(g, f->ex, cell_x1, cell_y2, im, gx, hy));
+    er_part = er_part + re_mul(e_m, patch(g, f->er, cell_x2, cell_y1, im, hx, gy));
+    et_part = et_part + re_mul(e_m, patch(g, f->et, cell_x2, cell_y2, im, hx, hy));
+    e_m = cmul(e_m, exp_min_itheta);
+  }
+  double ey_part = er_part * exp_min_itheta.re + PLANT(37, et_part, -et_part) * exp_min_itheta.im;
+  double ez_part = -(er_part * exp_min_itheta.im) + et_part * exp_min_itheta.re;
+  e_m = cc(1.0, 0.0);
+  double bx_part = 0.0, br_part = 0.0, bt_part = 0.0;
+  for (int im = 0; im < M; ++im) {
+    bx_part = bx_part + re_mul(e_m, patch(g, f->bx, cell_x2, cell_y1, im, hx, gy));
+    br_part = br_part + re_mul(e_m, patch(g, f->br, cell_x1, cell_y2, im, gx, hy));
+    bt_part = bt_part + re_mul(e_m, patch(g, f->bt, cell_x1, cell_y1, im, gx, gy));
+    e_m = cmul(e_m, exp_min_itheta);
+  }
+  double by_part = br_part * exp_min_itheta.re + bt_part * exp_min_itheta.im;
+  double bz_part = -(br_part * exp_min_itheta.im) + bt_part * exp_min_itheta.re;
+
+  /* Boris, particles.F90:404-451 */
+  double uxm = part_ux + cmratio * ex_part;
+  double uym = part_uy + cmratio * ey_part;
+  double uzm = part_uz + cmratio * ez_part;
+  gamma_rel = PLANT(31, sqrt(((uxm * uxm + uym * uym) + uzm * uzm) + 1.0),
+                    sqrt(((part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz) + 1.0));
+  root = ccmratio / gamma_rel;
+  double taux = bx_part * root, tauy = by_part * root, tauz = bz_part * root;
+  double taux2 = taux * taux, tauy2 = tauy * tauy, tauz2 = tauz * tauz;
+  double tau = 1.0 / (((1.0 + taux2) + tauy2) + tauz2);
+  double uxp = ((((1.0 + taux2) - tauy2) - tauz2) * uxm +
+                2.0 * ((taux * tauy + PLANT(33, tauz, -tauz)) * uym + (taux * tauz - tauy) * uzm)) * tau;
+  double uyp = ((((1.0 - taux2) + tauy2) - tauz2) * uym +
+                2.0 * ((tauy * tauz + taux) * uzm + (tauy * taux - tauz) * uxm)) * tau;
+  double uzp = ((((1.0 - taux2) - tauy2) + tauz2) * uzm +
+                2.0 * ((tauz * taux + tauy) * uxm + (tauz * tauy - taux) * uym)) * tau;
+  part_ux = uxp + PLANT(32, cmratio, -cmratio) * ex_part;
+  part_uy = uyp + cmratio * ey_part;
+  part_uz = uzp + cmratio * ez_part;
+
+  double part_u2 = (part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz;
+  gamma_rel = sqrt(part_u2 + 1.0);
+  double igamma = 1.0 / gamma_rel;
+  root = dtco2 * igamma;
+  double delta_x = part_ux * root, delta_y = part_uy * root, delta_z = part_uz * root;
+  part_x = part_x + delta_x;
+  part_y = part_y + delta_y;
+  part_z = part_z + delta_z;
+  s->x[ip] = part_x; s->y[ip] = part_y; s->z[ip] = part_z;
+  s->px[ip] = part_mc * part_ux; s->py[ip] = part_mc * part_uy; s->pz[ip] = part_mc * part_uz;
+
+  double part_vy = (part_uy * c) * igamma;
+  double part_vz = (part_uz * c) * igamma;
+  part_r = sqrt(part_y * part_y + part_z * part_z);
+  ocplx exp_itheta_10 = cc(part_y / part_r, part_z / part_r);
+  double part_vt = -(part_vy * exp_itheta_10.im) + part_vz * exp_itheta_10.re;
+
+  /* current at t+1.5dt, particles.F90:504-584 */
+  part_x_local = (part_x + delta_x) - g->x_grid_min;
+  part_y = part_y + delta_y;
+  part_z = part_z + delta_z;
+  part_r = sqrt(part_y * part_y + part_z * part_z);
+  part_r_local = part_r - g->y_grid_min;
+  ocplx exp_itheta_15 = cc(part_y / part_r, part_z / part_r);
+  double theta_15 = atan2(part_z, part_y);
+  ocplx exp_idtheta = cmul(exp_itheta_15, exp_min_itheta_05);
+  double dtheta = theta_15 - theta_05;
+
+  for (int k = -2; k <= 2; ++k) { gx[k] = hx[k]; gy[k] = hy[k]; }
+  cell_x_r = part_x_local * idx;
+  cell_y_r = part_r_local * idy;
+  int cell_x3 = (int)floor(cell_x_r);
+  cell_frac_x = ((double)cell_x3 - cell_x_r) + 0.5;
+  cell_x3 += 1;
+  int cell_y3 = (int)floor(cell_y_r);
+  cell_frac_y = ((double)cell_y3 - cell_y_r) + 0.5;
+  cell_y3 += 1;
+  for (int k = -2; k <= 2; ++k) { hx[k] = 0.0; hy[k] = 0.0; }
+  int dcellx = cell_x3 - cell_x2, dcelly = cell_y3 - cell_y2;
+  if (dcellx < -1 || dcellx > 1 || dcelly < -1 || dcelly > 1) {
+    /* a move of more than one cell per step (or a non-finite position):
+     * the reference's weight arrays would be overrun; flag and skip */
+    return 1;
+  }
+  weights3(cell_frac_x, hx + dcellx);
+  weights3(cell_frac_y, hy + dcelly);
+  for (int k = -2; k <= 2; ++k) { hx[k] = hx[k] - gx[k]; hy[k] = hy[k] - gy[k]; }
+
+  double q_weight_fac = (part_q * part_weight) * fac;
+  double fcx = q_weight_fac * idt, fcy = q_weight_fac * idt, fcz = q_weight_fac * part_vt;
+  orc_deposit_from_weights(gj, fj, gx_, hx_, gy_, hy_, cell_x2 - xoff, cell_y2, dcellx, dcelly,
+                           fcx, fcy, fcz, exp_itheta_05, exp_idtheta, dtheta,
+                           k->inv_area_rt, k->inv_area_xt, k->inv_volume, k->ratio_area_xt);
+  return 0;
+}
+
+static push_consts species_consts(const orc_grid *g, const orc_species *s, const double *tab, int ntab) {
+  push_consts k;
+  const double c = C_LIGHT, dt = g->dt;
+  const double dto2 = dt / 2.0;
+  const double fac = 0.25;                 /* (0.5)**c_ndims */
+  const double dtfac = (0.5 * dt) * fac;
+  k.c = c;
+  k.idx = 1.0 / g->dx;
+  k.idy = 1.0 / g->dy;
+  k.idt = 1.0 / dt;
+  k.dtco2 = c * dto2;
+  k.fac = fac;
+  k.part_q = s->q;
+  k.part_mc = c * s->m;
+  k.ipart_mc = 1.0 / k.part_mc;
+  k.cmratio = (k.part_q * dtfac) * k.ipart_mc;
+  k.ccmratio = c * k.cmratio;
+  k.inv_area_rt = tab;
+  k.inv_area_xt = tab + ntab;
+  k.inv_volume = tab + 2 * ntab;
+  k.ratio_area_xt = tab + 3 * ntab;
+  return k;
+}
+
 void orc_push_particles(const orc_grid *g, orc_fields *f, orc_species *sp, int n_species) {
-  const int ny = g->ny, M = g->n_mode, jng = g->ng;
+  const int ny = g->ny, jng = g->ng;
+  size_t nbytes = (size_t)NX2(g) * NY2(g) * g->n_mode * sizeof(ocplx);
+  memcpy(f->jx_old, f->jx, nbytes);
+  memcpy(f->jr_old, f->jr, nbytes);
+  memcpy(f->jt_old, f->jt, nbytes);
+  memset(f->jx, 0, nbytes);
+  memset(f->jr, 0, nbytes);
+  memset(f->jt, 0, nbytes);
+
+  int ntab = ny + 2 * jng + 1;
+  double *tab = calloc(4 * (size_t)ntab, sizeof(double));
+  orc_geometry(g, tab, tab + ntab, tab + 2 * ntab, tab + 3 * ntab);
+
+  for (int is = 0; is < n_species; ++is) {
+    orc_species *s = &sp[is];
+    if (s->immobile) continue;
+    const push_consts k = species_consts(g, s, tab, ntab);
+    for (int64_t ip = 0; ip < s->n; ++ip) orc_error |= push_one(g, f, g, f, 0, s, ip, &k);
+    /* current_bcs(species=ispecies): a no-op unless particle bcs are mixed */
+  }
+  free(tab);
+
+  orc_current_bcs_r_min_final(g, f);
+  orc_particle_bcs(g, sp, n_species);
+}
+
+/* The CPU baseline's push on every host core (timing only; the tests use
+ * orc_push_particles).  Each species' particles are cut by the x column of
+ * their position into nthreads slabs of columns, like the reference's MPI
+ * ranks; each thread pushes its slab's particles (stable order) and deposits
+ * into its own zeroed current arrays over the slab's columns and ng ghost
+ * columns each side; the copies are then added into J in slab order for
+ * every element, so the result does not depend on scheduling.  It differs
+ * from orc_push_particles only in the order of the current sums. */
+void orc_push_particles_threads(const orc_grid *g, orc_fields *f, orc_species *sp, int n_species, int nthreads) {
+  const int nx = g->nx, ny = g->ny, M = g->n_mode, jng = g->ng;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > nx) nthreads = nx;
   size_t nbytes = (size_t)NX2(g) * NY2(g) * M * sizeof(ocplx);
   memcpy(f->jx_old, f->jx, nbytes);
   memcpy(f->jr_old, f->jr, nbytes);
@@ -921,176 +1152,68 @@ void orc_push_particles(const orc_grid *g, orc_fields *f, orc_species *sp, int n
   memset(f->jr, 0, nbytes);
   memset(f->jt, 0, nbytes);
 
-  const double c = C_LIGHT, dt = g->dt;
-  const double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idt = 1.0 / dt;
-  const double dto2 = dt / 2.0, dtco2 = c * dto2;
-  const double fac = 0.25;                 /* (0.5)**c_ndims */
-  const double dtfac = (0.5 * dt) * fac;
-
   int ntab = ny + 2 * jng + 1;
-  double *inv_area_rt = calloc(ntab, sizeof(double));
-  double *inv_area_xt = calloc(ntab, sizeof(double));
-  double *inv_volume = calloc(ntab, sizeof(double));
-  double *ratio_area_xt = calloc(ntab, sizeof(double));
-  orc_geometry(g, inv_area_rt, inv_area_xt, inv_volume, ratio_area_xt);
-
+  double *tab = calloc(4 * (size_t)ntab, sizeof(double));
+  orc_geometry(g, tab, tab + ntab, tab + 2 * ntab, tab + 3 * ntab);
+  /* slab t: columns c0[t] .. c0[t+1]-1 */
+  int *c0 = malloc(sizeof(int) * (size_t)(nthreads + 1));
+  for (int t = 0; t <= nthreads; ++t) c0[t] = 1 + (int)(((int64_t)nx * t) / nthreads);
+  orc_grid *gs = malloc(sizeof(orc_grid) * (size_t)nthreads);
+  orc_fields *fs = calloc((size_t)nthreads, sizeof(orc_fields));
+  for (int t = 0; t < nthreads; ++t) {
+    gs[t] = *g;
+    gs[t].nx = c0[t + 1] - c0[t];
+    size_t n = (size_t)NX2(&gs[t]) * NY2(g) * M;
+    fs[t].jx = calloc(n, sizeof(ocplx));
+    fs[t].jr = calloc(n, sizeof(ocplx));
+    fs[t].jt = calloc(n, sizeof(ocplx));
+  }
+  int err = 0;
   for (int is = 0; is < n_species; ++is) {
     orc_species *s = &sp[is];
-    if (s->immobile) continue;
-    const double part_q = s->q, part_mc = c * s->m, ipart_mc = 1.0 / part_mc;
-    const double cmratio = (part_q * dtfac) * ipart_mc, ccmratio = c * cmratio;
-
+    if (s->immobile || s->n <= 0) continue;
+    const push_consts k = species_consts(g, s, tab, ntab);
+    /* the particles of each slab, in their order */
+    int *slab = malloc(sizeof(int) * (size_t)s->n);
+    int64_t *start = calloc((size_t)nthreads + 1, sizeof(int64_t));
     for (int64_t ip = 0; ip < s->n; ++ip) {
-      double part_weight = s->w[ip];
-      double part_x = s->x[ip], part_y = s->y[ip], part_z = s->z[ip];
-      double part_ux = s->px[ip] * ipart_mc;
-      double part_uy = s->py[ip] * ipart_mc;
-      double part_uz = s->pz[ip] * ipart_mc;
-
-      double gamma_rel = sqrt(((part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz) + 1.0);
-      double root = dtco2 / gamma_rel;
-      part_x = part_x + part_ux * root;
-      part_y = part_y + part_uy * root;
-      part_z = part_z + part_uz * root;
-
-      double part_x_local = part_x - g->x_grid_min;
-      double part_r = sqrt(part_y * part_y + part_z * part_z);
-      double part_r_local = part_r - g->y_grid_min;
-      ocplx exp_min_itheta = cc(part_y / part_r, (-part_z) / part_r);
-      double theta_05 = atan2(part_z, part_y);
-      ocplx exp_min_itheta_05 = exp_min_itheta;
-      ocplx exp_itheta_05 = crecip(exp_min_itheta_05);
-
-      double cell_x_r = part_x_local * idx;
-      double cell_y_r = part_r_local * idy;
-      int cell_x1 = (int)floor(cell_x_r + 0.5);
-      double cell_frac_x = (double)cell_x1 - cell_x_r;
-      cell_x1 += 1;
-      int cell_y1 = (int)floor(cell_y_r + 0.5);
-      double cell_frac_y = (double)cell_y1 - cell_y_r;
-      cell_y1 += 1;
-      double gx_[5] = {0}, gy_[5] = {0}, hx_[5] = {0}, hy_[5] = {0};
-      double *gx = gx_ + 2, *gy = gy_ + 2, *hx = hx_ + 2, *hy = hy_ + 2;
-      weights3(cell_frac_x, gx);
-      weights3(cell_frac_y, gy);
-
-      int cell_x2 = (int)floor(cell_x_r);
-      cell_frac_x = ((double)cell_x2 - cell_x_r) + 0.5;
-      cell_x2 += 1;
-      int cell_y2 = (int)floor(cell_y_r);
-      cell_frac_y = ((double)cell_y2 - cell_y_r) + 0.5;
-      cell_y2 += 1;
-      weights3(cell_frac_x, hx);
-      weights3(cell_frac_y, hy);
-      if (!(fabs(cell_x_r) < 1e9 && fabs(cell_y_r) < 1e9) || cell_x1 < 2 - g->ng || cell_x1 > g->nx + g->ng - 1 ||
-          cell_y1 < 2 - g->ng || cell_y1 > g->ny + g->ng - 1 || cell_x2 < 2 - g->ng || cell_x2 > g->nx + g->ng - 1 ||
-          cell_y2 < 2 - g->ng || cell_y2 > g->ny + g->ng - 1) {
-        /* a stencil outside the arrays (a position the reference's grid
-         * cannot hold, or not finite): flag and skip */
-        orc_error |= 2;
-        continue;
-      }
-
-      /* e_part.inc / b_part.inc */
-      ocplx e_m = cc(1.0, 0.0);
-      double ex_part = 0.0, er_part = 0.0, et_part = 0.0;
-      for (int im = 0; im < M; ++im) {
-        ex_part = ex_part + re_mul(e_m, patch(g, f->ex, cell_x1, cell_y2, im, gx, hy));
-        er_part = er_part + re_mul(e_m, patch(g, f->er, cell_x2, cell_y1, im, hx, gy));
-        et_part = et_part + re_mul(e_m, patch(g, f->et, cell_x2, cell_y2, im, hx, hy));
-        e_m = cmul(e_m, exp_min_itheta);
-      }
-      double ey_part = er_part * exp_min_itheta.re + PLANT(37, et_part, -et_part) * exp_min_itheta.im;
-      double ez_part = -(er_part * exp_min_itheta.im) + et_part * exp_min_itheta.re;
-      e_m = cc(1.0, 0.0);
-      double bx_part = 0.0, br_part = 0.0, bt_part = 0.0;
-      for (int im = 0; im < M; ++im) {
-        bx_part = bx_part + re_mul(e_m, patch(g, f->bx, cell_x2, cell_y1, im, hx, gy));
-        br_part = br_part + re_mul(e_m, patch(g, f->br, cell_x1, cell_y2, im, gx, hy));
-        bt_part = bt_part + re_mul(e_m, patch(g, f->bt, cell_x1, cell_y1, im, gx, gy));
-        e_m = cmul(e_m, exp_min_itheta);
-      }
-      double by_part = br_part * exp_min_itheta.re + bt_part * exp_min_itheta.im;
-      double bz_part = -(br_part * exp_min_itheta.im) + bt_part * exp_min_itheta.re;
-
-      /* Boris, particles.F90:404-451 */
-      double uxm = part_ux + cmratio * ex_part;
-      double uym = part_uy + cmratio * ey_part;
-      double uzm = part_uz + cmratio * ez_part;
-      gamma_rel = PLANT(31, sqrt(((uxm * uxm + uym * uym) + uzm * uzm) + 1.0),
-                        sqrt(((part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz) + 1.0));
-      root = ccmratio / gamma_rel;
-      double taux = bx_part * root, tauy = by_part * root, tauz = bz_part * root;
-      double taux2 = taux * taux, tauy2 = tauy * tauy, tauz2 = tauz * tauz;
-      double tau = 1.0 / (((1.0 + taux2) + tauy2) + tauz2);
-      double uxp = ((((1.0 + taux2) - tauy2) - tauz2) * uxm +
-                    2.0 * ((taux * tauy + PLANT(33, tauz, -tauz)) * uym + (taux * tauz - tauy) * uzm)) * tau;
-      double uyp = ((((1.0 - taux2) + tauy2) - tauz2) * uym +
-                    2.0 * ((tauy * tauz + taux) * uzm + (tauy * taux - tauz) * uxm)) * tau;
-      double uzp = ((((1.0 - taux2) - tauy2) + tauz2) * uzm +
-                    2.0 * ((tauz * taux + tauy) * uxm + (tauz * tauy - taux) * uym)) * tau;
-      part_ux = uxp + PLANT(32, cmratio, -cmratio) * ex_part;
-      part_uy = uyp + cmratio * ey_part;
-      part_uz = uzp + cmratio * ez_part;
-
-      double part_u2 = (part_ux * part_ux + part_uy * part_uy) + part_uz * part_uz;
-      gamma_rel = sqrt(part_u2 + 1.0);
-      double igamma = 1.0 / gamma_rel;
-      root = dtco2 * igamma;
-      double delta_x = part_ux * root, delta_y = part_uy * root, delta_z = part_uz * root;
-      part_x = part_x + delta_x;
-      part_y = part_y + delta_y;
-      part_z = part_z + delta_z;
-      s->x[ip] = part_x; s->y[ip] = part_y; s->z[ip] = part_z;
-      s->px[ip] = part_mc * part_ux; s->py[ip] = part_mc * part_uy; s->pz[ip] = part_mc * part_uz;
-
-      double part_vy = (part_uy * c) * igamma;
-      double part_vz = (part_uz * c) * igamma;
-      part_r = sqrt(part_y * part_y + part_z * part_z);
-      ocplx exp_itheta_10 = cc(part_y / part_r, part_z / part_r);
-      double part_vt = -(part_vy * exp_itheta_10.im) + part_vz * exp_itheta_10.re;
-
-      /* current at t+1.5dt, particles.F90:504-584 */
-      part_x_local = (part_x + delta_x) - g->x_grid_min;
-      part_y = part_y + delta_y;
-      part_z = part_z + delta_z;
-      part_r = sqrt(part_y * part_y + part_z * part_z);
-      part_r_local = part_r - g->y_grid_min;
-      ocplx exp_itheta_15 = cc(part_y / part_r, part_z / part_r);
-      double theta_15 = atan2(part_z, part_y);
-      ocplx exp_idtheta = cmul(exp_itheta_15, exp_min_itheta_05);
-      double dtheta = theta_15 - theta_05;
-
-      for (int k = -2; k <= 2; ++k) { gx[k] = hx[k]; gy[k] = hy[k]; }
-      cell_x_r = part_x_local * idx;
-      cell_y_r = part_r_local * idy;
-      int cell_x3 = (int)floor(cell_x_r);
-      cell_frac_x = ((double)cell_x3 - cell_x_r) + 0.5;
-      cell_x3 += 1;
-      int cell_y3 = (int)floor(cell_y_r);
-      cell_frac_y = ((double)cell_y3 - cell_y_r) + 0.5;
-      cell_y3 += 1;
-      for (int k = -2; k <= 2; ++k) { hx[k] = 0.0; hy[k] = 0.0; }
-      int dcellx = cell_x3 - cell_x2, dcelly = cell_y3 - cell_y2;
-      if (dcellx < -1 || dcellx > 1 || dcelly < -1 || dcelly > 1) {
-        /* a move of more than one cell per step (or a non-finite position):
-         * the reference's weight arrays would be overrun; flag and skip */
-        orc_error |= 1;
-        continue;
-      }
-      weights3(cell_frac_x, hx + dcellx);
-      weights3(cell_frac_y, hy + dcelly);
-      for (int k = -2; k <= 2; ++k) { hx[k] = hx[k] - gx[k]; hy[k] = hy[k] - gy[k]; }
-
-      double q_weight_fac = (part_q * part_weight) * fac;
-      double fcx = q_weight_fac * idt, fcy = q_weight_fac * idt, fcz = q_weight_fac * part_vt;
-      orc_deposit_from_weights(g, f, gx_, hx_, gy_, hy_, cell_x2, cell_y2, dcellx, dcelly,
-                               fcx, fcy, fcz, exp_itheta_05, exp_idtheta, dtheta,
-                               inv_area_rt, inv_area_xt, inv_volume, ratio_area_xt);
+      double cx = floor((s->x[ip] - g->x_grid_min) / g->dx + 0.5) + 1.0;
+      int c = cx < 1.0 ? 1 : (cx > (double)nx ? nx : (int)cx);
+      int t = (int)(((int64_t)(c - 1) * nthreads) / nx);
+      while (t + 1 < nthreads && c >= c0[t + 1]) ++t;
+      while (t > 0 && c < c0[t]) --t;
+      slab[ip] = t;
+      start[t + 1]++;
     }
-    /* current_bcs(species=ispecies): a no-op unless particle bcs are mixed */
+    for (int t = 0; t < nthreads; ++t) start[t + 1] += start[t];
+    int64_t *order = malloc(sizeof(int64_t) * (size_t)s->n);
+    int64_t *fill = malloc(sizeof(int64_t) * (size_t)nthreads);
+    for (int t = 0; t < nthreads; ++t) fill[t] = start[t];
+    for (int64_t ip = 0; ip < s->n; ++ip) order[fill[slab[ip]]++] = ip;
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1) reduction(| : err)
+    for (int t = 0; t < nthreads; ++t)
+      for (int64_t q = start[t]; q < start[t + 1]; ++q)
+        err |= push_one(g, f, &gs[t], &fs[t], c0[t] - 1, s, order[q], &k);
+    free(slab); free(start); free(order); free(fill);
   }
-  free(inv_area_rt); free(inv_area_xt); free(inv_volume); free(ratio_area_xt);
+  /* J += the slabs' currents, slab by slab for each element */
+#pragma omp parallel for num_threads(nthreads) collapse(2) schedule(static)
+  for (int m = 0; m < M; ++m)
+    for (int iy = 1 - jng; iy <= ny + jng; ++iy)
+      for (int t = 0; t < nthreads; ++t)
+        for (int lx = 1 - jng; lx <= gs[t].nx + jng; ++lx) {
+          const int ix = lx + c0[t] - 1;
+          if (ix < 1 - jng || ix > nx + jng) continue;
+          size_t a = I3(g, ix, iy, m), b = I3(&gs[t], lx, iy, m);
+          f->jx[a] = cadd(f->jx[a], fs[t].jx[b]);
+          f->jr[a] = cadd(f->jr[a], fs[t].jr[b]);
+          f->jt[a] = cadd(f->jt[a], fs[t].jt[b]);
+        }
+  for (int t = 0; t < nthreads; ++t) {
+    free(fs[t].jx); free(fs[t].jr); free(fs[t].jt);
+  }
+  free(fs); free(gs); free(c0); free(tab);
+  orc_error |= err;
 
   orc_current_bcs_r_min_final(g, f);
   orc_particle_bcs(g, sp, n_species);

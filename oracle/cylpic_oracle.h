@@ -95,6 +95,11 @@ void orc_setup_field_boundaries(const orc_grid *g, orc_fields *f);
 /* ---- particles (src/particles.F90, src/boundary.F90) ----------------- */
 void orc_push_particles(const orc_grid *g, orc_fields *f,
                         orc_species *sp, int n_species);
+/* The same push with the particles cut into nthreads x slabs pushed in
+ * parallel (OpenMP), each depositing into its own current arrays, added in
+ * slab order: for timing the CPU baseline on every core; it differs from
+ * orc_push_particles only in the order of the current sums. */
+void orc_push_particles_threads(const orc_grid *g, orc_fields *f, orc_species *sp, int n_species, int nthreads);
 void orc_current_bcs(const orc_grid *g, orc_fields *f);
 void orc_current_bcs_r_min_final(const orc_grid *g, orc_fields *f);
 void orc_current_finish(const orc_grid *g, orc_fields *f);

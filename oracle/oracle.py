@@ -86,6 +86,7 @@ def _declare(L):
         "orc_outflow_bcs_x_min": [gp, fp, dp], "orc_outflow_bcs_x_max": [gp, fp, dp],
         "orc_outflow_bcs_r_max": [gp, fp], "orc_setup_field_boundaries": [gp, fp],
         "orc_push_particles": [gp, fp, sp, C.c_int],
+        "orc_push_particles_threads": [gp, fp, sp, C.c_int, C.c_int],
         "orc_current_bcs": [gp, fp], "orc_current_bcs_r_min_final": [gp, fp],
         "orc_current_finish": [gp, fp], "orc_particle_bcs": [gp, sp, C.c_int],
         "orc_shift_window": [gp, fp, sp, C.c_int, C.c_int],
@@ -204,6 +205,16 @@ class State:
             raise FloatingPointError("oracle push: a particle moved more than one cell in a step "
                                      "(or its position is not finite)")
 
+    def push_particles_threads(self, nthreads):
+        """The push on nthreads host threads (x slabs, private currents added
+        in slab order): the CPU baseline's timing path."""
+        arr = self._species_array()
+        lib().orc_push_particles_threads(C.byref(self.g), C.byref(self.f), arr, len(self.species), int(nthreads))
+        self._species_back(arr)
+        if lib().orc_take_error():
+            raise FloatingPointError("oracle push: a particle moved more than one cell in a step "
+                                     "(or its position is not finite)")
+
     def current_finish(self, smooth=None):
         """current_finish (current_smooth.F90:29-46); smooth = (its,
         comp_its, strides) applies smooth_current after the current bcs."""
@@ -237,11 +248,15 @@ class State:
         lib().orc_shift_window(C.byref(self.g), C.byref(self.f), arr, len(self.species), int(ncells))
         self._species_back(arr)
 
-    def step(self, src_xmin=None, src_xmax=None):
+    def step(self, src_xmin=None, src_xmax=None, threads=0):
         """One timestep of the reference main loop, epoch2d.F90:213-265
-        (no moving window, no optional physics)."""
+        (no moving window, no optional physics); threads > 0 pushes on that
+        many host threads (push_particles_threads)."""
         self.update_eb_fields_half()
-        self.push_particles()
+        if threads > 0:
+            self.push_particles_threads(threads)
+        else:
+            self.push_particles()
         self.current_finish(smooth=self.smooth)
         self.g.time += self.g.dt
         self.update_eb_fields_final(src_xmin, src_xmax)
