@@ -284,6 +284,7 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   t.ntx = (h->g.nx + tx - 1) / tx + (periodic_x ? 0 : 1);
   t.ntr = (h->g.ny + tr - 1) / tr;
   t.ntiles = t.ntx * t.ntr;
+  if ((long)t.ntx * t.ntr >= (long)KEY_DEFERRED) return fail(CYLPIC_EINVAL, "too many tiles");
   size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t, h->acc_mode, h->gather_lds)
                                       : push_lds_bytes(h->g, t, h->gather_lds);
   // the device's LDS per workgroup, less the push kernels' 64 bytes of static LDS

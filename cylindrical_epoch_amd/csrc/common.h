@@ -24,6 +24,10 @@ enum { XMIN = 0, XMAX = 1, YMIN = 2, YMAX = 3 };
 // device error flags (bitmask, read back by the host after a step)
 enum { ERR_CELL_RANGE = 1, ERR_DCELL = 2, ERR_SORT_OVERFLOW = 4 };
 
+// FP64 push: a particle key with this bit is left for the second pass
+// (k_push_deposit_f64_rest); tile counts stay below it
+constexpr int KEY_DEFERRED = 1 << 30;
+
 template <class T>
 struct alignas(2 * sizeof(T)) cplx {
   T re, im;

@@ -67,6 +67,35 @@ def test_push_particles_parity(oracle, M, gather_lds, bcs):
     sim.close()
 
 
+@pytest.mark.parametrize("M,gather_lds", [(1, 0), (3, 0), (3, 1)])
+def test_unsorted_pushes_parity(oracle, M, gather_lds):
+    """Four pushes of a relativistic plasma without a sort, on 4 x 2 tiles:
+    particles that left their tile's neighbourhood (their stencil can reach
+    past the tile's current box) take the FP64 push's second pass, which
+    deposits straight into global J.  After every push the particles are
+    bit-exact and the currents within the push tolerances."""
+    log("unsorted pushes M", M, "gather_lds", gather_lds)
+    nx, ny, dx, dy = 32, 16, 1e-7, 1e-7
+    bf = ("periodic", "periodic", "other", "conduct")
+    bp = ("periodic", "periodic", "other", "reflect")
+    st, sim = make_pair(oracle, nx, ny, M, dx, dy, bc_field=bf, bc_particle=bp, tile=(4, 2), sort_interval=1000)
+    sim.tune(gather_lds=gather_lds)
+    rng = np.random.default_rng(10 + M)
+    load_fields(st, sim, random_fields(st.shape, rng, e_amp=3e10, b_amp=200.0, j_amp=0.0))
+    add_plasma(st, sim, nx, ny, dx, dy, ppc=6, temp_ev=2.0e5, drift=(0.4, 0.0, 0.0), seed=20 + M)
+    sp = st.species[0]
+    c0 = np.floor(sp["x"] / dx), np.floor(np.hypot(sp["y"], sp["z"]) / dy)
+    for k in range(4):
+        st.push_particles()
+        sim.push_particles()
+        _compare_particles(st, sim)
+        _compare_currents(st, sim)
+    moved = (np.abs(np.floor(sp["x"] / dx) - c0[0]) >= 2) | (np.abs(np.floor(np.hypot(sp["y"], sp["z"]) / dy) - c0[1]) >= 2)
+    log(f"  {int(moved.sum())} of {len(moved)} particles moved two cells or more without a sort")
+    assert moved.mean() > 0.05
+    sim.close()
+
+
 def test_current_finish_after_push(oracle):
     log("push + current_finish")
     nx, ny, dx, dy = 32, 16, 1e-7, 1e-7
