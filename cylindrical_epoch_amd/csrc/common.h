@@ -27,6 +27,9 @@ enum { ERR_CELL_RANGE = 1, ERR_DCELL = 2, ERR_SORT_OVERFLOW = 4 };
 // FP64 push: a particle key with this bit is left for the second pass
 // (k_push_deposit_f64_rest); tile counts stay below it
 constexpr int KEY_DEFERRED = 1 << 30;
+// ... and a tile holding such a particle has this in its removed-count slot
+// of the migration counts until the second pass counts it
+constexpr unsigned TILE_DEFERRED = 0xffffffffu;
 
 template <class T>
 struct alignas(2 * sizeof(T)) cplx {

@@ -474,8 +474,11 @@ template <int LDSF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LDSF ? 2 : 3))) void k_push_deposit_f64(
     Grid g, Tiles tl, PushConst pc, PartF64 sp, const int* __restrict__ tile_start, ExtraRange xr,
     FieldsF64 F, cplx<double>* __restrict__ jx, cplx<double>* __restrict__ jr, cplx<double>* __restrict__ jt,
-    Geo geo, int* __restrict__ err) {
+    Geo geo, int* __restrict__ err, MigCount migc) {
   extern __shared__ double lds[];
+  __shared__ uint32_t migs[MIG_DIRS];
+  __shared__ int deferred;
+  if (threadIdx.x == 0) deferred = 0;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   LdsDep box;
@@ -524,8 +527,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LDSF ? 2 : 
     if (sp.key[ip] >= tl.ntiles) continue;  // removed since the last sort
     // a particle that moved away from its tile since the last sort is left
     // to k_push_deposit_f64_rest
-    if (!push_one_f64<LDSF>(g, tl, pc, sp, ip, F, fr, geo, box, box, true, err))
+    if (!push_one_f64<LDSF>(g, tl, pc, sp, ip, F, fr, geo, box, box, true, err)) {
       sp.key[ip] |= KEY_DEFERRED;
+      deferred = 1;
+    }
   }
   __syncthreads();
   // one pass of global atomics per tile
@@ -549,18 +554,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LDSF ? 2 : 
     if (re != 0.0) unsafeAtomicAdd(p, re);
     if (imv != 0.0) unsafeAtomicAdd(p + 1, imv);
   }
+  // direction counts of the tile's new keys for the next migration sort,
+  // here if every particle is done; otherwise the second pass counts, and
+  // the removed-count slot marks the tile for it
+  if (migc.counts) {
+    __syncthreads();
+    if (!deferred) {
+      mig_count_tile((const uint32_t*)sp.key, tile, p0, p1, tl.ntx, tl.ntiles, migc.counts, migc.flag, migs);
+      if (xr.start)
+        mig_count_tile((const uint32_t*)sp.key, tile, q0, q1, tl.ntx, tl.ntiles, migc.counts_x, migc.flag, migs);
+    } else if (threadIdx.x == 0) {
+      migc.counts[(long)tile * MIG_DIRS + MIG_REMOVED] = TILE_DEFERRED;
+    }
+  }
 }
 
 // The particles the box kernel left (key marked KEY_DEFERRED), with their
 // whole deposit into global J; then the direction counts of the tile's new
 // keys for the next migration sort.  One workgroup per tile, the same tile
-// order; without deferred particles it only reads the keys.
+// order.  With migration counts, a tile the box kernel finished and counted
+// is skipped after one read; without them every tile's keys are scanned.
 __global__ __launch_bounds__(256) void k_push_deposit_f64_rest(
     Grid g, Tiles tl, PushConst pc, PartF64 sp, const int* __restrict__ tile_start, ExtraRange xr, FieldsF64 F,
     cplx<double>* __restrict__ jx, cplx<double>* __restrict__ jr, cplx<double>* __restrict__ jt, Geo geo,
     int* __restrict__ err, MigCount migc) {
   __shared__ uint32_t migs[MIG_DIRS];
+  __shared__ uint32_t mark;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (migc.counts) {
+    if (threadIdx.x == 0) mark = migc.counts[(long)tile * MIG_DIRS + MIG_REMOVED];
+    __syncthreads();
+    if (mark != TILE_DEFERRED) return;
+  }
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   LdsDep box;  // only its extent: the predicate is not evaluated here
   box.lds = nullptr;
@@ -632,10 +657,10 @@ void launch_push_deposit_f64(const Grid& g, const Tiles& tl, const PushConst& pc
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_push_deposit_f64<1>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_push_deposit_f64<1>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start, xr, fs,
-                       F.jx(), F.jr(), F.jt(), geo, err);
+                       F.jx(), F.jr(), F.jt(), geo, err, migc);
   } else {
     hipLaunchKernelGGL(k_push_deposit_f64<0>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, pc, sp, tile_start, xr, fs,
-                       F.jx(), F.jr(), F.jt(), geo, err);
+                       F.jx(), F.jr(), F.jt(), geo, err, migc);
   }
   hipLaunchKernelGGL(k_push_deposit_f64_rest, dim3(tl.ntiles), dim3(256), 0, s, g, tl, pc, sp, tile_start, xr, fs,
                      F.jx(), F.jr(), F.jt(), geo, err, migc);
