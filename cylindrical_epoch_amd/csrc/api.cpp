@@ -285,6 +285,8 @@ int set_tiles(cylpic_ctx* h, int tx, int tr) {
   t.ntr = (h->g.ny + tr - 1) / tr;
   t.ntiles = t.ntx * t.ntr;
   if ((long)t.ntx * t.ntr >= (long)KEY_DEFERRED) return fail(CYLPIC_EINVAL, "too many tiles");
+  // the upload's sort keys: 4 x the tile-major cell number + half-cell quadrant
+  if ((long)t.ntiles * tx * tr * 4 >= (1L << 31)) return fail(CYLPIC_EINVAL, "grid too large for 32-bit cell keys");
   size_t lds = h->prec == CYLPIC_MIXED ? push_lds_bytes_mixed(h->g, t, h->acc_mode, h->gather_lds)
                                       : push_lds_bytes(h->g, t, h->gather_lds);
   // the device's LDS per workgroup, less the push kernels' 64 bytes of static LDS
@@ -438,10 +440,13 @@ int sort_species_device(cylpic_ctx* h, Species& s, bool counted) {
 }
 
 // Stable sort of a species by tile key; removed particles (key ntiles) go
-// last.  With `interleave` (on upload) the keys are tile-major cell keys: the
-// sort groups each tile's particles by cell, and the gather then transposes
-// each tile so that consecutive particles sit in consecutive cells; later
-// stable sorts by tile keep that order.
+// last.  With `interleave` (on upload) the keys are tile-major cell keys with
+// the half-cell quadrant below them: the sort groups each tile's particles by
+// cell and within a cell by quadrant, and the gather then transposes each
+// tile so that consecutive particles sit in consecutive cells.  A wavefront
+// then holds particles of 64 cells in mostly the same quadrant, whose
+// deposit stencils start on distinct cells (no two lanes adding to one LDS
+// word); later stable sorts by tile keep that order.
 int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   Scope sc(h, "sort");
   s.dirty = false;
@@ -462,12 +467,13 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   // slots a device-decided sort left behind the kept particles sort last
   if (!s.n_exact) launch_mark_stale(s.key[0], s.tile_start, h->tl.ntiles, s.n - s.n_extra, h->stream);
   uint32_t* perm = nullptr;
-  // with `interleave` the keys are tile-major cell keys (upload)
+  // with `interleave` the keys are 4 x tile-major cell keys + half-cell
+  // quadrant (upload): each cell's particles grouped by quadrant
   const int cells = interleave ? h->tl.tx * h->tl.tr : 0;
-  radix_sort(s.key[0], s.n, interleave ? h->tl.ntiles * cells : h->tl.ntiles, h->slot_bits, h->sc, s.key[1], &perm,
-             h->stream);
+  radix_sort(s.key[0], s.n, interleave ? h->tl.ntiles * cells * 4 : h->tl.ntiles, h->slot_bits, h->sc, s.key[1],
+             &perm, h->stream);
   std::swap(s.key[0], s.key[1]);
-  if (interleave) launch_div_keys(s.key[0], s.n, (uint32_t)cells, h->stream);
+  if (interleave) launch_div_keys(s.key[0], s.n, (uint32_t)(cells * 4), h->stream);
   launch_tile_bounds(s.key[0], s.n, h->tl.ntiles, s.tile_start, h->stream);
   if (h->prec == CYLPIC_FP64) {
     launch_gather_f64(perm, s.n, s.a, s.b, 7, s.key[0], s.tile_start, cells, h->stream);

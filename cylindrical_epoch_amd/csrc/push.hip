@@ -632,10 +632,20 @@ void launch_window_f64(const Grid& g, const PartF64& sp, long n, int ntiles, hip
   if (n > 0) hipLaunchKernelGGL(k_window_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, sp, n, ntiles);
 }
 
+// With cell_key: 4 x the tile-major cell key + the particle's half-cell
+// quadrant (x, r: in the lower or upper half of its cell, which decides
+// whether its deposit stencil starts at its cell or the one below)
 __global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n, int cell_key) {
   const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= n) return;
-  sp.key[ip] = key_f64(g, tl, sp.x[ip], sp.y[ip], sp.z[ip], cell_key != 0);
+  const double x = sp.x[ip], y = sp.y[ip], z = sp.z[ip];
+  int k = key_f64(g, tl, x, y, z, cell_key != 0);
+  if (cell_key) {
+    const double fx = (x - g.x_grid_min) / g.dx + 1.5;
+    const double fy = (sqrt(y * y + z * z) - g.y_grid_min) / g.dy + 1.5;
+    k = 4 * k + 2 * (fy - floor(fy) < 0.5) + (fx - floor(fx) < 0.5);
+  }
+  sp.key[ip] = k;
 }
 
 // ------------------------------------------------------------------------

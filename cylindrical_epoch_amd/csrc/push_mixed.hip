@@ -851,13 +851,18 @@ __global__ __launch_bounds__(256) void k_cont_modes_end_mixed(Grid g, MixConst m
 }
 
 // ------------------------------------------------------------------------
-// key of an uploaded particle (no push)
+// key of an uploaded particle (no push); with cell_key, 4 x the tile-major
+// cell key + the half-cell quadrant (x offset below the centre, r in the
+// lower half of its cell: the deposit stencil then starts one cell lower)
 __global__ void k_keys_mixed(Grid g, Tiles tl, MixConst mc, PartMixed sp, long n, int cell_key) {
   const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= n) return;
   const float y = sp.y[ip], z = sp.z[ip];
-  const int cyk = (int)floorf(sqrtf(y * y + z * z) - mc.yoff + 1.5f);
-  sp.key[ip] = tl.key(sp.cell[ip], cyk, g.nx, g.ny, cell_key != 0);
+  const float fy = sqrtf(y * y + z * z) - mc.yoff + 1.5f;
+  const int cyk = (int)floorf(fy);
+  int k = tl.key(sp.cell[ip], cyk, g.nx, g.ny, cell_key != 0);
+  if (cell_key) k = 4 * k + 2 * (fy - (float)cyk < 0.5f) + (sp.ox[ip] < 0.0f);
+  sp.key[ip] = k;
 }
 
 // moving window (remove_particles, window.F90:330-350): the grid moved

@@ -132,20 +132,31 @@ def interleave_within_tiles(order, tiles, cells):
     return out
 
 
+def half_cell_quadrant(st, x, y, z):
+    """The upload's sub-key below the cell: 2 [r in the lower half of its
+    cell] + [x in the lower half], from the key's own expressions
+    (split_particle.F90:61-62 before the FLOOR)."""
+    fx = (x - st.g.x_grid_min) / st.g.dx + 1.5
+    fy = (np.sqrt(y * y + z * z) - st.g.y_grid_min) / st.g.dy + 1.5
+    return 2 * (fy - np.floor(fy) < 0.5).astype(np.int32) + (fx - np.floor(fx) < 0.5).astype(np.int32)
+
+
 @pytest.mark.parametrize("tile", [(4, 4), (16, 8), (7, 3)])
 def test_sort_matches_stable_cpu_sort(oracle, tile):
-    """The upload sorts particles by their tile-major cell key (stable) and
-    then interleaves each tile's particles; the downloaded order must equal
-    that permutation of the uploaded order computed on the CPU (bit-exact)."""
+    """The upload sorts particles by their tile-major cell key and, within a
+    cell, by half-cell quadrant (stable), then interleaves each tile's
+    particles; the downloaded order must equal that permutation of the
+    uploaded order computed on the CPU (bit-exact)."""
     log("sort tile", tile)
     nx, ny, dx, dy = 48, 20, 1e-7, 1e-7
     st, sim = make_pair(oracle, nx, ny, 1, dx, dy, tile=tile)
     p = add_plasma(st, sim, nx, ny, dx, dy, ppc=40, temp_ev=100.0, seed=11, shuffle=True)
     cells = tile[0] * tile[1]
-    keys = oracle.cell_keys(st, p["x"], p["y"], p["z"], tile[0], tile[1])
+    keys = 4 * oracle.cell_keys(st, p["x"], p["y"], p["z"], tile[0], tile[1]) + half_cell_quadrant(
+        st, p["x"], p["y"], p["z"])
     ntiles = ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
-    perm = oracle.stable_sort(keys, ntiles * cells)
-    want = interleave_within_tiles(p["id"][perm], keys[perm] // cells, cells)
+    perm = oracle.stable_sort(keys, ntiles * cells * 4)
+    want = interleave_within_tiles(p["id"][perm], keys[perm] // (cells * 4), cells)
     got = sim.download_particles(0)
     assert np.array_equal(got["id"], want)
     # a second, plain sort keeps the order (no particle moved)
