@@ -91,4 +91,22 @@ struct Tiles {
 };
 constexpr int DEPOSIT_HALO = 3;
 
+// The tile order that the XCD ranges above cut: bands of TILE_BAND tile rows,
+// each band column by column.  The ~100 workgroups one XCD holds at a time
+// then cover a block of about 12 x 8 tiles instead of a strip of one tile
+// row, so the vertical halos of their field and current footprints are
+// shared in that XCD's L2 instead of fetched again a whole tile row later.
+// A bijection of [0, ntiles).
+constexpr int TILE_BAND = 8;
+__device__ __forceinline__ int band_tile(int L, int ntx, int ntr) {
+  const int band = L / (TILE_BAND * ntx);
+  const int rows = min(TILE_BAND, ntr - band * TILE_BAND);
+  const int rem = L - band * TILE_BAND * ntx;
+  return (band * TILE_BAND + rem % rows) * ntx + rem / rows;
+}
+// the tile of workgroup b of a one-workgroup-per-tile launch
+__device__ __forceinline__ int tile_of_block(int b, const Tiles& tl) {
+  return band_tile(xcd_tile(b, tl.ntiles), tl.ntx, tl.ntr);
+}
+
 }  // namespace cylpic

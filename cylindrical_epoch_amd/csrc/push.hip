@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LDSF ? 2 : 
   __shared__ uint32_t migs[MIG_DIRS];
   __shared__ int deferred;
   if (threadIdx.x == 0) deferred = 0;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = tile_of_block(blockIdx.x, tl);
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   LdsDep box;
   box.lds = lds;
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_f64_rest(
     int* __restrict__ err, MigCount migc) {
   __shared__ uint32_t migs[MIG_DIRS];
   __shared__ uint32_t mark;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = tile_of_block(blockIdx.x, tl);
   if (migc.counts) {
     if (threadIdx.x == 0) mark = migc.counts[(long)tile * MIG_DIRS + MIG_REMOVED];
     __syncthreads();

@@ -714,7 +714,7 @@ __global__ __launch_bounds__(256) void k_push_deposit_mixed(
   __shared__ float wred[8];
   __shared__ uint32_t migs[MIG_DIRS];
   push_tile_mixed<M, ACC, LDSF, TX, TR>(g, tl, mc, sp, tile_start, xr, ex, er, et, bx, br, bt, jx, jr, jt, geo, err,
-                                        migc, xcd_tile(blockIdx.x, gridDim.x), lds_raw, wred, migs);
+                                        migc, tile_of_block(blockIdx.x, tl), lds_raw, wred, migs);
 }
 
 // The arguments of one (run, species) push for the batched kernel.

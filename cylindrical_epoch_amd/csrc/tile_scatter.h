@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void k_tile_scatter(Grid g, Tiles tl, Op op, c
                                                       const int* __restrict__ tile_start, ExtraRange xr,
                                                       double* __restrict__ out) {
   extern __shared__ double box[];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = tile_of_block(blockIdx.x, tl);
   const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
   const int bx0 = tl.x0(ttx) - 2, br0 = tty * tl.tr + 1 - 2;
   const int BX = tl.tx + 4, BR = tl.tr + 4, nbox = BX * BR;
