@@ -127,6 +127,7 @@ __device__ __forceinline__ void mfactors(float x, cplx<float>& f2, cplx<float>& 
 template <int M>
 struct Boxes {
   int fx0, fr0, FX, FR;  // field region origin (Fortran cell) and size
+  int FP;                // field region row pitch (field_pitch)
   int jx0, jr0, LX, LR;  // current box origin and size
 };
 
@@ -161,6 +162,19 @@ __host__ __device__ inline int box_pitch(int acc, int tx) {
 // int64 box planes per component (ACC 3): the m = 0 current has no
 // imaginary part, so mode 0 keeps only its real plane
 __host__ __device__ inline int acc3_planes(int M) { return 2 * M - 1; }
+
+// Field region rows (LDS gather): a wavefront reads a stencil tap of 4 rows
+// of 16 lanes (8 rows of 8 on 8-wide tiles), 8 bytes per lane.  A row pitch
+// of P complex values with P = 8 (mod 16) starts the rows' 32-bank blocks at
+// banks 0, 48, 32, 16: every bank serves two lanes, the minimum.  The gather
+// uses only the real part of mode 0 (the reference's REAL(e^{-i0 theta} F_0)
+// is Re F_0 for finite values), so mode 0 is staged as a real plane with the
+// same pitch (in floats); per component the region holds that plane, then
+// the complex planes of modes 1..M-1.
+__host__ __device__ inline int field_pitch(int fx) { return fx + ((8 - fx % 16) + 16) % 16; }
+__host__ __device__ inline long field_comp_floats(int M, int fr, int pitch) {
+  return (long)fr * pitch * (2 * M - 1);
+}
 
 template <int ACC>
 struct AccTraits {
@@ -199,13 +213,15 @@ __device__ __forceinline__ void push_tile_mixed(
   b.fr0 = tty * tr + 1 - FIELD_LO_R;
   b.FX = tx + FIELD_LO_X + FIELD_HI;
   b.FR = tr + FIELD_LO_R + FIELD_HI;
+  b.FP = field_pitch(b.FX);
   b.jx0 = ttx * tx + 1 - tl.shift - BOX_LO_X;
   b.jr0 = tty * tr + 1 - BOX_LO_R;
   b.LX = box_pitch(ACC, tx);
   b.LR = tr + BOX_LO_R + BOX_HI;
   const int nj = 3 * M * b.LR * b.LX;      // complex cells of the current box
   constexpr int NFC = LDSF == 2 ? 3 : 6;              // components staged in LDS
-  const int nf = LDSF ? NFC * M * b.FR * b.FX : 0;  // complex cells of the field region
+  const int nf = LDSF ? NFC * M * b.FR * b.FX : 0;  // field values staged (complex, mode 0 real)
+  const long fcomp = field_comp_floats(M, b.FR, b.FP);  // floats per staged component
   // current box: planes [3][M] of LR x LX cells; each plane holds the real
   // words then the imaginary words (FP64 for ACC 2, int64 for ACC 3, FP32 for
   // ACC 0, 1, where ACC 1 adds a second FP32 box for the low words), or one
@@ -222,7 +238,8 @@ __device__ __forceinline__ void push_tile_mixed(
   float* jlo = jf + 2 * nj;
   unsigned long long* jq = (unsigned long long*)lds_raw;
   const int jwords = ACC == 3 ? 2 * 3 * jcomp : AccTraits<ACC>::words * nj;  // 32-bit words
-  cplx<float>* fl = (cplx<float>*)(((float*)lds_raw) + ((jwords + 1) & ~1));  // [6][M][FR][FX]
+  // field region: per component [FR][FP] floats (mode 0) then [M-1][FR][FP] complex
+  float* fl = ((float*)lds_raw) + ((jwords + 1) & ~1);
 
   for (int e = threadIdx.x; e < jwords; e += blockDim.x) jf[e] = 0.0f;
   // the tile's particles: its main range, then its extra range
@@ -260,13 +277,15 @@ __device__ __forceinline__ void push_tile_mixed(
       cplx<float> v = mk(0.0f, 0.0f);
       const cplx<float>* src = c == 0 ? ex : (c == 1 ? er : (c == 2 ? et : (c == 3 ? bx : (c == 4 ? br : bt))));
       if (cx >= 1 - g.ng && cx <= g.nx + g.ng && cy >= 1 - g.ng && cy <= g.ny + g.ng) v = src[g.idx(cx, cy, m)];
-      fl[e] = v;
+      float* plane = fl + c * fcomp + (m == 0 ? 0 : (long)(2 * m - 1) * b.FR * b.FP);
+      const int o = fr * b.FP + fx;
+      if (m == 0) plane[o] = v.re;
+      else ((cplx<float>*)plane)[o] = v;
     }
   }
   __syncthreads();
 
   const int jng = g.ng;
-  const long fplane = (long)b.FR * b.FX;
 
   for (int k = threadIdx.x; k < np; k += blockDim.x) {
     const int ip = k < np_main ? p0 + k : q0 + (k - np_main);
@@ -332,16 +351,24 @@ __device__ __forceinline__ void push_tile_mixed(
         for (int m = 0; m < M; ++m) {
           cplx<float> S = mk(0.0f, 0.0f);
           if (in_lds && c < NFC) {
-            const cplx<float>* base =
-                fl + ((long)(c * M + m) * fplane + (long)(ccy[c] - 1 - b.fr0) * b.FX + (ccx[c] - 1 - b.fx0));
+            const long o = (long)(ccy[c] - 1 - b.fr0) * b.FP + (ccx[c] - 1 - b.fx0);
+            if (m == 0) {
+              const float* base = fl + c * fcomp + o;
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
+              for (int j = 0; j < 3; ++j)
 #pragma unroll
-              for (int i = 0; i < 3; ++i) {
-                const cplx<float> f = base[j * b.FX + i];
-                S.re = fmaf(W[3 * j + i], f.re, S.re);
-                S.im = fmaf(W[3 * j + i], f.im, S.im);
-              }
+                for (int i = 0; i < 3; ++i) S.re = fmaf(W[3 * j + i], base[j * b.FP + i], S.re);
+            } else {
+              const cplx<float>* base = (const cplx<float>*)(fl + c * fcomp + (long)(2 * m - 1) * b.FR * b.FP) + o;
+#pragma unroll
+              for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                  const cplx<float> f = base[j * b.FP + i];
+                  S.re = fmaf(W[3 * j + i], f.re, S.re);
+                  S.im = fmaf(W[3 * j + i], f.im, S.im);
+                }
+            }
           } else {
             const cplx<float>* base = gsrc[c] + g.idx(ccx[c] - 1, ccy[c] - 1, m);
 #pragma unroll
@@ -350,10 +377,11 @@ __device__ __forceinline__ void push_tile_mixed(
               for (int i = 0; i < 3; ++i) {
                 const cplx<float> f = base[j * g.nx2 + i];
                 S.re = fmaf(W[3 * j + i], f.re, S.re);
-                S.im = fmaf(W[3 * j + i], f.im, S.im);
+                if (m > 0) S.im = fmaf(W[3 * j + i], f.im, S.im);
               }
           }
-          acc = fmaf(e_m.re, S.re, fmaf(-e_m.im, S.im, acc));
+          // mode 0: e_m = 1 and only Re S_0 enters
+          acc = m == 0 ? acc + S.re : fmaf(e_m.re, S.re, fmaf(-e_m.im, S.im, acc));
           if (m + 1 < M) e_m = cmul(e_m, emi);
         }
         ef[c] = acc;
@@ -920,11 +948,12 @@ __global__ void k_from_mixed(Grid g, double mass, PartMixed sp, double* x, doubl
 size_t push_lds_bytes_mixed(const Grid& g, const Tiles& tl, int acc, int ldsf) {
   const size_t ncell = (size_t)(tl.tr + BOX_LO_R + BOX_HI) * box_pitch(acc, tl.tx);
   const size_t nj = 3 * (size_t)g.M * ncell;
-  const size_t nf = ldsf ? (ldsf == 2 ? 3 : 6) * (size_t)g.M * (tl.tr + FIELD_LO_R + FIELD_HI) *
-                               (tl.tx + FIELD_LO_X + FIELD_HI)
+  // staged field floats: per component a real mode-0 plane and M-1 complex planes
+  const size_t nf = ldsf ? (ldsf == 2 ? 3 : 6) * (size_t)field_comp_floats(g.M, tl.tr + FIELD_LO_R + FIELD_HI,
+                                                                             field_pitch(tl.tx + FIELD_LO_X + FIELD_HI))
                          : 0;
   const size_t jwords = acc == 3 ? 2 * 3 * acc3_planes(g.M) * ncell : ((acc == 0 || acc == 4) ? 2 * nj : 4 * nj);
-  return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(cplx<float>);
+  return ((jwords + 1) & ~(size_t)1) * sizeof(float) + nf * sizeof(float);
 }
 
 MixConst mix_const(const Grid& g, const PushConst& pc, double charge) {
