@@ -28,21 +28,26 @@ def test_mixed_fields_half(oracle, M):
     sim.close()
 
 
-@pytest.mark.parametrize("M,acc,gather_lds", [(1, 3, 1), (2, 3, 1), (3, 3, 1), (3, 3, 0), (4, 3, 0),
-                                               (3, 0, 1), (3, 1, 1), (3, 2, 0), (1, 4, 0), (3, 4, 1),
-                                               (2, 0, 0), (3, 3, 2), (1, 3, 2)])
-def test_mixed_push(oracle, M, acc, gather_lds):
+@pytest.mark.parametrize("M,acc,gather_lds,tile", [(1, 3, 1, (8, 4)), (2, 3, 1, (8, 4)), (3, 3, 1, (8, 4)),
+                                                    (3, 3, 0, (8, 4)), (4, 3, 0, (8, 4)), (3, 0, 1, (8, 4)),
+                                                    (3, 1, 1, (8, 4)), (3, 2, 0, (8, 4)), (1, 4, 0, (8, 4)),
+                                                    (3, 4, 1, (8, 4)), (2, 0, 0, (8, 4)), (3, 3, 2, (8, 4)),
+                                                    (1, 3, 2, (8, 4)), (3, 3, 1, (16, 4)), (2, 3, 1, (8, 8)),
+                                                    (3, 3, 2, (32, 4)), (3, 3, 1, (5, 3))])
+def test_mixed_push(oracle, M, acc, gather_lds, tile):
     """One push of a warm plasma in given fields, for each current
     accumulator (FP32, FP32 + 2Sum, FP64, int64 fixed point, packed 2 x 32-bit
     fixed point) and gather source (LDS, L1/L2, or E from LDS and B from
-    L1/L2).
+    L1/L2), on tiles with compile-time shapes (16 x 4, 8 x 8, 32 x 4, whose
+    staged field rows are padded to a bank-conflict-free pitch) and run-time
+    ones (8 x 4, 5 x 3).
     Positions and momenta carry FP32 rounding (~1e-7 relative per
     operation); the current of a particle is a difference of O(1) shape
     weights, so FP32 positions put ~1e-5 relative error on slow particles'
     contributions."""
-    log("mixed push M", M, "acc", acc, "gather_lds", gather_lds)
+    log("mixed push M", M, "acc", acc, "gather_lds", gather_lds, "tile", tile)
     nx, ny, dx, dy = 40, 20, 1e-7, 1e-7
-    st, sim = make_pair(oracle, nx, ny, M, dx, dy, tile=(8, 4), precision="mixed")
+    st, sim = make_pair(oracle, nx, ny, M, dx, dy, tile=tile, precision="mixed")
     sim.tune(acc, gather_lds)
     rng = np.random.default_rng(M)
     f = random_fields(st.shape, rng, e_amp=3e10, b_amp=200.0, j_amp=0.0)
