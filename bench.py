@@ -481,7 +481,7 @@ def main():
                          "the whole load)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-mixed", default=None, help="tile of the mixed path, e.g. 32x4 (library default)")
-    ap.add_argument("--tile-fp64", default=None, help="tile of the FP64 path, e.g. 16x8 (library default)")
+    ap.add_argument("--tile-fp64", default=None, help="tile of the FP64 path, e.g. 16x8 (library default 16x4)")
     ap.add_argument("--acc-mode", type=int, default=None, help="mixed current accumulator 0..4")
     ap.add_argument("--gather-lds", type=int, default=None, help="mixed gather from LDS (1) or L1/L2 (0)")
     ap.add_argument("--sort-method", default=None, choices=["radix", "migrate"])

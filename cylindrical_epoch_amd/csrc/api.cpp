@@ -898,9 +898,10 @@ int cylpic_init(cylpic_ctx** out, int nx, int ny, int n_mode, int ng, double dx,
     // VALU-bound FP64 path from L1/L2
     h->gather_lds = precision == CYLPIC_MIXED ? 1 : 0;
     // default tile: the first of these whose LDS needs fit (the fastest
-    // measured first: 16 x 4 for the mixed path, 16 x 8 for FP64)
+    // measured first on the 8192 x 512 x 3 wakefield: 16 x 4 for both paths;
+    // FP64 16 x 8 next, 1.7 % slower)
     const int cand_m[][2] = {{16, 4}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}, {1, 1}, {1, 1}};
-    const int cand_d[][2] = {{16, 8}, {16, 4}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}, {1, 1}};
+    const int cand_d[][2] = {{16, 4}, {16, 8}, {8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}, {1, 1}, {1, 1}};
     rc = CYLPIC_EINVAL;
     for (auto& c : (precision == CYLPIC_MIXED ? cand_m : cand_d)) {
       rc = set_tiles(h, std::min(c[0], nx), std::min(c[1], ny));
@@ -1645,9 +1646,8 @@ int cylpic_continuity_begin(cylpic_ctx* h) {
     if (s.immobile || s.n <= 0) continue;
     const int* ts = s.ranged ? s.tile_start : nullptr;
     if (h->prec == CYLPIC_FP64)
-      launch_cont_modes_begin_f64(h->g, view64(s), s.n, h->tl.ntiles, ts, extra_of(s), s.m, s.q, h->cm_planes,
-                                  h->stream);
-    else launch_cont_modes_begin_mixed(h->g, viewm(s), s.n, h->tl.ntiles, ts, extra_of(s), s.q, h->cm_planes, h->stream);
+      launch_cont_modes_begin_f64(h->g, h->tl, view64(s), s.n, ts, extra_of(s), s.m, s.q, h->cm_planes, h->stream);
+    else launch_cont_modes_begin_mixed(h->g, h->tl, viewm(s), s.n, ts, extra_of(s), s.q, h->cm_planes, h->stream);
   }
   HIPCHK(hipGetLastError());
   h->cont_armed = true;
@@ -1677,10 +1677,10 @@ int cylpic_continuity_end(cylpic_ctx* h, double* residual, double* dq, double* q
   for (auto& s : h->sp) {
     if (s.immobile || s.n <= 0) continue;
     const int* ts = s.ranged ? s.tile_start : nullptr;
+    // cq1, cz, czp are consecutive sets of w words
     if (h->prec == CYLPIC_FP64)
-      launch_cont_modes_end_f64(h->g, view64(s), s.n, h->tl.ntiles, ts, extra_of(s), s.m, s.q, cq1, cz, czp,
-                                h->stream);
-    else launch_cont_modes_end_mixed(h->g, viewm(s), s.n, h->tl.ntiles, ts, extra_of(s), s.q, cq1, cz, czp, h->stream);
+      launch_cont_modes_end_f64(h->g, h->tl, view64(s), s.n, ts, extra_of(s), s.m, s.q, cq1, w, h->stream);
+    else launch_cont_modes_end_mixed(h->g, h->tl, viewm(s), s.n, ts, extra_of(s), s.q, cq1, w, h->stream);
   }
   launch_zero_words((uint32_t*)h->cm_out, 8L * h->g.M, h->stream);
   with_fields(h, [&](auto& F) {

@@ -26,7 +26,7 @@
 // (v_perp dt / r)^2 / 4.
 #pragma once
 
-#include "common.h"
+#include "launch.h"
 
 namespace cylpic {
 
@@ -99,8 +99,36 @@ __device__ __forceinline__ bool cm_inside(const Grid& g, int cx, int cy, int wx,
   return cx >= 1 - g.ng && cx + wx - 1 <= g.nx + g.ng && cy >= 1 - g.ng && cy + wy - 1 <= g.ny + g.ng;
 }
 
-// charge modes of one particle at p into the M complex planes Q
-__device__ inline void cm_charge(const Grid& g, double qw, const ContPos& p, double* __restrict__ Q) {
+// Where a word of the check goes.  The check's planes form sets of M
+// complex planes, `stride` words apart (set 0: the charge; at the end also
+// set 1: Z, set 2: Zp); word part (0 real, 1 imaginary) of cell (cx, cy) of
+// mode m is base + set * stride + 2 idx(cx, cy, m) + part.
+struct CmGlobal {
+  double* base;
+  long stride;
+  __device__ __forceinline__ void add(const Grid& g, int set, int cx, int cy, int m, int part, double v) const {
+    unsafeAtomicAdd(base + set * stride + 2 * g.idx(cx, cy, m) + part, v);
+  }
+};
+// ... or a workgroup's LDS box of its tile plus CM_HALO cells, words
+// [set][m][part][BR][BX]; a cell outside the box goes to the planes
+constexpr int CM_HALO = 3;
+extern __shared__ double cm_box[];  // the dynamic LDS of k_cont_modes_tiles
+struct CmBox {
+  int bx0, br0, BX, BR, M;
+  CmGlobal out;
+  __device__ __forceinline__ void add(const Grid& g, int set, int cx, int cy, int m, int part, double v) const {
+    const int lx = cx - bx0, lr = cy - br0;
+    if ((unsigned)lx < (unsigned)BX && (unsigned)lr < (unsigned)BR)
+      atomicAdd(&cm_box[((set * M + m) * 2 + part) * BR * BX + lr * BX + lx], v);
+    else
+      out.add(g, set, cx, cy, m, part, v);
+  }
+};
+
+// charge modes of one particle at p into plane set `set`
+template <class To>
+__device__ inline void cm_charge(const Grid& g, double qw, const ContPos& p, const To& to, int set) {
   double wx[3], wy[3];
   const int cx = cm_weights(p.xr, wx, p.f32), cy = cm_weights(p.rr, wy, p.f32);
   if (!cm_inside(g, cx, cy, 3, 3)) return;
@@ -111,10 +139,9 @@ __device__ inline void cm_charge(const Grid& g, double qw, const ContPos& p, dou
     for (int j = 0; j < 3; ++j)
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        double* q = Q + 2 * g.idx(cx + i, cy + j, m);
         const double v = cm * qw * (wx[i] * wy[j]);
-        unsafeAtomicAdd(q, v * er);
-        unsafeAtomicAdd(q + 1, v * ei);
+        to.add(g, set, cx + i, cy + j, m, 0, v * er);
+        to.add(g, set, cx + i, cy + j, m, 1, v * ei);
       }
     const double nr = er * p.c - ei * p.s, ni = er * p.s + ei * p.c;
     er = nr;
@@ -122,11 +149,12 @@ __device__ inline void cm_charge(const Grid& g, double qw, const ContPos& p, dou
   }
 }
 
-// Z_m and Zp_m of one particle moving from a to b; vtdt_r = v_theta dt / r
-// at t + dt.  Returns false (nothing added) if the move spans more than one
-// cell in x or r or leaves the arrays.
+// Z_m (set 1) and Zp_m (set 2) of one particle moving from a to b;
+// vtdt_r = v_theta dt / r at t + dt.  Returns false (nothing added) if the
+// move spans more than one cell in x or r or leaves the arrays.
+template <class To>
 __device__ inline bool cm_azimuthal(const Grid& g, double qw, const ContPos& a, const ContPos& b, double vtdt_r,
-                                    double* __restrict__ Z, double* __restrict__ Zp) {
+                                    const To& to) {
   double wa[3], wb[3], ra[3], rb[3];
   const int ax = cm_weights(a.xr, wa, a.f32), bx = cm_weights(b.xr, wb, b.f32);
   const int ay = cm_weights(a.rr, ra, a.f32), by = cm_weights(b.rr, rb, b.f32);
@@ -162,17 +190,66 @@ __device__ inline bool cm_azimuthal(const Grid& g, double qw, const ContPos& a, 
         const double vr = f2.re * A + f3.re * B + f4.re * C, vi = f2.im * A + f3.im * B + f4.im * C;
         if (vr == 0.0 && vi == 0.0) continue;
         const double tr = pr * vr - pi * vi, ti = pr * vi + pi * vr;
-        const long k = 2 * g.idx(x0 + i, y0 + j, m);
-        unsafeAtomicAdd(Z + k, kz * tr);
-        unsafeAtomicAdd(Z + k + 1, kz * ti);
-        unsafeAtomicAdd(Zp + k, kp * tr);
-        unsafeAtomicAdd(Zp + k + 1, kp * ti);
+        to.add(g, 1, x0 + i, y0 + j, m, 0, kz * tr);
+        to.add(g, 1, x0 + i, y0 + j, m, 1, kz * ti);
+        to.add(g, 2, x0 + i, y0 + j, m, 0, kp * tr);
+        to.add(g, 2, x0 + i, y0 + j, m, 1, kp * ti);
       }
     const double nr = er * a.c - ei * a.s, ni = er * a.s + ei * a.c;
     er = nr;
     ei = ni;
   }
   return true;
+}
+
+// One workgroup per tile: the particles of the tile's ranges (its sorted
+// range, then its appended range) add through `part(g, i, box)` into the
+// LDS box of `nsets` plane sets, which is then added to the planes with
+// one atomic per nonzero word.  The box holds 2 nsets M (tx + 6)(tr + 6)
+// doubles.
+template <class Part>
+__global__ __launch_bounds__(256) void k_cont_modes_tiles(Grid g, Tiles tl, const int* __restrict__ key,
+                                                          const int* __restrict__ tile_start, ExtraRange xr,
+                                                          int nsets, Part part, CmGlobal out) {
+  double* cmbox = cm_box;
+  const int tile = tile_of_block(blockIdx.x, tl);
+  const int ttx = tile % tl.ntx, tty = tile / tl.ntx;
+  const CmBox b{tl.x0(ttx) - CM_HALO, tty * tl.tr + 1 - CM_HALO, tl.tx + 2 * CM_HALO, tl.tr + 2 * CM_HALO, g.M, out};
+  const int ncell = b.BX * b.BR;
+  const int nw = nsets * g.M * 2 * ncell;
+  for (int e = threadIdx.x; e < nw; e += blockDim.x) cmbox[e] = 0.0;
+  __syncthreads();
+  const int p0 = tile_start[tile], p1 = tile_start[tile + 1];
+  const int q0 = xr.start ? xr.base + xr.start[tile] : 0, q1 = xr.start ? xr.base + xr.start[tile + 1] : 0;
+  const int np_main = p1 - p0, np = np_main + (q1 - q0);
+  for (int k = threadIdx.x; k < np; k += blockDim.x) {
+    const int i = k < np_main ? p0 + k : q0 + (k - np_main);
+    if ((unsigned)key[i] >= (unsigned)tl.ntiles) continue;
+    part(g, i, b);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nw; e += blockDim.x) {
+    const double v = cmbox[e];
+    if (v == 0.0) continue;
+    int rem = e;
+    const int lx = rem % b.BX;
+    rem /= b.BX;
+    const int lr = rem % b.BR;
+    rem /= b.BR;
+    const int w = rem % 2;
+    rem /= 2;
+    const int m = rem % g.M, set = rem / g.M;
+    const int cx = b.bx0 + lx, cy = b.br0 + lr;
+    if (cx < 1 - g.ng || cx > g.nx + g.ng || cy < 1 - g.ng || cy > g.ny + g.ng) continue;
+    out.add(g, set, cx, cy, m, w, v);
+  }
+}
+
+// the LDS the tiled check needs, or 0 when it exceeds the default 64 KB
+// (the per-particle kernels run instead)
+inline size_t cm_tiles_lds(const Grid& g, const Tiles& tl, int nsets) {
+  const size_t b = (size_t)nsets * g.M * 2 * (tl.tx + 2 * CM_HALO) * (tl.tr + 2 * CM_HALO) * sizeof(double);
+  return b <= 65536 ? b : 0;
 }
 
 }  // namespace cylpic

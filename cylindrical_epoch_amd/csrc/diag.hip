@@ -485,45 +485,56 @@ __global__ __launch_bounds__(256) void k_gauss_m0(Grid g, const double* __restri
 
 // Continuity of every mode, FP64 path: the charge modes at the push's half
 // step (the arithmetic of k_charge_half) before the push ...
-__global__ __launch_bounds__(256) void k_cont_modes_begin_f64(Grid g, PartF64 p, long n, int ntiles,
-                                                              const int* __restrict__ ts, ExtraRange xr, double mc,
-                                                              double q, double* __restrict__ q0) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)p.key[i] >= (unsigned)ntiles) return;
-  const double ux = p.px[i] / mc, uy = p.py[i] / mc, uz = p.pz[i] / mc;
-  const double root = (C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
-  const double xh = p.x[i] + ux * root, yh = p.y[i] + uy * root, zh = p.z[i] + uz * root;
-  const double rh = sqrt(yh * yh + zh * zh);
-  const ContPos a{(xh - g.x_grid_min) / g.dx, (rh - g.y_grid_min) / g.dy, rh > 0.0 ? yh / rh : 1.0,
-                  rh > 0.0 ? zh / rh : 0.0, false};
-  cm_charge(g, q * p.w[i], a, q0);
-}
+struct CmBeginF64 {
+  PartF64 p;
+  double mc, q;
+  template <class To>
+  __device__ void operator()(const Grid& g, long i, const To& to) const {
+    const double ux = p.px[i] / mc, uy = p.py[i] / mc, uz = p.pz[i] / mc;
+    const double root = (C_LIGHT * (g.dt / 2.0)) / sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+    const double xh = p.x[i] + ux * root, yh = p.y[i] + uy * root, zh = p.z[i] + uz * root;
+    const double rh = sqrt(yh * yh + zh * zh);
+    const ContPos a{(xh - g.x_grid_min) / g.dx, (rh - g.y_grid_min) / g.dy, rh > 0.0 ? yh / rh : 1.0,
+                    rh > 0.0 ? zh / rh : 0.0, false};
+    cm_charge(g, q * p.w[i], a, to, 0);
+  }
+};
 
 // ... and after it: the charge modes at t + 3dt/2 = x(t+dt) + v dt/2 and the
 // azimuthal terms of the move from x(t+dt) - v dt/2 to there
-__global__ __launch_bounds__(256) void k_cont_modes_end_f64(Grid g, PartF64 p, long n, int ntiles,
-                                                            const int* __restrict__ ts, ExtraRange xr, double mc,
-                                                            double q, double* __restrict__ q1, double* __restrict__ z,
-                                                            double* __restrict__ zp) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)p.key[i] >= (unsigned)ntiles) return;
-  const double ux = p.px[i] / mc, uy = p.py[i] / mc, uz = p.pz[i] / mc;
-  const double gam = sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
-  const double root = (C_LIGHT * (g.dt / 2.0)) / gam;
-  const double x = p.x[i], y = p.y[i], zz = p.z[i];
-  ContPos pos[2];
-  for (int k = 0; k < 2; ++k) {
-    const double sg = k ? 1.0 : -1.0;
-    const double xs = x + sg * (ux * root), ys = y + sg * (uy * root), zs = zz + sg * (uz * root);
-    const double rs = sqrt(ys * ys + zs * zs);
-    pos[k] = ContPos{(xs - g.x_grid_min) / g.dx, (rs - g.y_grid_min) / g.dy, rs > 0.0 ? ys / rs : 1.0,
-                     rs > 0.0 ? zs / rs : 0.0, false};
+struct CmEndF64 {
+  PartF64 p;
+  double mc, q;
+  template <class To>
+  __device__ void operator()(const Grid& g, long i, const To& to) const {
+    const double ux = p.px[i] / mc, uy = p.py[i] / mc, uz = p.pz[i] / mc;
+    const double gam = sqrt(((ux * ux + uy * uy) + uz * uz) + 1.0);
+    const double root = (C_LIGHT * (g.dt / 2.0)) / gam;
+    const double x = p.x[i], y = p.y[i], zz = p.z[i];
+    ContPos pos[2];
+    for (int k = 0; k < 2; ++k) {
+      const double sg = k ? 1.0 : -1.0;
+      const double xs = x + sg * (ux * root), ys = y + sg * (uy * root), zs = zz + sg * (uz * root);
+      const double rs = sqrt(ys * ys + zs * zs);
+      pos[k] = ContPos{(xs - g.x_grid_min) / g.dx, (rs - g.y_grid_min) / g.dy, rs > 0.0 ? ys / rs : 1.0,
+                       rs > 0.0 ? zs / rs : 0.0, false};
+    }
+    const double qw = q * p.w[i];
+    cm_charge(g, qw, pos[1], to, 0);
+    const double r1 = sqrt(y * y + zz * zz);
+    const double vtdt_r = r1 > 0.0 ? ((-(uy * zz) + uz * y) / r1 / gam) * (C_LIGHT * g.dt) / r1 : 0.0;
+    cm_azimuthal(g, qw, pos[0], pos[1], vtdt_r, to);
   }
-  const double qw = q * p.w[i];
-  cm_charge(g, qw, pos[1], q1);
-  const double r1 = sqrt(y * y + zz * zz);
-  const double vtdt_r = r1 > 0.0 ? ((-(uy * zz) + uz * y) / r1 / gam) * (C_LIGHT * g.dt) / r1 : 0.0;
-  cm_azimuthal(g, qw, pos[0], pos[1], vtdt_r, z, zp);
+};
+
+// one thread per slot (species without tile ranges)
+template <class Part>
+__global__ __launch_bounds__(256) void k_cont_modes_flat(Grid g, long n, int ntiles, const int* __restrict__ key,
+                                                         const int* __restrict__ ts, ExtraRange xr, Part part,
+                                                         CmGlobal out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)key[i] >= (unsigned)ntiles) return;
+  part(g, i, out);
 }
 
 template <class T>
@@ -639,17 +650,27 @@ void launch_continuity_m0(const Grid& g, const double* q0, const double* q1, con
     hipLaunchKernelGGL(k_continuity_m0<T>, dim3(std::min(nblk(n), 1024u)), dim3(256), 0, s, g, q0, q1, jx, jr, geo,
                        out);
 }
-void launch_cont_modes_begin_f64(const Grid& g, const PartF64& p, long n, int ntiles, const int* ts, ExtraRange xr,
-                                 double mass, double q, double* q0, hipStream_t s) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_cont_modes_begin_f64, dim3(nblk(n)), dim3(256), 0, s, g, p, n, ntiles, ts, xr,
-                       mass * C_LIGHT, q, q0);
+// the charge set (nsets 1) or the charge, Z and Zp sets (3) of a check
+template <class Part>
+static void launch_cont_modes(const Grid& g, const Tiles& tl, long n, const int* key, const int* ts, ExtraRange xr,
+                              const Part& part, int nsets, double* planes, long stride, hipStream_t s) {
+  if (n <= 0) return;
+  const CmGlobal out{planes, stride};
+  const size_t lds = ts ? cm_tiles_lds(g, tl, nsets) : 0;
+  if (lds)
+    hipLaunchKernelGGL(k_cont_modes_tiles<Part>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, key, ts, xr, nsets, part,
+                       out);
+  else
+    hipLaunchKernelGGL(k_cont_modes_flat<Part>, dim3(nblk(n)), dim3(256), 0, s, g, n, tl.ntiles, key, ts, xr, part,
+                       out);
 }
-void launch_cont_modes_end_f64(const Grid& g, const PartF64& p, long n, int ntiles, const int* ts, ExtraRange xr,
-                               double mass, double q, double* q1, double* z, double* zp, hipStream_t s) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_cont_modes_end_f64, dim3(nblk(n)), dim3(256), 0, s, g, p, n, ntiles, ts, xr,
-                       mass * C_LIGHT, q, q1, z, zp);
+void launch_cont_modes_begin_f64(const Grid& g, const Tiles& tl, const PartF64& p, long n, const int* ts,
+                                 ExtraRange xr, double mass, double q, double* q0, hipStream_t s) {
+  launch_cont_modes(g, tl, n, p.key, ts, xr, CmBeginF64{p, mass * C_LIGHT, q}, 1, q0, 0, s);
+}
+void launch_cont_modes_end_f64(const Grid& g, const Tiles& tl, const PartF64& p, long n, const int* ts,
+                               ExtraRange xr, double mass, double q, double* planes, long stride, hipStream_t s) {
+  launch_cont_modes(g, tl, n, p.key, ts, xr, CmEndF64{p, mass * C_LIGHT, q}, 3, planes, stride, s);
 }
 template <class T>
 void launch_continuity_modes(const Grid& g, const double* q0, const double* q1, const double* z, const double* zp,

@@ -236,14 +236,16 @@ void launch_continuity_m0(const Grid&, const double* q0, const double* q1, const
 // out[4 m .. 4 m + 3] (bit patterns, zeroed by the caller)
 // (ts, xr: the species' tile ranges and appended ranges, which hold its
 // particles; ts null: every slot below n is a particle)
-void launch_cont_modes_begin_f64(const Grid&, const PartF64&, long n, int ntiles, const int* ts, ExtraRange xr,
+// (planes: the charge set at the end, then Z and Zp, `stride` words apart;
+// with tile ranges a workgroup per tile sums into LDS first)
+void launch_cont_modes_begin_f64(const Grid&, const Tiles&, const PartF64&, long n, const int* ts, ExtraRange xr,
                                  double mass, double q, double* q0, hipStream_t);
-void launch_cont_modes_end_f64(const Grid&, const PartF64&, long n, int ntiles, const int* ts, ExtraRange xr,
-                               double mass, double q, double* q1, double* z, double* zp, hipStream_t);
-void launch_cont_modes_begin_mixed(const Grid&, const PartMixed&, long n, int ntiles, const int* ts, ExtraRange xr,
+void launch_cont_modes_end_f64(const Grid&, const Tiles&, const PartF64&, long n, const int* ts, ExtraRange xr,
+                               double mass, double q, double* planes, long stride, hipStream_t);
+void launch_cont_modes_begin_mixed(const Grid&, const Tiles&, const PartMixed&, long n, const int* ts, ExtraRange xr,
                                    double q, double* q0, hipStream_t);
-void launch_cont_modes_end_mixed(const Grid&, const PartMixed&, long n, int ntiles, const int* ts, ExtraRange xr,
-                                 double q, double* q1, double* z, double* zp, hipStream_t);
+void launch_cont_modes_end_mixed(const Grid&, const Tiles&, const PartMixed&, long n, const int* ts, ExtraRange xr,
+                                 double q, double* planes, long stride, hipStream_t);
 template <class T>
 void launch_continuity_modes(const Grid&, const double* q0, const double* q1, const double* z, const double* zp,
                              const cplx<T>* jx, const cplx<T>* jr, const Geo& geo, unsigned long long* out,

@@ -834,48 +834,61 @@ __global__ __launch_bounds__(256) void k_charge_half_mixed(Grid g, MixConst mc, 
 
 // Continuity of every mode, mixed path: the charge modes at the push's own
 // FP32 half step before the push ...
-__global__ __launch_bounds__(256) void k_cont_modes_begin_mixed(Grid g, MixConst mc, PartMixed sp, long n, int ntiles,
-                                                                const int* __restrict__ ts, ExtraRange xr, double q,
-                                                                double* __restrict__ q0) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)sp.key[i] >= (unsigned)ntiles) return;
-  const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], sp.ux[i], sp.uy[i], sp.uz[i]);
-  const ContPos a{(double)(sp.cell[i] - 1) + (double)hs.oxh, (double)hs.cyr, (double)(hs.yh * hs.irh),
-                  (double)(hs.zh * hs.irh), true};
-  cm_charge(g, q * (double)sp.w[i], a, q0);
-}
+struct CmBeginMixed {
+  MixConst mc;
+  PartMixed sp;
+  double q;
+  template <class To>
+  __device__ void operator()(const Grid& g, long i, const To& to) const {
+    const HalfStep hs = half_step(mc, sp.ox[i], sp.y[i], sp.z[i], sp.ux[i], sp.uy[i], sp.uz[i]);
+    const ContPos a{(double)(sp.cell[i] - 1) + (double)hs.oxh, (double)hs.cyr, (double)(hs.yh * hs.irh),
+                    (double)(hs.zh * hs.irh), true};
+    cm_charge(g, q * (double)sp.w[i], a, to, 0);
+  }
+};
 
 // ... and after it, from the stored state t + dt and the push's FP32 half
 // move (y, z in units of dy, x as cell plus offset)
-__global__ __launch_bounds__(256) void k_cont_modes_end_mixed(Grid g, MixConst mc, PartMixed sp, long n, int ntiles,
-                                                              const int* __restrict__ ts, ExtraRange xr, double q,
-                                                              double* __restrict__ q1, double* __restrict__ z,
-                                                              double* __restrict__ zp) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)sp.key[i] >= (unsigned)ntiles) return;
-  const float ux = sp.ux[i], uy = sp.uy[i], uz = sp.uz[i];
-  const float ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
-  const float dxn = ux * (mc.kx * ig), dyn = uy * (mc.ky * ig), dzn = uz * (mc.ky * ig);
-  const float ox = sp.ox[i], y = sp.y[i], zz = sp.z[i];
-  // the radius and direction with the push's FP32 operations
-  // (t + 3dt/2: exactly its values; t + dt/2: within one rounding)
-  ContPos pos[2];
-  for (int k = 0; k < 2; ++k) {
-    const float oxs = k ? ox + dxn : ox - dxn, ys = k ? y + dyn : y - dyn, zs = k ? zz + dzn : zz - dzn;
-    const float rs2 = ys * ys + zs * zs;
-    const float irs = __builtin_amdgcn_rsqf(rs2);
-    const float rs = rs2 * irs;
-    pos[k] = ContPos{(double)(sp.cell[i] - 1) + (double)oxs, (double)(rs - mc.yoff), (double)(ys * irs),
-                     (double)(zs * irs), true};
+struct CmEndMixed {
+  MixConst mc;
+  PartMixed sp;
+  double q;
+  template <class To>
+  __device__ void operator()(const Grid& g, long i, const To& to) const {
+    const float ux = sp.ux[i], uy = sp.uy[i], uz = sp.uz[i];
+    const float ig = __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz));
+    const float dxn = ux * (mc.kx * ig), dyn = uy * (mc.ky * ig), dzn = uz * (mc.ky * ig);
+    const float ox = sp.ox[i], y = sp.y[i], zz = sp.z[i];
+    // the radius and direction with the push's FP32 operations
+    // (t + 3dt/2: exactly its values; t + dt/2: within one rounding)
+    ContPos pos[2];
+    for (int k = 0; k < 2; ++k) {
+      const float oxs = k ? ox + dxn : ox - dxn, ys = k ? y + dyn : y - dyn, zs = k ? zz + dzn : zz - dzn;
+      const float rs2 = ys * ys + zs * zs;
+      const float irs = __builtin_amdgcn_rsqf(rs2);
+      const float rs = rs2 * irs;
+      pos[k] = ContPos{(double)(sp.cell[i] - 1) + (double)oxs, (double)(rs - mc.yoff), (double)(ys * irs),
+                       (double)(zs * irs), true};
+    }
+    const double qw = q * (double)sp.w[i];
+    cm_charge(g, qw, pos[1], to, 0);
+    const double yd = y, zd = zz, r1 = sqrt(yd * yd + zd * zd);
+    const double ud[3] = {ux, uy, uz};
+    const double gam = sqrt(1.0 + (ud[0] * ud[0] + ud[1] * ud[1] + ud[2] * ud[2]));
+    // v_theta dt / r in cells: u_theta / gamma (c dt / dy) / r, c dt / dy = 2 ky
+    const double vtdt_r = r1 > 0.0 ? ((-(ud[1] * zd) + ud[2] * yd) / r1 / gam) * (2.0 * (double)mc.ky) / r1 : 0.0;
+    cm_azimuthal(g, qw, pos[0], pos[1], vtdt_r, to);
   }
-  const double qw = q * (double)sp.w[i];
-  cm_charge(g, qw, pos[1], q1);
-  const double yd = y, zd = zz, r1 = sqrt(yd * yd + zd * zd);
-  const double ud[3] = {ux, uy, uz};
-  const double gam = sqrt(1.0 + (ud[0] * ud[0] + ud[1] * ud[1] + ud[2] * ud[2]));
-  // v_theta dt / r in cells: u_theta / gamma (c dt / dy) / r, c dt / dy = 2 ky
-  const double vtdt_r = r1 > 0.0 ? ((-(ud[1] * zd) + ud[2] * yd) / r1 / gam) * (2.0 * (double)mc.ky) / r1 : 0.0;
-  cm_azimuthal(g, qw, pos[0], pos[1], vtdt_r, z, zp);
+};
+
+// one thread per slot (species without tile ranges)
+template <class Part>
+__global__ __launch_bounds__(256) void k_cont_modes_flat_mixed(Grid g, long n, int ntiles, const int* __restrict__ key,
+                                                               const int* __restrict__ ts, ExtraRange xr, Part part,
+                                                               CmGlobal out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !cm_live(i, ts, xr.start, xr.base, ntiles) || (unsigned)key[i] >= (unsigned)ntiles) return;
+  part(g, i, out);
 }
 
 // ------------------------------------------------------------------------
@@ -998,19 +1011,28 @@ struct ChargeOpMixed {
   __device__ long out_index(const Grid& gg, int cx, int cy, int) const { return gg.idx(cx, cy, 0); }
 };
 
-void launch_cont_modes_begin_mixed(const Grid& g, const PartMixed& sp, long n, int ntiles, const int* ts,
-                                   ExtraRange xr, double q, double* q0, hipStream_t s) {
+template <class Part>
+static void launch_cont_modes_mixed(const Grid& g, const Tiles& tl, long n, const int* key, const int* ts,
+                                    ExtraRange xr, const Part& part, int nsets, double* planes, long stride,
+                                    hipStream_t s) {
   if (n <= 0) return;
-  const MixConst mc = mix_const(g, PushConst{}, q);
-  hipLaunchKernelGGL(k_cont_modes_begin_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, mc, sp, n,
-                     ntiles, ts, xr, q, q0);
+  const CmGlobal out{planes, stride};
+  const size_t lds = ts ? cm_tiles_lds(g, tl, nsets) : 0;
+  if (lds)
+    hipLaunchKernelGGL(k_cont_modes_tiles<Part>, dim3(tl.ntiles), dim3(256), lds, s, g, tl, key, ts, xr, nsets, part,
+                       out);
+  else
+    hipLaunchKernelGGL(k_cont_modes_flat_mixed<Part>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, n,
+                       tl.ntiles, key, ts, xr, part, out);
 }
-void launch_cont_modes_end_mixed(const Grid& g, const PartMixed& sp, long n, int ntiles, const int* ts,
-                                 ExtraRange xr, double q, double* q1, double* z, double* zp, hipStream_t s) {
-  if (n <= 0) return;
-  const MixConst mc = mix_const(g, PushConst{}, q);
-  hipLaunchKernelGGL(k_cont_modes_end_mixed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, mc, sp, n,
-                     ntiles, ts, xr, q, q1, z, zp);
+void launch_cont_modes_begin_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, long n, const int* ts,
+                                   ExtraRange xr, double q, double* q0, hipStream_t s) {
+  launch_cont_modes_mixed(g, tl, n, sp.key, ts, xr, CmBeginMixed{mix_const(g, PushConst{}, q), sp, q}, 1, q0, 0, s);
+}
+void launch_cont_modes_end_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, long n, const int* ts,
+                                 ExtraRange xr, double q, double* planes, long stride, hipStream_t s) {
+  launch_cont_modes_mixed(g, tl, n, sp.key, ts, xr, CmEndMixed{mix_const(g, PushConst{}, q), sp, q}, 3, planes,
+                          stride, s);
 }
 
 void launch_charge_half_mixed(const Grid& g, const Tiles& tl, const PartMixed& sp, const int* tile_start,
