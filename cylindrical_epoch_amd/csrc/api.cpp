@@ -1091,7 +1091,8 @@ int cylpic_upload_particles(cylpic_ctx* h, int species, int64_t n, const double*
   s.n_extra = 0;
   s.ranged = false;
   s.counted = false;
-  // tile-major cell keys: the upload sort groups each tile's particles by cell
+  // tile-major cell keys and half-cell quadrants: the upload sort groups
+  // each tile's particles by cell, and each cell's by quadrant
   if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), n, 1, h->stream);
   else launch_keys_mixed(h->g, h->tl, viewm(s), n, 1, h->stream);
   rc = sort_species(h, s, 1);
