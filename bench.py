@@ -258,6 +258,9 @@ def measured_traffic(precision):
 # ds_add_u64 lane-operations per second over the 256 CUs, random addresses in
 # a 64 KB box (the per-tile current box's atomic), as measured on gfx950
 LDS_ATOMIC_PEAK = 3208e9
+# VALU lane-operations per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz,
+# one FP32 or FP64 operation per lane and cycle (not counting packed FP32)
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
 
 
 def roofline(w, n, prof_push, precision):
@@ -288,6 +291,14 @@ def roofline(w, n, prof_push, precision):
         out["lds_atomic"] = {"achieved": round(rate / 1e9, 1), "peak": LDS_ATOMIC_PEAK / 1e9, "unit": "Gops/s",
                              "frac": round(rate / LDS_ATOMIC_PEAK, 4), "ops_per_launch": ops,
                              "ops_per_particle": round(ops / n, 1)}
+    # the VALU issue rate: the committed SQ_INSTS_VALU reading (wave
+    # instructions, x 64 lanes) over this run's launch time
+    valu = _profile_summary(precision).get("valu_wave_instructions_per_launch")
+    if valu:
+        rate = valu * 64 / avg_s
+        out["valu"] = {"achieved": round(rate / 1e12, 2), "peak": round(VALU_PEAK / 1e12, 2),
+                       "unit": "T lane-ops/s", "frac": round(rate / VALU_PEAK, 4),
+                       "instructions_per_particle": round(valu * 64 / n, 1)}
     return out
 
 
