@@ -96,6 +96,7 @@ struct cylpic_ctx {
   uint32_t* mig = nullptr;  // migration sort scratch (mig_scratch_words(ntiles))
   int* d_sortstat = nullptr;  // device-decided sorts: [radix, migrate]
   bool migrate = true;      // use the neighbour migration sort when it applies
+  bool sort_cells = false;  // every sort re-keys by cell and re-interleaves the tiles
   double* stage = nullptr;  // FP64 staging of host particles for the mixed path
   long stage_cap = 0;
   cplx<double>* spare64 = nullptr;  // field-sized buffer of the moving window's shifts and smoothing
@@ -421,9 +422,9 @@ int sort_species_device(cylpic_ctx* h, Species& s, bool counted) {
   launch_mark_stale(s.key[0], s.tile_start, h->tl.ntiles, s.n - s.n_extra, h->stream, rad);
   radix_sort(s.key[0], s.n, h->tl.ntiles, h->slot_bits, h->sc, s.key[1], &perm, h->stream, rad);
   launch_tile_bounds(s.key[1], s.n, h->tl.ntiles, s.tile_start_alt, h->stream, rad);
-  if (f64) launch_gather_f64(perm, s.n, s.a, s.b, 7, s.key[1], s.tile_start_alt, 0, h->stream, rad);
-  else launch_gather_32(perm, s.n, s.ma, s.mb, 8, s.key[1], s.tile_start_alt, 0, h->stream, rad);
-  if (s.ids) launch_gather_64(perm, s.n, id_in, id_out, s.key[1], s.tile_start_alt, 0, h->stream, rad);
+  if (f64) launch_gather_f64(perm, s.n, s.a, s.b, 7, h->stream, rad);
+  else launch_gather_32(perm, s.n, s.ma, s.mb, 8, h->stream, rad);
+  if (s.ids) launch_gather_64(perm, s.n, id_in, id_out, h->stream, rad);
   launch_guard_count(h->d_sortstat + 0, h->stream, rad);
   HIPCHK(hipGetLastError());
   for (int k = 0; k < narr; ++k) {
@@ -442,11 +443,13 @@ int sort_species_device(cylpic_ctx* h, Species& s, bool counted) {
 // Stable sort of a species by tile key; removed particles (key ntiles) go
 // last.  With `interleave` (on upload) the keys are tile-major cell keys with
 // the half-cell quadrant below them: the sort groups each tile's particles by
-// cell and within a cell by quadrant, and the gather then transposes each
-// tile so that consecutive particles sit in consecutive cells.  A wavefront
-// then holds particles of 64 cells in mostly the same quadrant, whose
-// deposit stencils start on distinct cells (no two lanes adding to one LDS
-// word); later stable sorts by tile keep that order.
+// cell and within a cell by quadrant, and the gather then lays each tile out
+// in rounds (the r-th particle of every cell, cell after cell), so that
+// consecutive particles sit in consecutive cells.  A wavefront then holds
+// particles of 64 cells in mostly the same quadrant, whose deposit stencils
+// start on distinct cells (no two lanes adding to one LDS word).  Later
+// stable sorts by tile keep that order while particles drift; the cell sort
+// method (sort_cells) re-keys and re-interleaves at every sort.
 int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   Scope sc(h, "sort");
   s.dirty = false;
@@ -466,6 +469,13 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   h->sort_radix += 1;
   // slots a device-decided sort left behind the kept particles sort last
   if (!s.n_exact) launch_mark_stale(s.key[0], s.tile_start, h->tl.ntiles, s.n - s.n_extra, h->stream);
+  if (!interleave && h->sort_cells && s.ranged) {
+    // tile keys -> cell keys x 4 + quadrant; removed particles (tile key
+    // ntiles) -> ntiles * cells * 4
+    if (h->prec == CYLPIC_FP64) launch_keys_f64(h->g, h->tl, view64(s), s.n, 2, h->stream);
+    else launch_keys_mixed(h->g, h->tl, viewm(s), s.n, 2, h->stream);
+    interleave = 1;
+  }
   uint32_t* perm = nullptr;
   // with `interleave` the keys are 4 x tile-major cell keys + half-cell
   // quadrant (upload): each cell's particles grouped by quadrant
@@ -473,18 +483,22 @@ int sort_species(cylpic_ctx* h, Species& s, int interleave = 0) {
   radix_sort(s.key[0], s.n, interleave ? h->tl.ntiles * cells * 4 : h->tl.ntiles, h->slot_bits, h->sc, s.key[1],
              &perm, h->stream);
   std::swap(s.key[0], s.key[1]);
-  if (interleave) launch_div_keys(s.key[0], s.n, (uint32_t)(cells * 4), h->stream);
+  if (interleave) {
+    // the sources in rounds layout go to a radix key buffer, free after the sort
+    launch_round_map(s.key[0], perm, s.n, h->tl.ntiles, cells, h->sc.k[0], h->stream);
+    perm = h->sc.k[0];
+    launch_div_keys(s.key[0], s.n, (uint32_t)(cells * 4), h->stream);
+  }
   launch_tile_bounds(s.key[0], s.n, h->tl.ntiles, s.tile_start, h->stream);
   if (h->prec == CYLPIC_FP64) {
-    launch_gather_f64(perm, s.n, s.a, s.b, 7, s.key[0], s.tile_start, cells, h->stream);
+    launch_gather_f64(perm, s.n, s.a, s.b, 7, h->stream);
     for (int k = 0; k < 7; ++k) std::swap(s.a[k], s.b[k]);
   } else {
-    launch_gather_32(perm, s.n, s.ma, s.mb, 8, s.key[0], s.tile_start, cells, h->stream);
+    launch_gather_32(perm, s.n, s.ma, s.mb, 8, h->stream);
     for (int k = 0; k < 8; ++k) std::swap(s.ma[k], s.mb[k]);
   }
   if (s.ids) {
-    launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], s.key[0], s.tile_start, cells,
-                     h->stream);
+    launch_gather_64(perm, s.n, (const uint64_t*)s.id[0], (uint64_t*)s.id[1], h->stream);
     std::swap(s.id[0], s.id[1]);
   }
   HIPCHK(hipGetLastError());
@@ -1024,9 +1038,10 @@ int cylpic_tune(cylpic_ctx* h, int acc_mode, int gather_lds) {
 
 int cylpic_set_sort_method(cylpic_ctx* h, int method) {
   CHECK_H(h);
-  if (method != CYLPIC_SORT_RADIX && method != CYLPIC_SORT_MIGRATE)
-    return fail(CYLPIC_EINVAL, "sort method must be CYLPIC_SORT_RADIX or CYLPIC_SORT_MIGRATE");
+  if (method != CYLPIC_SORT_RADIX && method != CYLPIC_SORT_MIGRATE && method != CYLPIC_SORT_CELLS)
+    return fail(CYLPIC_EINVAL, "sort method must be CYLPIC_SORT_RADIX, CYLPIC_SORT_MIGRATE or CYLPIC_SORT_CELLS");
   h->migrate = method == CYLPIC_SORT_MIGRATE;
+  h->sort_cells = method == CYLPIC_SORT_CELLS;
   return CYLPIC_OK;
 }
 
@@ -1149,7 +1164,7 @@ int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double*
       sa[k] = s.a[k] + xbase;
       sb[k] = s.b[k] + xbase;
     }
-    launch_gather_f64(perm, nr, sa, sb, 7, nullptr, nullptr, 0, h->stream);
+    launch_gather_f64(perm, nr, sa, sb, 7, h->stream);
   } else {
     uint32_t* sa[8];
     uint32_t* sb[8];
@@ -1157,7 +1172,7 @@ int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double*
       sa[k] = s.ma[k] + xbase;
       sb[k] = s.mb[k] + xbase;
     }
-    launch_gather_32(perm, nr, sa, sb, 8, nullptr, nullptr, 0, h->stream);
+    launch_gather_32(perm, nr, sa, sb, 8, h->stream);
   }
   for (int k = 0; k < narr; ++k) {
     void* dst = f64 ? (void*)(s.a[k] + xbase) : (void*)(s.ma[k] + xbase);
@@ -1165,7 +1180,7 @@ int cylpic_inject_particles(cylpic_ctx* h, int species, int64_t n, const double*
     HIPCHK(hipMemcpyAsync(dst, from, nr * wb, hipMemcpyDeviceToDevice, h->stream));
   }
   if (s.ids) {
-    launch_gather_64(perm, nr, (const uint64_t*)s.id[0] + xbase, (uint64_t*)s.id[1] + xbase, nullptr, nullptr, 0,
+    launch_gather_64(perm, nr, (const uint64_t*)s.id[0] + xbase, (uint64_t*)s.id[1] + xbase,
                      h->stream);
     HIPCHK(hipMemcpyAsync(s.id[0] + xbase, s.id[1] + xbase, nr * sizeof(int64_t), hipMemcpyDeviceToDevice,
                           h->stream));
@@ -2172,7 +2187,7 @@ void run_signature(std::string& s, const cylpic_ctx* h, int period) {
                         (const void*)h->sc.v[1], (const void*)h->sc.counts, (const void*)h->sc.scan_tmp})
     put(s, p);
   put(s, h->sc.cap);
-  for (int v : {h->prec, h->ntab, h->sort_interval, h->slot_bits, (int)h->migrate, h->acc_mode, h->gather_lds,
+  for (int v : {h->prec, h->ntab, h->sort_interval, h->slot_bits, (int)h->migrate, (int)h->sort_cells, h->acc_mode, h->gather_lds,
                 h->smooth_its, h->smooth_comp_its, (int)h->cont_armed, (int)(h->push_count % period)})
     put(s, v);
   for (int v : h->smooth_strides) put(s, v);

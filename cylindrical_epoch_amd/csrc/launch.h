@@ -111,7 +111,9 @@ struct ExtraRange {
 void launch_push_deposit_f64(const Grid&, const Tiles&, const PushConst&, const PartF64&,
                              const int* tile_start, ExtraRange, FieldSet<double>&, const Geo&, int ldsf, int* err,
                              MigCount, hipStream_t);
-// key of every particle: its tile, or with cell_key its tile-major cell (Tiles::key)
+// key of every particle: its tile (cell_key 0), or 4 x its tile-major cell
+// (Tiles::key) + half-cell quadrant (1; 2: the same, except that a particle
+// whose key marks it removed, >= ntiles, gets ntiles * cells * 4)
 void launch_keys_f64(const Grid&, const Tiles&, const PartF64&, long n, int cell_key, hipStream_t);
 // push_mixed.hip
 int launch_push_deposit_mixed(const Grid&, const Tiles&, const PushConst&, double charge, const PartMixed&,
@@ -180,15 +182,19 @@ void launch_zero_words(void* p, long n, hipStream_t);
 void launch_relabel_tiles(uint32_t* key, long n, int ntiles, const int* ts_in, int* ts_out, hipStream_t);
 // key[i] /= div (tile-major cell keys -> tile keys)
 void launch_div_keys(uint32_t* key, long n, uint32_t div, hipStream_t);
-// Payload permutation dst[d(i)] = src[perm[i]]; d(i) = i, or with `inter`
-// = the cells of a tile (> 0) the per-tile interleave of sort.hip
-// (tile_start must describe the sorted keys).
+// map[dest(i)] = perm[i], dest(i) the place of sorted index i under the
+// per-tile rounds interleave, from the sorted tile-major cell keys (x 4 +
+// quadrant) of n particles, removed ones keyed ntiles * cells * 4: the
+// gather by map (in place of perm) writes that layout
+void launch_round_map(const uint32_t* sorted_cell_keys, const uint32_t* perm, long n, int ntiles, int cells,
+                      uint32_t* map, hipStream_t);
+// Payload permutation dst[i] = src[perm[i]]
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst,
-                       int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t, Guard = Guard{});
+                       int narr, hipStream_t, Guard = Guard{});
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst,
-                      int narr, const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t, Guard = Guard{});
-void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst,
-                      const uint32_t* sorted_tiles, const int* tile_start, int inter, hipStream_t, Guard = Guard{});
+                      int narr, hipStream_t, Guard = Guard{});
+void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, hipStream_t,
+                      Guard = Guard{});
 
 // Neighbour migration sort (keys tile-sorted before the last push).
 // launch_mig_count fills the direction counts (as the push kernels do);

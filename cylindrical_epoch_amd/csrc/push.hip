@@ -638,6 +638,11 @@ void launch_window_f64(const Grid& g, const PartF64& sp, long n, int ntiles, hip
 __global__ void k_keys_f64(Grid g, Tiles tl, PartF64 sp, long n, int cell_key) {
   const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= n) return;
+  // 2: re-key after pushes; removed particles keep sorting last
+  if (cell_key == 2 && sp.key[ip] >= tl.ntiles) {
+    sp.key[ip] = tl.ntiles * tl.tx * tl.tr * 4;
+    return;
+  }
   const double x = sp.x[ip], y = sp.y[ip], z = sp.z[ip];
   int k = key_f64(g, tl, x, y, z, cell_key != 0);
   if (cell_key) {

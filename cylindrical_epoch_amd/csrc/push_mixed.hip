@@ -898,6 +898,11 @@ __global__ __launch_bounds__(256) void k_cont_modes_flat_mixed(Grid g, long n, i
 __global__ void k_keys_mixed(Grid g, Tiles tl, MixConst mc, PartMixed sp, long n, int cell_key) {
   const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= n) return;
+  // 2: re-key after pushes; removed particles keep sorting last
+  if (cell_key == 2 && sp.key[ip] >= tl.ntiles) {
+    sp.key[ip] = tl.ntiles * tl.tx * tl.tr * 4;
+    return;
+  }
   const float y = sp.y[ip], z = sp.z[ip];
   const float fy = sqrtf(y * y + z * z) - mc.yoff + 1.5f;
   const int cyk = (int)floorf(fy);

@@ -311,38 +311,80 @@ struct Ptr8 {
   void* dst[8];
 };
 
-// Destination of sorted index i.  With `cells` > 0 (the cells of a tile),
-// the particles of each tile segment [s, s+L) are transposed as an S-column
-// table, S = max(1, L / cells) the tile's mean particles per cell: the one at
-// segment offset j goes to column j mod S, row j / S, columns laid out one
-// after another (ragged: the first L mod S columns hold one more row).
-// Consecutive destinations then come from positions S apart in the stable
-// order, i.e. (for a cell-grouped load) from consecutive cells, which puts
-// the 64 lanes of a wavefront on 64 different cells whose current-box words
-// are consecutive.
-__device__ __forceinline__ long dest_of(long i, const uint32_t* __restrict__ key, const int* __restrict__ ts,
-                                        int cells) {
-  if (cells <= 0) return i;
-  const uint32_t t = key[i];
-  const long s = ts[t], L = (long)ts[t + 1] - s, j = i - s;
-  const long S = L / cells > 1 ? L / cells : 1, A = L / S, R = L % S;
-  const long b = j % S, a = j / S;
-  return s + b * A + (b < R ? b : R) + a;
+// Interleave of each tile (rounds): the particles of a tile segment,
+// grouped by cell (ascending cell key, stable within a cell), are laid out as
+// rounds, round r holding the r-th particle of every cell that has more than
+// r, in cell order.  While every cell still has particles a round is the
+// tile's whole cell list, so 64 consecutive particles sit in 64 different
+// cells (and, for a quadrant-sorted load, mostly in one half-cell quadrant),
+// whose deposit stencils start on distinct current-box words, however uneven
+// the cell counts.  With perm[i] the source of sorted index i and dest(i) its
+// place in the rounds layout, map[dest(i)] = perm[i]: the payload gather
+// then reads map in order and writes coalesced.  One workgroup per tile;
+// workgroup ntiles keeps the removed particles (key ntiles * cells * 4,
+// sorted last) in place.  Destination of offset j in cell c at rank r: the
+// sum over cells c' of min(n_c', r) (the earlier rounds), plus the cells
+// c' < c with n_c' > r.
+__device__ __forceinline__ long lower_bound_u32(const uint32_t* __restrict__ a, long lo, long hi, uint32_t v) {
+  while (lo < hi) {
+    const long mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_round_dest(const uint32_t* __restrict__ key, const uint32_t* __restrict__ perm,
+                                                    long n, int ntiles, int cells, uint32_t* __restrict__ map) {
+  extern __shared__ int cnt[];  // cells counts, then cells starts within the segment
+  int* first = cnt + cells;
+  const int t = blockIdx.x;
+  const uint32_t K = (uint32_t)cells * 4u;
+  const long s = lower_bound_u32(key, 0, n, (uint32_t)t * K);
+  const long e = t == ntiles ? n : lower_bound_u32(key, s, n, (uint32_t)(t + 1) * K);
+  if (t == ntiles) {
+    for (long i = s + threadIdx.x; i < e; i += blockDim.x) map[i] = perm[i];
+    return;
+  }
+  for (int c = threadIdx.x; c < cells; c += blockDim.x)
+    first[c] = (int)(lower_bound_u32(key, s, e, ((uint32_t)t * (uint32_t)cells + (uint32_t)c) * 4u) - s);
+  __syncthreads();
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) cnt[c] = (c + 1 < cells ? first[c + 1] : (int)(e - s)) - first[c];
+  __syncthreads();
+  for (long i = s + threadIdx.x; i < e; i += blockDim.x) {
+    const int j = (int)(i - s);
+    const int c = (int)(key[i] / 4u - (uint32_t)t * (uint32_t)cells);
+    const int r = j - first[c];
+    int d = 0;
+    for (int q = 0; q < cells; ++q) {
+      const int nq = cnt[q];
+      d += min(nq, r) + (q < c && nq > r ? 1 : 0);
+    }
+    map[s + d] = perm[i];
+  }
+}
+
+void launch_round_map(const uint32_t* key, const uint32_t* perm, long n, int ntiles, int cells, uint32_t* map,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  const size_t lds = (size_t)(2 * cells) * sizeof(int);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_round_dest), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  hipLaunchKernelGGL(k_round_dest, dim3((unsigned)ntiles + 1), dim3(256), lds, s, key, perm, n, ntiles, cells, map);
 }
 
 template <class W>
-__global__ void k_gather(const uint32_t* __restrict__ perm, long n, Ptr8 p, int narr,
-                         const uint32_t* __restrict__ key, const int* __restrict__ ts, int inter, Guard gd) {
+__global__ void k_gather(const uint32_t* __restrict__ perm, long n, Ptr8 p, int narr, Guard gd) {
   if (guard_skip(gd)) return;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t j = perm[i];
-  const long d = dest_of(i, key, ts, inter);
-  for (int a = 0; a < narr; ++a) ((W*)p.dst[a])[d] = ((const W*)p.src[a])[j];
+  for (int a = 0; a < narr; ++a) ((W*)p.dst[a])[i] = ((const W*)p.src[a])[j];
 }
 
 void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, double* const* dst, int narr,
-                       const uint32_t* key, const int* ts, int inter, hipStream_t s, Guard gd) {
+                       hipStream_t s, Guard gd) {
   Ptr8 p;
   for (int a = 0; a < narr; ++a) {
     p.src[a] = src[a];
@@ -350,10 +392,10 @@ void launch_gather_f64(const uint32_t* perm, long n, const double* const* src, d
   }
   if (n > 0)
     hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr,
-                       key, ts, inter, gd);
+                       gd);
 }
 void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, uint32_t* const* dst, int narr,
-                      const uint32_t* key, const int* ts, int inter, hipStream_t s, Guard gd) {
+                      hipStream_t s, Guard gd) {
   Ptr8 p;
   for (int a = 0; a < narr; ++a) {
     p.src[a] = src[a];
@@ -361,16 +403,15 @@ void launch_gather_32(const uint32_t* perm, long n, const uint32_t* const* src, 
   }
   if (n > 0)
     hipLaunchKernelGGL(k_gather<uint32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, narr,
-                       key, ts, inter, gd);
+                       gd);
 }
-void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, const uint32_t* key,
-                      const int* ts, int inter, hipStream_t s, Guard gd) {
+void launch_gather_64(const uint32_t* perm, long n, const uint64_t* src, uint64_t* dst, hipStream_t s,
+                      Guard gd) {
   Ptr8 p;
   p.src[0] = src;
   p.dst[0] = dst;
   if (n > 0)
-    hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, 1, key,
-                       ts, inter, gd);
+    hipLaunchKernelGGL(k_gather<uint64_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, p, 1, gd);
 }
 
 // ---------------- neighbour migration sort -------------------------------

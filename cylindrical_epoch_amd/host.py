@@ -323,9 +323,10 @@ class Simulation:
         check(lib().cylpic_sort_particles(self.h))
 
     def set_sort_method(self, method):
-        """"radix" (full LSD radix sort every time) or "migrate" (neighbour
-        migration when it applies, radix otherwise)."""
-        check(lib().cylpic_set_sort_method(self.h, {"radix": 0, "migrate": 1}[method]))
+        """"radix" (full LSD radix sort every time), "migrate" (neighbour
+        migration when it applies, radix otherwise) or "cells" (the upload's
+        cell-keyed sort and per-tile rounds interleave every time)."""
+        check(lib().cylpic_set_sort_method(self.h, {"radix": 0, "migrate": 1, "cells": 2}[method]))
 
     def sort_counts(self):
         r, m = C.c_int64(), C.c_int64()

@@ -11,7 +11,8 @@ MODULE cylpic
   IMPLICIT NONE
 
   INTEGER(C_INT), PARAMETER :: CYLPIC_FP64 = 0, CYLPIC_MIXED = 1
-  INTEGER(C_INT), PARAMETER :: CYLPIC_SORT_RADIX = 0, CYLPIC_SORT_MIGRATE = 1
+  INTEGER(C_INT), PARAMETER :: CYLPIC_SORT_RADIX = 0, CYLPIC_SORT_MIGRATE = 1, &
+                              CYLPIC_SORT_CELLS = 2
   ! distribution function directions (c_dir_* of constants.F90)
   INTEGER(C_INT), PARAMETER :: CYLPIC_DIR_X = 1, CYLPIC_DIR_Y = 2, CYLPIC_DIR_PX = 3, &
       CYLPIC_DIR_PY = 4, CYLPIC_DIR_PZ = 5, CYLPIC_DIR_EN = 6, CYLPIC_DIR_GAMMA_M1 = 7, &

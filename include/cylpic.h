@@ -191,6 +191,11 @@ int cylpic_set_sort_interval(cylpic_ctx *h, int interval);
  * sort otherwise.  Both give the stable order by tile key. */
 #define CYLPIC_SORT_RADIX 0
 #define CYLPIC_SORT_MIGRATE 1
+/* CYLPIC_SORT_CELLS: every sort is the upload's sort (cell key and half-cell
+ * quadrant, then each tile laid out in rounds of one particle per cell), so
+ * that the push's wavefronts keep spanning distinct cells as particles drift
+ * between cells; counted as a radix sort. */
+#define CYLPIC_SORT_CELLS 2
 int cylpic_set_sort_method(cylpic_ctx *h, int method);
 /* Number of sorts done so far by each method. */
 int cylpic_sort_counts(cylpic_ctx *h, int64_t *radix, int64_t *migrate);

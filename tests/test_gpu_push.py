@@ -109,27 +109,30 @@ def test_current_finish_after_push(oracle):
     sim.close()
 
 
-def interleave_within_tiles(order, tiles, cells):
-    """The upload's per-tile interleave: the particles of a tile segment of
-    length L (stable order) are laid out as a table of S = max(1, L // cells)
-    columns (`cells` = cells per tile), offset j -> column j % S, row j // S,
-    columns one after the other (the first L % S columns one row longer)."""
-    out = np.empty_like(order)
-    i = 0
+def interleave_within_tiles(order, keys, cells, ntiles):
+    """The per-tile rounds interleave: `order` in the stable order of the
+    sort keys `keys` (4 x tile-major cell key + quadrant; removed particles
+    keyed ntiles * cells * 4, last); each tile's particles are laid out as
+    rounds, round r holding the r-th particle of every cell that has more
+    than r, in cell order.  Written as loops over tiles, rounds and cells."""
+    out = []
     n = len(order)
-    while i < n:
-        t = tiles[i]
-        e = i
-        while e < n and tiles[e] == t:
-            e += 1
-        L = e - i
-        columns = max(1, L // cells)
-        A, R = divmod(L, columns)
-        for j in range(L):
-            b, a = j % columns, j // columns
-            out[i + b * A + min(b, R) + a] = order[i + j]
-        i = e
-    return out
+    cell = keys // 4
+    i = 0
+    while i < n and keys[i] < ntiles * cells * 4:
+        t = keys[i] // (cells * 4)
+        members = {}
+        while i < n and keys[i] // (cells * 4) == t:
+            members.setdefault(int(cell[i]), []).append(order[i])
+            i += 1
+        r = 0
+        while any(len(v) > r for v in members.values()):
+            for c in sorted(members):
+                if len(members[c]) > r:
+                    out.append(members[c][r])
+            r += 1
+    out.extend(order[i:])
+    return np.array(out, dtype=order.dtype)
 
 
 def half_cell_quadrant(st, x, y, z):
@@ -156,7 +159,7 @@ def test_sort_matches_stable_cpu_sort(oracle, tile):
         st, p["x"], p["y"], p["z"])
     ntiles = ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
     perm = oracle.stable_sort(keys, ntiles * cells * 4)
-    want = interleave_within_tiles(p["id"][perm], keys[perm] // (cells * 4), cells)
+    want = interleave_within_tiles(p["id"][perm], keys[perm], cells, ntiles)
     got = sim.download_particles(0)
     assert np.array_equal(got["id"], want)
     # a second, plain sort keeps the order (no particle moved)
@@ -196,6 +199,10 @@ def test_multi_step_thermal(oracle):
     ("fp64", (1, 1), "periodic", 2.0e5, 5, "radix"),
     ("mixed", (8, 4), "periodic", 2.0e5, 1, "migrate"),
     ("mixed", (4, 2), "open", 2.0e5, 1, "migrate"),
+    ("fp64", (8, 4), "periodic", 100.0, 1, "cells"),
+    ("fp64", (4, 4), "open", 2.0e5, 3, "cells"),
+    ("fp64", (16, 8), "periodic", 2.0e5, 2, "cells"),
+    ("mixed", (8, 4), "open", 2.0e5, 2, "cells"),
 ])
 def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, temp_ev, pushes, method):
     """After `pushes` pushes without a sort, the next sort puts the particles
@@ -203,7 +210,8 @@ def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, te
     (bit-exact against the CPU stable sort of the keys).  One push moves a
     particle at most one tile, so the neighbour migration sort applies; five
     pushes of a hot plasma on 1 x 1 tiles move some farther, and the sort
-    falls back to the radix passes."""
+    falls back to the radix passes.  The cell method re-keys by cell and
+    half-cell quadrant and lays each tile out in rounds, as the upload does."""
     log("sort after push", precision, tile, bc, temp_ev, pushes)
     nx, ny, dx, dy = 24, 12, 1e-7, 1e-7
     bcf = (bc, bc, "other", "conduct") if bc == "periodic" else ("simple_outflow", "simple_outflow", "other",
@@ -211,6 +219,8 @@ def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, te
     bcp = (bc, bc, "other", "reflect") if bc == "periodic" else ("open", "open", "other", "open")
     st, sim = make_pair(oracle, nx, ny, 1, dx, dy, bc_field=bcf, bc_particle=bcp, precision=precision,
                         tile=tile, sort_interval=1000)
+    if method == "cells":
+        sim.set_sort_method("cells")
     sim_inputs = add_plasma(st, sim, nx, ny, dx, dy, ppc=6, temp_ev=temp_ev, seed=31, shuffle=True)
     before = sim.download_particles(0)["id"]
     c0 = sim.sort_counts()
@@ -220,8 +230,22 @@ def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, te
             st.push_particles()
     got = sim.download_particles(0)
     c1 = sim.sort_counts()
-    assert c1[method] == c0[method] + 1, (c0, c1)
+    counted = "radix" if method == "cells" else method
+    assert c1[counted] == c0[counted] + 1, (c0, c1)
     ntiles = ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
+    cells = tile[0] * tile[1]
+    if precision == "fp64" and method == "cells":
+        sp = st.species[0]
+        pos = {i: k for k, i in enumerate(sp["id"])}
+        ck = 4 * oracle.cell_keys(st, sp["x"], sp["y"], sp["z"], tile[0], tile[1]) + half_cell_quadrant(
+            st, sp["x"], sp["y"], sp["z"])
+        keys = np.array([ck[pos[i]] if i in pos else ntiles * cells * 4 for i in before], np.int64)
+        perm = oracle.stable_sort(keys, ntiles * cells * 4 + 1)
+        want = interleave_within_tiles(before[perm], keys[perm], cells, ntiles)[: int(np.sum(keys < ntiles * cells * 4))]
+        log(f"  {len(before)} -> {len(want)} particles, sort counts {c1}")
+        assert np.array_equal(got["id"], want)
+        sim.close()
+        return
     if precision == "fp64":
         # positions are bit-exact with the oracle: keys from the oracle's particles
         sp = st.species[0]
@@ -243,6 +267,9 @@ def test_sort_after_push_matches_stable_cpu_sort(oracle, precision, tile, bc, te
         want = ref.download_particles(0)
         assert ref.sort_counts()["migrate"] == 0
         ref.close()
+        if method == "cells":
+            # another order of the same particles: compare them by id
+            got, want = by_id(got), by_id(want)
         for k in ("id", "x", "y", "z", "px", "py", "pz", "w"):
             assert np.array_equal(got[k], want[k]), k
         sim.close()
