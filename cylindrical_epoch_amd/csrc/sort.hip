@@ -47,10 +47,6 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_count(const uint32_t* __re
   for (int d = threadIdx.x; d < RADIX; d += SORT_BLOCK) counts[(long)d * nb + blockIdx.x] = hist[d];
 }
 
-__device__ __forceinline__ uint32_t slot_get(int slot, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
-  return slot == 0 ? c0 : (slot == 1 ? c1 : (slot == 2 ? c2 : c3));
-}
-
 // compose_bits >= 0: first pass (keys are tile keys, values the index);
 // out_shift >= 0: last pass (write key >> out_shift, the tile key).
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ kin,
@@ -64,7 +60,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   const long base = (long)blockIdx.x * SORT_TILE + (long)w * WAVE_ITEMS;
-  uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // running count of digit lane + 64 * slot
+  // this wavefront's running count per digit (its own row of wcnt)
+  for (int d = lane; d < RADIX; d += 64) wcnt[w][d] = 0;
   uint32_t key[IPT], val[IPT], rank[IPT];
 #pragma unroll
   for (int r = 0; r < IPT; ++r) {
@@ -73,33 +70,27 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __
     const uint32_t k = valid ? comp_key(kin, i, compose_bits) : 0u;
     const uint32_t v = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     const uint32_t d = (k >> shift) & (RADIX - 1);
-    unsigned long long active = __ballot(valid);
+    // the lanes holding my digit: the AND over its 8 bits of the ballot of
+    // that bit (or of its complement), 9 ballots whatever the digits
+    unsigned long long same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const uint32_t before = (uint32_t)__popcll(same & lt_mask);
     uint32_t myrank = 0;
-    while (active) {
-      const int leader = __ffsll((long long)active) - 1;
-      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, leader);
-      const unsigned long long same = __ballot(valid && d == d0);
-      const int slot = (int)(d0 >> 6), owner = (int)(d0 & 63);
-      const uint32_t basecnt =
-          (uint32_t)__builtin_amdgcn_readlane((int)slot_get(slot, c0, c1, c2, c3), owner);
-      if (valid && d == d0) myrank = basecnt + (uint32_t)__popcll(same & lt_mask);
-      if (lane == owner) {
-        const uint32_t add = (uint32_t)__popcll(same);
-        if (slot == 0) c0 += add;
-        else if (slot == 1) c1 += add;
-        else if (slot == 2) c2 += add;
-        else c3 += add;
-      }
-      active &= ~same;
+    if (valid) {
+      const uint32_t cnt = wcnt[w][d];
+      myrank = cnt + before;
+      // the digit's first lane advances the count (after every lane read it)
+      if (before == 0) wcnt[w][d] = cnt + (uint32_t)__popcll(same);
     }
     key[r] = k;
     val[r] = v;
     rank[r] = myrank;
   }
-  wcnt[w][lane] = c0;
-  wcnt[w][lane + 64] = c1;
-  wcnt[w][lane + 128] = c2;
-  wcnt[w][lane + 192] = c3;
   __syncthreads();
   // exclusive prefix over the wavefronts of this block, per digit
   for (int d = threadIdx.x; d < RADIX; d += SORT_BLOCK) {
